@@ -1,0 +1,167 @@
+"""Benchmark: particle-steps/s of the p=1 FMM step at 1M bodies on one GPU.
+
+Workload (default): 1,048,576 equal-mass bodies in a seeded Plummer sphere
+(scale radius 0.1, cut at 1.5, virial velocities, seed 2), G=1, softening
+1e-3, dt 5e-4, planner depth 6; K timed steps after W warm-up steps.  A
+step is clear, deposit, upward, tree(level 1..6), integrate, replayed from a
+captured HIP graph with the bodies already resident in HBM.
+
+Prints ONE JSON line.  `roofline` describes the kernel with the largest
+mean duration (HIP events around each launch on the library's stream,
+outside the timed region): algorithmic bytes per launch / mean duration
+against the 8 TB/s HBM peak.  `cpu_baseline` times the oracle (the C++ CPU
+restatement of the reference step, OpenMP) on the same input for a bounded
+number of steps.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path[:0] = [str(ROOT / "fmm-rainbow_amd"), str(ROOT / "tests")]
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def phase_names(depth):
+    return ["clear", "deposit", "upward"] + [f"tree_l{l}" for l in range(1, depth + 1)] + ["integrate"]
+
+
+def algorithmic_bytes(n, depth, stats):
+    """Compulsory bytes per launch of each kernel (each unique address once).
+
+    clear      16 C                      zero the moments
+    deposit    16 N + 16 C_d             read positions, write finest moments
+    upward     16 (C - C_d) + 16 (C - 1) read children, write parents
+    tree(l)    16 T_l + 48 T_l + 48 P_l: read the level's target moments,
+               write their fields, read their parents' fields (the 6^3
+               source windows are served from cache)
+    integrate  64 N + 16 L + 48 L        read+write bodies, leaf moments+fields
+    with C cells, C_d finest cells, T_l targets of level l, P_l their
+    parents, L leaves.
+    """
+    cells = (8 ** (depth + 1) - 1) // 7
+    cd = 8 ** depth
+    leaves = stats[7]
+    out = {"clear": 16 * cells, "deposit": 16 * n + 16 * cd, "upward": 16 * (cells - cd) + 16 * (cells - 1)}
+    for l in range(1, depth + 1):
+        t = stats[l]
+        # P_l >= ceil(T_l / 8): each parent has at most 8 targets.
+        out[f"tree_l{l}"] = 16 * t + 48 * t + 48 * ((t + 7) // 8)
+    out["integrate"] = 64 * n + 16 * leaves + 48 * leaves
+    return out
+
+
+def cpu_baseline(parts, cfg, budget_s):
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as orc
+
+    slots = orc.build_plans(parts.shape[0], dt=cfg["dt"], gravity=cfg["gravity"], softening=cfg["softening"])
+    orc.step(slots, parts[:], 1)  # warm-up
+    steps = 0
+    t0 = time.perf_counter()
+    p = parts
+    while True:
+        p, _, _ = orc.step(slots, p, 1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s and steps >= 3:
+            break
+    return {
+        "value": parts.shape[0] * steps / el,
+        "unit": "particle-steps/s",
+        "cores": orc.threads(),
+        "kind": "port",
+        "sample": f"{steps} steps of the same {parts.shape[0]}-body workload on the host "
+                  f"({el:.1f} s, OpenMP threads={orc.threads()})",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--phase-steps", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "decomposition is a design note, not built.", file=sys.stderr)
+        sys.exit(2)
+
+    import rainbow_fmm as rf
+    import workloads as wl
+
+    cfg = dict(wl.CFG2)
+    n = args.n
+    parts = wl.cfg2_particles(n)
+    solver = rf.Solver(n, wl.params_for(cfg))
+    solver.upload(parts)
+    solver.step(args.warmup)
+    solver.sync()
+    wall0 = time.perf_counter()
+    ms = solver.time_steps(args.steps)
+    wall = time.perf_counter() - wall0
+    value = n * args.steps / (ms / 1e3)
+
+    # Per-kernel mean durations, HIP events around each launch (un-graphed).
+    solver.solve_fields()
+    stats = solver.tree_stats()
+    phases = solver.phase_ms(args.phase_steps)
+    names = phase_names(solver.depth)
+    per = dict(zip(names, phases))
+    bytes_per = algorithmic_bytes(n, solver.depth, stats)
+    dom = max(per, key=per.get)
+    achieved = bytes_per[dom] / (per[dom] * 1e-3) / 1e9
+    roofline = {
+        "bound": "hbm",
+        "kernel": dom,
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "mean_ms": round(per[dom], 5),
+        "bytes_per_launch": bytes_per[dom],
+    }
+    line = {
+        "metric": "particle-steps/s at 1M bodies on 1 GPU; p50/p99 latency for mixed-N serving",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plummer-{n}: seeded Plummer sphere a=0.1 (seed 2), G=1, softening 1e-3, dt 5e-4, "
+                        f"depth {solver.depth}",
+            "bodies": n,
+            "depth": solver.depth,
+            "parallelism": "single GPU",
+        },
+        "roofline": roofline,
+        "phases_ms": {k: round(v, 5) for k, v in per.items()},
+        "tree": {"targets_per_level": stats[1:1 + solver.depth], "leaves": stats[7]},
+        "wall_s_timed": wall,
+    }
+    solver.close()
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(parts, cfg, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

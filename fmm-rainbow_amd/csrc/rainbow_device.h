@@ -1,0 +1,98 @@
+// Device-side data layout shared by the gfx950 kernels and the host runtime.
+//
+// HBM layout of one system of n bodies and C = total_cells(depth) cells:
+//   pos     float4[n]   {x, y, z, hue}            \ the reference Particle,
+//   vel     float4[n]   {vx, vy, vz, excitation}  / split into two streams
+//   moments int4[C]     {count, sum_x, sum_y, sum_z}   reference Moment
+//   fields  float4[3C]  {a, phi}{Jxx,Jxy,Jxz,Jyy}{Jyz,Jzz,0,0}  reference Field
+//   source  float4[C]   {com.xyz, G m} per cell, G m = 0 for an empty cell
+//   active  uint8[C]    cell was an M2L target in the last solve
+// with the reference's cell numbering level_offset(l) + x + 2^l (y + 2^l z).
+// A step works on a set of systems (a plain solve is a set of one); each
+// system's descriptor carries its own buffer pointers, so systems can join
+// and leave the set without moving memory.
+#pragma once
+
+#include <cstdint>
+
+namespace rainbow {
+
+// One system's buffers, plan and physics (the per-system part of the
+// reference's Params uniform, src/rainbow_fmm.wgsl:18-50).
+struct alignas(16) SysDesc {
+  float4* pos;
+  float4* vel;
+  int4* moments;
+  float4* fields;
+  float4* source;
+  uint8_t* active;
+  uint32_t n;         // bodies (particle_count)
+  uint32_t depth;     // octree depth
+  float scale;        // fixed-point scale (control_settings.z)
+  float dt;
+  float gravity;
+  float softening;
+  float speed;
+  float control_strength;  // control_settings.x
+  float control_radius;    // control_settings.y
+  uint32_t live;           // 0: every kernel skips the system
+  float control_point[4];
+  float control_axis_mode[4];
+};
+static_assert(sizeof(SysDesc) == 128, "SysDesc layout");
+
+// Work list of the particle passes: a block covers bodies [first, first +
+// block size) of one system.
+struct alignas(8) BodyBlock {
+  uint32_t sys;
+  uint32_t first;
+};
+
+// Work list of the upward pass: a block owns the 8x8x8 finest cells under
+// one cell of level depth-3 of one system.
+struct alignas(8) SubtreeBlock {
+  uint32_t sys;
+  uint32_t root;  // linear index of the subtree root within level depth-3
+};
+
+// Work list of the clear pass: a block zeroes cells [first, first +
+// kClearCells) of one system.
+struct alignas(8) CellBlock {
+  uint32_t sys;
+  uint32_t first;
+};
+
+constexpr uint32_t kLevels = 7;  // levels 0..6
+
+// Everything a kernel of the step needs, passed by value.
+struct StepArgs {
+  const SysDesc* sys;
+  const CellBlock* clear_blocks;
+  const BodyBlock* deposit_blocks;
+  const BodyBlock* integrate_blocks;
+  const SubtreeBlock* subtree_blocks;
+  uint32_t* lists;         // internal cells (count > 16), grouped by level
+  uint32_t* list_count;    // kLevels counters
+  uint32_t* subtree_done;  // per-system completion counters of upward
+  uint32_t list_off[kLevels];
+  uint32_t list_cap[kLevels];
+  uint32_t nsys;
+  uint32_t compensated;
+};
+
+constexpr uint32_t kClearThreads = 256;
+constexpr uint32_t kClearCells = 4096;
+constexpr uint32_t kDepositThreads = 256;
+constexpr uint32_t kDepositPerThread = 4;
+constexpr uint32_t kDepositBodies = kDepositThreads * kDepositPerThread;  // per block
+constexpr uint32_t kDepositSlots = 2048;  // LDS hash slots, 2x bodies per block
+constexpr uint32_t kIntegrateThreads = 256;
+constexpr uint32_t kUpwardThreads = 256;
+constexpr uint32_t kTreeThreads = 256;
+constexpr uint32_t kTreeMaxBlocks = 4096;
+
+// List item: system index in the high 16 bits, cell index within its level
+// in the low 16 bits (levels <= 5 hold at most 32768 cells).
+inline constexpr uint32_t pack_item(uint32_t sys, uint32_t cell) { return (sys << 16) | cell; }
+
+}  // namespace rainbow

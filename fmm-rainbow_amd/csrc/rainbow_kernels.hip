@@ -1,0 +1,885 @@
+// gfx950 kernels of the p=1 Cartesian uniform-octree FMM step.
+//
+// One step is five passes over a batch of systems (a plain solve is a batch
+// of one), launched in order on one stream and captured in a HIP graph:
+//
+//   clear      zero moments and the per-level work-list counters
+//   deposit    P2M: finest-cell binning and fixed-point moment sums,
+//              pre-aggregated per block in an LDS hash table, then global
+//              integer atomics (exact in any order)
+//   upward     M2M: one block per 8x8x8 finest-cell subtree sums three levels
+//              in LDS; the last block of a system finishes the levels above.
+//              It also writes each cell's source record (centre of mass and
+//              G m), emits the per-level lists of internal cells (count > 16)
+//              and zeroes the fields of cells that stopped being M2L targets
+//   tree(l)    for l = 1..depth: M2L, L2L and the near field of every M2L
+//              target of level l (a non-empty child of an internal cell),
+//              eight lanes per internal parent
+//   integrate  adaptive descent, local expansion, own-leaf monopole,
+//              symplectic Euler kick-drift and the soft absorbing boundary
+//
+// The reference runs the same physics as 2d+5 dispatches
+// (native/main.rs:411-421): clear_cells, deposit, upward per level, m2l_all,
+// downward per level, near_all, integrate (src/rainbow_fmm.wgsl:394-771).
+//
+// Floating point follows the reference expressions operation for operation
+// (src/rainbow_fmm.wgsl:100-203, 457-771), with correctly rounded division
+// and square root and no contraction (-ffp-contract=off), so the fields and
+// particles equal a sequential f32 evaluation of the WGSL bit for bit.
+// Sums keep the reference order: M2L visits sources pz, py, px, cz, cy, cx
+// (src/rainbow_fmm.wgsl:485-517), then the L2L term is added
+// (src/rainbow_fmm.wgsl:610), then the near sources dz, dy, dx
+// (src/rainbow_fmm.wgsl:551-570).  Empty sources are skipped: the reference
+// adds +0 for them, and a running sum that starts at +0 is never -0, so
+// adding +0 never changes it.
+
+#include <hip/hip_runtime.h>
+
+#include "rainbow_device.h"
+#include "rainbow_launch.h"
+
+namespace rainbow {
+namespace {
+
+constexpr float kWorldHalfExtent = 1.60f;
+constexpr int kLeafCapacity = 16;
+constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint32_t level_off(uint32_t level) {
+  return ((1u << (3u * level)) - 1u) / 7u;
+}
+
+__device__ __forceinline__ uint32_t lin(uint32_t x, uint32_t y, uint32_t z, uint32_t dim) {
+  return x + dim * (y + dim * z);
+}
+
+// cell_center, src/rainbow_fmm.wgsl:136-142 (one axis).
+__device__ __forceinline__ float center1(uint32_t c, uint32_t dim) {
+  return ((static_cast<float>(c) + 0.5f) * (2.0f / static_cast<float>(dim)) - 1.0f) *
+         kWorldHalfExtent;
+}
+
+// position_cell, src/rainbow_fmm.wgsl:144-152, split in the level-independent
+// part clamp(p / 1.6) * 0.5 + 0.5 and the per-level floor(u * dim).  The
+// product by a power of two is exact, so this equals the reference at every
+// level.
+__device__ __forceinline__ float unit_coord(float p) {
+  float n = p / kWorldHalfExtent;
+  n = fminf(fmaxf(n, -0.99999f), 0.99999f);
+  return n * 0.5f + 0.5f;
+}
+
+__device__ __forceinline__ uint32_t cell1(float u, uint32_t dim) {
+  return static_cast<uint32_t>(floorf(u * static_cast<float>(dim)));
+}
+
+// WGSL i32(f32): truncation, saturated at the i32 range.
+__device__ __forceinline__ int32_t to_i32_sat(float x) {
+  if (!(x == x)) return 0;
+  if (x >= 2147483648.0f) return 2147483647;
+  if (x < -2147483648.0f) return static_cast<int32_t>(0x80000000u);
+  return static_cast<int32_t>(x);
+}
+
+__device__ __forceinline__ float clampf(float x, float lo, float hi) {
+  return fminf(fmaxf(x, lo), hi);
+}
+
+// Field accumulator: a.xyz, phi, Jxx, Jxy, Jxz, Jyy, Jyz, Jzz.  The last two
+// lanes of the reference Field (j1.zw) are always zero.
+struct Acc {
+  float v[10];
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int k = 0; k < 10; ++k) v[k] = 0.0f;
+  }
+  __device__ __forceinline__ void add(int k, float t) { v[k] = v[k] + t; }
+  __device__ __forceinline__ float get(int k) const { return v[k]; }
+  __device__ __forceinline__ void set(int k, float x) { v[k] = x; }
+};
+
+// Neumaier-compensated accumulator (the precision form).
+struct CompAcc {
+  float s[10];
+  float c[10];
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int k = 0; k < 10; ++k) s[k] = c[k] = 0.0f;
+  }
+  __device__ __forceinline__ void add(int k, float t) {
+    const float u = s[k] + t;
+    if (fabsf(s[k]) >= fabsf(t))
+      c[k] = c[k] + ((s[k] - u) + t);
+    else
+      c[k] = c[k] + ((t - u) + s[k]);
+    s[k] = u;
+  }
+  __device__ __forceinline__ float get(int k) const { return s[k] + c[k]; }
+  __device__ __forceinline__ void set(int k, float x) {
+    s[k] = x;
+    c[k] = 0.0f;
+  }
+};
+
+// pair_field + add_field, src/rainbow_fmm.wgsl:170-203 and 108-110, for a
+// source record {com.xyz, G m} (gm != 0).  com = f32(sum) * (1 / (scale
+// f32(count))) and gm = G (f32(count) / f32(N)) are formed once per cell by
+// the upward pass with the same operations the reference repeats per pair.
+template <class A>
+__device__ __forceinline__ void pair_add(const float4 src, float tx, float ty, float tz, float eps2,
+                                         A& f) {
+  const float rx = src.x - tx;
+  const float ry = src.y - ty;
+  const float rz = src.z - tz;
+  const float r2 = ((rx * rx + ry * ry) + rz * rz) + eps2;
+  const float inv_r = 1.0f / sqrtf(r2);
+  const float inv_r3 = inv_r * inv_r * inv_r;
+  const float inv_r5 = inv_r3 * inv_r * inv_r;
+  const float gm = src.w;
+  const float diagonal = -gm * inv_r3;
+  const float tidal = 3.0f * gm * inv_r5;
+  f.add(0, gm * rx * inv_r3);
+  f.add(1, gm * ry * inv_r3);
+  f.add(2, gm * rz * inv_r3);
+  f.add(3, -gm * inv_r);
+  f.add(4, diagonal + tidal * rx * rx);
+  f.add(5, tidal * rx * ry);
+  f.add(6, tidal * rx * rz);
+  f.add(7, diagonal + tidal * ry * ry);
+  f.add(8, tidal * ry * rz);
+  f.add(9, diagonal + tidal * rz * rz);
+}
+
+__device__ __forceinline__ void load_field(const float4* fields, uint32_t cell, float4& a, float4& j0,
+                                           float4& j1) {
+  a = fields[3u * cell + 0u];
+  j0 = fields[3u * cell + 1u];
+  j1 = fields[3u * cell + 2u];
+}
+
+// ------------------------------------------------------------------------
+// clear
+// ------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kClearThreads) void k_clear(StepArgs a) {
+  if (blockIdx.x == 0 && threadIdx.x < kLevels) a.list_count[threadIdx.x] = 0u;
+  const CellBlock blk = a.clear_blocks[blockIdx.x];
+  const SysDesc d = a.sys[blk.sys];
+  if (!d.live) return;
+  const uint32_t ncell = level_off(d.depth + 1u);
+  const uint32_t end = min(blk.first + kClearCells, ncell);
+  for (uint32_t i = blk.first + threadIdx.x; i < end; i += kClearThreads)
+    d.moments[i] = make_int4(0, 0, 0, 0);
+}
+
+// ------------------------------------------------------------------------
+// deposit (P2M), src/rainbow_fmm.wgsl:408-422
+// ------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kDepositThreads) void k_deposit(StepArgs a) {
+  __shared__ uint32_t keys[kDepositSlots];
+  __shared__ int vals[4][kDepositSlots];
+  const BodyBlock blk = a.deposit_blocks[blockIdx.x];
+  const SysDesc d = a.sys[blk.sys];
+  if (!d.live) return;
+  for (uint32_t s = threadIdx.x; s < kDepositSlots; s += kDepositThreads) {
+    keys[s] = kEmptyKey;
+    vals[0][s] = 0;
+    vals[1][s] = 0;
+    vals[2][s] = 0;
+    vals[3][s] = 0;
+  }
+  __syncthreads();
+  const uint32_t dim = 1u << d.depth;
+#pragma unroll
+  for (uint32_t k = 0; k < kDepositPerThread; ++k) {
+    const uint32_t i = blk.first + k * kDepositThreads + threadIdx.x;
+    if (i < d.n) {
+      const float4 p = d.pos[i];
+      const uint32_t key = lin(cell1(unit_coord(p.x), dim), cell1(unit_coord(p.y), dim),
+                               cell1(unit_coord(p.z), dim), dim);
+      const int ix = to_i32_sat(rintf(p.x * d.scale));
+      const int iy = to_i32_sat(rintf(p.y * d.scale));
+      const int iz = to_i32_sat(rintf(p.z * d.scale));
+      // Open addressing; at most kDepositBodies distinct keys in a table of
+      // 2 kDepositBodies slots, so an empty slot always exists.
+      uint32_t h = (key * 2654435761u) >> (32 - 11);
+      for (;;) {
+        const uint32_t prev = atomicCAS(&keys[h], kEmptyKey, key);
+        if (prev == kEmptyKey || prev == key) break;
+        h = (h + 1u) & (kDepositSlots - 1u);
+      }
+      atomicAdd(&vals[0][h], 1);
+      atomicAdd(&vals[1][h], ix);
+      atomicAdd(&vals[2][h], iy);
+      atomicAdd(&vals[3][h], iz);
+    }
+  }
+  __syncthreads();
+  int* finest = reinterpret_cast<int*>(d.moments + level_off(d.depth));
+  for (uint32_t s = threadIdx.x; s < kDepositSlots; s += kDepositThreads) {
+    const uint32_t key = keys[s];
+    if (key == kEmptyKey) continue;
+    int* m = finest + 4u * key;
+    atomicAdd(m + 0, vals[0][s]);
+    atomicAdd(m + 1, vals[1][s]);
+    atomicAdd(m + 2, vals[2][s]);
+    atomicAdd(m + 3, vals[3][s]);
+  }
+}
+
+// ------------------------------------------------------------------------
+// upward (M2M), src/rainbow_fmm.wgsl:424-455, plus per-cell bookkeeping
+// ------------------------------------------------------------------------
+
+__device__ __forceinline__ int4 sum8(const int4* src, uint32_t cx, uint32_t cy, uint32_t cz,
+                                     uint32_t side) {
+  uint32_t count = 0u;
+  uint32_t sx = 0u, sy = 0u, sz = 0u;  // wrapping i32 sums
+#pragma unroll
+  for (uint32_t dz = 0; dz < 2u; ++dz)
+#pragma unroll
+    for (uint32_t dy = 0; dy < 2u; ++dy)
+#pragma unroll
+      for (uint32_t dx = 0; dx < 2u; ++dx) {
+        const int4 m = src[lin(2u * cx + dx, 2u * cy + dy, 2u * cz + dz, side)];
+        count += static_cast<uint32_t>(m.x);
+        sx += static_cast<uint32_t>(m.y);
+        sy += static_cast<uint32_t>(m.z);
+        sz += static_cast<uint32_t>(m.w);
+      }
+  return make_int4(static_cast<int>(count), static_cast<int>(sx), static_cast<int>(sy),
+                   static_cast<int>(sz));
+}
+
+// Source record of a cell: {com, G m}, zero when empty
+// (moment_com_read and the mass/gm lines of pair_field,
+// src/rainbow_fmm.wgsl:163-168, 182-183).
+__device__ __forceinline__ float4 source_record(const int4 m, const SysDesc& d) {
+  const uint32_t count = static_cast<uint32_t>(m.x);
+  if (count == 0u) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  const float inv = 1.0f / (d.scale * static_cast<float>(count));
+  const float mass = static_cast<float>(count) / static_cast<float>(d.n);
+  return make_float4(static_cast<float>(m.y) * inv, static_cast<float>(m.z) * inv,
+                     static_cast<float>(m.w) * inv, d.gravity * mass);
+}
+
+// A cell of level >= 1 is an M2L target when it is non-empty and its parent
+// holds more than 16 bodies (src/rainbow_fmm.wgsl:472-476, 593-595).  Every
+// field outside the current target set is zero; a cell that leaves the set
+// has its field cleared here, before the tree passes run.
+__device__ __forceinline__ void update_active(const SysDesc& d, uint32_t cell, int count,
+                                              int parent_count) {
+  const uint8_t now = (count != 0 && parent_count > kLeafCapacity) ? 1u : 0u;
+  const uint8_t prev = d.active[cell];
+  if (prev != now) {
+    if (prev) {
+      const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      d.fields[3u * cell + 0u] = z;
+      d.fields[3u * cell + 1u] = z;
+      d.fields[3u * cell + 2u] = z;
+    }
+    d.active[cell] = now;
+  }
+}
+
+__device__ __forceinline__ void push_internal(const StepArgs& a, uint32_t level, uint32_t sys,
+                                              uint32_t local, int count) {
+  if (count > kLeafCapacity) {
+    const uint32_t slot = atomicAdd(&a.list_count[level], 1u);
+    if (slot < a.list_cap[level]) a.lists[a.list_off[level] + slot] = pack_item(sys, local);
+  }
+}
+
+__global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
+  // Levels of the subtree in one LDS array: 8^3 finest cells at 0, then
+  // 64, 8 and 1 cells of the three levels above.
+  __shared__ int4 lv[512 + 64 + 8 + 1];
+  __shared__ uint32_t is_last;
+  int4* const lvl0 = lv;
+  int4* const lvl1 = lv + 512;
+  int4* const lvl2 = lv + 576;
+  int4* const lvl3 = lv + 584;
+  const SubtreeBlock blk = a.subtree_blocks[blockIdx.x];
+  const SysDesc d = a.sys[blk.sys];
+  if (!d.live) return;
+  const uint32_t t = threadIdx.x;
+  const uint32_t D = d.depth;
+  const uint32_t r = D - 3u;
+  const uint32_t dim_r = 1u << r;
+  const uint32_t rz = blk.root / (dim_r * dim_r);
+  const uint32_t rrest = blk.root - rz * dim_r * dim_r;
+  const uint32_t ry = rrest / dim_r;
+  const uint32_t rx = rrest - ry * dim_r;
+  int4* M = d.moments;
+
+  // Finest level: 8x8x8 cells, x fastest (rows of 8 contiguous moments).
+  for (uint32_t i = t; i < 512u; i += kUpwardThreads) {
+    const uint32_t ix = i & 7u, iy = (i >> 3) & 7u, iz = i >> 6;
+    lvl0[i] = M[level_off(D) + lin(8u * rx + ix, 8u * ry + iy, 8u * rz + iz, 1u << D)];
+  }
+  __syncthreads();
+  if (t < 64u) {
+    const uint32_t cx = t & 3u, cy = (t >> 2) & 3u, cz = t >> 4;
+    const int4 s = sum8(lvl0, cx, cy, cz, 8u);
+    lvl1[t] = s;
+    M[level_off(D - 1u) + lin(4u * rx + cx, 4u * ry + cy, 4u * rz + cz, 1u << (D - 1u))] = s;
+  }
+  __syncthreads();
+  if (t < 8u) {
+    const uint32_t cx = t & 1u, cy = (t >> 1) & 1u, cz = t >> 2;
+    const int4 s = sum8(lvl1, cx, cy, cz, 4u);
+    lvl2[t] = s;
+    M[level_off(D - 2u) + lin(2u * rx + cx, 2u * ry + cy, 2u * rz + cz, 1u << (D - 2u))] = s;
+  }
+  __syncthreads();
+  if (t == 0u) {
+    const int4 s = sum8(lvl2, 0u, 0u, 0u, 2u);
+    lvl3[0] = s;
+    M[level_off(r) + blk.root] = s;
+  }
+  __syncthreads();
+
+  // Source records, target flags and internal-cell lists of the subtree.
+  for (uint32_t i = t; i < 512u; i += kUpwardThreads) {
+    const uint32_t ix = i & 7u, iy = (i >> 3) & 7u, iz = i >> 6;
+    const uint32_t cell = level_off(D) + lin(8u * rx + ix, 8u * ry + iy, 8u * rz + iz, 1u << D);
+    const int4 m = lvl0[i];
+    d.source[cell] = source_record(m, d);
+    update_active(d, cell, m.x, lvl1[lin(ix >> 1, iy >> 1, iz >> 1, 4u)].x);
+  }
+  if (t < 64u) {
+    const uint32_t cx = t & 3u, cy = (t >> 2) & 3u, cz = t >> 4;
+    const uint32_t local = lin(4u * rx + cx, 4u * ry + cy, 4u * rz + cz, 1u << (D - 1u));
+    const uint32_t cell = level_off(D - 1u) + local;
+    const int4 m = lvl1[t];
+    d.source[cell] = source_record(m, d);
+    update_active(d, cell, m.x, lvl2[lin(cx >> 1, cy >> 1, cz >> 1, 2u)].x);
+    push_internal(a, D - 1u, blk.sys, local, m.x);
+  }
+  if (t < 8u) {
+    const uint32_t cx = t & 1u, cy = (t >> 1) & 1u, cz = t >> 2;
+    const uint32_t local = lin(2u * rx + cx, 2u * ry + cy, 2u * rz + cz, 1u << (D - 2u));
+    const uint32_t cell = level_off(D - 2u) + local;
+    const int4 m = lvl2[t];
+    d.source[cell] = source_record(m, d);
+    update_active(d, cell, m.x, lvl3[0].x);
+    push_internal(a, D - 2u, blk.sys, local, m.x);
+  }
+  if (t == 0u) {
+    const int4 m = lvl3[0];
+    d.source[level_off(r) + blk.root] = source_record(m, d);
+    push_internal(a, r, blk.sys, blk.root, m.x);
+  }
+
+  // The last subtree block of this system to finish sums levels r-1..0.
+  __threadfence();
+  __syncthreads();
+  if (t == 0u) {
+    const uint32_t nsub = 1u << (3u * r);
+    const uint32_t prev = atomicAdd(&a.subtree_done[blk.sys], 1u);
+    is_last = (prev == nsub - 1u) ? 1u : 0u;
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+
+  // Level r (8^r <= 512 cells) into lvl0, then levels r-1, r-2, ... into
+  // lvl1, lvl2, lvl3.
+  auto bufs = [&](uint32_t j) -> int4* { return lv + (j == 0u ? 0u : (j == 1u ? 512u : (j == 2u ? 576u : 584u))); };
+  const uint32_t nr = 1u << (3u * r);
+  for (uint32_t i = t; i < nr; i += kUpwardThreads) lvl0[i] = M[level_off(r) + i];
+  __syncthreads();
+  for (uint32_t j = 1u; j <= r; ++j) {
+    const uint32_t k = r - j;  // level being formed
+    const uint32_t dk = 1u << k;
+    if (t < dk * dk * dk) {
+      const uint32_t cz = t / (dk * dk);
+      const uint32_t rest = t - cz * dk * dk;
+      const uint32_t cy = rest / dk;
+      const uint32_t cx = rest - cy * dk;
+      const int4 s = sum8(bufs(j - 1u), cx, cy, cz, 2u * dk);
+      bufs(j)[t] = s;
+      M[level_off(k) + t] = s;
+    }
+    __syncthreads();
+  }
+  // Levels 0..r-1: source records and internal lists; levels 1..r: flags.
+  for (uint32_t j = 0u; j <= r; ++j) {
+    const uint32_t k = r - j;
+    const uint32_t dk = 1u << k;
+    const uint32_t ncell = dk * dk * dk;
+    for (uint32_t i = t; i < ncell; i += kUpwardThreads) {
+      const int4 m = bufs(j)[i];
+      const uint32_t cell = level_off(k) + i;
+      if (k < r) {
+        d.source[cell] = source_record(m, d);
+        push_internal(a, k, blk.sys, i, m.x);
+      }
+      if (k >= 1u) {
+        const uint32_t cz = i / (dk * dk);
+        const uint32_t rest = i - cz * dk * dk;
+        const uint32_t cy = rest / dk;
+        const uint32_t cx = rest - cy * dk;
+        const uint32_t pd = dk >> 1;
+        update_active(d, cell, m.x, bufs(j + 1u)[lin(cx >> 1, cy >> 1, cz >> 1, pd)].x);
+      }
+    }
+  }
+  if (t == 0u) a.subtree_done[blk.sys] = 0u;
+}
+
+// ------------------------------------------------------------------------
+// tree(level): M2L (src/rainbow_fmm.wgsl:457-519), L2L (:585-611) and near
+// (:521-572) of the level's targets
+// ------------------------------------------------------------------------
+
+template <class A>
+__global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t level) {
+  const uint32_t nitems = min(a.list_count[level - 1u], a.list_cap[level - 1u]);
+  const uint32_t total = nitems * 8u;
+  const uint32_t* items = a.lists + a.list_off[level - 1u];
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t dim = 1u << level;
+  const uint32_t pdim = dim >> 1u;
+  const int dim_i = static_cast<int>(dim);
+  const int pdim_i = static_cast<int>(pdim);
+  const uint32_t off = level_off(level);
+  const uint32_t poff = level_off(level - 1u);
+  // cell_epsilon, src/rainbow_fmm.wgsl:507 (0.06 * WORLD_HALF_EXTENT / dim).
+  const float cell_epsilon = (0.06f * kWorldHalfExtent) / static_cast<float>(dim);
+  const float eps2_m2l = cell_epsilon * cell_epsilon;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+    const uint32_t item = items[i >> 3];
+    const uint32_t child = i & 7u;
+    const uint32_t s = item >> 16;
+    const uint32_t pidx = item & 0xFFFFu;
+    if (s >= a.nsys) continue;
+    const SysDesc d = a.sys[s];
+    const uint32_t pz = pidx / (pdim * pdim);
+    const uint32_t prest = pidx - pz * pdim * pdim;
+    const uint32_t py = prest / pdim;
+    const uint32_t px = prest - py * pdim;
+    const uint32_t tx = 2u * px + (child & 1u);
+    const uint32_t ty = 2u * py + ((child >> 1) & 1u);
+    const uint32_t tz = 2u * pz + (child >> 2);
+    const uint32_t g = off + lin(tx, ty, tz, dim);
+    const int count = d.moments[g].x;
+    if (count == 0) continue;
+    const float4* src = d.source + off;
+    const float cx = center1(tx, dim), cy = center1(ty, dim), cz = center1(tz, dim);
+
+    A f;
+    f.zero();
+    // M2L: children of the parent's 3x3x3 neighbourhood that are not in the
+    // target's own 3x3x3 neighbourhood.
+    for (int oz = -1; oz <= 1; ++oz) {
+      const int sz = static_cast<int>(pz) + oz;
+      if (sz < 0 || sz >= pdim_i) continue;
+      for (int oy = -1; oy <= 1; ++oy) {
+        const int sy = static_cast<int>(py) + oy;
+        if (sy < 0 || sy >= pdim_i) continue;
+        for (int ox = -1; ox <= 1; ++ox) {
+          const int sx = static_cast<int>(px) + ox;
+          if (sx < 0 || sx >= pdim_i) continue;
+#pragma unroll
+          for (int cz2 = 0; cz2 < 2; ++cz2)
+#pragma unroll
+            for (int cy2 = 0; cy2 < 2; ++cy2)
+#pragma unroll
+              for (int cx2 = 0; cx2 < 2; ++cx2) {
+                const int qx = 2 * sx + cx2, qy = 2 * sy + cy2, qz = 2 * sz + cz2;
+                if (abs(qx - static_cast<int>(tx)) <= 1 && abs(qy - static_cast<int>(ty)) <= 1 &&
+                    abs(qz - static_cast<int>(tz)) <= 1)
+                  continue;
+                const float4 sr = src[lin(static_cast<uint32_t>(qx), static_cast<uint32_t>(qy),
+                                          static_cast<uint32_t>(qz), dim)];
+                if (sr.w != 0.0f) pair_add(sr, cx, cy, cz, eps2_m2l, f);
+              }
+        }
+      }
+    }
+
+    // L2L: add the parent's field shifted to this cell's centre
+    // (downward, src/rainbow_fmm.wgsl:598-610).
+    float4 pa, pj0, pj1;
+    load_field(d.fields, poff + pidx, pa, pj0, pj1);
+    {
+      const float dx = cx - center1(px, pdim);
+      const float dy = cy - center1(py, pdim);
+      const float dz = cz - center1(pz, pdim);
+      const float jx = (pj0.x * dx + pj0.y * dy) + pj0.z * dz;
+      const float jy = (pj0.y * dx + pj0.w * dy) + pj1.x * dz;
+      const float jz = (pj0.z * dx + pj1.x * dy) + pj1.y * dz;
+      const float ia[4] = {pa.x + jx, pa.y + jy, pa.z + jz,
+                           pa.w - ((pa.x * dx + pa.y * dy) + pa.z * dz)};
+      const float ij[6] = {pj0.x, pj0.y, pj0.z, pj0.w, pj1.x, pj1.y};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) f.set(k, f.get(k) + ia[k]);
+#pragma unroll
+      for (int k = 0; k < 6; ++k) f.set(4 + k, f.get(4 + k) + ij[k]);
+    }
+
+    // Near field of a leaf: the 26 same-level neighbour monopoles at the
+    // cell centre (near_all, src/rainbow_fmm.wgsl:533-571).
+    if (count <= kLeafCapacity || level == d.depth) {
+      const float near_epsilon =
+          fmaxf(d.softening, (0.72f * kWorldHalfExtent) / static_cast<float>(dim));
+      const float eps2_near = near_epsilon * near_epsilon;
+      for (int oz = -1; oz <= 1; ++oz) {
+        const int qz = static_cast<int>(tz) + oz;
+        if (qz < 0 || qz >= dim_i) continue;
+        for (int oy = -1; oy <= 1; ++oy) {
+          const int qy = static_cast<int>(ty) + oy;
+          if (qy < 0 || qy >= dim_i) continue;
+          for (int ox = -1; ox <= 1; ++ox) {
+            if (ox == 0 && oy == 0 && oz == 0) continue;
+            const int qx = static_cast<int>(tx) + ox;
+            if (qx < 0 || qx >= dim_i) continue;
+            const float4 sr = src[lin(static_cast<uint32_t>(qx), static_cast<uint32_t>(qy),
+                                      static_cast<uint32_t>(qz), dim)];
+            if (sr.w != 0.0f) pair_add(sr, cx, cy, cz, eps2_near, f);
+          }
+        }
+      }
+    }
+
+    float4* out = d.fields + 3u * g;
+    out[0] = make_float4(f.get(0), f.get(1), f.get(2), f.get(3));
+    out[1] = make_float4(f.get(4), f.get(5), f.get(6), f.get(7));
+    out[2] = make_float4(f.get(8), f.get(9), 0.0f, 0.0f);
+  }
+}
+
+// ------------------------------------------------------------------------
+// integrate, src/rainbow_fmm.wgsl:613-771
+// ------------------------------------------------------------------------
+
+struct Leaf {
+  uint32_t cell;   // global cell index of the chosen leaf
+  uint32_t dim;    // its level's side
+  uint32_t count;  // bodies in it
+  float ax, ay, az;
+  float phi;
+};
+
+// Adaptive descent (src/rainbow_fmm.wgsl:622-639), local expansion
+// (:640-642) and own-leaf monopole (:644-657).  phi is the matching
+// potential phi_c - a_c.d - d.(J d)/2 - G (count-1)/N / r_soft, which the
+// reference does not form (energy diagnostic only).
+template <bool kComp, bool kPhi>
+__device__ __forceinline__ Leaf leaf_field(const SysDesc& d, float px, float py, float pz) {
+  const float ux = unit_coord(px), uy = unit_coord(py), uz = unit_coord(pz);
+  const int* counts = reinterpret_cast<const int*>(d.moments);
+  Leaf out;
+  uint32_t c[3] = {0u, 0u, 0u};
+  out.dim = 1u << d.depth;
+  out.cell = 0u;
+  for (uint32_t level = 1u; level <= d.depth; ++level) {
+    const uint32_t dim = 1u << level;
+    c[0] = cell1(ux, dim);
+    c[1] = cell1(uy, dim);
+    c[2] = cell1(uz, dim);
+    const uint32_t index = level_off(level) + lin(c[0], c[1], c[2], dim);
+    if (counts[4u * index] <= kLeafCapacity || level == d.depth) {
+      out.dim = dim;
+      out.cell = index;
+      break;
+    }
+  }
+  const uint32_t cell = out.cell;
+  float4 fa, fj0, fj1;
+  load_field(d.fields, cell, fa, fj0, fj1);
+  const float dx = px - center1(c[0], out.dim);
+  const float dy = py - center1(c[1], out.dim);
+  const float dz = pz - center1(c[2], out.dim);
+  const uint32_t count = static_cast<uint32_t>(counts[4u * cell]);
+  out.count = count;
+  float self[3] = {0.0f, 0.0f, 0.0f};
+  float self_phi = 0.0f;
+  const bool has_self = count > 1u;
+  if (has_self) {
+    const float4 com = d.source[cell];
+    const float ex = com.x - px, ey = com.y - py, ez = com.z - pz;
+    const float epsilon =
+        fmaxf(d.softening, (0.62f * kWorldHalfExtent) / static_cast<float>(out.dim));
+    const float inv_r = 1.0f / sqrtf(((ex * ex + ey * ey) + ez * ez) + epsilon * epsilon);
+    const float self_mass = static_cast<float>(count - 1u) / static_cast<float>(d.n);
+    self[0] = d.gravity * self_mass * ex * inv_r * inv_r * inv_r;
+    self[1] = d.gravity * self_mass * ey * inv_r * inv_r * inv_r;
+    self[2] = d.gravity * self_mass * ez * inv_r * inv_r * inv_r;
+    if (kPhi) self_phi = -(d.gravity * self_mass) * inv_r;
+  }
+  const float jx = (fj0.x * dx + fj0.y * dy) + fj0.z * dz;
+  const float jy = (fj0.y * dx + fj0.w * dy) + fj1.x * dz;
+  const float jz = (fj0.z * dx + fj1.x * dy) + fj1.y * dz;
+  if (!kComp) {
+    out.ax = fa.x + jx;
+    out.ay = fa.y + jy;
+    out.az = fa.z + jz;
+    if (has_self) {
+      out.ax = out.ax + self[0];
+      out.ay = out.ay + self[1];
+      out.az = out.az + self[2];
+    }
+  } else {
+    const float row[3][3] = {{fj0.x, fj0.y, fj0.z}, {fj0.y, fj0.w, fj1.x}, {fj0.z, fj1.x, fj1.y}};
+    const float base[3] = {fa.x, fa.y, fa.z};
+    const float dd[3] = {dx, dy, dz};
+    float res[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      CompAcc s;
+      s.set(0, base[k]);
+      s.add(0, row[k][0] * dd[0]);
+      s.add(0, row[k][1] * dd[1]);
+      s.add(0, row[k][2] * dd[2]);
+      if (has_self) s.add(0, self[k]);
+      res[k] = s.get(0);
+    }
+    out.ax = res[0];
+    out.ay = res[1];
+    out.az = res[2];
+  }
+  out.phi = 0.0f;
+  if (kPhi)
+    out.phi = ((fa.w - ((fa.x * dx + fa.y * dy) + fa.z * dz)) - 0.5f * ((dx * jx + dy * jy) + dz * jz)) +
+              self_phi;
+  return out;
+}
+
+__device__ __forceinline__ float length3(float x, float y, float z) {
+  return sqrtf((x * x + y * y) + z * z);
+}
+
+template <bool kComp>
+__global__ __launch_bounds__(kIntegrateThreads) void k_integrate(StepArgs a) {
+  const BodyBlock blk = a.integrate_blocks[blockIdx.x];
+  const SysDesc d = a.sys[blk.sys];
+  if (!d.live) return;
+  const uint32_t i = blk.first + threadIdx.x;
+  if (i >= d.n) return;
+  const float4 p4 = d.pos[i];
+  const float4 v4 = d.vel[i];
+  float pos[3] = {p4.x, p4.y, p4.z};
+  float velocity[3] = {v4.x, v4.y, v4.z};
+  const Leaf leaf = leaf_field<kComp, false>(d, p4.x, p4.y, p4.z);
+  float acceleration[3] = {leaf.ax, leaf.ay, leaf.az};
+  const uint32_t count = leaf.count;
+
+  // Shepherd control field, src/rainbow_fmm.wgsl:659-689 (off unless
+  // control_point.w > 0.5).
+  float control_influence = 0.0f;
+  if (d.control_point[3] > 0.5f) {
+    const float cdx = d.control_point[0] - pos[0];
+    const float cdy = d.control_point[1] - pos[1];
+    const float cdz = d.control_point[2] - pos[2];
+    const float distance2 = (cdx * cdx + cdy * cdy) + cdz * cdz;
+    const float control_radius = fmaxf(d.control_radius, 0.02f);
+    control_influence = expf(-2.0f * distance2 / (control_radius * control_radius));
+    const float control_strength = d.control_strength;
+    const float isq = 1.0f / sqrtf(distance2 + control_radius * control_radius * 0.018f);
+    const float dir[3] = {cdx * isq, cdy * isq, cdz * isq};
+    const float mode = d.control_axis_mode[3];
+    if (mode < 1.5f) {
+      for (int k = 0; k < 3; ++k)
+        acceleration[k] = acceleration[k] + dir[k] * control_strength * control_influence;
+    } else if (mode < 2.5f) {
+      for (int k = 0; k < 3; ++k)
+        acceleration[k] = acceleration[k] - dir[k] * control_strength * control_influence;
+    } else {
+      const float al =
+          length3(d.control_axis_mode[0], d.control_axis_mode[1], d.control_axis_mode[2]);
+      const float ax = d.control_axis_mode[0] / al;
+      const float ay = d.control_axis_mode[1] / al;
+      const float az = d.control_axis_mode[2] / al;
+      const float tg[3] = {ay * cdz - az * cdy, az * cdx - ax * cdz, ax * cdy - ay * cdx};
+      const float tl = length3(tg[0], tg[1], tg[2]);
+      if (tl > 0.00001f)
+        for (int k = 0; k < 3; ++k)
+          acceleration[k] = acceleration[k] + tg[k] / tl * control_strength * control_influence;
+    }
+  }
+
+  // Symplectic Euler with the excitation/hue state (src/rainbow_fmm.wgsl:691-770).
+  const float dt = d.dt * d.speed;
+  const float radius = fmaxf(length3(pos[0], pos[1], pos[2]), 0.0001f);
+  const float velocity_length = fmaxf(length3(velocity[0], velocity[1], velocity[2]), 0.0001f);
+  const float signed_radial_motion = clampf(
+      ((pos[0] * velocity[0] + pos[1] * velocity[1]) + pos[2] * velocity[2]) / (radius * velocity_length),
+      -1.0f, 1.0f);
+  const float radial_fraction = fabsf(signed_radial_motion);
+  const float radial_shock = radial_fraction * radial_fraction * radial_fraction;
+  const float density_pulse =
+      clampf((static_cast<float>(count) - static_cast<float>(kLeafCapacity)) / 72.0f, 0.0f, 1.0f);
+  const float field_strength = clampf(
+      length3(acceleration[0], acceleration[1], acceleration[2]) * (radius + 0.12f) * 0.58f, 0.0f, 1.0f);
+  const float target_excitation =
+      clampf(0.025f + radial_shock * 0.56f + density_pulse * 0.26f + field_strength * 0.13f +
+                 control_influence * 0.42f,
+             0.0f, 1.0f);
+  float excitation = v4.w;
+  const float response = target_excitation > excitation ? 10.0f : 1.05f;
+  const float w = clampf(dt * response, 0.0f, 1.0f);
+  excitation = excitation * (1.0f - w) + target_excitation * w;
+
+#pragma unroll
+  for (int k = 0; k < 3; ++k) velocity[k] = velocity[k] + acceleration[k] * dt;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) pos[k] = pos[k] + velocity[k] * dt;
+
+  const float world_edge = kWorldHalfExtent * 0.985f;
+  const float edge_band = 0.12f * kWorldHalfExtent;
+#pragma unroll
+  for (int axis = 0; axis < 3; ++axis) {
+    const float over = fabsf(pos[axis]) - (world_edge - edge_band);
+    if (over > 0.0f) {
+      const float absorb = clampf(over / edge_band, 0.0f, 1.0f);
+      velocity[axis] = velocity[axis] * (1.0f - absorb * 0.16f);
+    }
+    if (fabsf(pos[axis]) > world_edge) {
+      pos[axis] = clampf(pos[axis], -world_edge, world_edge);
+      velocity[axis] = velocity[axis] * -0.55f;
+    }
+  }
+
+  const float st = clampf((excitation - 0.30f) / (0.70f - 0.30f), 0.0f, 1.0f);
+  const float hue_drift = 1.0f - st * st * (3.0f - 2.0f * st);
+  const float h = p4.w + dt * hue_drift *
+                             (0.012f + signed_radial_motion * 0.030f + excitation * 0.036f +
+                              control_influence * 0.080f);
+  const float dynamical_hue = h - floorf(h);
+  d.pos[i] = make_float4(pos[0], pos[1], pos[2], dynamical_hue);
+  d.vel[i] = make_float4(velocity[0], velocity[1], velocity[2], excitation);
+}
+
+// ------------------------------------------------------------------------
+// diagnostics: per-body field, FP64 energy and momentum reductions
+// ------------------------------------------------------------------------
+
+template <bool kComp>
+__global__ __launch_bounds__(kIntegrateThreads) void k_accel(StepArgs a, uint32_t sys, float4* out) {
+  const BodyBlock blk = a.integrate_blocks[blockIdx.x];
+  if (blk.sys != sys) return;
+  const SysDesc d = a.sys[blk.sys];
+  const uint32_t i = blk.first + threadIdx.x;
+  if (i >= d.n) return;
+  const float4 p4 = d.pos[i];
+  const Leaf leaf = leaf_field<kComp, true>(d, p4.x, p4.y, p4.z);
+  out[i] = make_float4(leaf.ax, leaf.ay, leaf.az, leaf.phi);
+}
+
+// Block partials of {KE, PE, Px, Py, Pz, Lx, Ly, Lz} in FP64 (equal masses
+// 1/N), reduced in a fixed order so repeated runs agree bit for bit.
+template <bool kComp>
+__global__ __launch_bounds__(kIntegrateThreads) void k_diag_partial(StepArgs a, uint32_t sys,
+                                                                     double* partial) {
+  __shared__ double red[8][kIntegrateThreads];
+  const BodyBlock blk = a.integrate_blocks[blockIdx.x];
+  const SysDesc d = a.sys[blk.sys];
+  const uint32_t i = blk.first + threadIdx.x;
+  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (blk.sys == sys && i < d.n) {
+    const float4 p4 = d.pos[i];
+    const float4 v4 = d.vel[i];
+    const Leaf leaf = leaf_field<kComp, true>(d, p4.x, p4.y, p4.z);
+    const double m = 1.0 / static_cast<double>(d.n);
+    const double x = p4.x, y = p4.y, z = p4.z, vx = v4.x, vy = v4.y, vz = v4.z;
+    v[0] = 0.5 * m * (vx * vx + vy * vy + vz * vz);
+    v[1] = 0.5 * m * static_cast<double>(leaf.phi);
+    v[2] = m * vx;
+    v[3] = m * vy;
+    v[4] = m * vz;
+    v[5] = m * (y * vz - z * vy);
+    v[6] = m * (z * vx - x * vz);
+    v[7] = m * (x * vy - y * vx);
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) red[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (uint32_t s = kIntegrateThreads / 2u; s > 0u; s >>= 1) {
+    if (threadIdx.x < s)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) red[k][threadIdx.x] = red[k][threadIdx.x] + red[k][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 8u) partial[8u * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(64) void k_diag_final(const double* partial, uint32_t nblocks,
+                                                   double* out) {
+  if (threadIdx.x < 8u) {
+    double s = 0.0;
+    for (uint32_t b = 0; b < nblocks; ++b) s = s + partial[8u * b + threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------
+// launch wrappers
+// ------------------------------------------------------------------------
+
+hipError_t launch_clear(const StepArgs& a, uint32_t nblocks, hipStream_t stream) {
+  if (nblocks == 0u) return hipSuccess;
+  hipLaunchKernelGGL(k_clear, dim3(nblocks), dim3(kClearThreads), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_deposit(const StepArgs& a, uint32_t nblocks, hipStream_t stream) {
+  if (nblocks == 0u) return hipSuccess;
+  hipLaunchKernelGGL(k_deposit, dim3(nblocks), dim3(kDepositThreads), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_upward(const StepArgs& a, uint32_t nblocks, hipStream_t stream) {
+  if (nblocks == 0u) return hipSuccess;
+  hipLaunchKernelGGL(k_upward, dim3(nblocks), dim3(kUpwardThreads), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_tree(const StepArgs& a, uint32_t level, uint32_t nblocks, hipStream_t stream) {
+  if (nblocks == 0u) return hipSuccess;
+  if (a.compensated)
+    hipLaunchKernelGGL(k_tree<CompAcc>, dim3(nblocks), dim3(kTreeThreads), 0, stream, a, level);
+  else
+    hipLaunchKernelGGL(k_tree<Acc>, dim3(nblocks), dim3(kTreeThreads), 0, stream, a, level);
+  return hipGetLastError();
+}
+
+hipError_t launch_integrate(const StepArgs& a, uint32_t nblocks, hipStream_t stream) {
+  if (nblocks == 0u) return hipSuccess;
+  if (a.compensated)
+    hipLaunchKernelGGL(k_integrate<true>, dim3(nblocks), dim3(kIntegrateThreads), 0, stream, a);
+  else
+    hipLaunchKernelGGL(k_integrate<false>, dim3(nblocks), dim3(kIntegrateThreads), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_accel(const StepArgs& a, uint32_t nblocks, uint32_t sys, float4* out,
+                        hipStream_t stream) {
+  if (nblocks == 0u) return hipSuccess;
+  if (a.compensated)
+    hipLaunchKernelGGL(k_accel<true>, dim3(nblocks), dim3(kIntegrateThreads), 0, stream, a, sys, out);
+  else
+    hipLaunchKernelGGL(k_accel<false>, dim3(nblocks), dim3(kIntegrateThreads), 0, stream, a, sys, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_diag(const StepArgs& a, uint32_t nblocks, uint32_t sys, double* partial,
+                       double* out, hipStream_t stream) {
+  if (nblocks == 0u) return hipSuccess;
+  if (a.compensated)
+    hipLaunchKernelGGL(k_diag_partial<true>, dim3(nblocks), dim3(kIntegrateThreads), 0, stream, a,
+                       sys, partial);
+  else
+    hipLaunchKernelGGL(k_diag_partial<false>, dim3(nblocks), dim3(kIntegrateThreads), 0, stream, a,
+                       sys, partial);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, stream, partial, nblocks, out);
+  return hipGetLastError();
+}
+
+}  // namespace rainbow

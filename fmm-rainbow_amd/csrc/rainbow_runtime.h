@@ -1,0 +1,122 @@
+// Host runtime objects shared by the solver, batch and server C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "rainbow_device.h"
+#include "rainbow_plan.h"
+
+struct rainbow_params;
+
+namespace rainbow {
+
+// The physics part of the reference's Params (src/rainbow_fmm.wgsl:28-49).
+struct Physics {
+  float dt = 0.0009f;
+  float gravity = 0.34f;
+  float softening = 0.012f;
+  float speed = 1.0f;
+  float control_strength = 0.7f;
+  float control_radius = 0.22f;
+  float scale = 0.0f;  // 0: planner scale
+  float control_point[4] = {0, 0, 0, 0};
+  float control_axis_mode[4] = {0, 0, 0, 0};
+};
+
+Physics physics_from(const rainbow_params* p);
+void apply_physics(SysDesc& s, const Physics& ph);
+
+int fail(int code, const char* fmt, ...);
+void clear_error();
+const char* last_error();
+
+// Steps captured in the multi-step graph.
+constexpr uint32_t kGraphSteps = 16;
+
+// One device allocation holding a system's bodies and cells, and its
+// descriptor pointing into it.
+struct SystemStorage {
+  void* base = nullptr;
+  size_t bytes = 0;
+  SysDesc desc{};
+};
+
+// Bytes of the allocation of a system of n bodies.
+size_t system_storage_bytes(uint32_t n);
+// Allocates and zeroes a system's storage; scale 0 = planner scale.
+int alloc_system(uint32_t n, const Physics& ph, float scale, hipStream_t stream, SystemStorage* out);
+void free_system(SystemStorage* s);
+// Copies reference-layout particles in / a buffer out (synchronous on stream).
+int upload_system(const SysDesc& d, const void* data, size_t bytes, hipStream_t stream);
+int download_system(const SysDesc& d, int which, void* dst, size_t bytes, hipStream_t stream);
+
+// A device buffer that only grows.
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t bytes);
+  void release();
+};
+
+// Work tables, lists, stream and graphs of a step over a set of systems.
+class Engine {
+ public:
+  Engine() = default;
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+  ~Engine();
+
+  int open(uint32_t options);
+  // Installs the system set (descriptors with device pointers).  Rebuilds the
+  // work tables and drops captured graphs.  Waits for the stream first.
+  int configure(const std::vector<SysDesc>& systems);
+  int push_descriptors();
+  int enqueue(bool integrate);
+  int solve_fields();
+  int step(uint32_t steps);
+  int sync();
+  int time_steps(uint32_t steps, float* ms);
+  int phase_ms(uint32_t steps, float* out, uint32_t n);
+  int upload(uint32_t sys, const void* data, size_t bytes);
+  int download(uint32_t sys, int which, void* dst, size_t bytes);
+  int accelerations(uint32_t sys, void* dst, size_t bytes);
+  int diagnostics(uint32_t sys, double out[8]);
+
+  hipStream_t stream = nullptr;
+  uint32_t options = 0;
+  uint32_t dmax = 0;
+  uint64_t total_bodies = 0;
+  std::vector<SysDesc> sys;
+  StepArgs args{};
+  uint32_t n_clear = 0;
+  uint32_t n_deposit = 0;
+  uint32_t n_integrate = 0;
+  uint32_t n_subtree = 0;
+  uint32_t tree_blocks[kLevels + 1] = {};
+
+ private:
+  int capture(uint32_t steps, hipGraphExec_t* out);
+  void drop_graphs();
+
+  DevBuf sys_buf, clear_buf, deposit_buf, integrate_buf, subtree_buf, list_buf, count_buf,
+      done_buf, diag_buf;
+  hipGraphExec_t graph_one = nullptr;
+  hipGraphExec_t graph_many = nullptr;
+  hipEvent_t ev_a = nullptr;
+  hipEvent_t ev_b = nullptr;
+};
+
+// A fixed set of systems (the solver is a batch of one).
+class Batch {
+ public:
+  ~Batch();
+  int init(const std::vector<uint32_t>& counts, const Physics& ph, uint32_t options,
+           const std::vector<float>* scales);
+  Engine engine;
+  std::vector<SystemStorage> storage;
+};
+
+}  // namespace rainbow

@@ -1,0 +1,713 @@
+// Host runtime: per-system device storage, work tables, the step schedule on
+// one HIP stream, HIP-graph capture, and the solver/batch C ABI.
+//
+// The schedule per step is clear, deposit, upward, tree(1..depth_max),
+// integrate -- the physics of the reference's encode_frame
+// (native/main.rs:411-421) in depth+4 launches instead of 2 depth+5.  A plain
+// solver is a batch of one system.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "rainbow_device.h"
+#include "rainbow_fmm.h"
+#include "rainbow_launch.h"
+#include "rainbow_plan.h"
+#include "rainbow_runtime.h"
+
+namespace rainbow {
+
+namespace {
+thread_local std::string g_error;
+
+constexpr size_t kAlign = 256;
+
+size_t align_up(size_t x) { return (x + kAlign - 1u) & ~(kAlign - 1u); }
+}  // namespace
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_error = buf;
+  return code;
+}
+
+void clear_error() { g_error.clear(); }
+
+const char* last_error() { return g_error.c_str(); }
+
+#define RB_HIP(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return fail(RAINBOW_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),       \
+                  __FILE__, __LINE__);                                                        \
+  } while (0)
+
+Physics physics_from(const rainbow_params* p) {
+  if (p == nullptr) {
+    rainbow_params_t def;
+    rainbow_write_params(0, 0, 0, 0, 1, 0, 1, 0, 0.0f, &def);
+    return physics_from(&def);
+  }
+  Physics ph;
+  ph.dt = p->dt;
+  ph.gravity = p->gravity;
+  ph.softening = p->softening;
+  ph.speed = p->speed;
+  ph.control_strength = p->control_settings[0];
+  ph.control_radius = p->control_settings[1];
+  ph.scale = p->control_settings[2];
+  for (int k = 0; k < 4; ++k) {
+    ph.control_point[k] = p->control_point[k];
+    ph.control_axis_mode[k] = p->control_axis_mode[k];
+  }
+  return ph;
+}
+
+void apply_physics(SysDesc& s, const Physics& ph) {
+  s.dt = ph.dt;
+  s.gravity = ph.gravity;
+  s.softening = ph.softening;
+  s.speed = ph.speed;
+  s.control_strength = ph.control_strength;
+  s.control_radius = ph.control_radius;
+  for (int k = 0; k < 4; ++k) {
+    s.control_point[k] = ph.control_point[k];
+    s.control_axis_mode[k] = ph.control_axis_mode[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Per-system storage: pos, vel, moments, fields, source, active in one block.
+// ---------------------------------------------------------------------------
+
+size_t system_storage_bytes(uint32_t n) {
+  const size_t cells = total_cells(depth_for_particles(n));
+  return 2u * align_up(16u * static_cast<size_t>(n)) + align_up(16u * cells) +
+         align_up(48u * cells) + align_up(16u * cells) + align_up(cells);
+}
+
+int alloc_system(uint32_t n, const Physics& ph, float scale, hipStream_t stream, SystemStorage* out) {
+  if (n == 0u) return fail(RAINBOW_E_INVALID, "a system needs at least one body");
+  const Plan plan = make_plan(n);
+  const size_t bytes = system_storage_bytes(n);
+  void* base = nullptr;
+  hipError_t e = hipMalloc(&base, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(RAINBOW_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  }
+  e = hipMemsetAsync(base, 0, bytes, stream);
+  if (e != hipSuccess) {
+    (void)hipFree(base);
+    return fail(RAINBOW_E_HIP, "hipMemsetAsync failed: %s", hipGetErrorString(e));
+  }
+  const size_t cells = plan.cells;
+  char* p = static_cast<char*>(base);
+  SysDesc d{};
+  d.pos = reinterpret_cast<float4*>(p);
+  p += align_up(16u * static_cast<size_t>(n));
+  d.vel = reinterpret_cast<float4*>(p);
+  p += align_up(16u * static_cast<size_t>(n));
+  d.moments = reinterpret_cast<int4*>(p);
+  p += align_up(16u * cells);
+  d.fields = reinterpret_cast<float4*>(p);
+  p += align_up(48u * cells);
+  d.source = reinterpret_cast<float4*>(p);
+  p += align_up(16u * cells);
+  d.active = reinterpret_cast<uint8_t*>(p);
+  d.n = n;
+  d.depth = plan.depth;
+  d.scale = scale != 0.0f ? scale : plan.scale;
+  d.live = 1u;
+  apply_physics(d, ph);
+  out->base = base;
+  out->bytes = bytes;
+  out->desc = d;
+  return RAINBOW_OK;
+}
+
+void free_system(SystemStorage* s) {
+  if (s->base) (void)hipFree(s->base);
+  s->base = nullptr;
+  s->bytes = 0;
+}
+
+int DevBuf::ensure(size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  if (bytes <= cap) return RAINBOW_OK;
+  release();
+  const size_t want = std::max<size_t>(bytes + bytes / 4u, 4096u);
+  hipError_t e = hipMalloc(&p, want);
+  if (e != hipSuccess) {
+    p = nullptr;
+    (void)hipGetLastError();
+    return fail(RAINBOW_E_NOMEM, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+  }
+  cap = want;
+  return RAINBOW_OK;
+}
+
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  cap = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Engine
+// ---------------------------------------------------------------------------
+
+Engine::~Engine() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  drop_graphs();
+  for (DevBuf* b : {&sys_buf, &clear_buf, &deposit_buf, &integrate_buf, &subtree_buf, &list_buf,
+                    &count_buf, &done_buf, &diag_buf})
+    b->release();
+  if (ev_a) (void)hipEventDestroy(ev_a);
+  if (ev_b) (void)hipEventDestroy(ev_b);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Engine::drop_graphs() {
+  if (graph_one) (void)hipGraphExecDestroy(graph_one);
+  if (graph_many) (void)hipGraphExecDestroy(graph_many);
+  graph_one = graph_many = nullptr;
+}
+
+int Engine::open(uint32_t opts) {
+  options = opts;
+  RB_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  RB_HIP(hipEventCreate(&ev_a));
+  RB_HIP(hipEventCreate(&ev_b));
+  return RAINBOW_OK;
+}
+
+int Engine::configure(const std::vector<SysDesc>& systems) {
+  if (systems.size() > 65535u) return fail(RAINBOW_E_INVALID, "at most 65535 systems");
+  RB_HIP(hipStreamSynchronize(stream));
+  drop_graphs();
+  sys = systems;
+  const uint32_t nsys = static_cast<uint32_t>(sys.size());
+  std::vector<CellBlock> clr;
+  std::vector<BodyBlock> dep, integ;
+  std::vector<SubtreeBlock> sub;
+  uint32_t list_cap[kLevels] = {};
+  dmax = 0;
+  total_bodies = 0;
+  for (uint32_t s = 0; s < nsys; ++s) {
+    const SysDesc& d = sys[s];
+    const uint32_t cells = total_cells(d.depth);
+    for (uint32_t f = 0; f < cells; f += kClearCells) clr.push_back(CellBlock{s, f});
+    for (uint32_t f = 0; f < d.n; f += kDepositBodies) dep.push_back(BodyBlock{s, f});
+    for (uint32_t f = 0; f < d.n; f += kIntegrateThreads) integ.push_back(BodyBlock{s, f});
+    for (uint32_t r = 0; r < level_cells(d.depth - 3u); ++r) sub.push_back(SubtreeBlock{s, r});
+    for (uint32_t k = 0; k < d.depth; ++k)
+      list_cap[k] += std::min<uint32_t>(level_cells(k), d.n / (kAdaptiveLeafCapacity + 1u));
+    dmax = std::max(dmax, d.depth);
+    total_bodies += d.n;
+  }
+  n_clear = static_cast<uint32_t>(clr.size());
+  n_deposit = static_cast<uint32_t>(dep.size());
+  n_integrate = static_cast<uint32_t>(integ.size());
+  n_subtree = static_cast<uint32_t>(sub.size());
+  uint32_t list_total = 0;
+  for (uint32_t k = 0; k < kLevels; ++k) {
+    args.list_off[k] = list_total;
+    args.list_cap[k] = list_cap[k];
+    list_total += list_cap[k];
+  }
+  int rc;
+  if ((rc = sys_buf.ensure(sizeof(SysDesc) * nsys)) || (rc = clear_buf.ensure(sizeof(CellBlock) * n_clear)) ||
+      (rc = deposit_buf.ensure(sizeof(BodyBlock) * n_deposit)) ||
+      (rc = integrate_buf.ensure(sizeof(BodyBlock) * n_integrate)) ||
+      (rc = subtree_buf.ensure(sizeof(SubtreeBlock) * n_subtree)) ||
+      (rc = list_buf.ensure(sizeof(uint32_t) * list_total)) ||
+      (rc = count_buf.ensure(sizeof(uint32_t) * (kLevels + 1u))) ||
+      (rc = done_buf.ensure(sizeof(uint32_t) * nsys)) ||
+      (rc = diag_buf.ensure(sizeof(double) * (8u * static_cast<size_t>(n_integrate) + 8u))))
+    return rc;
+  args.sys = static_cast<const SysDesc*>(sys_buf.p);
+  args.clear_blocks = static_cast<const CellBlock*>(clear_buf.p);
+  args.deposit_blocks = static_cast<const BodyBlock*>(deposit_buf.p);
+  args.integrate_blocks = static_cast<const BodyBlock*>(integrate_buf.p);
+  args.subtree_blocks = static_cast<const SubtreeBlock*>(subtree_buf.p);
+  args.lists = static_cast<uint32_t*>(list_buf.p);
+  args.list_count = static_cast<uint32_t*>(count_buf.p);
+  args.subtree_done = static_cast<uint32_t*>(done_buf.p);
+  args.nsys = nsys;
+  args.compensated = (options & RAINBOW_OPT_COMPENSATED) ? 1u : 0u;
+  for (uint32_t l = 1; l <= kLevels - 1u; ++l) {
+    const uint64_t lanes = 8ull * list_cap[l - 1u];
+    const uint64_t blocks = (lanes + kTreeThreads - 1u) / kTreeThreads;
+    tree_blocks[l] = static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(blocks, 1u), kTreeMaxBlocks));
+  }
+  if (nsys) {
+    RB_HIP(hipMemcpyAsync(sys_buf.p, sys.data(), sizeof(SysDesc) * nsys, hipMemcpyHostToDevice, stream));
+    RB_HIP(hipMemcpyAsync(clear_buf.p, clr.data(), sizeof(CellBlock) * n_clear, hipMemcpyHostToDevice, stream));
+    RB_HIP(hipMemcpyAsync(deposit_buf.p, dep.data(), sizeof(BodyBlock) * n_deposit, hipMemcpyHostToDevice, stream));
+    RB_HIP(hipMemcpyAsync(integrate_buf.p, integ.data(), sizeof(BodyBlock) * n_integrate, hipMemcpyHostToDevice,
+                          stream));
+    RB_HIP(hipMemcpyAsync(subtree_buf.p, sub.data(), sizeof(SubtreeBlock) * n_subtree, hipMemcpyHostToDevice,
+                          stream));
+    RB_HIP(hipMemsetAsync(done_buf.p, 0, sizeof(uint32_t) * nsys, stream));
+  }
+  RB_HIP(hipMemsetAsync(count_buf.p, 0, sizeof(uint32_t) * (kLevels + 1u), stream));
+  RB_HIP(hipStreamSynchronize(stream));
+  return RAINBOW_OK;
+}
+
+int Engine::push_descriptors() {
+  RB_HIP(hipStreamSynchronize(stream));
+  if (!sys.empty())
+    RB_HIP(hipMemcpyAsync(sys_buf.p, sys.data(), sizeof(SysDesc) * sys.size(), hipMemcpyHostToDevice, stream));
+  RB_HIP(hipStreamSynchronize(stream));
+  return RAINBOW_OK;
+}
+
+int Engine::enqueue(bool integrate) {
+  if (sys.empty()) return RAINBOW_OK;
+  RB_HIP(launch_clear(args, n_clear, stream));
+  RB_HIP(launch_deposit(args, n_deposit, stream));
+  RB_HIP(launch_upward(args, n_subtree, stream));
+  for (uint32_t l = 1; l <= dmax; ++l) RB_HIP(launch_tree(args, l, tree_blocks[l], stream));
+  if (integrate) RB_HIP(launch_integrate(args, n_integrate, stream));
+  return RAINBOW_OK;
+}
+
+int Engine::capture(uint32_t steps, hipGraphExec_t* out) {
+  hipGraph_t graph = nullptr;
+  RB_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  int rc = RAINBOW_OK;
+  for (uint32_t s = 0; s < steps && rc == RAINBOW_OK; ++s) rc = enqueue(true);
+  const hipError_t e = hipStreamEndCapture(stream, &graph);
+  if (rc != RAINBOW_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(RAINBOW_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  const hipError_t ei = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) return fail(RAINBOW_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+  return RAINBOW_OK;
+}
+
+int Engine::step(uint32_t steps) {
+  if (sys.empty() || steps == 0u) return RAINBOW_OK;
+  int rc;
+  if (steps >= kGraphSteps && graph_many == nullptr && (rc = capture(kGraphSteps, &graph_many)) != 0)
+    return rc;
+  if (steps % kGraphSteps != 0u && graph_one == nullptr && (rc = capture(1u, &graph_one)) != 0) return rc;
+  for (uint32_t s = 0; s + kGraphSteps <= steps; s += kGraphSteps) RB_HIP(hipGraphLaunch(graph_many, stream));
+  for (uint32_t s = 0; s < steps % kGraphSteps; ++s) RB_HIP(hipGraphLaunch(graph_one, stream));
+  return RAINBOW_OK;
+}
+
+int Engine::solve_fields() { return enqueue(false); }
+
+int Engine::sync() {
+  RB_HIP(hipStreamSynchronize(stream));
+  return RAINBOW_OK;
+}
+
+int Engine::time_steps(uint32_t steps, float* ms) {
+  int rc;
+  // Graphs are captured before the timed region.
+  if (steps >= kGraphSteps && graph_many == nullptr && (rc = capture(kGraphSteps, &graph_many)) != 0)
+    return rc;
+  if (steps % kGraphSteps != 0u && graph_one == nullptr && (rc = capture(1u, &graph_one)) != 0) return rc;
+  RB_HIP(hipStreamSynchronize(stream));
+  RB_HIP(hipEventRecord(ev_a, stream));
+  if ((rc = step(steps)) != 0) return rc;
+  RB_HIP(hipEventRecord(ev_b, stream));
+  RB_HIP(hipEventSynchronize(ev_b));
+  RB_HIP(hipEventElapsedTime(ms, ev_a, ev_b));
+  return RAINBOW_OK;
+}
+
+int Engine::phase_ms(uint32_t steps, float* out, uint32_t n) {
+  const uint32_t nphase = 4u + dmax;  // clear, deposit, upward, tree 1..dmax, integrate
+  if (out == nullptr || n < nphase) return fail(RAINBOW_E_INVALID, "phase_ms needs %u entries", nphase);
+  if (steps == 0u) return fail(RAINBOW_E_INVALID, "steps must be >= 1");
+  std::vector<hipEvent_t> ev(nphase + 1u, nullptr);
+  for (auto& e : ev) RB_HIP(hipEventCreate(&e));
+  std::vector<double> acc(nphase, 0.0);
+  hipError_t err = hipSuccess;
+  for (uint32_t s = 0; s < steps && err == hipSuccess; ++s) {
+    uint32_t k = 0;
+    auto mark = [&]() {
+      if (err == hipSuccess) err = hipEventRecord(ev[k++], stream);
+    };
+    mark();
+    if (err == hipSuccess) err = launch_clear(args, n_clear, stream);
+    mark();
+    if (err == hipSuccess) err = launch_deposit(args, n_deposit, stream);
+    mark();
+    if (err == hipSuccess) err = launch_upward(args, n_subtree, stream);
+    mark();
+    for (uint32_t l = 1; l <= dmax; ++l) {
+      if (err == hipSuccess) err = launch_tree(args, l, tree_blocks[l], stream);
+      mark();
+    }
+    if (err == hipSuccess) err = launch_integrate(args, n_integrate, stream);
+    mark();
+    if (err == hipSuccess) err = hipEventSynchronize(ev[nphase]);
+    for (uint32_t i = 0; i < nphase && err == hipSuccess; ++i) {
+      float ms = 0.0f;
+      err = hipEventElapsedTime(&ms, ev[i], ev[i + 1u]);
+      acc[i] += ms;
+    }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (err != hipSuccess) return fail(RAINBOW_E_HIP, "phase timing: %s", hipGetErrorString(err));
+  for (uint32_t i = 0; i < nphase; ++i) out[i] = static_cast<float>(acc[i] / steps);
+  return static_cast<int>(nphase);
+}
+
+int upload_system(const SysDesc& d, const void* data, size_t bytes, hipStream_t stream) {
+  if (data == nullptr || bytes != 32ull * d.n)
+    return fail(RAINBOW_E_INVALID, "upload expects %llu bytes, got %zu",
+                static_cast<unsigned long long>(32ull * d.n), bytes);
+  std::vector<float4> pos(d.n), vel(d.n);
+  const float* f = static_cast<const float*>(data);
+  for (uint32_t i = 0; i < d.n; ++i) {
+    pos[i] = make_float4(f[8u * i + 0u], f[8u * i + 1u], f[8u * i + 2u], f[8u * i + 3u]);
+    vel[i] = make_float4(f[8u * i + 4u], f[8u * i + 5u], f[8u * i + 6u], f[8u * i + 7u]);
+  }
+  RB_HIP(hipMemcpyAsync(d.pos, pos.data(), 16ull * d.n, hipMemcpyHostToDevice, stream));
+  RB_HIP(hipMemcpyAsync(d.vel, vel.data(), 16ull * d.n, hipMemcpyHostToDevice, stream));
+  RB_HIP(hipStreamSynchronize(stream));
+  return RAINBOW_OK;
+}
+
+int download_system(const SysDesc& d, int which, void* dst, size_t bytes, hipStream_t stream) {
+  const uint64_t cells = total_cells(d.depth);
+  if (dst == nullptr) return fail(RAINBOW_E_INVALID, "null destination");
+  if (which == RAINBOW_PARTICLES) {
+    if (bytes != 32ull * d.n)
+      return fail(RAINBOW_E_INVALID, "particles need %llu bytes", static_cast<unsigned long long>(32ull * d.n));
+    std::vector<float4> pos(d.n), vel(d.n);
+    RB_HIP(hipMemcpyAsync(pos.data(), d.pos, 16ull * d.n, hipMemcpyDeviceToHost, stream));
+    RB_HIP(hipMemcpyAsync(vel.data(), d.vel, 16ull * d.n, hipMemcpyDeviceToHost, stream));
+    RB_HIP(hipStreamSynchronize(stream));
+    float* f = static_cast<float*>(dst);
+    for (uint32_t i = 0; i < d.n; ++i) {
+      f[8u * i + 0u] = pos[i].x;
+      f[8u * i + 1u] = pos[i].y;
+      f[8u * i + 2u] = pos[i].z;
+      f[8u * i + 3u] = pos[i].w;
+      f[8u * i + 4u] = vel[i].x;
+      f[8u * i + 5u] = vel[i].y;
+      f[8u * i + 6u] = vel[i].z;
+      f[8u * i + 7u] = vel[i].w;
+    }
+    return RAINBOW_OK;
+  }
+  if (which == RAINBOW_MOMENTS) {
+    if (bytes != 16ull * cells)
+      return fail(RAINBOW_E_INVALID, "moments need %llu bytes", static_cast<unsigned long long>(16ull * cells));
+    RB_HIP(hipMemcpyAsync(dst, d.moments, bytes, hipMemcpyDeviceToHost, stream));
+    RB_HIP(hipStreamSynchronize(stream));
+    return RAINBOW_OK;
+  }
+  if (which == RAINBOW_FIELDS) {
+    if (bytes != 48ull * cells)
+      return fail(RAINBOW_E_INVALID, "fields need %llu bytes", static_cast<unsigned long long>(48ull * cells));
+    RB_HIP(hipMemcpyAsync(dst, d.fields, bytes, hipMemcpyDeviceToHost, stream));
+    RB_HIP(hipStreamSynchronize(stream));
+    return RAINBOW_OK;
+  }
+  return fail(RAINBOW_E_INVALID, "unknown buffer %d", which);
+}
+
+int Engine::upload(uint32_t s, const void* data, size_t bytes) {
+  if (s >= sys.size()) return fail(RAINBOW_E_INVALID, "system %u out of range", s);
+  RB_HIP(hipStreamSynchronize(stream));
+  return upload_system(sys[s], data, bytes, stream);
+}
+
+int Engine::download(uint32_t s, int which, void* dst, size_t bytes) {
+  if (s >= sys.size()) return fail(RAINBOW_E_INVALID, "system %u out of range", s);
+  return download_system(sys[s], which, dst, bytes, stream);
+}
+
+int Engine::accelerations(uint32_t s, void* dst, size_t bytes) {
+  if (s >= sys.size()) return fail(RAINBOW_E_INVALID, "system %u out of range", s);
+  const SysDesc& d = sys[s];
+  if (dst == nullptr || bytes != 16ull * d.n)
+    return fail(RAINBOW_E_INVALID, "accelerations need %llu bytes", static_cast<unsigned long long>(16ull * d.n));
+  float4* out = nullptr;
+  RB_HIP(hipMalloc(reinterpret_cast<void**>(&out), 16ull * d.n));
+  hipError_t e = launch_accel(args, n_integrate, s, out, stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dst, out, bytes, hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  (void)hipFree(out);
+  if (e != hipSuccess) return fail(RAINBOW_E_HIP, "accelerations: %s", hipGetErrorString(e));
+  return RAINBOW_OK;
+}
+
+int Engine::diagnostics(uint32_t s, double out[8]) {
+  if (s >= sys.size()) return fail(RAINBOW_E_INVALID, "system %u out of range", s);
+  double* partial = static_cast<double*>(diag_buf.p);
+  double* result = partial + 8u * static_cast<size_t>(n_integrate);
+  RB_HIP(launch_diag(args, n_integrate, s, partial, result, stream));
+  RB_HIP(hipMemcpyAsync(out, result, 8u * sizeof(double), hipMemcpyDeviceToHost, stream));
+  RB_HIP(hipStreamSynchronize(stream));
+  return RAINBOW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batch
+// ---------------------------------------------------------------------------
+
+Batch::~Batch() {
+  if (engine.stream) (void)hipStreamSynchronize(engine.stream);
+  for (auto& s : storage) free_system(&s);
+}
+
+int Batch::init(const std::vector<uint32_t>& counts, const Physics& ph, uint32_t options,
+                const std::vector<float>* scales) {
+  if (counts.empty() || counts.size() > 65535u) return fail(RAINBOW_E_INVALID, "systems must be in [1, 65535]");
+  int rc = engine.open(options);
+  if (rc) return rc;
+  storage.resize(counts.size());
+  std::vector<SysDesc> descs(counts.size());
+  for (size_t s = 0; s < counts.size(); ++s) {
+    const float scale = scales != nullptr ? (*scales)[s] : 0.0f;
+    if ((rc = alloc_system(counts[s], ph, scale, engine.stream, &storage[s])) != 0) return rc;
+    descs[s] = storage[s].desc;
+  }
+  return engine.configure(descs);
+}
+
+}  // namespace rainbow
+
+// ---------------------------------------------------------------------------
+// C ABI: solver (one system) and batch.
+// ---------------------------------------------------------------------------
+
+using rainbow::Batch;
+using rainbow::fail;
+
+struct rainbow_hip {
+  Batch batch;
+};
+
+struct rainbow_batch {
+  Batch batch;
+};
+
+#define RB_CHECK_HANDLE(h) \
+  if ((h) == nullptr) return fail(RAINBOW_E_INVALID, "null handle")
+
+extern "C" {
+
+const char* rainbow_last_error(void) { return rainbow::last_error(); }
+
+int rainbow_hip_create_ex(uint32_t particles, const rainbow_params_t* params, uint32_t options,
+                          rainbow_hip_t** out) {
+  rainbow::clear_error();
+  if (out == nullptr) return fail(RAINBOW_E_INVALID, "null output handle");
+  *out = nullptr;
+  if (particles == 0u) return fail(RAINBOW_E_INVALID, "particles must be >= 1");
+  if (options & ~static_cast<uint32_t>(RAINBOW_OPT_COMPENSATED))
+    return fail(RAINBOW_E_INVALID, "unknown option bits 0x%x", options);
+  rainbow_hip* h = new (std::nothrow) rainbow_hip;
+  if (h == nullptr) return fail(RAINBOW_E_NOMEM, "out of host memory");
+  const rainbow::Physics ph = rainbow::physics_from(params);
+  const std::vector<float> scales(1, ph.scale);
+  const int rc = h->batch.init(std::vector<uint32_t>(1, particles), ph, options, &scales);
+  if (rc != RAINBOW_OK) {
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return RAINBOW_OK;
+}
+
+int rainbow_hip_create(uint32_t particles, const rainbow_params_t* params, rainbow_hip_t** out) {
+  return rainbow_hip_create_ex(particles, params, 0u, out);
+}
+
+void rainbow_hip_destroy(rainbow_hip_t* h) { delete h; }
+
+int rainbow_hip_set_params(rainbow_hip_t* h, const rainbow_params_t* params) {
+  RB_CHECK_HANDLE(h);
+  const rainbow::Physics ph = rainbow::physics_from(params);
+  for (auto& s : h->batch.engine.sys) {
+    rainbow::apply_physics(s, ph);
+    if (ph.scale != 0.0f) s.scale = ph.scale;
+  }
+  return h->batch.engine.push_descriptors();
+}
+
+int rainbow_hip_plan(rainbow_hip_t* h, uint32_t* depth, uint32_t* cells, float* scale) {
+  RB_CHECK_HANDLE(h);
+  const rainbow::SysDesc& d = h->batch.engine.sys[0];
+  if (depth) *depth = d.depth;
+  if (cells) *cells = rainbow::total_cells(d.depth);
+  if (scale) *scale = d.scale;
+  return RAINBOW_OK;
+}
+
+int rainbow_hip_upload_particles(rainbow_hip_t* h, const void* particles, size_t bytes) {
+  RB_CHECK_HANDLE(h);
+  return h->batch.engine.upload(0u, particles, bytes);
+}
+
+int rainbow_hip_solve_fields(rainbow_hip_t* h) {
+  RB_CHECK_HANDLE(h);
+  return h->batch.engine.solve_fields();
+}
+
+int rainbow_hip_step(rainbow_hip_t* h, uint32_t steps) {
+  RB_CHECK_HANDLE(h);
+  return h->batch.engine.step(steps);
+}
+
+int rainbow_hip_synchronize(rainbow_hip_t* h) {
+  RB_CHECK_HANDLE(h);
+  return h->batch.engine.sync();
+}
+
+int rainbow_hip_download(rainbow_hip_t* h, int which, void* dst, size_t bytes) {
+  RB_CHECK_HANDLE(h);
+  return h->batch.engine.download(0u, which, dst, bytes);
+}
+
+int rainbow_hip_digest(rainbow_hip_t* h, int which, uint32_t* out) {
+  RB_CHECK_HANDLE(h);
+  if (out == nullptr) return fail(RAINBOW_E_INVALID, "null output");
+  const rainbow::SysDesc& d = h->batch.engine.sys[0];
+  const size_t cells = rainbow::total_cells(d.depth);
+  size_t bytes = 0;
+  if (which == RAINBOW_PARTICLES)
+    bytes = 32u * static_cast<size_t>(d.n);
+  else if (which == RAINBOW_MOMENTS)
+    bytes = 16u * cells;
+  else if (which == RAINBOW_FIELDS)
+    bytes = 48u * cells;
+  else
+    return fail(RAINBOW_E_INVALID, "unknown buffer %d", which);
+  std::vector<uint8_t> buf(bytes);
+  const int rc = h->batch.engine.download(0u, which, buf.data(), bytes);
+  if (rc) return rc;
+  *out = rainbow_fnv1a(buf.data(), bytes);
+  return RAINBOW_OK;
+}
+
+int rainbow_hip_time_steps(rainbow_hip_t* h, uint32_t steps, float* ms) {
+  RB_CHECK_HANDLE(h);
+  if (ms == nullptr) return fail(RAINBOW_E_INVALID, "null output");
+  return h->batch.engine.time_steps(steps, ms);
+}
+
+int rainbow_hip_phase_ms(rainbow_hip_t* h, uint32_t steps, float* out, uint32_t n) {
+  RB_CHECK_HANDLE(h);
+  return h->batch.engine.phase_ms(steps, out, n);
+}
+
+int rainbow_hip_tree_stats(rainbow_hip_t* h, uint32_t out[8]) {
+  RB_CHECK_HANDLE(h);
+  if (out == nullptr) return fail(RAINBOW_E_INVALID, "null output");
+  const rainbow::SysDesc& d = h->batch.engine.sys[0];
+  const uint32_t cells = rainbow::total_cells(d.depth);
+  std::vector<int32_t> m(4u * static_cast<size_t>(cells));
+  const int rc = h->batch.engine.download(0u, RAINBOW_MOMENTS, m.data(), 16u * static_cast<size_t>(cells));
+  if (rc) return rc;
+  for (int k = 0; k < 8; ++k) out[k] = 0u;
+  for (uint32_t l = 1; l <= d.depth; ++l) {
+    const uint32_t dim = 1u << l, pdim = dim >> 1;
+    for (uint32_t i = 0; i < rainbow::level_cells(l); ++i) {
+      const uint32_t z = i / (dim * dim), y = (i / dim) % dim, x = i % dim;
+      const uint32_t pi = rainbow::level_offset(l - 1u) + (x / 2u) + pdim * ((y / 2u) + pdim * (z / 2u));
+      const int32_t c = m[4u * (rainbow::level_offset(l) + i)];
+      if (c != 0 && m[4u * pi] > 16) {
+        out[l] += 1u;
+        if (c <= 16 || l == d.depth) out[7] += 1u;
+      }
+    }
+  }
+  return RAINBOW_OK;
+}
+
+int rainbow_hip_diagnostics(rainbow_hip_t* h, double out[8]) {
+  RB_CHECK_HANDLE(h);
+  if (out == nullptr) return fail(RAINBOW_E_INVALID, "null output");
+  return h->batch.engine.diagnostics(0u, out);
+}
+
+int rainbow_hip_accelerations(rainbow_hip_t* h, void* dst, size_t bytes) {
+  RB_CHECK_HANDLE(h);
+  return h->batch.engine.accelerations(0u, dst, bytes);
+}
+
+int rainbow_batch_create(uint32_t systems, const uint32_t* counts, const rainbow_params_t* params,
+                         uint32_t options, rainbow_batch_t** out) {
+  rainbow::clear_error();
+  if (out == nullptr || counts == nullptr) return fail(RAINBOW_E_INVALID, "null argument");
+  *out = nullptr;
+  if (options & ~static_cast<uint32_t>(RAINBOW_OPT_COMPENSATED))
+    return fail(RAINBOW_E_INVALID, "unknown option bits 0x%x", options);
+  for (uint32_t s = 0; s < systems; ++s)
+    if (counts[s] == 0u) return fail(RAINBOW_E_INVALID, "system %u has no bodies", s);
+  rainbow_batch* b = new (std::nothrow) rainbow_batch;
+  if (b == nullptr) return fail(RAINBOW_E_NOMEM, "out of host memory");
+  rainbow::Physics ph = rainbow::physics_from(params);
+  ph.scale = 0.0f;  // every system keeps its own planner scale
+  const int rc = b->batch.init(std::vector<uint32_t>(counts, counts + systems), ph, options, nullptr);
+  if (rc != RAINBOW_OK) {
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return RAINBOW_OK;
+}
+
+void rainbow_batch_destroy(rainbow_batch_t* b) { delete b; }
+
+int rainbow_batch_upload(rainbow_batch_t* b, uint32_t system, const void* particles, size_t bytes) {
+  RB_CHECK_HANDLE(b);
+  return b->batch.engine.upload(system, particles, bytes);
+}
+
+int rainbow_batch_download(rainbow_batch_t* b, uint32_t system, int which, void* dst, size_t bytes) {
+  RB_CHECK_HANDLE(b);
+  return b->batch.engine.download(system, which, dst, bytes);
+}
+
+int rainbow_batch_solve_fields(rainbow_batch_t* b) {
+  RB_CHECK_HANDLE(b);
+  return b->batch.engine.solve_fields();
+}
+
+int rainbow_batch_step(rainbow_batch_t* b, uint32_t steps) {
+  RB_CHECK_HANDLE(b);
+  return b->batch.engine.step(steps);
+}
+
+int rainbow_batch_time_steps(rainbow_batch_t* b, uint32_t steps, float* ms) {
+  RB_CHECK_HANDLE(b);
+  if (ms == nullptr) return fail(RAINBOW_E_INVALID, "null output");
+  return b->batch.engine.time_steps(steps, ms);
+}
+
+int rainbow_batch_synchronize(rainbow_batch_t* b) {
+  RB_CHECK_HANDLE(b);
+  return b->batch.engine.sync();
+}
+
+uint64_t rainbow_batch_particles(rainbow_batch_t* b) { return b ? b->batch.engine.total_bodies : 0u; }
+
+}  // extern "C"

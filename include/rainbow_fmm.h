@@ -1,0 +1,338 @@
+/*
+ * rainbow_fmm.h -- C ABI of the MI355X (gfx950) p=1 Cartesian uniform-octree
+ * FMM step.
+ *
+ * Part 1 restates the reference planner's exported symbols one for one
+ * (src/lib.rs:88-157), with the same names, argument meaning and clamping
+ * behaviour: the functions are pure, never allocate, have no error codes and
+ * return 0 for out-of-range levels.
+ *
+ * Part 2 restates the headless harness's plan construction
+ * (native/main.rs:38-128) and FNV-1a digest (native/main.rs:152-163).
+ *
+ * Part 3 is the solver, which in the reference is host code driving WGSL
+ * entry points (native/main.rs:355-421, app.js:609-648).  Here it is a
+ * library: a handle owns device memory and one HIP stream, and every call
+ * returns 0 on success or a negative status, with a thread-local message in
+ * rainbow_last_error().  Nothing aborts and no exception crosses the ABI.
+ *
+ * Part 4 (batch) advances many independent systems in one launch sequence per
+ * step; Part 5 (serve) is the continuous-batching request loop built on it.
+ *
+ * Buffer layouts seen through this ABI are the reference's
+ * (src/rainbow_fmm.wgsl:18-73):
+ *   Particle 32 B = {pos.xyz, hue} {vel.xyz, excitation}       (f32 x 8)
+ *   Moment   16 B = {count u32, sum_x, sum_y, sum_z i32}       fixed point
+ *   Field    48 B = {a.xyz, phi} {Jxx,Jxy,Jxz,Jyy} {Jyz,Jzz,0,0}
+ * Cell g of level l is level_offset(l) + x + 2^l (y + 2^l z).
+ */
+#ifndef RAINBOW_FMM_H
+#define RAINBOW_FMM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Part 1. Planner ABI (src/lib.rs:88-157).                                  */
+/* ------------------------------------------------------------------------ */
+
+/* == 1.  Replaces rainbow_api_version, src/lib.rs:88-91. */
+uint32_t rainbow_api_version(void);
+
+/* Static payload of the solver kernels (the gfx950 HIP source compiled into
+ * this library), no NUL terminator, never freed.  Replaces
+ * rainbow_shader_ptr/len, src/lib.rs:93-101, which point at the WGSL text. */
+const uint8_t* rainbow_shader_ptr(void);
+size_t rainbow_shader_len(void);
+
+/* Static algorithm name.  Replaces src/lib.rs:103-111. */
+const uint8_t* rainbow_algorithm_ptr(void);
+size_t rainbow_algorithm_len(void);
+
+/* depth_for_particles(clamp_particle_count(n)).  Replaces src/lib.rs:113-116. */
+uint32_t rainbow_depth_for_particles(uint32_t particles);
+/* 8^level, or 0 when level > 6.  Replaces src/lib.rs:118-125. */
+uint32_t rainbow_level_cells(uint32_t level);
+/* (8^level - 1) / 7, or 0 when level > 7.  Replaces src/lib.rs:127-134. */
+uint32_t rainbow_level_offset(uint32_t level);
+/* total_cells(clamp(depth, 4, 6)).  Replaces src/lib.rs:136-139. */
+uint32_t rainbow_total_cells(uint32_t depth);
+/* clamp(n, 2^14, 2^23).  Replaces src/lib.rs:141-144. */
+uint32_t rainbow_clamp_particle_count(uint32_t requested);
+/* fixed_point_scale(clamp_particle_count(n)).  Replaces src/lib.rs:146-149. */
+float rainbow_fixed_point_scale(uint32_t particles);
+/* Low 32 bits of 3 n + 199 total_cells(d), n and d clamped.
+ * Replaces src/lib.rs:151-157. */
+uint32_t rainbow_linear_work_lo(uint32_t particles, uint32_t depth);
+
+/* The unclamped planner functions the native harness calls directly
+ * (native/main.rs:90-92): depth_for_particles (src/lib.rs:45-52),
+ * fixed_point_scale (src/lib.rs:59-69) and linear_work_bound
+ * (src/lib.rs:83-86).  Solves use these, as the harness does. */
+uint32_t rainbow_plan_depth(uint32_t particles);
+float rainbow_plan_fixed_point_scale(uint32_t particles);
+uint64_t rainbow_plan_linear_work(uint32_t particles, uint32_t depth);
+
+/* ------------------------------------------------------------------------ */
+/* Part 2. Params image, plan table and digest (native/main.rs).            */
+/* ------------------------------------------------------------------------ */
+
+/* The 128-byte uniform block, field for field as struct Params in
+ * src/rainbow_fmm.wgsl:18-50 (eight u32, then twenty-four f32). */
+typedef struct rainbow_params {
+  uint32_t particle_count;
+  uint32_t depth;
+  uint32_t level;
+  uint32_t level_offset;
+  uint32_t level_dim;
+  uint32_t parent_offset;
+  uint32_t parent_dim;
+  uint32_t total_cells;
+  float dt;
+  float gravity;
+  float softening;
+  float point_size;
+  float time;
+  float aspect;
+  float camera_yaw;
+  float camera_pitch;
+  float viewport_width;
+  float viewport_height;
+  float exposure;
+  float speed;
+  float control_point[4];     /* xyz brush centre, w > 0.5 = active      */
+  float control_axis_mode[4]; /* xyz vortex axis, w: 1 pull 2 push 3 spin */
+  float control_settings[4];  /* x strength, y radius, z fixed-point scale */
+} rainbow_params_t;
+
+#define RAINBOW_PARAM_BYTES 128u
+#define RAINBOW_PARAM_STRIDE 256u
+#define RAINBOW_PARAM_SLOTS 16u
+#define RAINBOW_PARTICLE_BYTES 32u
+#define RAINBOW_MOMENT_BYTES 16u
+#define RAINBOW_FIELD_BYTES 48u
+
+/* Fills one Params block with the harness constants (dt 0.0009, gravity 0.34,
+ * softening 0.012, speed 1, control off, scale in control_settings.z).
+ * Replaces write_params, native/main.rs:38-87. */
+void rainbow_write_params(uint32_t count, uint32_t depth, uint32_t level,
+                          uint32_t level_offset, uint32_t level_dim,
+                          uint32_t parent_offset, uint32_t parent_dim,
+                          uint32_t cells, float scale, rainbow_params_t* out);
+
+/* Fills the 16 x 256-byte slot table: slot l (l = 0..depth) is the level plan
+ * with parent = l-1 (saturating), slot 8+l (l = 0..depth-1) is the upward
+ * plan whose "parent" is the child level l+1.  Unused bytes are zero.
+ * Uses the unclamped planner.  Returns the depth.
+ * Replaces build_plans, native/main.rs:89-128. */
+uint32_t rainbow_build_plans(uint32_t count, uint8_t out_slots[4096]);
+
+/* FNV-1a-32 over bytes (basis 2166136261, prime 16777619).
+ * Replaces digest, native/main.rs:152-163. */
+uint32_t rainbow_fnv1a(const void* data, size_t bytes);
+
+/* Seeded initial conditions (host).  The reference seeds its galaxy on the
+ * GPU (initialize, src/rainbow_fmm.wgsl:232-374); these produce the
+ * synthetic inputs of the benchmark workloads.  SplitMix64 stream, f64
+ * generation rounded to f32, equal masses 1/N, hue uniform in [0,1),
+ * excitation 0.  `out` receives 32 N bytes in the reference layout.
+ * Return 0 or RAINBOW_E_INVALID. */
+/* Uniform in a ball of `radius`, at rest. */
+int rainbow_ic_uniform_ball(uint32_t n, uint64_t seed, double radius, void* out);
+/* Plummer sphere of scale radius a and total mass 1 truncated at rmax, with
+ * isotropic virial velocities for gravity G (Aarseth-Henon-Wielen). */
+int rainbow_ic_plummer(uint32_t n, uint64_t seed, double a, double rmax, double gravity,
+                       void* out);
+/* `clumps` Plummer clumps of scale radius a truncated at rcut_over_a * a,
+ * clump masses 10^U(0,2) normalised, bodies per clump in proportion,
+ * centres uniform in [-0.5, 0.5]^3, virial velocities within each clump. */
+int rainbow_ic_clumps(uint32_t n, uint64_t seed, uint32_t clumps, double a, double rcut_over_a,
+                      double gravity, void* out);
+
+/* ------------------------------------------------------------------------ */
+/* Part 3. Solver (one system).                                              */
+/* ------------------------------------------------------------------------ */
+
+typedef struct rainbow_hip rainbow_hip_t;
+
+enum {
+  RAINBOW_OK = 0,
+  RAINBOW_E_INVALID = -1, /* bad argument, size or handle          */
+  RAINBOW_E_HIP = -2,     /* a HIP runtime call failed             */
+  RAINBOW_E_NOMEM = -3,   /* device or host allocation failed      */
+  RAINBOW_E_BUDGET = -4,  /* request exceeds the admission budget  */
+  RAINBOW_E_STATE = -5    /* call not valid in the current state   */
+};
+
+enum { RAINBOW_PARTICLES = 0, RAINBOW_MOMENTS = 1, RAINBOW_FIELDS = 2 };
+
+/* Solver options (bit flags). */
+enum {
+  /* Neumaier-compensated FP32 sums in M2L, near field and the own-leaf term
+   * (the precision form).  Off by default: the plain step follows the
+   * reference's straight FP32 sums bit for bit. */
+  RAINBOW_OPT_COMPENSATED = 1u
+};
+
+/* Creates a solver for `particles` bodies.  Depth, cell count and scale come
+ * from the unclamped planner (as native/main.rs:90-92); dt, gravity,
+ * softening, speed and the control block come from `params` (NULL = the
+ * harness constants of native/main.rs:57-82).  A nonzero
+ * params->control_settings[2] overrides the planner's fixed-point scale.
+ * Particles start zeroed; fields start zeroed. */
+int rainbow_hip_create(uint32_t particles, const rainbow_params_t* params,
+                       rainbow_hip_t** out);
+/* Same, with option flags. */
+int rainbow_hip_create_ex(uint32_t particles, const rainbow_params_t* params,
+                          uint32_t options, rainbow_hip_t** out);
+void rainbow_hip_destroy(rainbow_hip_t* h);
+
+/* Physics parameters for later steps (same meaning as in create). */
+int rainbow_hip_set_params(rainbow_hip_t* h, const rainbow_params_t* params);
+
+/* Plan chosen at create: depth, total cells, fixed-point scale. */
+int rainbow_hip_plan(rainbow_hip_t* h, uint32_t* depth, uint32_t* cells,
+                     float* scale);
+
+/* Copies 32 * particles bytes of reference-layout particles in (host memory,
+ * caller-owned, copied before return). */
+int rainbow_hip_upload_particles(rainbow_hip_t* h, const void* particles,
+                                 size_t bytes);
+
+/* One field solve without integrate: clear, deposit, upward, M2L, downward,
+ * near (the digest point of native/main.rs:436-454).  Asynchronous. */
+int rainbow_hip_solve_fields(rainbow_hip_t* h);
+
+/* `steps` full steps (native/main.rs:412-421), replayed from a captured HIP
+ * graph.  Asynchronous. */
+int rainbow_hip_step(rainbow_hip_t* h, uint32_t steps);
+
+/* Waits for all work on the handle's stream. */
+int rainbow_hip_synchronize(rainbow_hip_t* h);
+
+/* Copies a buffer out in the reference layout: particles 32 N bytes, moments
+ * 16 C bytes, fields 48 C bytes.  `bytes` must equal that size.  Synchronous. */
+int rainbow_hip_download(rainbow_hip_t* h, int which, void* dst, size_t bytes);
+
+/* FNV-1a-32 of a downloaded buffer (as native/main.rs:450-454). */
+int rainbow_hip_digest(rainbow_hip_t* h, int which, uint32_t* out);
+
+/* Runs `steps` steps, timed with HIP events on the handle's stream; writes
+ * the elapsed milliseconds.  The timed region holds only the step graph
+ * replays; inputs are already resident. */
+int rainbow_hip_time_steps(rainbow_hip_t* h, uint32_t steps, float* ms);
+
+/* Runs `steps` steps without the graph, with a HIP event pair around every
+ * kernel launch, and writes the mean duration in ms of each phase:
+ * out[0] clear, out[1] deposit, out[2] upward, out[3 + l - 1] the tree pass
+ * of level l (M2L + L2L + near, l = 1..depth).  `n` is the capacity of out
+ * (at least 3 + depth).  Returns the number of entries written (>= 0) or a
+ * negative status. */
+int rainbow_hip_phase_ms(rainbow_hip_t* h, uint32_t steps, float* out,
+                         uint32_t n);
+
+/* Number of cells active in the last solve at each level (M2L targets:
+ * non-empty cells whose parent holds more than 16 bodies), plus the
+ * number of near-field leaves.  out[l] for l = 0..6, out[7] = leaves. */
+int rainbow_hip_tree_stats(rainbow_hip_t* h, uint32_t out[8]);
+
+/* FP64 diagnostics of the current particle state (precision form):
+ * out[0] kinetic energy, out[1] potential energy (the tree's own potential
+ * at each body, halved), out[2..4] linear momentum, out[5..7] angular
+ * momentum.  Uses the fields of the last solve; call after
+ * rainbow_hip_solve_fields for a state that matches the particles. */
+int rainbow_hip_diagnostics(rainbow_hip_t* h, double out[8]);
+
+/* Per-body acceleration the integrator applies (local expansion plus the
+ * own-leaf monopole), as float4 {ax, ay, az, phi} per body, for the current
+ * particles and the fields of the last solve.  `bytes` = 16 N. */
+int rainbow_hip_accelerations(rainbow_hip_t* h, void* dst, size_t bytes);
+
+/* Error message of the last failing call on this thread ("" if none). */
+const char* rainbow_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* Part 4. Batch: many independent systems advanced together.               */
+/* ------------------------------------------------------------------------ */
+
+typedef struct rainbow_batch rainbow_batch_t;
+
+/* Creates `systems` independent systems with the given body counts (each
+ * >= 1, planner depth per system), sharing the physics of `params` (NULL =
+ * harness constants).  Each system's scale is its own planner scale. */
+int rainbow_batch_create(uint32_t systems, const uint32_t* counts,
+                         const rainbow_params_t* params, uint32_t options,
+                         rainbow_batch_t** out);
+void rainbow_batch_destroy(rainbow_batch_t* b);
+int rainbow_batch_upload(rainbow_batch_t* b, uint32_t system,
+                         const void* particles, size_t bytes);
+int rainbow_batch_download(rainbow_batch_t* b, uint32_t system, int which,
+                           void* dst, size_t bytes);
+/* Solve without integrate, for every system. */
+int rainbow_batch_solve_fields(rainbow_batch_t* b);
+/* `steps` steps of every system, one captured graph replay per step. */
+int rainbow_batch_step(rainbow_batch_t* b, uint32_t steps);
+int rainbow_batch_time_steps(rainbow_batch_t* b, uint32_t steps, float* ms);
+int rainbow_batch_synchronize(rainbow_batch_t* b);
+/* Total bodies over all systems. */
+uint64_t rainbow_batch_particles(rainbow_batch_t* b);
+
+/* ------------------------------------------------------------------------ */
+/* Part 5. Serving: continuous batching of N-body solve requests.           */
+/* ------------------------------------------------------------------------ */
+
+typedef struct rainbow_server rainbow_server_t;
+
+/* A request: body count, steps to run, and its particles (32 N bytes,
+ * reference layout, caller-owned, copied at submit). */
+typedef struct rainbow_request {
+  uint64_t id;
+  uint32_t particles;
+  uint32_t steps;
+  const void* data;
+} rainbow_request_t;
+
+/* Completion record. */
+typedef struct rainbow_completion {
+  uint64_t id;
+  uint32_t particles;
+  uint32_t steps;
+  double submit_s;   /* host clock at submit                */
+  double admit_s;    /* host clock when it got a slot       */
+  double finish_s;   /* host clock when its last step ended */
+} rainbow_completion_t;
+
+/* A server with `budget_bytes` of device memory for resident systems and at
+ * most `max_slots` systems in flight. */
+int rainbow_server_create(uint64_t budget_bytes, uint32_t max_slots,
+                          const rainbow_params_t* params, uint32_t options,
+                          rainbow_server_t** out);
+void rainbow_server_destroy(rainbow_server_t* s);
+/* Device bytes a request of `particles` bodies holds while resident. */
+uint64_t rainbow_server_request_bytes(uint32_t particles);
+/* Queues a request (copies its particles).  RAINBOW_E_BUDGET if it could
+ * never fit the budget. */
+int rainbow_server_submit(rainbow_server_t* s, const rainbow_request_t* r);
+/* One scheduling iteration: retire finished systems, admit queued requests
+ * while budget and slots allow, run one batched step over every resident
+ * system.  Writes the number of systems stepped. */
+int rainbow_server_poll(rainbow_server_t* s, uint32_t* stepped);
+/* Pops up to `cap` completions; writes how many. */
+int rainbow_server_completions(rainbow_server_t* s, rainbow_completion_t* out,
+                               uint32_t cap, uint32_t* n);
+/* Copies a completed request's final particles out (32 N bytes), once. */
+int rainbow_server_result(rainbow_server_t* s, uint64_t id, void* dst,
+                          size_t bytes);
+/* Requests queued, resident, device bytes resident. */
+int rainbow_server_state(rainbow_server_t* s, uint32_t* queued,
+                         uint32_t* resident, uint64_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RAINBOW_FMM_H */

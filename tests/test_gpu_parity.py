@@ -1,0 +1,309 @@
+"""GPU parity: the HIP path against the oracle, through the C ABI.
+
+The bar is bit-exactness for everything: the integer moments (order-free
+fixed-point sums), and the f32 fields and particles, because the kernels
+evaluate the reference expressions operation for operation with correctly
+rounded division/sqrt and no contraction, and keep the reference's
+summation order.  The only exception is the optional control field, which
+calls exp (device and host libm differ in the last bits): it is checked to
+a relative tolerance of 1e-5.
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as orc
+import rainbow_fmm as rf
+import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def assert_bits(got, want, what):
+    got = np.ascontiguousarray(got)
+    want = np.ascontiguousarray(want)
+    assert got.shape == want.shape, f"{what}: shape {got.shape} != {want.shape}"
+    if got.tobytes() == want.tobytes():
+        return
+    g = got.view(np.uint32) if got.dtype.itemsize == 4 else got
+    w = want.view(np.uint32) if want.dtype.itemsize == 4 else want
+    bad = np.argwhere(g != w)
+    first = tuple(bad[0])
+    raise AssertionError(
+        f"{what}: {len(bad)} of {g.size} words differ; first at {first}: got {got[first]!r} want {want[first]!r}"
+    )
+
+
+def slots_for(n, dt, gravity, softening, control=None):
+    return orc.build_plans(n, dt=dt, gravity=gravity, softening=softening, control=control)
+
+
+def run_solve(parts, dt, gravity, softening, compensated=False):
+    n = parts.shape[0]
+    s = rf.Solver(n, rf.physics(dt=dt, gravity=gravity, softening=softening), compensated=compensated)
+    s.upload(parts)
+    s.solve_fields()
+    return s, s.moments(), s.fields()
+
+
+def test_cfg1_field_solve_bit_exact(golden):
+    say("cfg1 solve_fields, 65536 uniform ball")
+    parts = wl.cfg1_particles()
+    assert orc.fnv1a(parts) == golden["cfg1"]["input_digest"]
+    c = wl.CFG1
+    s, m, f = run_solve(parts, c["dt"], c["gravity"], c["softening"])
+    om, of = orc.solve_fields(slots_for(c["n"], c["dt"], c["gravity"], c["softening"]), parts)
+    assert_bits(m, om, "moments")
+    assert_bits(f, of, "fields")
+    assert s.digest(rf.MOMENTS) == golden["cfg1"]["moments_digest"]
+    assert s.digest(rf.FIELDS) == golden["cfg1"]["fields_digest"]
+    s.close()
+
+
+def test_cfg1_ten_steps_bit_exact(golden):
+    say("cfg1 10 steps")
+    parts = wl.cfg1_particles()
+    c = wl.CFG1
+    s = rf.Solver(c["n"], wl.params_for(c))
+    s.upload(parts)
+    s.step(10)
+    p = s.particles()
+    op, om, of = orc.step(slots_for(c["n"], c["dt"], c["gravity"], c["softening"]), parts, 10)
+    assert_bits(p, op, "particles")
+    assert_bits(s.moments(), om, "moments")
+    assert_bits(s.fields(), of, "fields")
+    assert orc.fnv1a(p) == golden["cfg1"]["step_particles_digest"]
+    s.close()
+
+
+def test_cfg2_plummer_million_bit_exact(golden):
+    say("cfg2 1M Plummer solve + 3 steps")
+    parts = wl.cfg2_particles()
+    c = wl.CFG2
+    assert orc.fnv1a(parts) == golden["cfg2"]["input_digest"]
+    slots = slots_for(c["n"], c["dt"], c["gravity"], c["softening"])
+    s, m, f = run_solve(parts, c["dt"], c["gravity"], c["softening"])
+    om, of = orc.solve_fields(slots, parts)
+    assert_bits(m, om, "moments")
+    assert_bits(f, of, "fields")
+    s.upload(parts)
+    s.step(3)
+    op, om3, of3 = orc.step(slots, parts, 3)
+    assert_bits(s.particles(), op, "particles after 3 steps")
+    assert_bits(s.fields(), of3, "fields after 3 steps")
+    assert orc.fnv1a(op) == golden["cfg2"]["step_particles_digest"]
+    s.close()
+
+
+def _edge_input(kind, n, seed):
+    rng = np.random.default_rng(seed)
+    if kind == "plummer":
+        return rf.ic_plummer(n, seed, 0.1, 1.5, 1.0)
+    if kind == "ball":
+        return rf.ic_uniform_ball(n, seed, 1.0)
+    p = np.zeros((n, 8), dtype=np.float32)
+    if kind == "point":
+        p[:, :3] = np.float32(0.25)
+    elif kind == "faces":
+        # Coordinates exactly on cell faces of several levels, and beyond
+        # the world edge on both sides.
+        grid = np.array([-1.7, -1.6, -0.8, -0.4, 0.0, 0.2, 0.4, 0.8, 1.2, 1.6, 1.7, 2.5], dtype=np.float32)
+        p[:, :3] = rng.choice(grid, size=(n, 3))
+        p[:, 4:7] = rng.normal(0, 0.3, size=(n, 3)).astype(np.float32)
+    elif kind == "ties":
+        # x * scale lands exactly on .5: round-half-to-even cases.
+        scale = rf.plan_scale(n)
+        k = rng.integers(-1000, 1000, size=(n, 3))
+        p[:, :3] = ((k + 0.5) / scale).astype(np.float32)
+    elif kind == "clustered":
+        centres = rng.uniform(-1.2, 1.2, size=(5, 3))
+        idx = rng.integers(0, 5, size=n)
+        p[:, :3] = (centres[idx] + rng.normal(0, 0.02, size=(n, 3))).astype(np.float32)
+        p[:, 4:7] = rng.normal(0, 0.5, size=(n, 3)).astype(np.float32)
+    p[:, 3] = rng.uniform(0, 1, size=n).astype(np.float32)
+    p[:, 7] = rng.uniform(0, 1, size=n).astype(np.float32)
+    return p
+
+
+EDGE_CASES = [
+    ("ball", 1), ("ball", 2), ("point", 17), ("clustered", 100), ("faces", 777), ("ties", 4096),
+    ("plummer", 4096), ("clustered", 16385), ("faces", 32769), ("plummer", 262145), ("point", 300000),
+]
+
+
+@pytest.mark.parametrize("kind,n", EDGE_CASES, ids=[f"{k}-{n}" for k, n in EDGE_CASES])
+def test_edge_cases_bit_exact(kind, n):
+    say(f"edge {kind} n={n}")
+    parts = _edge_input(kind, n, 1234 + n)
+    dt, g, soft = 2e-3, 0.7, 2e-3
+    slots = slots_for(n, dt, g, soft)
+    s, m, f = run_solve(parts, dt, g, soft)
+    om, of = orc.solve_fields(slots, parts)
+    assert_bits(m, om, "moments")
+    assert_bits(f, of, "fields")
+    s.upload(parts)
+    s.step(3)
+    op, om3, of3 = orc.step(slots, parts, 3)
+    assert_bits(s.particles(), op, "particles")
+    assert_bits(s.moments(), om3, "moments after steps")
+    assert_bits(s.fields(), of3, "fields after steps")
+    s.close()
+
+
+def test_compensated_form_bit_exact():
+    say("compensated 200k Plummer")
+    n = 200000
+    parts = rf.ic_plummer(n, 77, 0.1, 1.5, 1.0)
+    dt, g, soft = 5e-4, 1.0, 1e-3
+    slots = slots_for(n, dt, g, soft)
+    s, m, f = run_solve(parts, dt, g, soft, compensated=True)
+    om, of = orc.solve_fields(slots, parts, compensated=True)
+    assert_bits(m, om, "moments")
+    assert_bits(f, of, "fields (compensated)")
+    acc = s.accelerations()
+    oacc = orc.accelerations(slots, parts, om, of, compensated=True)
+    assert_bits(acc[:, :3], oacc[:, :3], "accelerations (compensated)")
+    s.upload(parts)
+    s.step(2)
+    op, _, _ = orc.step(slots, parts, 2, compensated=True)
+    assert_bits(s.particles(), op, "particles (compensated)")
+    s.close()
+
+
+def test_accelerations_match_oracle():
+    say("accelerations 65536")
+    parts = wl.cfg1_particles()
+    c = wl.CFG1
+    slots = slots_for(c["n"], c["dt"], c["gravity"], c["softening"])
+    s, m, f = run_solve(parts, c["dt"], c["gravity"], c["softening"])
+    acc = s.accelerations()
+    oacc = orc.accelerations(slots, parts, m, f)
+    assert_bits(acc, oacc, "accelerations and potential")
+    s.close()
+
+
+def test_control_field_within_tolerance():
+    say("control field on")
+    n = 50000
+    parts = rf.ic_plummer(n, 5, 0.1, 1.5, 1.0)
+    control = ((0.1, 0.05, -0.02, 1.0), (0.0, 1.0, 0.0, 3.0), 0.7, 0.22)
+    dt, g, soft = 1e-3, 1.0, 1e-3
+    slots = slots_for(n, dt, g, soft, control=control)
+    s = rf.Solver(n, rf.physics(dt=dt, gravity=g, softening=soft, control=control))
+    s.upload(parts)
+    s.step(1)
+    op, _, _ = orc.step(slots, parts, 1)
+    got = s.particles()
+    # Relative tolerance 1e-5 on position and velocity (exp differs by ulps
+    # between the device and host libraries).
+    for sl in (slice(0, 3), slice(4, 7)):
+        err = np.linalg.norm(got[:, sl] - op[:, sl], axis=1)
+        ref = np.linalg.norm(op[:, sl], axis=1) + 1e-6
+        assert np.max(err / ref) < 1e-5
+    s.close()
+
+
+def test_batch_equals_solo():
+    say("batch of 6 uneven systems")
+    counts = [4096, 70000, 1, 300000, 20000, 262144]
+    dt, g, soft = 5e-4, 1.0, 1e-3
+    inputs = [rf.ic_plummer(n, 4000 + i, 0.1, 1.5, 1.0) for i, n in enumerate(counts)]
+    b = rf.Batch(counts, rf.physics(dt=dt, gravity=g, softening=soft))
+    for i, p in enumerate(inputs):
+        b.upload(i, p)
+    b.step(5)
+    for i, n in enumerate(counts):
+        op, om, of = orc.step(slots_for(n, dt, g, soft), inputs[i], 5)
+        assert_bits(b.download(i, rf.PARTICLES), op, f"system {i} particles")
+        assert_bits(b.download(i, rf.MOMENTS), om, f"system {i} moments")
+        assert_bits(b.download(i, rf.FIELDS), of, f"system {i} fields")
+    b.close()
+
+
+def test_graph_replay_consistency():
+    say("graph replays: 16-step graph vs single steps vs eager oracle")
+    n = 100000
+    parts = rf.ic_plummer(n, 9, 0.1, 1.5, 1.0)
+    dt, g, soft = 5e-4, 1.0, 1e-3
+    a = rf.Solver(n, rf.physics(dt=dt, gravity=g, softening=soft))
+    a.upload(parts)
+    a.step(37)  # two 16-step graph replays + five single-step replays
+    b = rf.Solver(n, rf.physics(dt=dt, gravity=g, softening=soft))
+    b.upload(parts)
+    for _ in range(37):
+        b.step(1)
+        b.sync()
+    op, _, _ = orc.step(slots_for(n, dt, g, soft), parts, 37)
+    assert_bits(a.particles(), op, "37 steps through graphs")
+    assert_bits(b.particles(), op, "37 single-step replays")
+    a.close()
+    b.close()
+
+
+def test_server_results_equal_solo():
+    say("server: 7 requests, results alone == in the mix")
+    dt, g, soft = 5e-4, 1.0, 1e-3
+    prm = rf.physics(dt=dt, gravity=g, softening=soft)
+    reqs = [(i, n, st) for i, (n, st) in enumerate([(16384, 5), (50000, 2), (1000, 9), (262145, 3), (30000, 1),
+                                                       (70000, 7), (20000, 4)])]
+    inputs = {i: rf.ic_uniform_ball(n, 5000 + i, 1.0) if i % 2 else rf.ic_plummer(n, 5000 + i, 0.1, 1.5, 1.0)
+              for i, n, _ in reqs}
+    budget = sum(rf.Server.request_bytes(n) for _, n, _ in reqs[:4])
+    srv = rf.Server(budget, 3, prm)
+    for i, n, st in reqs:
+        srv.submit(i, inputs[i], st)
+    done = {}
+    for _ in range(200):
+        srv.poll()
+        for c in srv.completions():
+            done[c.id] = c
+        if len(done) == len(reqs):
+            break
+    assert len(done) == len(reqs)
+    for i, n, st in reqs:
+        got = srv.result(i, n)
+        op, _, _ = orc.step(slots_for(n, dt, g, soft), inputs[i], st)
+        assert_bits(got, op, f"request {i}")
+        assert done[i].finish_s >= done[i].admit_s >= done[i].submit_s
+    q, r, used = srv.state()
+    assert (q, r, used) == (0, 0, 0)
+    srv.close()
+
+
+def test_determinism_two_runs():
+    say("determinism")
+    n = 1 << 20
+    parts = wl.cfg2_particles()
+    out = []
+    for _ in range(2):
+        s = rf.Solver(n, wl.params_for(wl.CFG2))
+        s.upload(parts)
+        s.step(2)
+        out.append((s.particles(), s.moments(), s.fields()))
+        s.close()
+    for k in range(3):
+        assert_bits(out[0][k], out[1][k], f"run-to-run buffer {k}")
+
+
+def test_diagnostics_fp64():
+    say("FP64 diagnostics")
+    n = 65536
+    parts = rf.ic_plummer(n, 11, 0.1, 1.5, 1.0)
+    c = dict(dt=5e-4, gravity=1.0, softening=1e-3)
+    slots = slots_for(n, **c)
+    s, m, f = run_solve(parts, **c)
+    d = s.diagnostics()
+    cons = orc.conserved_f64(parts)
+    acc = orc.accelerations(slots, parts, m, f)
+    pe = 0.5 * np.sum(acc[:, 3].astype(np.float64)) / n
+    assert abs(d[0] - cons[0]) <= 1e-12 * abs(cons[0]) + 1e-15
+    assert abs(d[1] - pe) <= 1e-12 * abs(pe)
+    np.testing.assert_allclose(d[2:], cons[2:], rtol=0, atol=1e-14)
+    s.close()
