@@ -1,1 +1,112 @@
-int main() { return 0; }
+// Headless harness for the gfx950 FMM step, the counterpart of the
+// reference's native/main.rs: plans a system, seeds it, prints the plan, the
+// fields/moments FNV-1a digests of one solve without integrate, per-phase
+// GPU times, the amortised time of a full step and a finiteness check over
+// a particle slice -- the same report lines, read side by side.
+//
+//   rainbow_native [particles (default 2^20)] [steps (default 40)]
+//
+// The reference seeds its galaxy on the GPU (src/rainbow_fmm.wgsl:232-374);
+// this harness seeds a Plummer sphere (a = 0.1, cut at 1.5, seed 1) with the
+// harness physics constants of native/main.rs:57-82.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "rainbow_fmm.h"
+
+static int check(int rc, const char* what) {
+  if (rc != RAINBOW_OK) {
+    std::fprintf(stderr, "%s failed (%d): %s\n", what, rc, rainbow_last_error());
+    std::exit(1);
+  }
+  return rc;
+}
+
+int main(int argc, char** argv) {
+  const uint32_t count = argc > 1 ? static_cast<uint32_t>(std::strtoul(argv[1], nullptr, 10)) : (1u << 20);
+  const uint32_t repeat = argc > 2 ? static_cast<uint32_t>(std::strtoul(argv[2], nullptr, 10)) : 40u;
+  if (count == 0u || repeat == 0u) {
+    std::fprintf(stderr, "usage: rainbow_native [particles] [steps]\n");
+    return 2;
+  }
+
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) != hipSuccess) {
+    std::fprintf(stderr, "no HIP device\n");
+    return 1;
+  }
+  std::printf("adapter: %s (HIP, %s)\n", prop.name, prop.gcnArchName);
+
+  uint8_t slots[4096];
+  const uint32_t depth = rainbow_build_plans(count, slots);
+  rainbow_params_t params;
+  std::memcpy(&params, slots + depth * RAINBOW_PARAM_STRIDE, sizeof(params));
+  std::printf("count=%u depth=%u cells=%u scale=%g\n", count, depth, params.total_cells,
+              static_cast<double>(params.control_settings[2]));
+
+  std::vector<float> parts(8ull * count);
+  check(rainbow_ic_plummer(count, 1u, 0.1, 1.5, params.gravity, parts.data()), "seed");
+
+  rainbow_hip_t* h = nullptr;
+  check(rainbow_hip_create(count, &params, &h), "create");
+  check(rainbow_hip_upload_particles(h, parts.data(), parts.size() * sizeof(float)), "upload");
+
+  // Content digest of one field solve (native/main.rs:436-454).
+  check(rainbow_hip_solve_fields(h), "solve");
+  uint32_t fh = 0, mh = 0;
+  check(rainbow_hip_digest(h, RAINBOW_FIELDS, &fh), "digest");
+  check(rainbow_hip_digest(h, RAINBOW_MOMENTS, &mh), "digest");
+  std::printf("fieldsHash=%08x momentsHash=%08x\n", fh, mh);
+
+  // Warm up (native/main.rs:456-464).
+  check(rainbow_hip_step(h, 300u), "warm-up");
+  check(rainbow_hip_synchronize(h), "sync");
+
+  // Per-phase GPU time of one step, events around each kernel.
+  float phase[16];
+  const int nphase = rainbow_hip_phase_ms(h, 5u, phase, 16u);
+  if (nphase < 0) check(nphase, "phase timing");
+  const char* fixed[3] = {"clear", "deposit", "upward"};
+  double total = 0.0;
+  for (int i = 0; i < nphase; ++i) {
+    char name[32];
+    if (i < 3)
+      std::snprintf(name, sizeof(name), "%s", fixed[i]);
+    else if (i < nphase - 1)
+      std::snprintf(name, sizeof(name), "tree_l%d", i - 2);
+    else
+      std::snprintf(name, sizeof(name), "integrate");
+    std::printf("  ts %-10s %.4f ms\n", name, static_cast<double>(phase[i]));
+    total += phase[i];
+  }
+  std::printf("  ts TOTAL     %.4f ms\n", total);
+
+  // Amortised time of a full step (native/main.rs:522-534): graph replays.
+  float ms = 0.0f;
+  check(rainbow_hip_time_steps(h, repeat, &ms), "time");
+  std::printf("full solve: %.4f ms/step  (%u steps)  %.4g particle-steps/s\n", ms / repeat, repeat,
+              static_cast<double>(count) * repeat / (ms * 1e-3));
+
+  // Finiteness over a particle slice (native/main.rs:536-549).
+  std::vector<float> out(8ull * count);
+  check(rainbow_hip_download(h, RAINBOW_PARTICLES, out.data(), out.size() * sizeof(float)), "download");
+  const size_t probe = std::min<size_t>(count, 8192u) * 8u;
+  size_t non_finite = 0;
+  float max_abs = 0.0f;
+  for (size_t i = 0; i < probe; ++i) {
+    if (std::isfinite(out[i]))
+      max_abs = std::max(max_abs, std::fabs(out[i]));
+    else
+      ++non_finite;
+  }
+  std::printf("non-finite=%zu maxAbs=%.6f\n", non_finite, static_cast<double>(max_abs));
+  rainbow_hip_destroy(h);
+  return 0;
+}

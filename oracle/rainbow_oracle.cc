@@ -94,7 +94,7 @@ uint32_t octree_offset(uint32_t level) { return (level_cells(level) - 1u) / 7u; 
 uint32_t total_cells(uint32_t depth) { return octree_offset(depth + 1u); }
 
 uint32_t depth_for(uint32_t n) {
-  const uint32_t target = (n + 7u) / 8u;  // div_ceil(n, 8) without overflow for n < 2^32 - 7
+  const uint32_t target = n / 8u + (n % 8u != 0u ? 1u : 0u);  // u32::div_ceil(n, 8)
   uint32_t d = 4u;
   while (d < 6u && level_cells(d) < target) ++d;
   return d;
