@@ -82,6 +82,134 @@ def cpu_baseline(parts, cfg, budget_s):
     }
 
 
+def batch_form(args):
+    """256 uneven Plummer systems advanced together (one launch sequence per
+    step, graph-captured) against the same systems stepped one after another
+    with their own solvers."""
+    import numpy as np
+
+    import rainbow_fmm as rf
+
+    rng = np.random.default_rng(4)
+    counts = [int(round(np.exp(rng.uniform(np.log(4096), np.log(262144))))) for _ in range(args.systems)]
+    prm = rf.physics(dt=5e-4, gravity=1.0, softening=1e-3)
+    inputs = [rf.ic_plummer(n, 4000 + i, 0.1, 1.5, 1.0) for i, n in enumerate(counts)]
+    total = sum(counts)
+    b = rf.Batch(counts, prm)
+    for i, p in enumerate(inputs):
+        b.upload(i, p)
+    b.step(args.warmup)
+    b.sync()
+    ms_batch = b.time_steps(args.steps)
+    b.close()
+    # Sequential plain launches: each system on its own solver.
+    solvers = []
+    for i, n in enumerate(counts):
+        s = rf.Solver(n, prm)
+        s.upload(inputs[i])
+        s.step(args.warmup)
+        s.sync()
+        solvers.append(s)
+    t0 = time.perf_counter()
+    for s in solvers:
+        s.step(args.steps)
+    for s in solvers:
+        s.sync()
+    seq_s = time.perf_counter() - t0
+    for s in solvers:
+        s.close()
+    batch_rate = total * args.steps / (ms_batch / 1e3)
+    seq_rate = total * args.steps / seq_s
+    return {
+        "metric": "batched particle-steps/s, 256 uneven Plummer systems",
+        "value": batch_rate,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_batch / args.steps,
+        "higher_is_better": True,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{len(counts)} Plummer systems, N log-uniform in [4096, 262144] (seed 4), "
+                               f"{total} bodies, dt 5e-4, softening 1e-3"},
+        "sequential_value": seq_rate,
+        "speedup_vs_sequential": batch_rate / seq_rate,
+    }
+
+
+def serve_form(args):
+    """Poisson arrivals of N-body requests into the continuous-batching
+    server; latency = completion - arrival."""
+    import numpy as np
+
+    import rainbow_fmm as rf
+
+    rng = np.random.default_rng(5)
+    nreq = args.requests
+    reqs = []
+    for j in range(nreq):
+        n = int(round(np.exp(rng.uniform(np.log(16384), np.log(args.serve_nmax)))))
+        steps = int(rng.integers(1, 201))
+        kind = "plummer" if rng.uniform() < 0.5 else "ball"
+        reqs.append((j, n, steps, kind))
+    data = {}
+    for j, n, steps, kind in reqs:
+        data[j] = (rf.ic_plummer(n, 5000 + j, 0.1, 1.5, 1.0) if kind == "plummer"
+                   else rf.ic_uniform_ball(n, 5000 + j, 1.0))
+    prm = rf.physics(dt=5e-4, gravity=1.0, softening=1e-3)
+    budget = int(args.budget_gb * 1e9)
+    # Capacity probe: a mix of the request sizes stepped as one batch.
+    probe = [n for _, n, _, _ in reqs[:32]]
+    b = rf.Batch(probe, prm)
+    for i, n in enumerate(probe):
+        b.upload(i, data[i])
+    b.step(2)
+    b.sync()
+    ms = b.time_steps(10)
+    b.close()
+    capacity = sum(probe) * 10 / (ms / 1e3)
+    mean_work = float(np.mean([n * s for _, n, s, _ in reqs]))
+    lam = args.load * capacity / mean_work
+    gaps = rng.exponential(1.0 / lam, size=nreq)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    srv = rf.Server(budget, args.max_slots, prm)
+    done = {}
+    t0 = time.monotonic()
+    nxt = 0
+    work = 0
+    while len(done) < nreq:
+        now = time.monotonic() - t0
+        while nxt < nreq and arrivals[nxt] <= now:
+            j, n, steps, _ = reqs[nxt]
+            srv.submit(j, data[j], steps)
+            nxt += 1
+        stepped = srv.poll()
+        for c in srv.completions():
+            done[c.id] = c
+            work += c.particles * c.steps
+        if stepped == 0 and nxt < nreq:
+            time.sleep(max(0.0, min(0.001, arrivals[nxt] - (time.monotonic() - t0))))
+    wall = time.monotonic() - t0
+    lat = np.array([done[j].finish_s - (t0 + arrivals[j]) for j in range(nreq)])
+    srv.close()
+    return {
+        "metric": "serving: p50/p99 request latency and aggregate particle-steps/s",
+        "value": work / wall,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "p50_ms": float(np.percentile(lat, 50) * 1e3),
+        "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "higher_is_better": True,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{nreq} requests, Poisson arrivals at {lam:.1f}/s ({args.load} x probed capacity "
+                               f"{capacity:.3g} particle-steps/s), N log-uniform in [16384, {args.serve_nmax}], "
+                               f"steps U{{1..200}}, Plummer or uniform ball, budget {args.budget_gb} GB, "
+                               f"max {args.max_slots} resident"},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +219,13 @@ def main():
     ap.add_argument("--phase-steps", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--form", choices=["plain", "batch", "serve"], default="plain")
+    ap.add_argument("--systems", type=int, default=256)
+    ap.add_argument("--requests", type=int, default=96)
+    ap.add_argument("--serve-nmax", type=int, default=2097152)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--max-slots", type=int, default=256)
+    ap.add_argument("--load", type=float, default=0.8)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -99,6 +234,13 @@ def main():
 
     import rainbow_fmm as rf
     import workloads as wl
+
+    if args.form == "batch":
+        print(json.dumps(batch_form(args)), flush=True)
+        return
+    if args.form == "serve":
+        print(json.dumps(serve_form(args)), flush=True)
+        return
 
     cfg = dict(wl.CFG2)
     n = args.n

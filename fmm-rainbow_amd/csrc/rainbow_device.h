@@ -78,7 +78,13 @@ struct StepArgs {
   uint32_t list_cap[kLevels];
   uint32_t nsys;
   uint32_t compensated;
+  uint32_t tree_kernel;  // kTreeLane or kTreeWave
 };
+
+// Tree-pass variants (identical results): one lane per target, or one
+// wave per internal parent.
+constexpr uint32_t kTreeLane = 0;
+constexpr uint32_t kTreeWave = 1;
 
 constexpr uint32_t kClearThreads = 256;
 constexpr uint32_t kClearCells = 4096;

@@ -552,6 +552,217 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t leve
 }
 
 // ------------------------------------------------------------------------
+// tree(level), one wave per internal parent.  The 8 children are the
+// targets; the M2L window is walked one source-parent block (8 sources) at
+// a time: the 64 lanes evaluate the 8 x 8 (target, source) pair fields in
+// parallel into LDS, then 80 lanes-worth of chains -- one per (target,
+// component) -- add the 8 terms in the reference order.  Same sums, same
+// order, same bits as the lane-per-target kernel, with 8x the parallelism.
+// ------------------------------------------------------------------------
+
+// One pair's ten field terms (pair_field, src/rainbow_fmm.wgsl:170-203).
+__device__ __forceinline__ void pair_terms(const float4 src, float tx, float ty, float tz, float eps2,
+                                           float out[10]) {
+  const float rx = src.x - tx;
+  const float ry = src.y - ty;
+  const float rz = src.z - tz;
+  const float r2 = ((rx * rx + ry * ry) + rz * rz) + eps2;
+  const float inv_r = 1.0f / sqrtf(r2);
+  const float inv_r3 = inv_r * inv_r * inv_r;
+  const float inv_r5 = inv_r3 * inv_r * inv_r;
+  const float gm = src.w;
+  const float diagonal = -gm * inv_r3;
+  const float tidal = 3.0f * gm * inv_r5;
+  out[0] = gm * rx * inv_r3;
+  out[1] = gm * ry * inv_r3;
+  out[2] = gm * rz * inv_r3;
+  out[3] = -gm * inv_r;
+  out[4] = diagonal + tidal * rx * rx;
+  out[5] = tidal * rx * ry;
+  out[6] = tidal * rx * rz;
+  out[7] = diagonal + tidal * ry * ry;
+  out[8] = tidal * ry * rz;
+  out[9] = diagonal + tidal * rz * rz;
+}
+
+// One running sum of a chain (plain or Neumaier-compensated).
+template <bool kComp>
+struct Chain {
+  float s = 0.0f, c = 0.0f;
+  __device__ __forceinline__ void add(float t) {
+    if (!kComp) {
+      s = s + t;
+    } else {
+      const float u = s + t;
+      if (fabsf(s) >= fabsf(t))
+        c = c + ((s - u) + t);
+      else
+        c = c + ((t - u) + s);
+      s = u;
+    }
+  }
+  __device__ __forceinline__ float value() const { return kComp ? s + c : s; }
+  __device__ __forceinline__ void reset(float x) {
+    s = x;
+    c = 0.0f;
+  }
+};
+
+constexpr uint32_t kWaveTreeThreads = 64;
+constexpr uint32_t kWaveTreeMaxBlocks = 16384;
+
+template <bool kComp>
+__global__ __launch_bounds__(kWaveTreeThreads) void k_tree_wave(StepArgs a, uint32_t level) {
+  __shared__ float terms[10][kWaveTreeThreads];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t nitems = min(a.list_count[level - 1u], a.list_cap[level - 1u]);
+  const uint32_t* items = a.lists + a.list_off[level - 1u];
+  const uint32_t dim = 1u << level;
+  const uint32_t pdim = dim >> 1u;
+  const int dim_i = static_cast<int>(dim);
+  const int pdim_i = static_cast<int>(pdim);
+  const uint32_t off = level_off(level);
+  const uint32_t poff = level_off(level - 1u);
+  const float cell_epsilon = (0.06f * kWorldHalfExtent) / static_cast<float>(dim);
+  const float eps2_m2l = cell_epsilon * cell_epsilon;
+  // Pair lane: target tt (child of the parent), source cc (child of the
+  // source-parent block, or near-offset slot).
+  const uint32_t tt = lane >> 3, cc = lane & 7u;
+  // Chain lanes: chain q = target q / 10, component q % 10; lanes < 16 own
+  // a second chain q + 64.
+  const bool two = lane < 16u;
+  const uint32_t ct0 = lane / 10u, ck0 = lane % 10u;
+  const uint32_t ct1 = (lane + 64u) / 10u, ck1 = (lane + 64u) % 10u;
+
+  for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const uint32_t item = items[it];
+    const uint32_t s = item >> 16;
+    if (s >= a.nsys) continue;
+    const uint32_t pidx = item & 0xFFFFu;
+    const SysDesc d = a.sys[s];
+    const uint32_t pz = pidx / (pdim * pdim);
+    const uint32_t prest = pidx - pz * pdim * pdim;
+    const uint32_t py = prest / pdim;
+    const uint32_t px = prest - py * pdim;
+    const float4* src = d.source + off;
+    const int* counts = reinterpret_cast<const int*>(d.moments);
+
+    // This lane's pair target.
+    const uint32_t tx = 2u * px + (tt & 1u), ty = 2u * py + ((tt >> 1) & 1u), tz = 2u * pz + (tt >> 2);
+    const int tcount = counts[4u * (off + lin(tx, ty, tz, dim))];
+    const float tcx = center1(tx, dim), tcy = center1(ty, dim), tcz = center1(tz, dim);
+
+    Chain<kComp> ch0, ch1;
+    for (int oz = -1; oz <= 1; ++oz) {
+      const int sz = static_cast<int>(pz) + oz;
+      if (sz < 0 || sz >= pdim_i) continue;
+      for (int oy = -1; oy <= 1; ++oy) {
+        const int sy = static_cast<int>(py) + oy;
+        if (sy < 0 || sy >= pdim_i) continue;
+        for (int ox = -1; ox <= 1; ++ox) {
+          const int sx = static_cast<int>(px) + ox;
+          if (sx < 0 || sx >= pdim_i) continue;
+          float t[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+          const int qx = 2 * sx + static_cast<int>(cc & 1u);
+          const int qy = 2 * sy + static_cast<int>((cc >> 1) & 1u);
+          const int qz = 2 * sz + static_cast<int>(cc >> 2);
+          const bool near = abs(qx - static_cast<int>(tx)) <= 1 && abs(qy - static_cast<int>(ty)) <= 1 &&
+                            abs(qz - static_cast<int>(tz)) <= 1;
+          if (tcount != 0 && !near) {
+            const float4 sr = src[lin(static_cast<uint32_t>(qx), static_cast<uint32_t>(qy),
+                                      static_cast<uint32_t>(qz), dim)];
+            if (sr.w != 0.0f) pair_terms(sr, tcx, tcy, tcz, eps2_m2l, t);
+          }
+#pragma unroll
+          for (int k = 0; k < 10; ++k) terms[k][lane] = t[k];
+          __syncthreads();
+#pragma unroll
+          for (uint32_t j = 0; j < 8u; ++j) ch0.add(terms[ck0][ct0 * 8u + j]);
+          if (two) {
+#pragma unroll
+            for (uint32_t j = 0; j < 8u; ++j) ch1.add(terms[ck1][ct1 * 8u + j]);
+          }
+          __syncthreads();
+        }
+      }
+    }
+
+    // L2L (downward, src/rainbow_fmm.wgsl:598-610): each chain adds its
+    // component of the parent's field shifted to its target's centre.
+    float4 pa, pj0, pj1;
+    load_field(d.fields, poff + pidx, pa, pj0, pj1);
+    const float pcx = center1(px, pdim), pcy = center1(py, pdim), pcz = center1(pz, pdim);
+    auto inherit = [&](uint32_t t_, uint32_t k_) -> float {
+      const float dx = center1(2u * px + (t_ & 1u), dim) - pcx;
+      const float dy = center1(2u * py + ((t_ >> 1) & 1u), dim) - pcy;
+      const float dz = center1(2u * pz + (t_ >> 2), dim) - pcz;
+      switch (k_) {
+        case 0: return pa.x + ((pj0.x * dx + pj0.y * dy) + pj0.z * dz);
+        case 1: return pa.y + ((pj0.y * dx + pj0.w * dy) + pj1.x * dz);
+        case 2: return pa.z + ((pj0.z * dx + pj1.x * dy) + pj1.y * dz);
+        case 3: return pa.w - ((pa.x * dx + pa.y * dy) + pa.z * dz);
+        case 4: return pj0.x;
+        case 5: return pj0.y;
+        case 6: return pj0.z;
+        case 7: return pj0.w;
+        case 8: return pj1.x;
+        default: return pj1.y;
+      }
+    };
+    ch0.reset(ch0.value() + inherit(ct0, ck0));
+    if (two) ch1.reset(ch1.value() + inherit(ct1, ck1));
+
+    // Near field of leaf targets (near_all, src/rainbow_fmm.wgsl:533-571):
+    // the 27 offsets dz, dy, dx in four blocks of 8, the centre skipped.
+    const bool leaf = tcount != 0 && (tcount <= kLeafCapacity || level == d.depth);
+    if (__syncthreads_or(leaf)) {
+      const float near_epsilon =
+          fmaxf(d.softening, (0.72f * kWorldHalfExtent) / static_cast<float>(dim));
+      const float eps2_near = near_epsilon * near_epsilon;
+      for (uint32_t blk = 0; blk < 4u; ++blk) {
+        const uint32_t idx = blk * 8u + cc;
+        float t[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (leaf && idx < 27u && idx != 13u) {
+          const int qx = static_cast<int>(tx) + static_cast<int>(idx % 3u) - 1;
+          const int qy = static_cast<int>(ty) + static_cast<int>((idx / 3u) % 3u) - 1;
+          const int qz = static_cast<int>(tz) + static_cast<int>(idx / 9u) - 1;
+          if (qx >= 0 && qy >= 0 && qz >= 0 && qx < dim_i && qy < dim_i && qz < dim_i) {
+            const float4 sr = src[lin(static_cast<uint32_t>(qx), static_cast<uint32_t>(qy),
+                                      static_cast<uint32_t>(qz), dim)];
+            if (sr.w != 0.0f) pair_terms(sr, tcx, tcy, tcz, eps2_near, t);
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < 10; ++k) terms[k][lane] = t[k];
+        __syncthreads();
+#pragma unroll
+        for (uint32_t j = 0; j < 8u; ++j) ch0.add(terms[ck0][ct0 * 8u + j]);
+        if (two) {
+#pragma unroll
+          for (uint32_t j = 0; j < 8u; ++j) ch1.add(terms[ck1][ct1 * 8u + j]);
+        }
+        __syncthreads();
+      }
+    }
+
+    // Store each chain's component of its (non-empty) target's field.
+    auto store = [&](uint32_t t_, uint32_t k_, float v) {
+      const uint32_t x = 2u * px + (t_ & 1u), y = 2u * py + ((t_ >> 1) & 1u), z = 2u * pz + (t_ >> 2);
+      const uint32_t g = off + lin(x, y, z, dim);
+      if (counts[4u * g] == 0) return;
+      float* f = reinterpret_cast<float*>(d.fields + 3u * g);
+      f[k_] = v;
+      if (k_ == 9u) {
+        f[10] = 0.0f;
+        f[11] = 0.0f;
+      }
+    };
+    store(ct0, ck0, ch0.value());
+    if (two) store(ct1, ck1, ch1.value());
+  }
+}
+
+// ------------------------------------------------------------------------
 // integrate, src/rainbow_fmm.wgsl:613-771
 // ------------------------------------------------------------------------
 
@@ -572,21 +783,26 @@ __device__ __forceinline__ Leaf leaf_field(const SysDesc& d, float px, float py,
   const float ux = unit_coord(px), uy = unit_coord(py), uz = unit_coord(pz);
   const int* counts = reinterpret_cast<const int*>(d.moments);
   Leaf out;
-  uint32_t c[3] = {0u, 0u, 0u};
-  out.dim = 1u << d.depth;
-  out.cell = 0u;
-  for (uint32_t level = 1u; level <= d.depth; ++level) {
-    const uint32_t dim = 1u << level;
-    c[0] = cell1(ux, dim);
-    c[1] = cell1(uy, dim);
-    c[2] = cell1(uz, dim);
-    const uint32_t index = level_off(level) + lin(c[0], c[1], c[2], dim);
-    if (counts[4u * index] <= kLeafCapacity || level == d.depth) {
-      out.dim = dim;
-      out.cell = index;
-      break;
+  // The cell of every level depends only on the position, so all counts are
+  // fetched at once; the leaf is then the first level with count <= 16, or
+  // the finest level.
+  const uint32_t fx = cell1(ux, 1u << d.depth), fy = cell1(uy, 1u << d.depth), fz = cell1(uz, 1u << d.depth);
+  int lc[kLevels];
+#pragma unroll
+  for (uint32_t level = 1u; level < kLevels; ++level) {
+    if (level <= d.depth) {
+      const uint32_t sh = d.depth - level;
+      lc[level] = counts[4u * (level_off(level) + lin(fx >> sh, fy >> sh, fz >> sh, 1u << level))];
     }
   }
+  uint32_t chosen = d.depth;
+#pragma unroll
+  for (uint32_t level = kLevels - 1u; level >= 1u; --level)
+    if (level < d.depth && lc[level] <= kLeafCapacity) chosen = level;
+  const uint32_t sh = d.depth - chosen;
+  const uint32_t c[3] = {fx >> sh, fy >> sh, fz >> sh};
+  out.dim = 1u << chosen;
+  out.cell = level_off(chosen) + lin(c[0], c[1], c[2], out.dim);
   const uint32_t cell = out.cell;
   float4 fa, fj0, fj1;
   load_field(d.fields, cell, fa, fj0, fj1);
@@ -841,6 +1057,16 @@ hipError_t launch_upward(const StepArgs& a, uint32_t nblocks, hipStream_t stream
 
 hipError_t launch_tree(const StepArgs& a, uint32_t level, uint32_t nblocks, hipStream_t stream) {
   if (nblocks == 0u) return hipSuccess;
+  if (a.tree_kernel == kTreeWave) {
+    // One 64-lane block per internal parent of the level above.
+    const uint32_t items = a.list_cap[level - 1u];
+    const uint32_t blocks = items < 1u ? 1u : (items > kWaveTreeMaxBlocks ? kWaveTreeMaxBlocks : items);
+    if (a.compensated)
+      hipLaunchKernelGGL(k_tree_wave<true>, dim3(blocks), dim3(kWaveTreeThreads), 0, stream, a, level);
+    else
+      hipLaunchKernelGGL(k_tree_wave<false>, dim3(blocks), dim3(kWaveTreeThreads), 0, stream, a, level);
+    return hipGetLastError();
+  }
   if (a.compensated)
     hipLaunchKernelGGL(k_tree<CompAcc>, dim3(nblocks), dim3(kTreeThreads), 0, stream, a, level);
   else

@@ -96,6 +96,7 @@ class Engine {
   uint32_t n_integrate = 0;
   uint32_t n_subtree = 0;
   uint32_t tree_blocks[kLevels + 1] = {};
+  uint32_t tree_kernel = kTreeLane;
 
  private:
   int capture(uint32_t steps, hipGraphExec_t* out);

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -188,6 +189,9 @@ void Engine::drop_graphs() {
 
 int Engine::open(uint32_t opts) {
   options = opts;
+  // Tree-pass variant (both give identical bits): RAINBOW_TREE=lane|wave.
+  const char* tk = std::getenv("RAINBOW_TREE");
+  tree_kernel = (tk != nullptr && std::strcmp(tk, "wave") == 0) ? kTreeWave : kTreeLane;
   RB_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   RB_HIP(hipEventCreate(&ev_a));
   RB_HIP(hipEventCreate(&ev_b));
@@ -248,6 +252,7 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
   args.subtree_done = static_cast<uint32_t*>(done_buf.p);
   args.nsys = nsys;
   args.compensated = (options & RAINBOW_OPT_COMPENSATED) ? 1u : 0u;
+  args.tree_kernel = tree_kernel;
   for (uint32_t l = 1; l <= kLevels - 1u; ++l) {
     const uint64_t lanes = 8ull * list_cap[l - 1u];
     const uint64_t blocks = (lanes + kTreeThreads - 1u) / kTreeThreads;

@@ -277,6 +277,38 @@ def test_server_results_equal_solo():
     srv.close()
 
 
+@pytest.mark.parametrize("tree", ["lane", "wave"])
+@pytest.mark.parametrize("compensated", [False, True])
+def test_tree_variants_bit_exact(tree, compensated, monkeypatch):
+    say(f"tree={tree} compensated={compensated}, 1M Plummer solve + 2 steps")
+    monkeypatch.setenv("RAINBOW_TREE", tree)
+    parts = wl.cfg2_particles()
+    c = wl.CFG2
+    slots = slots_for(c["n"], c["dt"], c["gravity"], c["softening"])
+    s, m, f = run_solve(parts, c["dt"], c["gravity"], c["softening"], compensated=compensated)
+    om, of = orc.solve_fields(slots, parts, compensated=compensated)
+    assert_bits(m, om, "moments")
+    assert_bits(f, of, "fields")
+    s.upload(parts)
+    s.step(2)
+    op, _, _ = orc.step(slots, parts, 2, compensated=compensated)
+    assert_bits(s.particles(), op, "particles")
+    s.close()
+    # and a small uneven batch through the same variant
+    counts = [4096, 1, 70000]
+    ins = [rf.ic_plummer(n, 60 + i, 0.1, 1.5, 1.0) for i, n in enumerate(counts)]
+    b = rf.Batch(counts, rf.physics(dt=c["dt"], gravity=c["gravity"], softening=c["softening"]),
+                 compensated=compensated)
+    for i, p in enumerate(ins):
+        b.upload(i, p)
+    b.step(3)
+    for i, n in enumerate(counts):
+        op, _, of = orc.step(slots_for(n, c["dt"], c["gravity"], c["softening"]), ins[i], 3, compensated=compensated)
+        assert_bits(b.download(i, rf.PARTICLES), op, f"batch system {i}")
+        assert_bits(b.download(i, rf.FIELDS), of, f"batch system {i} fields")
+    b.close()
+
+
 def test_determinism_two_runs():
     say("determinism")
     n = 1 << 20
