@@ -29,7 +29,7 @@ FP32_PEAK_TFLOPS = 157.3
 
 
 def phase_names(depth):
-    return ["clear", "deposit", "upward"] + [f"tree_l{l}" for l in range(1, depth + 1)] + ["integrate"]
+    return ["clear", "deposit", "upward", "top"] + [f"tree_l{l}" for l in range(1, depth + 1)] + ["integrate"]
 
 
 def algorithmic_bytes(n, depth, stats):
@@ -37,7 +37,9 @@ def algorithmic_bytes(n, depth, stats):
 
     clear      16 C                      zero the moments
     deposit    16 N + 16 C_d             read positions, write finest moments
-    upward     16 (C - C_d) + 16 (C - 1) read children, write parents
+    upward     read the finest cells and the levels below depth-3 once,
+               write levels depth-1..depth-3
+    top        read level depth-3, write the levels above
     tree(l)    16 T_l + 48 T_l + 48 P_l: read the level's target moments,
                write their fields, read their parents' fields (the 6^3
                source windows are served from cache)
@@ -48,7 +50,10 @@ def algorithmic_bytes(n, depth, stats):
     cells = (8 ** (depth + 1) - 1) // 7
     cd = 8 ** depth
     leaves = stats[7]
-    out = {"clear": 16 * cells, "deposit": 16 * n + 16 * cd, "upward": 16 * (cells - cd) + 16 * (cells - 1)}
+    top = (8 ** (depth - 2) - 1) // 7  # cells of levels 0..depth-3
+    out = {"clear": 16 * cells, "deposit": 16 * n + 16 * cd,
+           "upward": 16 * (cells - top) + 16 * (cells - cd - top + 8 ** (depth - 3)),
+           "top": 16 * (8 ** (depth - 3)) + 16 * (top - 8 ** (depth - 3))}
     for l in range(1, depth + 1):
         t = stats[l]
         # P_l >= ceil(T_l / 8): each parent has at most 8 targets.

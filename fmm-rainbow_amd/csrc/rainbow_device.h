@@ -64,6 +64,10 @@ struct alignas(8) CellBlock {
 
 constexpr uint32_t kLevels = 7;  // levels 0..6
 
+// Internal cells are listed per level in kStripes stripes; a block appends
+// to stripe (block index mod kStripes) with one atomic per level.
+constexpr uint32_t kStripes = 32;
+
 // Everything a kernel of the step needs, passed by value.
 struct StepArgs {
   const SysDesc* sys;
@@ -71,20 +75,13 @@ struct StepArgs {
   const BodyBlock* deposit_blocks;
   const BodyBlock* integrate_blocks;
   const SubtreeBlock* subtree_blocks;
-  uint32_t* lists;         // internal cells (count > 16), grouped by level
-  uint32_t* list_count;    // kLevels counters
-  uint32_t* subtree_done;  // per-system completion counters of upward
-  uint32_t list_off[kLevels];
-  uint32_t list_cap[kLevels];
+  uint32_t* lists;             // internal cells (count > 16)
+  uint32_t* list_count;        // [kLevels][kStripes] items per stripe
+  const uint32_t* stripe_off;  // [kLevels][kStripes] first slot of a stripe
+  const uint32_t* stripe_cap;  // [kLevels][kStripes] slots of a stripe
   uint32_t nsys;
   uint32_t compensated;
-  uint32_t tree_kernel;  // kTreeLane or kTreeWave
 };
-
-// Tree-pass variants (identical results): one lane per target, or one
-// wave per internal parent.
-constexpr uint32_t kTreeLane = 0;
-constexpr uint32_t kTreeWave = 1;
 
 constexpr uint32_t kClearThreads = 256;
 constexpr uint32_t kClearCells = 4096;
@@ -94,8 +91,8 @@ constexpr uint32_t kDepositBodies = kDepositThreads * kDepositPerThread;  // per
 constexpr uint32_t kDepositSlots = 2048;  // LDS hash slots, 2x bodies per block
 constexpr uint32_t kIntegrateThreads = 256;
 constexpr uint32_t kUpwardThreads = 256;
-constexpr uint32_t kTreeThreads = 256;
-constexpr uint32_t kTreeMaxBlocks = 4096;
+constexpr uint32_t kTreeThreads = 64;  // one wave per internal parent
+constexpr uint32_t kTreeMaxBlocks = 16384;
 
 // List item: system index in the high 16 bits, cell index within its level
 // in the low 16 bits (levels <= 5 hold at most 32768 cells).

@@ -1,20 +1,20 @@
 // gfx950 kernels of the p=1 Cartesian uniform-octree FMM step.
 //
-// One step is five passes over a batch of systems (a plain solve is a batch
+// One step is six passes over a batch of systems (a plain solve is a batch
 // of one), launched in order on one stream and captured in a HIP graph:
 //
-//   clear      zero moments and the per-level work-list counters
+//   clear      zero moments and the work-list counters
 //   deposit    P2M: finest-cell binning and fixed-point moment sums,
 //              pre-aggregated per block in an LDS hash table, then global
 //              integer atomics (exact in any order)
 //   upward     M2M: one block per 8x8x8 finest-cell subtree sums three levels
-//              in LDS; the last block of a system finishes the levels above.
-//              It also writes each cell's source record (centre of mass and
-//              G m), emits the per-level lists of internal cells (count > 16)
-//              and zeroes the fields of cells that stopped being M2L targets
+//              in LDS, writes each cell's source record (centre of mass and
+//              G m), lists the internal cells (count > 16) and zeroes the
+//              fields of cells that stopped being M2L targets
+//   top        the levels above the subtrees, one block per system
 //   tree(l)    for l = 1..depth: M2L, L2L and the near field of every M2L
 //              target of level l (a non-empty child of an internal cell),
-//              eight lanes per internal parent
+//              one 64-lane block per internal parent
 //   integrate  adaptive descent, local expansion, own-leaf monopole,
 //              symplectic Euler kick-drift and the soft absorbing boundary
 //
@@ -85,71 +85,6 @@ __device__ __forceinline__ float clampf(float x, float lo, float hi) {
   return fminf(fmaxf(x, lo), hi);
 }
 
-// Field accumulator: a.xyz, phi, Jxx, Jxy, Jxz, Jyy, Jyz, Jzz.  The last two
-// lanes of the reference Field (j1.zw) are always zero.
-struct Acc {
-  float v[10];
-  __device__ __forceinline__ void zero() {
-#pragma unroll
-    for (int k = 0; k < 10; ++k) v[k] = 0.0f;
-  }
-  __device__ __forceinline__ void add(int k, float t) { v[k] = v[k] + t; }
-  __device__ __forceinline__ float get(int k) const { return v[k]; }
-  __device__ __forceinline__ void set(int k, float x) { v[k] = x; }
-};
-
-// Neumaier-compensated accumulator (the precision form).
-struct CompAcc {
-  float s[10];
-  float c[10];
-  __device__ __forceinline__ void zero() {
-#pragma unroll
-    for (int k = 0; k < 10; ++k) s[k] = c[k] = 0.0f;
-  }
-  __device__ __forceinline__ void add(int k, float t) {
-    const float u = s[k] + t;
-    if (fabsf(s[k]) >= fabsf(t))
-      c[k] = c[k] + ((s[k] - u) + t);
-    else
-      c[k] = c[k] + ((t - u) + s[k]);
-    s[k] = u;
-  }
-  __device__ __forceinline__ float get(int k) const { return s[k] + c[k]; }
-  __device__ __forceinline__ void set(int k, float x) {
-    s[k] = x;
-    c[k] = 0.0f;
-  }
-};
-
-// pair_field + add_field, src/rainbow_fmm.wgsl:170-203 and 108-110, for a
-// source record {com.xyz, G m} (gm != 0).  com = f32(sum) * (1 / (scale
-// f32(count))) and gm = G (f32(count) / f32(N)) are formed once per cell by
-// the upward pass with the same operations the reference repeats per pair.
-template <class A>
-__device__ __forceinline__ void pair_add(const float4 src, float tx, float ty, float tz, float eps2,
-                                         A& f) {
-  const float rx = src.x - tx;
-  const float ry = src.y - ty;
-  const float rz = src.z - tz;
-  const float r2 = ((rx * rx + ry * ry) + rz * rz) + eps2;
-  const float inv_r = 1.0f / sqrtf(r2);
-  const float inv_r3 = inv_r * inv_r * inv_r;
-  const float inv_r5 = inv_r3 * inv_r * inv_r;
-  const float gm = src.w;
-  const float diagonal = -gm * inv_r3;
-  const float tidal = 3.0f * gm * inv_r5;
-  f.add(0, gm * rx * inv_r3);
-  f.add(1, gm * ry * inv_r3);
-  f.add(2, gm * rz * inv_r3);
-  f.add(3, -gm * inv_r);
-  f.add(4, diagonal + tidal * rx * rx);
-  f.add(5, tidal * rx * ry);
-  f.add(6, tidal * rx * rz);
-  f.add(7, diagonal + tidal * ry * ry);
-  f.add(8, tidal * ry * rz);
-  f.add(9, diagonal + tidal * rz * rz);
-}
-
 __device__ __forceinline__ void load_field(const float4* fields, uint32_t cell, float4& a, float4& j0,
                                            float4& j1) {
   a = fields[3u * cell + 0u];
@@ -162,7 +97,8 @@ __device__ __forceinline__ void load_field(const float4* fields, uint32_t cell, 
 // ------------------------------------------------------------------------
 
 __global__ __launch_bounds__(kClearThreads) void k_clear(StepArgs a) {
-  if (blockIdx.x == 0 && threadIdx.x < kLevels) a.list_count[threadIdx.x] = 0u;
+  if (blockIdx.x == 0)
+    for (uint32_t i = threadIdx.x; i < kLevels * kStripes; i += kClearThreads) a.list_count[i] = 0u;
   const CellBlock blk = a.clear_blocks[blockIdx.x];
   const SysDesc d = a.sys[blk.sys];
   if (!d.live) return;
@@ -283,19 +219,38 @@ __device__ __forceinline__ void update_active(const SysDesc& d, uint32_t cell, i
   }
 }
 
-__device__ __forceinline__ void push_internal(const StepArgs& a, uint32_t level, uint32_t sys,
-                                              uint32_t local, int count) {
-  if (count > kLeafCapacity) {
-    const uint32_t slot = atomicAdd(&a.list_count[level], 1u);
-    if (slot < a.list_cap[level]) a.lists[a.list_off[level] + slot] = pack_item(sys, local);
-  }
+// Internal cells (count > 16) are collected per level into kStripes striped
+// lists; a block reserves its items of one level with a single atomic on
+// the counter of its stripe, so no counter sees more than a few atomics.
+struct PushStage {
+  uint32_t n[4];
+  uint32_t base[4];
+};
+
+__device__ __forceinline__ uint32_t stage_item(PushStage& ps, uint32_t k) { return atomicAdd(&ps.n[k], 1u); }
+
+// Thread 0: reserve each staged level's items in stripe `stripe`.
+__device__ __forceinline__ void reserve_items(const StepArgs& a, PushStage& ps, const uint32_t levels[4],
+                                              uint32_t nlev, uint32_t stripe) {
+  for (uint32_t k = 0; k < nlev; ++k)
+    ps.base[k] = ps.n[k] ? atomicAdd(&a.list_count[levels[k] * kStripes + stripe], ps.n[k]) : 0u;
 }
 
+__device__ __forceinline__ void write_item(const StepArgs& a, uint32_t level, uint32_t stripe, uint32_t slot,
+                                           uint32_t item) {
+  const uint32_t si = level * kStripes + stripe;
+  if (slot < a.stripe_cap[si]) a.lists[a.stripe_off[si] + slot] = item;
+}
+
+// upward (M2M), src/rainbow_fmm.wgsl:424-455: one block per 8x8x8 finest
+// cells of a subtree rooted at level r = depth-3 sums levels depth-1..r in
+// LDS, writes them with their source records, updates the target flags of
+// the levels whose parents it holds, and lists the internal cells.
 __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
   // Levels of the subtree in one LDS array: 8^3 finest cells at 0, then
   // 64, 8 and 1 cells of the three levels above.
   __shared__ int4 lv[512 + 64 + 8 + 1];
-  __shared__ uint32_t is_last;
+  __shared__ PushStage ps;
   int4* const lvl0 = lv;
   int4* const lvl1 = lv + 512;
   int4* const lvl2 = lv + 576;
@@ -312,6 +267,7 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
   const uint32_t ry = rrest / dim_r;
   const uint32_t rx = rrest - ry * dim_r;
   int4* M = d.moments;
+  if (t < 4u) ps.n[t] = 0u;
 
   // Finest level: 8x8x8 cells, x fastest (rows of 8 contiguous moments).
   for (uint32_t i = t; i < 512u; i += kUpwardThreads) {
@@ -340,7 +296,6 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
   }
   __syncthreads();
 
-  // Source records, target flags and internal-cell lists of the subtree.
   for (uint32_t i = t; i < 512u; i += kUpwardThreads) {
     const uint32_t ix = i & 7u, iy = (i >> 3) & 7u, iz = i >> 6;
     const uint32_t cell = level_off(D) + lin(8u * rx + ix, 8u * ry + iy, 8u * rz + iz, 1u << D);
@@ -348,6 +303,8 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
     d.source[cell] = source_record(m, d);
     update_active(d, cell, m.x, lvl1[lin(ix >> 1, iy >> 1, iz >> 1, 4u)].x);
   }
+  uint32_t my_item = 0u, my_slot = 0u, my_level = 0u;
+  bool pushes = false;
   if (t < 64u) {
     const uint32_t cx = t & 3u, cy = (t >> 2) & 3u, cz = t >> 4;
     const uint32_t local = lin(4u * rx + cx, 4u * ry + cy, 4u * rz + cz, 1u << (D - 1u));
@@ -355,212 +312,139 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
     const int4 m = lvl1[t];
     d.source[cell] = source_record(m, d);
     update_active(d, cell, m.x, lvl2[lin(cx >> 1, cy >> 1, cz >> 1, 2u)].x);
-    push_internal(a, D - 1u, blk.sys, local, m.x);
+    if (m.x > kLeafCapacity) {
+      pushes = true;
+      my_level = 0u;
+      my_item = pack_item(blk.sys, local);
+      my_slot = stage_item(ps, 0u);
+    }
   }
-  if (t < 8u) {
-    const uint32_t cx = t & 1u, cy = (t >> 1) & 1u, cz = t >> 2;
+  uint32_t my_item2 = 0u, my_slot2 = 0u, my_level2 = 0u;
+  bool pushes2 = false;
+  if (t >= 64u && t < 72u) {
+    const uint32_t u = t - 64u;
+    const uint32_t cx = u & 1u, cy = (u >> 1) & 1u, cz = u >> 2;
     const uint32_t local = lin(2u * rx + cx, 2u * ry + cy, 2u * rz + cz, 1u << (D - 2u));
     const uint32_t cell = level_off(D - 2u) + local;
-    const int4 m = lvl2[t];
+    const int4 m = lvl2[u];
     d.source[cell] = source_record(m, d);
     update_active(d, cell, m.x, lvl3[0].x);
-    push_internal(a, D - 2u, blk.sys, local, m.x);
+    if (m.x > kLeafCapacity) {
+      pushes2 = true;
+      my_level2 = 1u;
+      my_item2 = pack_item(blk.sys, local);
+      my_slot2 = stage_item(ps, 1u);
+    }
   }
-  if (t == 0u) {
+  if (t == 72u) {
     const int4 m = lvl3[0];
     d.source[level_off(r) + blk.root] = source_record(m, d);
-    push_internal(a, r, blk.sys, blk.root, m.x);
+    if (m.x > kLeafCapacity) {
+      pushes2 = true;
+      my_level2 = 2u;
+      my_item2 = pack_item(blk.sys, blk.root);
+      my_slot2 = stage_item(ps, 2u);
+    }
   }
-
-  // The last subtree block of this system to finish sums levels r-1..0.
-  __threadfence();
   __syncthreads();
+  const uint32_t stripe = blockIdx.x % kStripes;
   if (t == 0u) {
-    const uint32_t nsub = 1u << (3u * r);
-    const uint32_t prev = atomicAdd(&a.subtree_done[blk.sys], 1u);
-    is_last = (prev == nsub - 1u) ? 1u : 0u;
+    const uint32_t levels[4] = {D - 1u, D - 2u, r, 0u};
+    reserve_items(a, ps, levels, 3u, stripe);
   }
   __syncthreads();
-  if (!is_last) return;
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  const uint32_t lev_of[3] = {D - 1u, D - 2u, r};
+  if (pushes) write_item(a, lev_of[my_level], stripe, ps.base[my_level] + my_slot, my_item);
+  if (pushes2) write_item(a, lev_of[my_level2], stripe, ps.base[my_level2] + my_slot2, my_item2);
+}
 
-  // Level r (8^r <= 512 cells) into lvl0, then levels r-1, r-2, ... into
-  // lvl1, lvl2, lvl3.
-  auto bufs = [&](uint32_t j) -> int4* { return lv + (j == 0u ? 0u : (j == 1u ? 512u : (j == 2u ? 576u : 584u))); };
+// The levels above the subtrees, one block per system: levels r-1..0 from
+// the 8^r subtree roots, their source records and lists, and the target
+// flags of levels 1..r.
+__global__ __launch_bounds__(kUpwardThreads) void k_top(StepArgs a) {
+  __shared__ int4 lv[512 + 64 + 8 + 1];
+  __shared__ PushStage ps;
+  const uint32_t s = blockIdx.x;
+  const SysDesc d = a.sys[s];
+  if (!d.live) return;
+  const uint32_t t = threadIdx.x;
+  const uint32_t r = d.depth - 3u;
+  int4* M = d.moments;
+  auto buf = [&](uint32_t j) -> int4* {
+    return lv + (j == 0u ? 0u : (j == 1u ? 512u : (j == 2u ? 576u : 584u)));
+  };
+  if (t < 4u) ps.n[t] = 0u;
+  // Level r (8^r <= 512 cells) into buf(0), then levels r-1, r-2, ... into
+  // buf(1), buf(2), buf(3).
   const uint32_t nr = 1u << (3u * r);
-  for (uint32_t i = t; i < nr; i += kUpwardThreads) lvl0[i] = M[level_off(r) + i];
+  for (uint32_t i = t; i < nr; i += kUpwardThreads) buf(0)[i] = M[level_off(r) + i];
   __syncthreads();
   for (uint32_t j = 1u; j <= r; ++j) {
-    const uint32_t k = r - j;  // level being formed
+    const uint32_t k = r - j;
     const uint32_t dk = 1u << k;
     if (t < dk * dk * dk) {
       const uint32_t cz = t / (dk * dk);
       const uint32_t rest = t - cz * dk * dk;
       const uint32_t cy = rest / dk;
       const uint32_t cx = rest - cy * dk;
-      const int4 s = sum8(bufs(j - 1u), cx, cy, cz, 2u * dk);
-      bufs(j)[t] = s;
-      M[level_off(k) + t] = s;
+      const int4 v = sum8(buf(j - 1u), cx, cy, cz, 2u * dk);
+      buf(j)[t] = v;
+      M[level_off(k) + t] = v;
     }
     __syncthreads();
   }
-  // Levels 0..r-1: source records and internal lists; levels 1..r: flags.
+  // Levels 0..r-1: source records and lists; levels 1..r: flags.
+  uint32_t items[4], slots[4], levs[4];
+  uint32_t nmine = 0u;
   for (uint32_t j = 0u; j <= r; ++j) {
     const uint32_t k = r - j;
     const uint32_t dk = 1u << k;
     const uint32_t ncell = dk * dk * dk;
     for (uint32_t i = t; i < ncell; i += kUpwardThreads) {
-      const int4 m = bufs(j)[i];
+      const int4 m = buf(j)[i];
       const uint32_t cell = level_off(k) + i;
       if (k < r) {
         d.source[cell] = source_record(m, d);
-        push_internal(a, k, blk.sys, i, m.x);
+        if (m.x > kLeafCapacity && nmine < 4u) {
+          items[nmine] = pack_item(s, i);
+          levs[nmine] = k;
+          slots[nmine] = stage_item(ps, k);
+          ++nmine;
+        }
       }
       if (k >= 1u) {
         const uint32_t cz = i / (dk * dk);
         const uint32_t rest = i - cz * dk * dk;
         const uint32_t cy = rest / dk;
         const uint32_t cx = rest - cy * dk;
-        const uint32_t pd = dk >> 1;
-        update_active(d, cell, m.x, bufs(j + 1u)[lin(cx >> 1, cy >> 1, cz >> 1, pd)].x);
+        update_active(d, cell, m.x, buf(j + 1u)[lin(cx >> 1, cy >> 1, cz >> 1, dk >> 1)].x);
       }
     }
   }
-  if (t == 0u) a.subtree_done[blk.sys] = 0u;
+  __syncthreads();
+  const uint32_t stripe = s % kStripes;
+  if (t == 0u) {
+    const uint32_t levels[4] = {0u, 1u, 2u, 3u};
+    reserve_items(a, ps, levels, r, stripe);
+  }
+  __syncthreads();
+  for (uint32_t q = 0; q < nmine; ++q) write_item(a, levs[q], stripe, ps.base[levs[q]] + slots[q], items[q]);
 }
 
 // ------------------------------------------------------------------------
 // tree(level): M2L (src/rainbow_fmm.wgsl:457-519), L2L (:585-611) and near
-// (:521-572) of the level's targets
-// ------------------------------------------------------------------------
-
-template <class A>
-__global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t level) {
-  const uint32_t nitems = min(a.list_count[level - 1u], a.list_cap[level - 1u]);
-  const uint32_t total = nitems * 8u;
-  const uint32_t* items = a.lists + a.list_off[level - 1u];
-  const uint32_t stride = gridDim.x * blockDim.x;
-  const uint32_t dim = 1u << level;
-  const uint32_t pdim = dim >> 1u;
-  const int dim_i = static_cast<int>(dim);
-  const int pdim_i = static_cast<int>(pdim);
-  const uint32_t off = level_off(level);
-  const uint32_t poff = level_off(level - 1u);
-  // cell_epsilon, src/rainbow_fmm.wgsl:507 (0.06 * WORLD_HALF_EXTENT / dim).
-  const float cell_epsilon = (0.06f * kWorldHalfExtent) / static_cast<float>(dim);
-  const float eps2_m2l = cell_epsilon * cell_epsilon;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
-    const uint32_t item = items[i >> 3];
-    const uint32_t child = i & 7u;
-    const uint32_t s = item >> 16;
-    const uint32_t pidx = item & 0xFFFFu;
-    if (s >= a.nsys) continue;
-    const SysDesc d = a.sys[s];
-    const uint32_t pz = pidx / (pdim * pdim);
-    const uint32_t prest = pidx - pz * pdim * pdim;
-    const uint32_t py = prest / pdim;
-    const uint32_t px = prest - py * pdim;
-    const uint32_t tx = 2u * px + (child & 1u);
-    const uint32_t ty = 2u * py + ((child >> 1) & 1u);
-    const uint32_t tz = 2u * pz + (child >> 2);
-    const uint32_t g = off + lin(tx, ty, tz, dim);
-    const int count = d.moments[g].x;
-    if (count == 0) continue;
-    const float4* src = d.source + off;
-    const float cx = center1(tx, dim), cy = center1(ty, dim), cz = center1(tz, dim);
-
-    A f;
-    f.zero();
-    // M2L: children of the parent's 3x3x3 neighbourhood that are not in the
-    // target's own 3x3x3 neighbourhood.
-    for (int oz = -1; oz <= 1; ++oz) {
-      const int sz = static_cast<int>(pz) + oz;
-      if (sz < 0 || sz >= pdim_i) continue;
-      for (int oy = -1; oy <= 1; ++oy) {
-        const int sy = static_cast<int>(py) + oy;
-        if (sy < 0 || sy >= pdim_i) continue;
-        for (int ox = -1; ox <= 1; ++ox) {
-          const int sx = static_cast<int>(px) + ox;
-          if (sx < 0 || sx >= pdim_i) continue;
-#pragma unroll
-          for (int cz2 = 0; cz2 < 2; ++cz2)
-#pragma unroll
-            for (int cy2 = 0; cy2 < 2; ++cy2)
-#pragma unroll
-              for (int cx2 = 0; cx2 < 2; ++cx2) {
-                const int qx = 2 * sx + cx2, qy = 2 * sy + cy2, qz = 2 * sz + cz2;
-                if (abs(qx - static_cast<int>(tx)) <= 1 && abs(qy - static_cast<int>(ty)) <= 1 &&
-                    abs(qz - static_cast<int>(tz)) <= 1)
-                  continue;
-                const float4 sr = src[lin(static_cast<uint32_t>(qx), static_cast<uint32_t>(qy),
-                                          static_cast<uint32_t>(qz), dim)];
-                if (sr.w != 0.0f) pair_add(sr, cx, cy, cz, eps2_m2l, f);
-              }
-        }
-      }
-    }
-
-    // L2L: add the parent's field shifted to this cell's centre
-    // (downward, src/rainbow_fmm.wgsl:598-610).
-    float4 pa, pj0, pj1;
-    load_field(d.fields, poff + pidx, pa, pj0, pj1);
-    {
-      const float dx = cx - center1(px, pdim);
-      const float dy = cy - center1(py, pdim);
-      const float dz = cz - center1(pz, pdim);
-      const float jx = (pj0.x * dx + pj0.y * dy) + pj0.z * dz;
-      const float jy = (pj0.y * dx + pj0.w * dy) + pj1.x * dz;
-      const float jz = (pj0.z * dx + pj1.x * dy) + pj1.y * dz;
-      const float ia[4] = {pa.x + jx, pa.y + jy, pa.z + jz,
-                           pa.w - ((pa.x * dx + pa.y * dy) + pa.z * dz)};
-      const float ij[6] = {pj0.x, pj0.y, pj0.z, pj0.w, pj1.x, pj1.y};
-#pragma unroll
-      for (int k = 0; k < 4; ++k) f.set(k, f.get(k) + ia[k]);
-#pragma unroll
-      for (int k = 0; k < 6; ++k) f.set(4 + k, f.get(4 + k) + ij[k]);
-    }
-
-    // Near field of a leaf: the 26 same-level neighbour monopoles at the
-    // cell centre (near_all, src/rainbow_fmm.wgsl:533-571).
-    if (count <= kLeafCapacity || level == d.depth) {
-      const float near_epsilon =
-          fmaxf(d.softening, (0.72f * kWorldHalfExtent) / static_cast<float>(dim));
-      const float eps2_near = near_epsilon * near_epsilon;
-      for (int oz = -1; oz <= 1; ++oz) {
-        const int qz = static_cast<int>(tz) + oz;
-        if (qz < 0 || qz >= dim_i) continue;
-        for (int oy = -1; oy <= 1; ++oy) {
-          const int qy = static_cast<int>(ty) + oy;
-          if (qy < 0 || qy >= dim_i) continue;
-          for (int ox = -1; ox <= 1; ++ox) {
-            if (ox == 0 && oy == 0 && oz == 0) continue;
-            const int qx = static_cast<int>(tx) + ox;
-            if (qx < 0 || qx >= dim_i) continue;
-            const float4 sr = src[lin(static_cast<uint32_t>(qx), static_cast<uint32_t>(qy),
-                                      static_cast<uint32_t>(qz), dim)];
-            if (sr.w != 0.0f) pair_add(sr, cx, cy, cz, eps2_near, f);
-          }
-        }
-      }
-    }
-
-    float4* out = d.fields + 3u * g;
-    out[0] = make_float4(f.get(0), f.get(1), f.get(2), f.get(3));
-    out[1] = make_float4(f.get(4), f.get(5), f.get(6), f.get(7));
-    out[2] = make_float4(f.get(8), f.get(9), 0.0f, 0.0f);
-  }
-}
-
-// ------------------------------------------------------------------------
-// tree(level), one wave per internal parent.  The 8 children are the
-// targets; the M2L window is walked one source-parent block (8 sources) at
-// a time: the 64 lanes evaluate the 8 x 8 (target, source) pair fields in
-// parallel into LDS, then 80 lanes-worth of chains -- one per (target,
-// component) -- add the 8 terms in the reference order.  Same sums, same
-// order, same bits as the lane-per-target kernel, with 8x the parallelism.
+// (:521-572) of the targets of one level, one 64-lane block per internal
+// parent.  The parent's 6x6x6 source window is staged in LDS with one
+// round of loads; the window is walked one source-parent block (8 sources)
+// at a time: the 64 lanes evaluate the 8 x 8 (target, source) pair fields
+// in parallel, then one lane per (target, component) chain adds the 8
+// terms in the reference order.
 // ------------------------------------------------------------------------
 
 // One pair's ten field terms (pair_field, src/rainbow_fmm.wgsl:170-203).
+// The source record {com, G m} holds com = f32(sum) * (1 / (scale
+// f32(count))) and gm = G (f32(count) / f32(N)), formed once per cell with
+// the same operations the reference repeats per pair.
 __device__ __forceinline__ void pair_terms(const float4 src, float tx, float ty, float tz, float eps2,
                                            float out[10]) {
   const float rx = src.x - tx;
@@ -585,7 +469,7 @@ __device__ __forceinline__ void pair_terms(const float4 src, float tx, float ty,
   out[9] = diagonal + tidal * rz * rz;
 }
 
-// One running sum of a chain (plain or Neumaier-compensated).
+// One running sum: plain f32, or Neumaier-compensated (the precision form).
 template <bool kComp>
 struct Chain {
   float s = 0.0f, c = 0.0f;
@@ -608,34 +492,46 @@ struct Chain {
   }
 };
 
-constexpr uint32_t kWaveTreeThreads = 64;
-constexpr uint32_t kWaveTreeMaxBlocks = 16384;
-
 template <bool kComp>
-__global__ __launch_bounds__(kWaveTreeThreads) void k_tree_wave(StepArgs a, uint32_t level) {
-  __shared__ float terms[10][kWaveTreeThreads];
+__global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t level) {
+  __shared__ float4 win[216];
+  __shared__ float terms[2][10][kTreeThreads];
   const uint32_t lane = threadIdx.x;
-  const uint32_t nitems = min(a.list_count[level - 1u], a.list_cap[level - 1u]);
-  const uint32_t* items = a.lists + a.list_off[level - 1u];
+  // Items of this level: stripe counts and their prefix across lanes 0..31.
+  const uint32_t lv = level - 1u;
+  uint32_t cnt = 0u;
+  if (lane < kStripes) cnt = min(a.list_count[lv * kStripes + lane], a.stripe_cap[lv * kStripes + lane]);
+  uint32_t incl = cnt;
+#pragma unroll
+  for (uint32_t o = 1u; o < kStripes; o <<= 1) {
+    const uint32_t v = __shfl_up(incl, o);
+    if (lane >= o) incl += v;
+  }
+  const uint32_t total = __shfl(incl, kStripes - 1u);
+  const uint32_t excl = incl - cnt;
+
   const uint32_t dim = 1u << level;
   const uint32_t pdim = dim >> 1u;
-  const int dim_i = static_cast<int>(dim);
   const int pdim_i = static_cast<int>(pdim);
+  const int dim_i = static_cast<int>(dim);
   const uint32_t off = level_off(level);
   const uint32_t poff = level_off(level - 1u);
+  // cell_epsilon, src/rainbow_fmm.wgsl:507 (0.06 * WORLD_HALF_EXTENT / dim).
   const float cell_epsilon = (0.06f * kWorldHalfExtent) / static_cast<float>(dim);
   const float eps2_m2l = cell_epsilon * cell_epsilon;
-  // Pair lane: target tt (child of the parent), source cc (child of the
-  // source-parent block, or near-offset slot).
+  // Pair lane: target tt (child of the parent), source cc.
   const uint32_t tt = lane >> 3, cc = lane & 7u;
-  // Chain lanes: chain q = target q / 10, component q % 10; lanes < 16 own
-  // a second chain q + 64.
+  const uint32_t wtx = 2u + (tt & 1u), wty = 2u + ((tt >> 1) & 1u), wtz = 2u + (tt >> 2);
+  // Chain lanes: chain q = target q / 10, component q % 10; lanes < 16 own a
+  // second chain q + 64.
   const bool two = lane < 16u;
   const uint32_t ct0 = lane / 10u, ck0 = lane % 10u;
   const uint32_t ct1 = (lane + 64u) / 10u, ck1 = (lane + 64u) % 10u;
 
-  for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
-    const uint32_t item = items[it];
+  for (uint32_t it = blockIdx.x; it < total; it += gridDim.x) {
+    const uint32_t j = static_cast<uint32_t>(__popcll(__ballot(lane < kStripes && incl <= it)));
+    const uint32_t base = __shfl(excl, j);
+    const uint32_t item = a.lists[a.stripe_off[lv * kStripes + j] + (it - base)];
     const uint32_t s = item >> 16;
     if (s >= a.nsys) continue;
     const uint32_t pidx = item & 0xFFFFu;
@@ -644,53 +540,61 @@ __global__ __launch_bounds__(kWaveTreeThreads) void k_tree_wave(StepArgs a, uint
     const uint32_t prest = pidx - pz * pdim * pdim;
     const uint32_t py = prest / pdim;
     const uint32_t px = prest - py * pdim;
-    const float4* src = d.source + off;
     const int* counts = reinterpret_cast<const int*>(d.moments);
 
-    // This lane's pair target.
+    // Stage the window of cells 2P-2 .. 2P+3 per axis; cells outside the
+    // level read as empty.
+    const float4* src = d.source + off;
+    for (uint32_t w = lane; w < 216u; w += kTreeThreads) {
+      const uint32_t wz = w / 36u, wy = (w / 6u) % 6u, wx = w % 6u;
+      const int qx = 2 * static_cast<int>(px) - 2 + static_cast<int>(wx);
+      const int qy = 2 * static_cast<int>(py) - 2 + static_cast<int>(wy);
+      const int qz = 2 * static_cast<int>(pz) - 2 + static_cast<int>(wz);
+      float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (qx >= 0 && qy >= 0 && qz >= 0 && qx < dim_i && qy < dim_i && qz < dim_i)
+        v = src[lin(static_cast<uint32_t>(qx), static_cast<uint32_t>(qy), static_cast<uint32_t>(qz), dim)];
+      win[w] = v;
+    }
+    float4 pa, pj0, pj1;
+    load_field(d.fields, poff + pidx, pa, pj0, pj1);
     const uint32_t tx = 2u * px + (tt & 1u), ty = 2u * py + ((tt >> 1) & 1u), tz = 2u * pz + (tt >> 2);
     const int tcount = counts[4u * (off + lin(tx, ty, tz, dim))];
     const float tcx = center1(tx, dim), tcy = center1(ty, dim), tcz = center1(tz, dim);
+    __syncthreads();
 
     Chain<kComp> ch0, ch1;
+    uint32_t b = 0u;
     for (int oz = -1; oz <= 1; ++oz) {
-      const int sz = static_cast<int>(pz) + oz;
-      if (sz < 0 || sz >= pdim_i) continue;
+      if (static_cast<int>(pz) + oz < 0 || static_cast<int>(pz) + oz >= pdim_i) continue;
       for (int oy = -1; oy <= 1; ++oy) {
-        const int sy = static_cast<int>(py) + oy;
-        if (sy < 0 || sy >= pdim_i) continue;
+        if (static_cast<int>(py) + oy < 0 || static_cast<int>(py) + oy >= pdim_i) continue;
         for (int ox = -1; ox <= 1; ++ox) {
-          const int sx = static_cast<int>(px) + ox;
-          if (sx < 0 || sx >= pdim_i) continue;
+          if (static_cast<int>(px) + ox < 0 || static_cast<int>(px) + ox >= pdim_i) continue;
+          const uint32_t wsx = static_cast<uint32_t>(2 * (ox + 1)) + (cc & 1u);
+          const uint32_t wsy = static_cast<uint32_t>(2 * (oy + 1)) + ((cc >> 1) & 1u);
+          const uint32_t wsz = static_cast<uint32_t>(2 * (oz + 1)) + (cc >> 2);
+          const bool near = (wsx + 1u - wtx) <= 2u && (wsy + 1u - wty) <= 2u && (wsz + 1u - wtz) <= 2u;
           float t[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-          const int qx = 2 * sx + static_cast<int>(cc & 1u);
-          const int qy = 2 * sy + static_cast<int>((cc >> 1) & 1u);
-          const int qz = 2 * sz + static_cast<int>(cc >> 2);
-          const bool near = abs(qx - static_cast<int>(tx)) <= 1 && abs(qy - static_cast<int>(ty)) <= 1 &&
-                            abs(qz - static_cast<int>(tz)) <= 1;
           if (tcount != 0 && !near) {
-            const float4 sr = src[lin(static_cast<uint32_t>(qx), static_cast<uint32_t>(qy),
-                                      static_cast<uint32_t>(qz), dim)];
+            const float4 sr = win[wsx + 6u * (wsy + 6u * wsz)];
             if (sr.w != 0.0f) pair_terms(sr, tcx, tcy, tcz, eps2_m2l, t);
           }
 #pragma unroll
-          for (int k = 0; k < 10; ++k) terms[k][lane] = t[k];
+          for (int k = 0; k < 10; ++k) terms[b][k][lane] = t[k];
           __syncthreads();
 #pragma unroll
-          for (uint32_t j = 0; j < 8u; ++j) ch0.add(terms[ck0][ct0 * 8u + j]);
+          for (uint32_t q = 0; q < 8u; ++q) ch0.add(terms[b][ck0][ct0 * 8u + q]);
           if (two) {
 #pragma unroll
-            for (uint32_t j = 0; j < 8u; ++j) ch1.add(terms[ck1][ct1 * 8u + j]);
+            for (uint32_t q = 0; q < 8u; ++q) ch1.add(terms[b][ck1][ct1 * 8u + q]);
           }
-          __syncthreads();
+          b ^= 1u;
         }
       }
     }
 
     // L2L (downward, src/rainbow_fmm.wgsl:598-610): each chain adds its
     // component of the parent's field shifted to its target's centre.
-    float4 pa, pj0, pj1;
-    load_field(d.fields, poff + pidx, pa, pj0, pj1);
     const float pcx = center1(px, pdim), pcy = center1(py, pdim), pcz = center1(pz, pdim);
     auto inherit = [&](uint32_t t_, uint32_t k_) -> float {
       const float dx = center1(2u * px + (t_ & 1u), dim) - pcx;
@@ -713,35 +617,33 @@ __global__ __launch_bounds__(kWaveTreeThreads) void k_tree_wave(StepArgs a, uint
     if (two) ch1.reset(ch1.value() + inherit(ct1, ck1));
 
     // Near field of leaf targets (near_all, src/rainbow_fmm.wgsl:533-571):
-    // the 27 offsets dz, dy, dx in four blocks of 8, the centre skipped.
+    // the 27 offsets dz, dy, dx in four blocks of 8, the centre skipped;
+    // neighbours outside the level were staged as empty.
     const bool leaf = tcount != 0 && (tcount <= kLeafCapacity || level == d.depth);
     if (__syncthreads_or(leaf)) {
       const float near_epsilon =
           fmaxf(d.softening, (0.72f * kWorldHalfExtent) / static_cast<float>(dim));
       const float eps2_near = near_epsilon * near_epsilon;
-      for (uint32_t blk = 0; blk < 4u; ++blk) {
-        const uint32_t idx = blk * 8u + cc;
+      for (uint32_t nb = 0; nb < 4u; ++nb) {
+        const uint32_t idx = nb * 8u + cc;
         float t[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         if (leaf && idx < 27u && idx != 13u) {
-          const int qx = static_cast<int>(tx) + static_cast<int>(idx % 3u) - 1;
-          const int qy = static_cast<int>(ty) + static_cast<int>((idx / 3u) % 3u) - 1;
-          const int qz = static_cast<int>(tz) + static_cast<int>(idx / 9u) - 1;
-          if (qx >= 0 && qy >= 0 && qz >= 0 && qx < dim_i && qy < dim_i && qz < dim_i) {
-            const float4 sr = src[lin(static_cast<uint32_t>(qx), static_cast<uint32_t>(qy),
-                                      static_cast<uint32_t>(qz), dim)];
-            if (sr.w != 0.0f) pair_terms(sr, tcx, tcy, tcz, eps2_near, t);
-          }
+          const uint32_t wx = wtx + idx % 3u - 1u;
+          const uint32_t wy = wty + (idx / 3u) % 3u - 1u;
+          const uint32_t wz = wtz + idx / 9u - 1u;
+          const float4 sr = win[wx + 6u * (wy + 6u * wz)];
+          if (sr.w != 0.0f) pair_terms(sr, tcx, tcy, tcz, eps2_near, t);
         }
 #pragma unroll
-        for (int k = 0; k < 10; ++k) terms[k][lane] = t[k];
+        for (int k = 0; k < 10; ++k) terms[b][k][lane] = t[k];
         __syncthreads();
 #pragma unroll
-        for (uint32_t j = 0; j < 8u; ++j) ch0.add(terms[ck0][ct0 * 8u + j]);
+        for (uint32_t q = 0; q < 8u; ++q) ch0.add(terms[b][ck0][ct0 * 8u + q]);
         if (two) {
 #pragma unroll
-          for (uint32_t j = 0; j < 8u; ++j) ch1.add(terms[ck1][ct1 * 8u + j]);
+          for (uint32_t q = 0; q < 8u; ++q) ch1.add(terms[b][ck1][ct1 * 8u + q]);
         }
-        __syncthreads();
+        b ^= 1u;
       }
     }
 
@@ -759,6 +661,7 @@ __global__ __launch_bounds__(kWaveTreeThreads) void k_tree_wave(StepArgs a, uint
     };
     store(ct0, ck0, ch0.value());
     if (two) store(ct1, ck1, ch1.value());
+    __syncthreads();
   }
 }
 
@@ -845,13 +748,13 @@ __device__ __forceinline__ Leaf leaf_field(const SysDesc& d, float px, float py,
     float res[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      CompAcc s;
-      s.set(0, base[k]);
-      s.add(0, row[k][0] * dd[0]);
-      s.add(0, row[k][1] * dd[1]);
-      s.add(0, row[k][2] * dd[2]);
-      if (has_self) s.add(0, self[k]);
-      res[k] = s.get(0);
+      Chain<true> s;
+      s.reset(base[k]);
+      s.add(row[k][0] * dd[0]);
+      s.add(row[k][1] * dd[1]);
+      s.add(row[k][2] * dd[2]);
+      if (has_self) s.add(self[k]);
+      res[k] = s.value();
     }
     out.ax = res[0];
     out.ay = res[1];
@@ -1055,22 +958,18 @@ hipError_t launch_upward(const StepArgs& a, uint32_t nblocks, hipStream_t stream
   return hipGetLastError();
 }
 
+hipError_t launch_top(const StepArgs& a, hipStream_t stream) {
+  if (a.nsys == 0u) return hipSuccess;
+  hipLaunchKernelGGL(k_top, dim3(a.nsys), dim3(kUpwardThreads), 0, stream, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_tree(const StepArgs& a, uint32_t level, uint32_t nblocks, hipStream_t stream) {
   if (nblocks == 0u) return hipSuccess;
-  if (a.tree_kernel == kTreeWave) {
-    // One 64-lane block per internal parent of the level above.
-    const uint32_t items = a.list_cap[level - 1u];
-    const uint32_t blocks = items < 1u ? 1u : (items > kWaveTreeMaxBlocks ? kWaveTreeMaxBlocks : items);
-    if (a.compensated)
-      hipLaunchKernelGGL(k_tree_wave<true>, dim3(blocks), dim3(kWaveTreeThreads), 0, stream, a, level);
-    else
-      hipLaunchKernelGGL(k_tree_wave<false>, dim3(blocks), dim3(kWaveTreeThreads), 0, stream, a, level);
-    return hipGetLastError();
-  }
   if (a.compensated)
-    hipLaunchKernelGGL(k_tree<CompAcc>, dim3(nblocks), dim3(kTreeThreads), 0, stream, a, level);
+    hipLaunchKernelGGL(k_tree<true>, dim3(nblocks), dim3(kTreeThreads), 0, stream, a, level);
   else
-    hipLaunchKernelGGL(k_tree<Acc>, dim3(nblocks), dim3(kTreeThreads), 0, stream, a, level);
+    hipLaunchKernelGGL(k_tree<false>, dim3(nblocks), dim3(kTreeThreads), 0, stream, a, level);
   return hipGetLastError();
 }
 

@@ -96,14 +96,13 @@ class Engine {
   uint32_t n_integrate = 0;
   uint32_t n_subtree = 0;
   uint32_t tree_blocks[kLevels + 1] = {};
-  uint32_t tree_kernel = kTreeLane;
 
  private:
   int capture(uint32_t steps, hipGraphExec_t* out);
   void drop_graphs();
 
   DevBuf sys_buf, clear_buf, deposit_buf, integrate_buf, subtree_buf, list_buf, count_buf,
-      done_buf, diag_buf;
+      stripe_buf, diag_buf;
   hipGraphExec_t graph_one = nullptr;
   hipGraphExec_t graph_many = nullptr;
   hipEvent_t ev_a = nullptr;

@@ -174,7 +174,7 @@ Engine::~Engine() {
   if (stream) (void)hipStreamSynchronize(stream);
   drop_graphs();
   for (DevBuf* b : {&sys_buf, &clear_buf, &deposit_buf, &integrate_buf, &subtree_buf, &list_buf,
-                    &count_buf, &done_buf, &diag_buf})
+                    &count_buf, &stripe_buf, &diag_buf})
     b->release();
   if (ev_a) (void)hipEventDestroy(ev_a);
   if (ev_b) (void)hipEventDestroy(ev_b);
@@ -189,9 +189,6 @@ void Engine::drop_graphs() {
 
 int Engine::open(uint32_t opts) {
   options = opts;
-  // Tree-pass variant (both give identical bits): RAINBOW_TREE=lane|wave.
-  const char* tk = std::getenv("RAINBOW_TREE");
-  tree_kernel = (tk != nullptr && std::strcmp(tk, "wave") == 0) ? kTreeWave : kTreeLane;
   RB_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   RB_HIP(hipEventCreate(&ev_a));
   RB_HIP(hipEventCreate(&ev_b));
@@ -207,18 +204,29 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
   std::vector<CellBlock> clr;
   std::vector<BodyBlock> dep, integ;
   std::vector<SubtreeBlock> sub;
-  uint32_t list_cap[kLevels] = {};
+  // Capacity of every (level, stripe) list: a subtree block appends at most
+  // 64 / 8 / 1 internal cells of levels depth-1 / -2 / -3 to stripe (block
+  // index mod kStripes), the top block of system s appends the levels above
+  // to stripe (s mod kStripes); a level never holds more than n / 17
+  // internal cells.
+  std::vector<uint32_t> caps(kLevels * kStripes, 0u);
   dmax = 0;
   total_bodies = 0;
   for (uint32_t s = 0; s < nsys; ++s) {
     const SysDesc& d = sys[s];
     const uint32_t cells = total_cells(d.depth);
+    const uint32_t ib = d.n / (kAdaptiveLeafCapacity + 1u);
     for (uint32_t f = 0; f < cells; f += kClearCells) clr.push_back(CellBlock{s, f});
     for (uint32_t f = 0; f < d.n; f += kDepositBodies) dep.push_back(BodyBlock{s, f});
     for (uint32_t f = 0; f < d.n; f += kIntegrateThreads) integ.push_back(BodyBlock{s, f});
-    for (uint32_t r = 0; r < level_cells(d.depth - 3u); ++r) sub.push_back(SubtreeBlock{s, r});
-    for (uint32_t k = 0; k < d.depth; ++k)
-      list_cap[k] += std::min<uint32_t>(level_cells(k), d.n / (kAdaptiveLeafCapacity + 1u));
+    for (uint32_t r = 0; r < level_cells(d.depth - 3u); ++r) {
+      const uint32_t stripe = static_cast<uint32_t>(sub.size()) % kStripes;
+      caps[(d.depth - 1u) * kStripes + stripe] += std::min(64u, ib);
+      caps[(d.depth - 2u) * kStripes + stripe] += std::min(8u, ib);
+      caps[(d.depth - 3u) * kStripes + stripe] += std::min(1u, ib);
+      sub.push_back(SubtreeBlock{s, r});
+    }
+    for (uint32_t k = 0; k + 3u < d.depth; ++k) caps[k * kStripes + s % kStripes] += std::min(level_cells(k), ib);
     dmax = std::max(dmax, d.depth);
     total_bodies += d.n;
   }
@@ -226,11 +234,12 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
   n_deposit = static_cast<uint32_t>(dep.size());
   n_integrate = static_cast<uint32_t>(integ.size());
   n_subtree = static_cast<uint32_t>(sub.size());
+  std::vector<uint32_t> stripes(2u * kLevels * kStripes);
   uint32_t list_total = 0;
-  for (uint32_t k = 0; k < kLevels; ++k) {
-    args.list_off[k] = list_total;
-    args.list_cap[k] = list_cap[k];
-    list_total += list_cap[k];
+  for (uint32_t i = 0; i < kLevels * kStripes; ++i) {
+    stripes[i] = list_total;
+    stripes[kLevels * kStripes + i] = caps[i];
+    list_total += caps[i];
   }
   int rc;
   if ((rc = sys_buf.ensure(sizeof(SysDesc) * nsys)) || (rc = clear_buf.ensure(sizeof(CellBlock) * n_clear)) ||
@@ -238,8 +247,8 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
       (rc = integrate_buf.ensure(sizeof(BodyBlock) * n_integrate)) ||
       (rc = subtree_buf.ensure(sizeof(SubtreeBlock) * n_subtree)) ||
       (rc = list_buf.ensure(sizeof(uint32_t) * list_total)) ||
-      (rc = count_buf.ensure(sizeof(uint32_t) * (kLevels + 1u))) ||
-      (rc = done_buf.ensure(sizeof(uint32_t) * nsys)) ||
+      (rc = count_buf.ensure(sizeof(uint32_t) * kLevels * kStripes)) ||
+      (rc = stripe_buf.ensure(sizeof(uint32_t) * stripes.size())) ||
       (rc = diag_buf.ensure(sizeof(double) * (8u * static_cast<size_t>(n_integrate) + 8u))))
     return rc;
   args.sys = static_cast<const SysDesc*>(sys_buf.p);
@@ -249,14 +258,14 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
   args.subtree_blocks = static_cast<const SubtreeBlock*>(subtree_buf.p);
   args.lists = static_cast<uint32_t*>(list_buf.p);
   args.list_count = static_cast<uint32_t*>(count_buf.p);
-  args.subtree_done = static_cast<uint32_t*>(done_buf.p);
+  args.stripe_off = static_cast<const uint32_t*>(stripe_buf.p);
+  args.stripe_cap = static_cast<const uint32_t*>(stripe_buf.p) + kLevels * kStripes;
   args.nsys = nsys;
   args.compensated = (options & RAINBOW_OPT_COMPENSATED) ? 1u : 0u;
-  args.tree_kernel = tree_kernel;
   for (uint32_t l = 1; l <= kLevels - 1u; ++l) {
-    const uint64_t lanes = 8ull * list_cap[l - 1u];
-    const uint64_t blocks = (lanes + kTreeThreads - 1u) / kTreeThreads;
-    tree_blocks[l] = static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(blocks, 1u), kTreeMaxBlocks));
+    uint64_t items = 0;
+    for (uint32_t j = 0; j < kStripes; ++j) items += caps[(l - 1u) * kStripes + j];
+    tree_blocks[l] = static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(items, 1u), kTreeMaxBlocks));
   }
   if (nsys) {
     RB_HIP(hipMemcpyAsync(sys_buf.p, sys.data(), sizeof(SysDesc) * nsys, hipMemcpyHostToDevice, stream));
@@ -266,9 +275,10 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
                           stream));
     RB_HIP(hipMemcpyAsync(subtree_buf.p, sub.data(), sizeof(SubtreeBlock) * n_subtree, hipMemcpyHostToDevice,
                           stream));
-    RB_HIP(hipMemsetAsync(done_buf.p, 0, sizeof(uint32_t) * nsys, stream));
   }
-  RB_HIP(hipMemsetAsync(count_buf.p, 0, sizeof(uint32_t) * (kLevels + 1u), stream));
+  RB_HIP(hipMemcpyAsync(stripe_buf.p, stripes.data(), sizeof(uint32_t) * stripes.size(), hipMemcpyHostToDevice,
+                        stream));
+  RB_HIP(hipMemsetAsync(count_buf.p, 0, sizeof(uint32_t) * kLevels * kStripes, stream));
   RB_HIP(hipStreamSynchronize(stream));
   return RAINBOW_OK;
 }
@@ -286,6 +296,7 @@ int Engine::enqueue(bool integrate) {
   RB_HIP(launch_clear(args, n_clear, stream));
   RB_HIP(launch_deposit(args, n_deposit, stream));
   RB_HIP(launch_upward(args, n_subtree, stream));
+  RB_HIP(launch_top(args, stream));
   for (uint32_t l = 1; l <= dmax; ++l) RB_HIP(launch_tree(args, l, tree_blocks[l], stream));
   if (integrate) RB_HIP(launch_integrate(args, n_integrate, stream));
   return RAINBOW_OK;
@@ -342,7 +353,7 @@ int Engine::time_steps(uint32_t steps, float* ms) {
 }
 
 int Engine::phase_ms(uint32_t steps, float* out, uint32_t n) {
-  const uint32_t nphase = 4u + dmax;  // clear, deposit, upward, tree 1..dmax, integrate
+  const uint32_t nphase = 5u + dmax;  // clear, deposit, upward, top, tree 1..dmax, integrate
   if (out == nullptr || n < nphase) return fail(RAINBOW_E_INVALID, "phase_ms needs %u entries", nphase);
   if (steps == 0u) return fail(RAINBOW_E_INVALID, "steps must be >= 1");
   std::vector<hipEvent_t> ev(nphase + 1u, nullptr);
@@ -360,6 +371,8 @@ int Engine::phase_ms(uint32_t steps, float* out, uint32_t n) {
     if (err == hipSuccess) err = launch_deposit(args, n_deposit, stream);
     mark();
     if (err == hipSuccess) err = launch_upward(args, n_subtree, stream);
+    mark();
+    if (err == hipSuccess) err = launch_top(args, stream);
     mark();
     for (uint32_t l = 1; l <= dmax; ++l) {
       if (err == hipSuccess) err = launch_tree(args, l, tree_blocks[l], stream);

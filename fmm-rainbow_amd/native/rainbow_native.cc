@@ -73,14 +73,14 @@ int main(int argc, char** argv) {
   float phase[16];
   const int nphase = rainbow_hip_phase_ms(h, 5u, phase, 16u);
   if (nphase < 0) check(nphase, "phase timing");
-  const char* fixed[3] = {"clear", "deposit", "upward"};
+  const char* fixed[4] = {"clear", "deposit", "upward", "top"};
   double total = 0.0;
   for (int i = 0; i < nphase; ++i) {
     char name[32];
-    if (i < 3)
+    if (i < 4)
       std::snprintf(name, sizeof(name), "%s", fixed[i]);
     else if (i < nphase - 1)
-      std::snprintf(name, sizeof(name), "tree_l%d", i - 2);
+      std::snprintf(name, sizeof(name), "tree_l%d", i - 3);
     else
       std::snprintf(name, sizeof(name), "integrate");
     std::printf("  ts %-10s %.4f ms\n", name, static_cast<double>(phase[i]));

@@ -277,11 +277,9 @@ def test_server_results_equal_solo():
     srv.close()
 
 
-@pytest.mark.parametrize("tree", ["lane", "wave"])
 @pytest.mark.parametrize("compensated", [False, True])
-def test_tree_variants_bit_exact(tree, compensated, monkeypatch):
-    say(f"tree={tree} compensated={compensated}, 1M Plummer solve + 2 steps")
-    monkeypatch.setenv("RAINBOW_TREE", tree)
+def test_million_and_uneven_batch_bit_exact(compensated):
+    say(f"compensated={compensated}, 1M Plummer solve + 2 steps, uneven batch")
     parts = wl.cfg2_particles()
     c = wl.CFG2
     slots = slots_for(c["n"], c["dt"], c["gravity"], c["softening"])
