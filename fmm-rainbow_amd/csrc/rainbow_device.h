@@ -3,6 +3,8 @@
 // HBM layout of one system of n bodies and C = total_cells(depth) cells:
 //   pos     float4[n]   {x, y, z, hue}            \ the reference Particle,
 //   vel     float4[n]   {vx, vy, vz, excitation}  / split into two streams
+//   perm    uint32[n]   caller index of the body in each slot: bodies are
+//                       kept in Morton order of their finest cell
 //   moments int4[C]     {count, sum_x, sum_y, sum_z}   reference Moment
 //   fields  float4[3C]  {a, phi}{Jxx,Jxy,Jxz,Jyy}{Jyz,Jzz,0,0}  reference Field
 //   source  float4[C]   {com.xyz, G m} per cell, G m = 0 for an empty cell
@@ -22,12 +24,15 @@ namespace rainbow {
 struct alignas(16) SysDesc {
   float4* pos;
   float4* vel;
+  uint32_t* perm;  // caller index of the body stored in each slot
   int4* moments;
   float4* fields;
   float4* source;
   uint8_t* active;
-  uint32_t n;         // bodies (particle_count)
-  uint32_t depth;     // octree depth
+  uint32_t n;          // bodies (particle_count)
+  uint32_t depth;      // octree depth
+  uint32_t body_base;  // first index of the system in the batch's body space
+  uint32_t pad;
   float scale;        // fixed-point scale (control_settings.z)
   float dt;
   float gravity;
@@ -39,7 +44,7 @@ struct alignas(16) SysDesc {
   float control_point[4];
   float control_axis_mode[4];
 };
-static_assert(sizeof(SysDesc) == 128, "SysDesc layout");
+static_assert(sizeof(SysDesc) == 144, "SysDesc layout");
 
 // Work list of the particle passes: a block covers bodies [first, first +
 // block size) of one system.

@@ -127,16 +127,40 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(StepArgs a) {
   }
   __syncthreads();
   const uint32_t dim = 1u << d.depth;
+  const uint32_t lane = threadIdx.x & 63u;
 #pragma unroll
   for (uint32_t k = 0; k < kDepositPerThread; ++k) {
     const uint32_t i = blk.first + k * kDepositThreads + threadIdx.x;
-    if (i < d.n) {
+    const bool valid = i < d.n;
+    uint32_t key = kEmptyKey;
+    int v[4] = {0, 0, 0, 0};
+    if (valid) {
       const float4 p = d.pos[i];
-      const uint32_t key = lin(cell1(unit_coord(p.x), dim), cell1(unit_coord(p.y), dim),
-                               cell1(unit_coord(p.z), dim), dim);
-      const int ix = to_i32_sat(rintf(p.x * d.scale));
-      const int iy = to_i32_sat(rintf(p.y * d.scale));
-      const int iz = to_i32_sat(rintf(p.z * d.scale));
+      key = lin(cell1(unit_coord(p.x), dim), cell1(unit_coord(p.y), dim), cell1(unit_coord(p.z), dim), dim);
+      v[0] = 1;
+      v[1] = to_i32_sat(rintf(p.x * d.scale));
+      v[2] = to_i32_sat(rintf(p.y * d.scale));
+      v[3] = to_i32_sat(rintf(p.z * d.scale));
+    }
+    // Bodies are kept in Morton order, so equal keys form runs across the
+    // wave: a segmented inclusive scan sums each run into its last lane
+    // (integer sums, exact in any order).
+    const uint32_t prev_key = __shfl_up(key, 1u);
+    const unsigned long long heads = __ballot(lane == 0u || key != prev_key);
+    const uint32_t start = 63u - static_cast<uint32_t>(__clzll(heads & ((2ull << lane) - 1ull)));
+#pragma unroll
+    for (uint32_t off = 1u; off < 64u; off <<= 1) {
+      int t[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) t[c] = __shfl_up(v[c], off);
+      if (lane >= start + off) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          v[c] = static_cast<int>(static_cast<uint32_t>(v[c]) + static_cast<uint32_t>(t[c]));
+      }
+    }
+    const bool tail = lane == 63u || ((heads >> (lane + 1u)) & 1ull) != 0ull;
+    if (valid && tail) {
       // Open addressing; at most kDepositBodies distinct keys in a table of
       // 2 kDepositBodies slots, so an empty slot always exists.
       uint32_t h = (key * 2654435761u) >> (32 - 11);
@@ -145,23 +169,77 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(StepArgs a) {
         if (prev == kEmptyKey || prev == key) break;
         h = (h + 1u) & (kDepositSlots - 1u);
       }
-      atomicAdd(&vals[0][h], 1);
-      atomicAdd(&vals[1][h], ix);
-      atomicAdd(&vals[2][h], iy);
-      atomicAdd(&vals[3][h], iz);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) atomicAdd(&vals[c][h], v[c]);
     }
   }
   __syncthreads();
-  int* finest = reinterpret_cast<int*>(d.moments + level_off(d.depth));
+  int4* finest = d.moments + level_off(d.depth);
   for (uint32_t s = threadIdx.x; s < kDepositSlots; s += kDepositThreads) {
     const uint32_t key = keys[s];
     if (key == kEmptyKey) continue;
-    int* m = finest + 4u * key;
-    atomicAdd(m + 0, vals[0][s]);
-    atomicAdd(m + 1, vals[1][s]);
-    atomicAdd(m + 2, vals[2][s]);
-    atomicAdd(m + 3, vals[3][s]);
+    // count and sum_x share one 64-bit atomic: the count never carries out
+    // of its 32 bits, and the high word wraps as an i32 sum.
+    const unsigned long long packed = static_cast<unsigned long long>(static_cast<uint32_t>(vals[0][s])) |
+                                      (static_cast<unsigned long long>(static_cast<uint32_t>(vals[1][s])) << 32);
+    atomicAdd(reinterpret_cast<unsigned long long*>(finest + key), packed);
+    atomicAdd(reinterpret_cast<int*>(finest + key) + 2, vals[2][s]);
+    atomicAdd(reinterpret_cast<int*>(finest + key) + 3, vals[3][s]);
   }
+}
+
+// ------------------------------------------------------------------------
+// body order: Morton keys, gather and copy-back around a radix sort
+// ------------------------------------------------------------------------
+
+// Interleave the low 10 bits of v with two zero bits between each.
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+  v &= 0x3FFu;
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+// key = system << 18 | Morton code of the finest cell; value = slot.
+__global__ __launch_bounds__(kIntegrateThreads) void k_sort_keys(StepArgs a, uint32_t* keys, uint32_t* vals) {
+  const BodyBlock blk = a.integrate_blocks[blockIdx.x];
+  const SysDesc d = a.sys[blk.sys];
+  const uint32_t i = blk.first + threadIdx.x;
+  if (i >= d.n) return;
+  const float4 p = d.pos[i];
+  const uint32_t dim = 1u << d.depth;
+  const uint32_t m = spread3(cell1(unit_coord(p.x), dim)) | (spread3(cell1(unit_coord(p.y), dim)) << 1) |
+                     (spread3(cell1(unit_coord(p.z), dim)) << 2);
+  keys[d.body_base + i] = (blk.sys << 18) | m;
+  vals[d.body_base + i] = i;
+}
+
+__global__ __launch_bounds__(kIntegrateThreads) void k_gather(StepArgs a, const uint32_t* vals, float4* tpos,
+                                                               float4* tvel, uint32_t* tperm) {
+  const BodyBlock blk = a.integrate_blocks[blockIdx.x];
+  const SysDesc d = a.sys[blk.sys];
+  const uint32_t i = blk.first + threadIdx.x;
+  if (i >= d.n) return;
+  const uint32_t g = d.body_base + i;
+  const uint32_t j = vals[g];  // slot of the body that goes to position i
+  if (j >= d.n) return;
+  tpos[g] = d.pos[j];
+  tvel[g] = d.vel[j];
+  tperm[g] = d.perm[j];
+}
+
+__global__ __launch_bounds__(kIntegrateThreads) void k_copy_back(StepArgs a, const float4* tpos,
+                                                                  const float4* tvel, const uint32_t* tperm) {
+  const BodyBlock blk = a.integrate_blocks[blockIdx.x];
+  const SysDesc d = a.sys[blk.sys];
+  const uint32_t i = blk.first + threadIdx.x;
+  if (i >= d.n) return;
+  const uint32_t g = d.body_base + i;
+  d.pos[i] = tpos[g];
+  d.vel[i] = tvel[g];
+  d.perm[i] = tperm[g];
 }
 
 // ------------------------------------------------------------------------
@@ -885,7 +963,7 @@ __global__ __launch_bounds__(kIntegrateThreads) void k_accel(StepArgs a, uint32_
   if (i >= d.n) return;
   const float4 p4 = d.pos[i];
   const Leaf leaf = leaf_field<kComp, true>(d, p4.x, p4.y, p4.z);
-  out[i] = make_float4(leaf.ax, leaf.ay, leaf.az, leaf.phi);
+  out[d.perm[i]] = make_float4(leaf.ax, leaf.ay, leaf.az, leaf.phi);
 }
 
 // Block partials of {KE, PE, Px, Py, Pz, Lx, Ly, Lz} in FP64 (equal masses
@@ -970,6 +1048,23 @@ hipError_t launch_tree(const StepArgs& a, uint32_t level, uint32_t nblocks, hipS
     hipLaunchKernelGGL(k_tree<true>, dim3(nblocks), dim3(kTreeThreads), 0, stream, a, level);
   else
     hipLaunchKernelGGL(k_tree<false>, dim3(nblocks), dim3(kTreeThreads), 0, stream, a, level);
+  return hipGetLastError();
+}
+
+hipError_t launch_sort_keys(const StepArgs& a, uint32_t nblocks, uint32_t* keys, uint32_t* vals,
+                            hipStream_t stream) {
+  if (nblocks == 0u) return hipSuccess;
+  hipLaunchKernelGGL(k_sort_keys, dim3(nblocks), dim3(kIntegrateThreads), 0, stream, a, keys, vals);
+  return hipGetLastError();
+}
+
+hipError_t launch_reorder(const StepArgs& a, uint32_t nblocks, const uint32_t* vals, float4* tpos, float4* tvel,
+                          uint32_t* tperm, hipStream_t stream) {
+  if (nblocks == 0u) return hipSuccess;
+  hipLaunchKernelGGL(k_gather, dim3(nblocks), dim3(kIntegrateThreads), 0, stream, a, vals, tpos, tvel, tperm);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_copy_back, dim3(nblocks), dim3(kIntegrateThreads), 0, stream, a, tpos, tvel, tperm);
   return hipGetLastError();
 }
 

@@ -35,6 +35,8 @@ const char* last_error();
 
 // Steps captured in the multi-step graph.
 constexpr uint32_t kGraphSteps = 16;
+// Bodies are re-sorted into Morton order before every this many steps.
+constexpr uint32_t kSortInterval = 16;
 
 // One device allocation holding a system's bodies and cells, and its
 // descriptor pointing into it.
@@ -84,6 +86,7 @@ class Engine {
   int download(uint32_t sys, int which, void* dst, size_t bytes);
   int accelerations(uint32_t sys, void* dst, size_t bytes);
   int diagnostics(uint32_t sys, double out[8]);
+  int sort_bodies();
 
   hipStream_t stream = nullptr;
   uint32_t options = 0;
@@ -102,7 +105,8 @@ class Engine {
   void drop_graphs();
 
   DevBuf sys_buf, clear_buf, deposit_buf, integrate_buf, subtree_buf, list_buf, count_buf,
-      stripe_buf, diag_buf;
+      stripe_buf, diag_buf, key_buf, body_buf, sort_tmp;
+  uint32_t since_sort = kSortInterval;
   hipGraphExec_t graph_one = nullptr;
   hipGraphExec_t graph_many = nullptr;
   hipEvent_t ev_a = nullptr;

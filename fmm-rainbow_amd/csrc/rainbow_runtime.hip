@@ -95,8 +95,8 @@ void apply_physics(SysDesc& s, const Physics& ph) {
 
 size_t system_storage_bytes(uint32_t n) {
   const size_t cells = total_cells(depth_for_particles(n));
-  return 2u * align_up(16u * static_cast<size_t>(n)) + align_up(16u * cells) +
-         align_up(48u * cells) + align_up(16u * cells) + align_up(cells);
+  return 2u * align_up(16u * static_cast<size_t>(n)) + align_up(4u * static_cast<size_t>(n)) +
+         align_up(16u * cells) + align_up(48u * cells) + align_up(16u * cells) + align_up(cells);
 }
 
 int alloc_system(uint32_t n, const Physics& ph, float scale, hipStream_t stream, SystemStorage* out) {
@@ -121,6 +121,8 @@ int alloc_system(uint32_t n, const Physics& ph, float scale, hipStream_t stream,
   p += align_up(16u * static_cast<size_t>(n));
   d.vel = reinterpret_cast<float4*>(p);
   p += align_up(16u * static_cast<size_t>(n));
+  d.perm = reinterpret_cast<uint32_t*>(p);
+  p += align_up(4u * static_cast<size_t>(n));
   d.moments = reinterpret_cast<int4*>(p);
   p += align_up(16u * cells);
   d.fields = reinterpret_cast<float4*>(p);
@@ -174,7 +176,7 @@ Engine::~Engine() {
   if (stream) (void)hipStreamSynchronize(stream);
   drop_graphs();
   for (DevBuf* b : {&sys_buf, &clear_buf, &deposit_buf, &integrate_buf, &subtree_buf, &list_buf,
-                    &count_buf, &stripe_buf, &diag_buf})
+                    &count_buf, &stripe_buf, &diag_buf, &key_buf, &body_buf, &sort_tmp})
     b->release();
   if (ev_a) (void)hipEventDestroy(ev_a);
   if (ev_b) (void)hipEventDestroy(ev_b);
@@ -213,6 +215,7 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
   dmax = 0;
   total_bodies = 0;
   for (uint32_t s = 0; s < nsys; ++s) {
+    sys[s].body_base = static_cast<uint32_t>(total_bodies);
     const SysDesc& d = sys[s];
     const uint32_t cells = total_cells(d.depth);
     const uint32_t ib = d.n / (kAdaptiveLeafCapacity + 1u);
@@ -280,6 +283,35 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
                         stream));
   RB_HIP(hipMemsetAsync(count_buf.p, 0, sizeof(uint32_t) * kLevels * kStripes, stream));
   RB_HIP(hipStreamSynchronize(stream));
+  since_sort = kSortInterval;
+  return RAINBOW_OK;
+}
+
+// Reorders every system's bodies by the Morton code of their finest cell
+// (keys carry the system index above 18 code bits).  The order is internal:
+// perm maps slots back to caller indices, and no result depends on it.
+int Engine::sort_bodies() {
+  if (sys.empty() || sys.size() > (1u << 14) || total_bodies > 0xFFFFFFFFull) return RAINBOW_OK;
+  const uint32_t n = static_cast<uint32_t>(total_bodies);
+  uint32_t sys_bits = 0;
+  while ((1u << sys_bits) < sys.size()) ++sys_bits;
+  const uint32_t end_bit = 18u + sys_bits;
+  size_t temp_bytes = 0;
+  RB_HIP(sort_pairs(nullptr, temp_bytes, nullptr, nullptr, nullptr, nullptr, n, end_bit, stream));
+  int rc;
+  if ((rc = key_buf.ensure(4ull * 4u * n)) || (rc = body_buf.ensure(36ull * n)) ||
+      (rc = sort_tmp.ensure(temp_bytes)))
+    return rc;
+  uint32_t* keys0 = static_cast<uint32_t*>(key_buf.p);
+  uint32_t* keys1 = keys0 + n;
+  uint32_t* vals0 = keys1 + n;
+  uint32_t* vals1 = vals0 + n;
+  float4* tpos = static_cast<float4*>(body_buf.p);
+  float4* tvel = tpos + n;
+  uint32_t* tperm = reinterpret_cast<uint32_t*>(tvel + n);
+  RB_HIP(launch_sort_keys(args, n_integrate, keys0, vals0, stream));
+  RB_HIP(sort_pairs(sort_tmp.p, temp_bytes, keys0, keys1, vals0, vals1, n, end_bit, stream));
+  RB_HIP(launch_reorder(args, n_integrate, vals1, tpos, tvel, tperm, stream));
   return RAINBOW_OK;
 }
 
@@ -325,8 +357,25 @@ int Engine::step(uint32_t steps) {
   if (steps >= kGraphSteps && graph_many == nullptr && (rc = capture(kGraphSteps, &graph_many)) != 0)
     return rc;
   if (steps % kGraphSteps != 0u && graph_one == nullptr && (rc = capture(1u, &graph_one)) != 0) return rc;
-  for (uint32_t s = 0; s + kGraphSteps <= steps; s += kGraphSteps) RB_HIP(hipGraphLaunch(graph_many, stream));
-  for (uint32_t s = 0; s < steps % kGraphSteps; ++s) RB_HIP(hipGraphLaunch(graph_one, stream));
+  // Bodies are re-sorted before every kSortInterval steps; the graphs hold
+  // the step kernels only.
+  uint32_t done = 0;
+  while (done < steps) {
+    if (since_sort >= kSortInterval) {
+      if ((rc = sort_bodies()) != 0) return rc;
+      since_sort = 0;
+    }
+    if (steps - done >= kGraphSteps && since_sort == 0u) {
+      RB_HIP(hipGraphLaunch(graph_many, stream));
+      done += kGraphSteps;
+      since_sort += kGraphSteps;
+    } else {
+      if (graph_one == nullptr && (rc = capture(1u, &graph_one)) != 0) return rc;
+      RB_HIP(hipGraphLaunch(graph_one, stream));
+      done += 1u;
+      since_sort += 1u;
+    }
+  }
   return RAINBOW_OK;
 }
 
@@ -398,13 +447,16 @@ int upload_system(const SysDesc& d, const void* data, size_t bytes, hipStream_t 
     return fail(RAINBOW_E_INVALID, "upload expects %llu bytes, got %zu",
                 static_cast<unsigned long long>(32ull * d.n), bytes);
   std::vector<float4> pos(d.n), vel(d.n);
+  std::vector<uint32_t> perm(d.n);
   const float* f = static_cast<const float*>(data);
   for (uint32_t i = 0; i < d.n; ++i) {
     pos[i] = make_float4(f[8u * i + 0u], f[8u * i + 1u], f[8u * i + 2u], f[8u * i + 3u]);
     vel[i] = make_float4(f[8u * i + 4u], f[8u * i + 5u], f[8u * i + 6u], f[8u * i + 7u]);
+    perm[i] = i;
   }
   RB_HIP(hipMemcpyAsync(d.pos, pos.data(), 16ull * d.n, hipMemcpyHostToDevice, stream));
   RB_HIP(hipMemcpyAsync(d.vel, vel.data(), 16ull * d.n, hipMemcpyHostToDevice, stream));
+  RB_HIP(hipMemcpyAsync(d.perm, perm.data(), 4ull * d.n, hipMemcpyHostToDevice, stream));
   RB_HIP(hipStreamSynchronize(stream));
   return RAINBOW_OK;
 }
@@ -416,19 +468,22 @@ int download_system(const SysDesc& d, int which, void* dst, size_t bytes, hipStr
     if (bytes != 32ull * d.n)
       return fail(RAINBOW_E_INVALID, "particles need %llu bytes", static_cast<unsigned long long>(32ull * d.n));
     std::vector<float4> pos(d.n), vel(d.n);
+    std::vector<uint32_t> perm(d.n);
     RB_HIP(hipMemcpyAsync(pos.data(), d.pos, 16ull * d.n, hipMemcpyDeviceToHost, stream));
     RB_HIP(hipMemcpyAsync(vel.data(), d.vel, 16ull * d.n, hipMemcpyDeviceToHost, stream));
+    RB_HIP(hipMemcpyAsync(perm.data(), d.perm, 4ull * d.n, hipMemcpyDeviceToHost, stream));
     RB_HIP(hipStreamSynchronize(stream));
     float* f = static_cast<float*>(dst);
     for (uint32_t i = 0; i < d.n; ++i) {
-      f[8u * i + 0u] = pos[i].x;
-      f[8u * i + 1u] = pos[i].y;
-      f[8u * i + 2u] = pos[i].z;
-      f[8u * i + 3u] = pos[i].w;
-      f[8u * i + 4u] = vel[i].x;
-      f[8u * i + 5u] = vel[i].y;
-      f[8u * i + 6u] = vel[i].z;
-      f[8u * i + 7u] = vel[i].w;
+      const size_t o = 8u * static_cast<size_t>(perm[i] < d.n ? perm[i] : i);
+      f[o + 0u] = pos[i].x;
+      f[o + 1u] = pos[i].y;
+      f[o + 2u] = pos[i].z;
+      f[o + 3u] = pos[i].w;
+      f[o + 4u] = vel[i].x;
+      f[o + 5u] = vel[i].y;
+      f[o + 6u] = vel[i].z;
+      f[o + 7u] = vel[i].w;
     }
     return RAINBOW_OK;
   }
@@ -452,6 +507,7 @@ int download_system(const SysDesc& d, int which, void* dst, size_t bytes, hipStr
 int Engine::upload(uint32_t s, const void* data, size_t bytes) {
   if (s >= sys.size()) return fail(RAINBOW_E_INVALID, "system %u out of range", s);
   RB_HIP(hipStreamSynchronize(stream));
+  since_sort = kSortInterval;
   return upload_system(sys[s], data, bytes, stream);
 }
 
