@@ -572,10 +572,14 @@ struct Chain {
 
 template <bool kComp>
 __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t level) {
+  constexpr uint32_t kWaves = kTreeThreads / 64u;
   __shared__ float4 win[216];
-  __shared__ float terms[2][10][kTreeThreads];
-  const uint32_t lane = threadIdx.x;
-  // Items of this level: stripe counts and their prefix across lanes 0..31.
+  __shared__ float terms[kWaves][10][64];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = tid & 63u;
+  const uint32_t wave = tid >> 6;
+  // Items of this level: stripe counts and their prefix across lanes 0..31
+  // (every wave computes the same values).
   const uint32_t lv = level - 1u;
   uint32_t cnt = 0u;
   if (lane < kStripes) cnt = min(a.list_count[lv * kStripes + lane], a.stripe_cap[lv * kStripes + lane]);
@@ -597,14 +601,12 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t leve
   // cell_epsilon, src/rainbow_fmm.wgsl:507 (0.06 * WORLD_HALF_EXTENT / dim).
   const float cell_epsilon = (0.06f * kWorldHalfExtent) / static_cast<float>(dim);
   const float eps2_m2l = cell_epsilon * cell_epsilon;
-  // Pair lane: target tt (child of the parent), source cc.
+  // Pair lane: target tt (child of the parent), source cc of the chunk.
   const uint32_t tt = lane >> 3, cc = lane & 7u;
   const uint32_t wtx = 2u + (tt & 1u), wty = 2u + ((tt >> 1) & 1u), wtz = 2u + (tt >> 2);
-  // Chain lanes: chain q = target q / 10, component q % 10; lanes < 16 own a
-  // second chain q + 64.
-  const bool two = lane < 16u;
-  const uint32_t ct0 = lane / 10u, ck0 = lane % 10u;
-  const uint32_t ct1 = (lane + 64u) / 10u, ck1 = (lane + 64u) % 10u;
+  // Chain threads 0..79: target tid / 10, component tid % 10.
+  const bool chain = tid < 80u;
+  const uint32_t ct = tid / 10u, ck = tid % 10u;
 
   for (uint32_t it = blockIdx.x; it < total; it += gridDim.x) {
     const uint32_t j = static_cast<uint32_t>(__popcll(__ballot(lane < kStripes && incl <= it)));
@@ -623,7 +625,7 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t leve
     // Stage the window of cells 2P-2 .. 2P+3 per axis; cells outside the
     // level read as empty.
     const float4* src = d.source + off;
-    for (uint32_t w = lane; w < 216u; w += kTreeThreads) {
+    for (uint32_t w = tid; w < 216u; w += kTreeThreads) {
       const uint32_t wz = w / 36u, wy = (w / 6u) % 6u, wx = w % 6u;
       const int qx = 2 * static_cast<int>(px) - 2 + static_cast<int>(wx);
       const int qy = 2 * static_cast<int>(py) - 2 + static_cast<int>(wy);
@@ -633,78 +635,69 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t leve
         v = src[lin(static_cast<uint32_t>(qx), static_cast<uint32_t>(qy), static_cast<uint32_t>(qz), dim)];
       win[w] = v;
     }
-    float4 pa, pj0, pj1;
-    load_field(d.fields, poff + pidx, pa, pj0, pj1);
     const uint32_t tx = 2u * px + (tt & 1u), ty = 2u * py + ((tt >> 1) & 1u), tz = 2u * pz + (tt >> 2);
     const int tcount = counts[4u * (off + lin(tx, ty, tz, dim))];
     const float tcx = center1(tx, dim), tcy = center1(ty, dim), tcz = center1(tz, dim);
-    __syncthreads();
+    const bool leaf = tcount != 0 && (tcount <= kLeafCapacity || level == d.depth);
+    // Chunks: the in-range source-parent blocks in pz, py, px order (8 M2L
+    // sources each), then, if any target is a leaf, 4 blocks of 8 near
+    // offsets.
+    const int ox0 = px > 0u ? -1 : 0, oy0 = py > 0u ? -1 : 0, oz0 = pz > 0u ? -1 : 0;
+    const int nx = (px + 1u < pdim ? 1 : 0) - ox0 + 1;
+    const int ny = (py + 1u < pdim ? 1 : 0) - oy0 + 1;
+    const int nz = (pz + 1u < pdim ? 1 : 0) - oz0 + 1;
+    const uint32_t m2l_chunks = static_cast<uint32_t>(nx * ny * nz);
+    const uint32_t chunks = m2l_chunks + (__syncthreads_or(leaf) ? 4u : 0u);
+    const float near_epsilon =
+        fmaxf(d.softening, (0.72f * kWorldHalfExtent) / static_cast<float>(dim));
+    const float eps2_near = near_epsilon * near_epsilon;
 
-    Chain<kComp> ch0, ch1;
-    uint32_t b = 0u;
-    for (int oz = -1; oz <= 1; ++oz) {
-      if (static_cast<int>(pz) + oz < 0 || static_cast<int>(pz) + oz >= pdim_i) continue;
-      for (int oy = -1; oy <= 1; ++oy) {
-        if (static_cast<int>(py) + oy < 0 || static_cast<int>(py) + oy >= pdim_i) continue;
-        for (int ox = -1; ox <= 1; ++ox) {
-          if (static_cast<int>(px) + ox < 0 || static_cast<int>(px) + ox >= pdim_i) continue;
-          const uint32_t wsx = static_cast<uint32_t>(2 * (ox + 1)) + (cc & 1u);
-          const uint32_t wsy = static_cast<uint32_t>(2 * (oy + 1)) + ((cc >> 1) & 1u);
-          const uint32_t wsz = static_cast<uint32_t>(2 * (oz + 1)) + (cc >> 2);
-          const bool near = (wsx + 1u - wtx) <= 2u && (wsy + 1u - wty) <= 2u && (wsz + 1u - wtz) <= 2u;
-          float t[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-          if (tcount != 0 && !near) {
-            const float4 sr = win[wsx + 6u * (wsy + 6u * wsz)];
-            if (sr.w != 0.0f) pair_terms(sr, tcx, tcy, tcz, eps2_m2l, t);
-          }
-#pragma unroll
-          for (int k = 0; k < 10; ++k) terms[b][k][lane] = t[k];
-          __syncthreads();
-#pragma unroll
-          for (uint32_t q = 0; q < 8u; ++q) ch0.add(terms[b][ck0][ct0 * 8u + q]);
-          if (two) {
-#pragma unroll
-            for (uint32_t q = 0; q < 8u; ++q) ch1.add(terms[b][ck1][ct1 * 8u + q]);
-          }
-          b ^= 1u;
-        }
+    Chain<kComp> ch;
+    float inh = 0.0f;
+    if (chain) {
+      // L2L (downward, src/rainbow_fmm.wgsl:598-610): this chain's
+      // component of the parent's field shifted to its target's centre.
+      float4 pa, pj0, pj1;
+      load_field(d.fields, poff + pidx, pa, pj0, pj1);
+      const float dx = center1(2u * px + (ct & 1u), dim) - center1(px, pdim);
+      const float dy = center1(2u * py + ((ct >> 1) & 1u), dim) - center1(py, pdim);
+      const float dz = center1(2u * pz + (ct >> 2), dim) - center1(pz, pdim);
+      switch (ck) {
+        case 0: inh = pa.x + ((pj0.x * dx + pj0.y * dy) + pj0.z * dz); break;
+        case 1: inh = pa.y + ((pj0.y * dx + pj0.w * dy) + pj1.x * dz); break;
+        case 2: inh = pa.z + ((pj0.z * dx + pj1.x * dy) + pj1.y * dz); break;
+        case 3: inh = pa.w - ((pa.x * dx + pa.y * dy) + pa.z * dz); break;
+        case 4: inh = pj0.x; break;
+        case 5: inh = pj0.y; break;
+        case 6: inh = pj0.z; break;
+        case 7: inh = pj0.w; break;
+        case 8: inh = pj1.x; break;
+        default: inh = pj1.y; break;
       }
     }
+    __syncthreads();
 
-    // L2L (downward, src/rainbow_fmm.wgsl:598-610): each chain adds its
-    // component of the parent's field shifted to its target's centre.
-    const float pcx = center1(px, pdim), pcy = center1(py, pdim), pcz = center1(pz, pdim);
-    auto inherit = [&](uint32_t t_, uint32_t k_) -> float {
-      const float dx = center1(2u * px + (t_ & 1u), dim) - pcx;
-      const float dy = center1(2u * py + ((t_ >> 1) & 1u), dim) - pcy;
-      const float dz = center1(2u * pz + (t_ >> 2), dim) - pcz;
-      switch (k_) {
-        case 0: return pa.x + ((pj0.x * dx + pj0.y * dy) + pj0.z * dz);
-        case 1: return pa.y + ((pj0.y * dx + pj0.w * dy) + pj1.x * dz);
-        case 2: return pa.z + ((pj0.z * dx + pj1.x * dy) + pj1.y * dz);
-        case 3: return pa.w - ((pa.x * dx + pa.y * dy) + pa.z * dz);
-        case 4: return pj0.x;
-        case 5: return pj0.y;
-        case 6: return pj0.z;
-        case 7: return pj0.w;
-        case 8: return pj1.x;
-        default: return pj1.y;
-      }
-    };
-    ch0.reset(ch0.value() + inherit(ct0, ck0));
-    if (two) ch1.reset(ch1.value() + inherit(ct1, ck1));
-
-    // Near field of leaf targets (near_all, src/rainbow_fmm.wgsl:533-571):
-    // the 27 offsets dz, dy, dx in four blocks of 8, the centre skipped;
-    // neighbours outside the level were staged as empty.
-    const bool leaf = tcount != 0 && (tcount <= kLeafCapacity || level == d.depth);
-    if (__syncthreads_or(leaf)) {
-      const float near_epsilon =
-          fmaxf(d.softening, (0.72f * kWorldHalfExtent) / static_cast<float>(dim));
-      const float eps2_near = near_epsilon * near_epsilon;
-      for (uint32_t nb = 0; nb < 4u; ++nb) {
-        const uint32_t idx = nb * 8u + cc;
-        float t[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t r0 = 0; r0 < chunks; r0 += kWaves) {
+      // Each wave evaluates one chunk's 8 x 8 pair fields.
+      const uint32_t q = r0 + wave;
+      float t[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+      if (q < m2l_chunks) {
+        const int ox = ox0 + static_cast<int>(q % static_cast<uint32_t>(nx));
+        const int oy = oy0 + static_cast<int>((q / static_cast<uint32_t>(nx)) % static_cast<uint32_t>(ny));
+        const int oz = oz0 + static_cast<int>(q / static_cast<uint32_t>(nx * ny));
+        const uint32_t wsx = static_cast<uint32_t>(2 * (ox + 1)) + (cc & 1u);
+        const uint32_t wsy = static_cast<uint32_t>(2 * (oy + 1)) + ((cc >> 1) & 1u);
+        const uint32_t wsz = static_cast<uint32_t>(2 * (oz + 1)) + (cc >> 2);
+        const bool near = (wsx + 1u - wtx) <= 2u && (wsy + 1u - wty) <= 2u && (wsz + 1u - wtz) <= 2u;
+        if (tcount != 0 && !near) {
+          const float4 sr = win[wsx + 6u * (wsy + 6u * wsz)];
+          if (sr.w != 0.0f) pair_terms(sr, tcx, tcy, tcz, eps2_m2l, t);
+        }
+      } else if (q < chunks) {
+        // Near field of leaf targets (near_all, src/rainbow_fmm.wgsl:533-571):
+        // offsets dz, dy, dx; the centre is skipped; neighbours outside the
+        // level were staged as empty.
+        const uint32_t idx = (q - m2l_chunks) * 8u + cc;
         if (leaf && idx < 27u && idx != 13u) {
           const uint32_t wx = wtx + idx % 3u - 1u;
           const uint32_t wy = wty + (idx / 3u) % 3u - 1u;
@@ -712,34 +705,38 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t leve
           const float4 sr = win[wx + 6u * (wy + 6u * wz)];
           if (sr.w != 0.0f) pair_terms(sr, tcx, tcy, tcz, eps2_near, t);
         }
-#pragma unroll
-        for (int k = 0; k < 10; ++k) terms[b][k][lane] = t[k];
-        __syncthreads();
-#pragma unroll
-        for (uint32_t q = 0; q < 8u; ++q) ch0.add(terms[b][ck0][ct0 * 8u + q]);
-        if (two) {
-#pragma unroll
-          for (uint32_t q = 0; q < 8u; ++q) ch1.add(terms[b][ck1][ct1 * 8u + q]);
-        }
-        b ^= 1u;
       }
+#pragma unroll
+      for (int k = 0; k < 10; ++k) terms[wave][k][lane] = t[k];
+      __syncthreads();
+      // The chains add the chunks of this round in order; the L2L term goes
+      // in between the last M2L chunk and the first near chunk.
+      if (chain) {
+        for (uint32_t w = 0; w < kWaves; ++w) {
+          const uint32_t qq = r0 + w;
+          if (qq >= chunks) break;
+          if (qq == m2l_chunks) ch.reset(ch.value() + inh);
+#pragma unroll
+          for (uint32_t c = 0; c < 8u; ++c) ch.add(terms[w][ck][ct * 8u + c]);
+        }
+      }
+      __syncthreads();
     }
+    if (chain && chunks == m2l_chunks) ch.reset(ch.value() + inh);
 
     // Store each chain's component of its (non-empty) target's field.
-    auto store = [&](uint32_t t_, uint32_t k_, float v) {
-      const uint32_t x = 2u * px + (t_ & 1u), y = 2u * py + ((t_ >> 1) & 1u), z = 2u * pz + (t_ >> 2);
+    if (chain) {
+      const uint32_t x = 2u * px + (ct & 1u), y = 2u * py + ((ct >> 1) & 1u), z = 2u * pz + (ct >> 2);
       const uint32_t g = off + lin(x, y, z, dim);
-      if (counts[4u * g] == 0) return;
-      float* f = reinterpret_cast<float*>(d.fields + 3u * g);
-      f[k_] = v;
-      if (k_ == 9u) {
-        f[10] = 0.0f;
-        f[11] = 0.0f;
+      if (counts[4u * g] != 0) {
+        float* f = reinterpret_cast<float*>(d.fields + 3u * g);
+        f[ck] = ch.value();
+        if (ck == 9u) {
+          f[10] = 0.0f;
+          f[11] = 0.0f;
+        }
       }
-    };
-    store(ct0, ck0, ch0.value());
-    if (two) store(ct1, ck1, ch1.value());
-    __syncthreads();
+    }
   }
 }
 
