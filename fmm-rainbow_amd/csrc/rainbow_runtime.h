@@ -35,8 +35,9 @@ const char* last_error();
 
 // Steps captured in the multi-step graph.
 constexpr uint32_t kGraphSteps = 16;
-// Bodies are re-sorted into Morton order before every this many steps.
-constexpr uint32_t kSortInterval = 16;
+// Bodies are re-sorted into Morton order before every this many steps by
+// default (tunable per engine; 0 = never).
+constexpr uint32_t kDefaultSortInterval = 8;
 
 // One device allocation holding a system's bodies and cells, and its
 // descriptor pointing into it.
@@ -87,6 +88,9 @@ class Engine {
   int accelerations(uint32_t sys, void* dst, size_t bytes);
   int diagnostics(uint32_t sys, double out[8]);
   int sort_bodies();
+  int set_sort_interval(uint32_t interval);
+  int autotune(uint32_t steps, uint32_t* chosen);
+  int snapshot(bool restore);
 
   hipStream_t stream = nullptr;
   uint32_t options = 0;
@@ -102,11 +106,14 @@ class Engine {
 
  private:
   int capture(uint32_t steps, hipGraphExec_t* out);
+  int prepare_graphs(uint32_t steps);
+  uint32_t chunk_steps() const;
   void drop_graphs();
 
   DevBuf sys_buf, clear_buf, deposit_buf, integrate_buf, subtree_buf, list_buf, count_buf,
-      stripe_buf, diag_buf, key_buf, body_buf, sort_tmp;
-  uint32_t since_sort = kSortInterval;
+      stripe_buf, diag_buf, key_buf, body_buf, sort_tmp, snap_buf;
+  uint32_t sort_interval = kDefaultSortInterval;
+  uint32_t since_sort = kDefaultSortInterval;
   hipGraphExec_t graph_one = nullptr;
   hipGraphExec_t graph_many = nullptr;
   hipEvent_t ev_a = nullptr;

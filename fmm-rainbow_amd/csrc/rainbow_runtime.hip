@@ -176,7 +176,7 @@ Engine::~Engine() {
   if (stream) (void)hipStreamSynchronize(stream);
   drop_graphs();
   for (DevBuf* b : {&sys_buf, &clear_buf, &deposit_buf, &integrate_buf, &subtree_buf, &list_buf,
-                    &count_buf, &stripe_buf, &diag_buf, &key_buf, &body_buf, &sort_tmp})
+                    &count_buf, &stripe_buf, &diag_buf, &key_buf, &body_buf, &sort_tmp, &snap_buf})
     b->release();
   if (ev_a) (void)hipEventDestroy(ev_a);
   if (ev_b) (void)hipEventDestroy(ev_b);
@@ -283,7 +283,7 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
                         stream));
   RB_HIP(hipMemsetAsync(count_buf.p, 0, sizeof(uint32_t) * kLevels * kStripes, stream));
   RB_HIP(hipStreamSynchronize(stream));
-  since_sort = kSortInterval;
+  since_sort = sort_interval;
   return RAINBOW_OK;
 }
 
@@ -351,31 +351,104 @@ int Engine::capture(uint32_t steps, hipGraphExec_t* out) {
   return RAINBOW_OK;
 }
 
+// Steps per captured multi-step graph: the sort interval, at most
+// kGraphSteps (kGraphSteps when bodies are never re-sorted).
+uint32_t Engine::chunk_steps() const {
+  return sort_interval == 0u ? kGraphSteps : std::min(sort_interval, kGraphSteps);
+}
+
+int Engine::prepare_graphs(uint32_t steps) {
+  int rc;
+  const uint32_t chunk = chunk_steps();
+  if (steps >= chunk && graph_many == nullptr && (rc = capture(chunk, &graph_many)) != 0) return rc;
+  if (graph_one == nullptr && (rc = capture(1u, &graph_one)) != 0) return rc;
+  return RAINBOW_OK;
+}
+
 int Engine::step(uint32_t steps) {
   if (sys.empty() || steps == 0u) return RAINBOW_OK;
   int rc;
-  if (steps >= kGraphSteps && graph_many == nullptr && (rc = capture(kGraphSteps, &graph_many)) != 0)
-    return rc;
-  if (steps % kGraphSteps != 0u && graph_one == nullptr && (rc = capture(1u, &graph_one)) != 0) return rc;
-  // Bodies are re-sorted before every kSortInterval steps; the graphs hold
+  if ((rc = prepare_graphs(steps)) != 0) return rc;
+  // Bodies are re-sorted before every sort_interval steps; the graphs hold
   // the step kernels only.
+  const uint32_t chunk = chunk_steps();
   uint32_t done = 0;
   while (done < steps) {
-    if (since_sort >= kSortInterval) {
+    if (sort_interval != 0u && since_sort >= sort_interval) {
       if ((rc = sort_bodies()) != 0) return rc;
       since_sort = 0;
     }
-    if (steps - done >= kGraphSteps && since_sort == 0u) {
+    const bool aligned = sort_interval == 0u || since_sort % chunk == 0u;
+    if (steps - done >= chunk && aligned && graph_many != nullptr) {
       RB_HIP(hipGraphLaunch(graph_many, stream));
-      done += kGraphSteps;
-      since_sort += kGraphSteps;
+      done += chunk;
+      since_sort += chunk;
     } else {
-      if (graph_one == nullptr && (rc = capture(1u, &graph_one)) != 0) return rc;
       RB_HIP(hipGraphLaunch(graph_one, stream));
       done += 1u;
       since_sort += 1u;
     }
   }
+  return RAINBOW_OK;
+}
+
+int Engine::set_sort_interval(uint32_t interval) {
+  if (interval > 1024u) return fail(RAINBOW_E_INVALID, "sort interval must be <= 1024");
+  RB_HIP(hipStreamSynchronize(stream));
+  if (interval != sort_interval) drop_graphs();
+  sort_interval = interval;
+  since_sort = interval;  // sort before the next step
+  return RAINBOW_OK;
+}
+
+// Copies every system's bodies (pos, vel, perm) to a device snapshot, or
+// back from it.
+int Engine::snapshot(bool restore) {
+  if ((total_bodies * 36ull) > snap_buf.cap && !restore) {
+    const int rc = snap_buf.ensure(36ull * total_bodies);
+    if (rc) return rc;
+  }
+  char* p = static_cast<char*>(snap_buf.p);
+  for (const SysDesc& d : sys) {
+    const size_t b16 = 16ull * d.n, b4 = 4ull * d.n;
+    char* sp = p + 36ull * d.body_base;
+    if (restore) {
+      RB_HIP(hipMemcpyAsync(d.pos, sp, b16, hipMemcpyDeviceToDevice, stream));
+      RB_HIP(hipMemcpyAsync(d.vel, sp + b16, b16, hipMemcpyDeviceToDevice, stream));
+      RB_HIP(hipMemcpyAsync(d.perm, sp + 2 * b16, b4, hipMemcpyDeviceToDevice, stream));
+    } else {
+      RB_HIP(hipMemcpyAsync(sp, d.pos, b16, hipMemcpyDeviceToDevice, stream));
+      RB_HIP(hipMemcpyAsync(sp + b16, d.vel, b16, hipMemcpyDeviceToDevice, stream));
+      RB_HIP(hipMemcpyAsync(sp + 2 * b16, d.perm, b4, hipMemcpyDeviceToDevice, stream));
+    }
+  }
+  RB_HIP(hipStreamSynchronize(stream));
+  return RAINBOW_OK;
+}
+
+// Times `steps` steps at each candidate sort interval from the same
+// starting state and keeps the fastest; the bodies are restored afterwards.
+int Engine::autotune(uint32_t steps, uint32_t* chosen) {
+  static const uint32_t kCandidates[] = {2u, 4u, 8u, 16u, 32u, 0u};
+  if (steps == 0u) return fail(RAINBOW_E_INVALID, "steps must be >= 1");
+  int rc = snapshot(false);
+  if (rc) return rc;
+  float best = 0.0f;
+  uint32_t best_interval = sort_interval;
+  for (uint32_t c : kCandidates) {
+    if ((rc = set_sort_interval(c)) != 0) return rc;
+    float ms = 0.0f;
+    if ((rc = time_steps(std::min<uint32_t>(steps, 4u), &ms)) != 0) return rc;  // warm graphs
+    if ((rc = snapshot(true)) != 0) return rc;
+    if ((rc = time_steps(steps, &ms)) != 0) return rc;
+    if ((rc = snapshot(true)) != 0) return rc;
+    if (best == 0.0f || ms < best) {
+      best = ms;
+      best_interval = c;
+    }
+  }
+  if ((rc = set_sort_interval(best_interval)) != 0) return rc;
+  if (chosen) *chosen = best_interval;
   return RAINBOW_OK;
 }
 
@@ -389,9 +462,7 @@ int Engine::sync() {
 int Engine::time_steps(uint32_t steps, float* ms) {
   int rc;
   // Graphs are captured before the timed region.
-  if (steps >= kGraphSteps && graph_many == nullptr && (rc = capture(kGraphSteps, &graph_many)) != 0)
-    return rc;
-  if (steps % kGraphSteps != 0u && graph_one == nullptr && (rc = capture(1u, &graph_one)) != 0) return rc;
+  if ((rc = prepare_graphs(steps)) != 0) return rc;
   RB_HIP(hipStreamSynchronize(stream));
   RB_HIP(hipEventRecord(ev_a, stream));
   if ((rc = step(steps)) != 0) return rc;
@@ -507,7 +578,7 @@ int download_system(const SysDesc& d, int which, void* dst, size_t bytes, hipStr
 int Engine::upload(uint32_t s, const void* data, size_t bytes) {
   if (s >= sys.size()) return fail(RAINBOW_E_INVALID, "system %u out of range", s);
   RB_HIP(hipStreamSynchronize(stream));
-  since_sort = kSortInterval;
+  since_sort = sort_interval;
   return upload_system(sys[s], data, bytes, stream);
 }
 
@@ -692,6 +763,16 @@ int rainbow_hip_phase_ms(rainbow_hip_t* h, uint32_t steps, float* out, uint32_t 
   return h->batch.engine.phase_ms(steps, out, n);
 }
 
+int rainbow_hip_set_sort_interval(rainbow_hip_t* h, uint32_t interval) {
+  RB_CHECK_HANDLE(h);
+  return h->batch.engine.set_sort_interval(interval);
+}
+
+int rainbow_hip_autotune(rainbow_hip_t* h, uint32_t steps, uint32_t* interval) {
+  RB_CHECK_HANDLE(h);
+  return h->batch.engine.autotune(steps, interval);
+}
+
 int rainbow_hip_tree_stats(rainbow_hip_t* h, uint32_t out[8]) {
   RB_CHECK_HANDLE(h);
   if (out == nullptr) return fail(RAINBOW_E_INVALID, "null output");
@@ -783,5 +864,15 @@ int rainbow_batch_synchronize(rainbow_batch_t* b) {
 }
 
 uint64_t rainbow_batch_particles(rainbow_batch_t* b) { return b ? b->batch.engine.total_bodies : 0u; }
+
+int rainbow_batch_set_sort_interval(rainbow_batch_t* b, uint32_t interval) {
+  RB_CHECK_HANDLE(b);
+  return b->batch.engine.set_sort_interval(interval);
+}
+
+int rainbow_batch_autotune(rainbow_batch_t* b, uint32_t steps, uint32_t* interval) {
+  RB_CHECK_HANDLE(b);
+  return b->batch.engine.autotune(steps, interval);
+}
 
 }  // extern "C"

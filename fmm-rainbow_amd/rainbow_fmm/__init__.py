@@ -131,6 +131,10 @@ _SIGNATURES = {
     "rainbow_hip_time_steps": (_i32, [_vp, _u32, _P(_f32)]),
     "rainbow_hip_phase_ms": (_i32, [_vp, _u32, _P(_f32), _u32]),
     "rainbow_hip_tree_stats": (_i32, [_vp, _P(_u32)]),
+    "rainbow_hip_set_sort_interval": (_i32, [_vp, _u32]),
+    "rainbow_hip_autotune": (_i32, [_vp, _u32, _P(_u32)]),
+    "rainbow_batch_set_sort_interval": (_i32, [_vp, _u32]),
+    "rainbow_batch_autotune": (_i32, [_vp, _u32, _P(_u32)]),
     "rainbow_hip_diagnostics": (_i32, [_vp, _P(_f64)]),
     "rainbow_hip_accelerations": (_i32, [_vp, _vp, _sz]),
     "rainbow_last_error": (ctypes.c_char_p, []),
@@ -399,6 +403,14 @@ class Solver:
         n = _check(lib().rainbow_hip_phase_ms(self._h, steps, buf, 16))
         return [buf[i] for i in range(n)]
 
+    def set_sort_interval(self, interval: int):
+        _check(lib().rainbow_hip_set_sort_interval(self._h, interval))
+
+    def autotune(self, steps: int = 32) -> int:
+        v = _u32()
+        _check(lib().rainbow_hip_autotune(self._h, steps, ctypes.byref(v)))
+        return v.value
+
     def tree_stats(self) -> list[int]:
         buf = (_u32 * 8)()
         _check(lib().rainbow_hip_tree_stats(self._h, buf))
@@ -466,6 +478,14 @@ class Batch:
         ms = _f32()
         _check(lib().rainbow_batch_time_steps(self._h, steps, ctypes.byref(ms)))
         return ms.value
+
+    def set_sort_interval(self, interval: int):
+        _check(lib().rainbow_batch_set_sort_interval(self._h, interval))
+
+    def autotune(self, steps: int = 32) -> int:
+        v = _u32()
+        _check(lib().rainbow_batch_autotune(self._h, steps, ctypes.byref(v)))
+        return v.value
 
     @property
     def particles_total(self) -> int:

@@ -236,6 +236,16 @@ int rainbow_hip_time_steps(rainbow_hip_t* h, uint32_t steps, float* ms);
 int rainbow_hip_phase_ms(rainbow_hip_t* h, uint32_t steps, float* out,
                          uint32_t n);
 
+/* Bodies are stored in Morton order of their finest cell and re-sorted
+ * before every `interval` steps (0 = never; default 8).  The order is
+ * internal: downloads return caller order and no result depends on it. */
+int rainbow_hip_set_sort_interval(rainbow_hip_t* h, uint32_t interval);
+
+/* Autotuner: times `steps` steps at each candidate sort interval (2, 4, 8,
+ * 16, 32, never) from the same starting state, keeps the fastest and writes
+ * it to *interval.  The bodies are restored afterwards. */
+int rainbow_hip_autotune(rainbow_hip_t* h, uint32_t steps, uint32_t* interval);
+
 /* Number of cells active in the last solve at each level (M2L targets:
  * non-empty cells whose parent holds more than 16 bodies), plus the
  * number of near-field leaves.  out[l] for l = 0..6, out[7] = leaves. */
@@ -281,6 +291,9 @@ int rainbow_batch_time_steps(rainbow_batch_t* b, uint32_t steps, float* ms);
 int rainbow_batch_synchronize(rainbow_batch_t* b);
 /* Total bodies over all systems. */
 uint64_t rainbow_batch_particles(rainbow_batch_t* b);
+/* As rainbow_hip_set_sort_interval / rainbow_hip_autotune, for the batch. */
+int rainbow_batch_set_sort_interval(rainbow_batch_t* b, uint32_t interval);
+int rainbow_batch_autotune(rainbow_batch_t* b, uint32_t steps, uint32_t* interval);
 
 /* ------------------------------------------------------------------------ */
 /* Part 5. Serving: continuous batching of N-body solve requests.           */

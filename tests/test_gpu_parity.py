@@ -337,3 +337,34 @@ def test_diagnostics_fp64():
     assert abs(d[1] - pe) <= 1e-12 * abs(pe)
     np.testing.assert_allclose(d[2:], cons[2:], rtol=0, atol=1e-14)
     s.close()
+
+
+@pytest.mark.parametrize("interval", [0, 1, 3, 32])
+def test_sort_interval_does_not_change_results(interval):
+    say(f"sort interval {interval}")
+    n = 150000
+    parts = rf.ic_plummer(n, 91, 0.1, 1.5, 1.0)
+    dt, g, soft = 5e-4, 1.0, 1e-3
+    s = rf.Solver(n, rf.physics(dt=dt, gravity=g, softening=soft))
+    s.set_sort_interval(interval)
+    s.upload(parts)
+    s.step(21)
+    op, om, of = orc.step(slots_for(n, dt, g, soft), parts, 21)
+    assert_bits(s.particles(), op, "particles")
+    assert_bits(s.moments(), om, "moments")
+    assert_bits(s.fields(), of, "fields")
+    s.close()
+
+
+def test_autotune_restores_state():
+    say("autotune")
+    n = 200000
+    parts = rf.ic_plummer(n, 92, 0.1, 1.5, 1.0)
+    s = rf.Solver(n, rf.physics(dt=5e-4, gravity=1.0, softening=1e-3))
+    s.upload(parts)
+    s.step(5)
+    before = s.particles()
+    chosen = s.autotune(8)
+    assert chosen in (0, 2, 4, 8, 16, 32)
+    assert_bits(s.particles(), before, "bodies after autotune")
+    s.close()
