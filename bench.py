@@ -29,7 +29,7 @@ FP32_PEAK_TFLOPS = 157.3
 
 
 def phase_names(depth):
-    return ["clear", "deposit", "upward", "top"] + [f"tree_l{l}" for l in range(1, depth + 1)] + ["integrate"]
+    return ["clear", "deposit", "upward", "top"] + [f"tree_l{l}" for l in range(2, depth + 1)] + ["integrate"]
 
 
 def algorithmic_bytes(n, depth, stats):
@@ -54,7 +54,7 @@ def algorithmic_bytes(n, depth, stats):
     out = {"clear": 16 * cells, "deposit": 16 * n + 16 * cd,
            "upward": 16 * (cells - top) + 16 * (cells - cd - top + 8 ** (depth - 3)),
            "top": 16 * (8 ** (depth - 3)) + 16 * (top - 8 ** (depth - 3))}
-    for l in range(1, depth + 1):
+    for l in range(2, depth + 1):
         t = stats[l]
         # P_l >= ceil(T_l / 8): each parent has at most 8 targets.
         out[f"tree_l{l}"] = 16 * t + 48 * t + 48 * ((t + 7) // 8)

@@ -11,10 +11,11 @@
 //              in LDS, writes each cell's source record (centre of mass and
 //              G m), lists the internal cells (count > 16) and zeroes the
 //              fields of cells that stopped being M2L targets
-//   top        the levels above the subtrees, one block per system
-//   tree(l)    for l = 1..depth: M2L, L2L and the near field of every M2L
+//   top        the levels above the subtrees and the fields of level 1,
+//              one block per system
+//   tree(l)    for l = 2..depth: M2L, L2L and the near field of every M2L
 //              target of level l (a non-empty child of an internal cell),
-//              one 64-lane block per internal parent
+//              four waves per internal parent
 //   integrate  adaptive descent, local expansion, own-leaf monopole,
 //              symplectic Euler kick-drift and the soft absorbing boundary
 //
@@ -266,6 +267,57 @@ __device__ __forceinline__ int4 sum8(const int4* src, uint32_t cx, uint32_t cy, 
                    static_cast<int>(sz));
 }
 
+// One pair's ten field terms (pair_field, src/rainbow_fmm.wgsl:170-203).
+// The source record {com, G m} holds com = f32(sum) * (1 / (scale
+// f32(count))) and gm = G (f32(count) / f32(N)), formed once per cell with
+// the same operations the reference repeats per pair.
+__device__ __forceinline__ void pair_terms(const float4 src, float tx, float ty, float tz, float eps2,
+                                           float out[10]) {
+  const float rx = src.x - tx;
+  const float ry = src.y - ty;
+  const float rz = src.z - tz;
+  const float r2 = ((rx * rx + ry * ry) + rz * rz) + eps2;
+  const float inv_r = 1.0f / sqrtf(r2);
+  const float inv_r3 = inv_r * inv_r * inv_r;
+  const float inv_r5 = inv_r3 * inv_r * inv_r;
+  const float gm = src.w;
+  const float diagonal = -gm * inv_r3;
+  const float tidal = 3.0f * gm * inv_r5;
+  out[0] = gm * rx * inv_r3;
+  out[1] = gm * ry * inv_r3;
+  out[2] = gm * rz * inv_r3;
+  out[3] = -gm * inv_r;
+  out[4] = diagonal + tidal * rx * rx;
+  out[5] = tidal * rx * ry;
+  out[6] = tidal * rx * rz;
+  out[7] = diagonal + tidal * ry * ry;
+  out[8] = tidal * ry * rz;
+  out[9] = diagonal + tidal * rz * rz;
+}
+
+// One running sum: plain f32, or Neumaier-compensated (the precision form).
+template <bool kComp>
+struct Chain {
+  float s = 0.0f, c = 0.0f;
+  __device__ __forceinline__ void add(float t) {
+    if (!kComp) {
+      s = s + t;
+    } else {
+      const float u = s + t;
+      if (fabsf(s) >= fabsf(t))
+        c = c + ((s - u) + t);
+      else
+        c = c + ((t - u) + s);
+      s = u;
+    }
+  }
+  __device__ __forceinline__ float value() const { return kComp ? s + c : s; }
+  __device__ __forceinline__ void reset(float x) {
+    s = x;
+    c = 0.0f;
+  }
+};
+
 // Source record of a cell: {com, G m}, zero when empty
 // (moment_com_read and the mass/gm lines of pair_field,
 // src/rainbow_fmm.wgsl:163-168, 182-183).
@@ -437,8 +489,9 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
 }
 
 // The levels above the subtrees, one block per system: levels r-1..0 from
-// the 8^r subtree roots, their source records and lists, and the target
-// flags of levels 1..r.
+// the 8^r subtree roots, their source records and lists, the target flags
+// of levels 1..r, and the fields of level 1.
+template <bool kComp>
 __global__ __launch_bounds__(kUpwardThreads) void k_top(StepArgs a) {
   __shared__ int4 lv[512 + 64 + 8 + 1];
   __shared__ PushStage ps;
@@ -507,6 +560,41 @@ __global__ __launch_bounds__(kUpwardThreads) void k_top(StepArgs a) {
   }
   __syncthreads();
   for (uint32_t q = 0; q < nmine; ++q) write_item(a, levs[q], stripe, ps.base[levs[q]] + slots[q], items[q]);
+
+  // Level 1 (tree pass of level 1, src/rainbow_fmm.wgsl:457-611): the root's
+  // children have no well-separated source (every level-1 cell is a
+  // neighbour) and inherit the root's zero field, so a level-1 target's
+  // field is +0 plus, for a leaf, its near field: the other level-1 cells
+  // in dz, dy, dx order.
+  if (t < 8u) {
+    const int4* l1 = buf(r - 1u);
+    const int root = buf(r)[0].x;
+    const int count = l1[t].x;
+    if (count != 0 && root > kLeafCapacity) {
+      const uint32_t tx = t & 1u, ty = (t >> 1) & 1u, tz = t >> 2;
+      const float cx = center1(tx, 2u), cy = center1(ty, 2u), cz = center1(tz, 2u);
+      Chain<kComp> ch[10];
+      if (count <= kLeafCapacity) {  // level 1 is never the finest (depth >= 4)
+        const float near_epsilon = fmaxf(d.softening, (0.72f * kWorldHalfExtent) / 2.0f);
+        const float eps2 = near_epsilon * near_epsilon;
+        for (uint32_t zz = 0; zz < 2u; ++zz)
+          for (uint32_t yy = 0; yy < 2u; ++yy)
+            for (uint32_t xx = 0; xx < 2u; ++xx) {
+              if (xx == tx && yy == ty && zz == tz) continue;
+              const float4 sr = source_record(l1[lin(xx, yy, zz, 2u)], d);
+              if (sr.w == 0.0f) continue;
+              float v[10];
+              pair_terms(sr, cx, cy, cz, eps2, v);
+#pragma unroll
+              for (int k = 0; k < 10; ++k) ch[k].add(v[k]);
+            }
+      }
+      float4* f = d.fields + 3u * (level_off(1u) + t);
+      f[0] = make_float4(ch[0].value(), ch[1].value(), ch[2].value(), ch[3].value());
+      f[1] = make_float4(ch[4].value(), ch[5].value(), ch[6].value(), ch[7].value());
+      f[2] = make_float4(ch[8].value(), ch[9].value(), 0.0f, 0.0f);
+    }
+  }
 }
 
 // ------------------------------------------------------------------------
@@ -518,57 +606,6 @@ __global__ __launch_bounds__(kUpwardThreads) void k_top(StepArgs a) {
 // in parallel, then one lane per (target, component) chain adds the 8
 // terms in the reference order.
 // ------------------------------------------------------------------------
-
-// One pair's ten field terms (pair_field, src/rainbow_fmm.wgsl:170-203).
-// The source record {com, G m} holds com = f32(sum) * (1 / (scale
-// f32(count))) and gm = G (f32(count) / f32(N)), formed once per cell with
-// the same operations the reference repeats per pair.
-__device__ __forceinline__ void pair_terms(const float4 src, float tx, float ty, float tz, float eps2,
-                                           float out[10]) {
-  const float rx = src.x - tx;
-  const float ry = src.y - ty;
-  const float rz = src.z - tz;
-  const float r2 = ((rx * rx + ry * ry) + rz * rz) + eps2;
-  const float inv_r = 1.0f / sqrtf(r2);
-  const float inv_r3 = inv_r * inv_r * inv_r;
-  const float inv_r5 = inv_r3 * inv_r * inv_r;
-  const float gm = src.w;
-  const float diagonal = -gm * inv_r3;
-  const float tidal = 3.0f * gm * inv_r5;
-  out[0] = gm * rx * inv_r3;
-  out[1] = gm * ry * inv_r3;
-  out[2] = gm * rz * inv_r3;
-  out[3] = -gm * inv_r;
-  out[4] = diagonal + tidal * rx * rx;
-  out[5] = tidal * rx * ry;
-  out[6] = tidal * rx * rz;
-  out[7] = diagonal + tidal * ry * ry;
-  out[8] = tidal * ry * rz;
-  out[9] = diagonal + tidal * rz * rz;
-}
-
-// One running sum: plain f32, or Neumaier-compensated (the precision form).
-template <bool kComp>
-struct Chain {
-  float s = 0.0f, c = 0.0f;
-  __device__ __forceinline__ void add(float t) {
-    if (!kComp) {
-      s = s + t;
-    } else {
-      const float u = s + t;
-      if (fabsf(s) >= fabsf(t))
-        c = c + ((s - u) + t);
-      else
-        c = c + ((t - u) + s);
-      s = u;
-    }
-  }
-  __device__ __forceinline__ float value() const { return kComp ? s + c : s; }
-  __device__ __forceinline__ void reset(float x) {
-    s = x;
-    c = 0.0f;
-  }
-};
 
 template <bool kComp>
 __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t level) {
@@ -1035,7 +1072,10 @@ hipError_t launch_upward(const StepArgs& a, uint32_t nblocks, hipStream_t stream
 
 hipError_t launch_top(const StepArgs& a, hipStream_t stream) {
   if (a.nsys == 0u) return hipSuccess;
-  hipLaunchKernelGGL(k_top, dim3(a.nsys), dim3(kUpwardThreads), 0, stream, a);
+  if (a.compensated)
+    hipLaunchKernelGGL(k_top<true>, dim3(a.nsys), dim3(kUpwardThreads), 0, stream, a);
+  else
+    hipLaunchKernelGGL(k_top<false>, dim3(a.nsys), dim3(kUpwardThreads), 0, stream, a);
   return hipGetLastError();
 }
 

@@ -329,7 +329,7 @@ int Engine::enqueue(bool integrate) {
   RB_HIP(launch_deposit(args, n_deposit, stream));
   RB_HIP(launch_upward(args, n_subtree, stream));
   RB_HIP(launch_top(args, stream));
-  for (uint32_t l = 1; l <= dmax; ++l) RB_HIP(launch_tree(args, l, tree_blocks[l], stream));
+  for (uint32_t l = 2; l <= dmax; ++l) RB_HIP(launch_tree(args, l, tree_blocks[l], stream));
   if (integrate) RB_HIP(launch_integrate(args, n_integrate, stream));
   return RAINBOW_OK;
 }
@@ -473,7 +473,7 @@ int Engine::time_steps(uint32_t steps, float* ms) {
 }
 
 int Engine::phase_ms(uint32_t steps, float* out, uint32_t n) {
-  const uint32_t nphase = 5u + dmax;  // clear, deposit, upward, top, tree 1..dmax, integrate
+  const uint32_t nphase = 4u + dmax;  // clear, deposit, upward, top, tree 2..dmax, integrate
   if (out == nullptr || n < nphase) return fail(RAINBOW_E_INVALID, "phase_ms needs %u entries", nphase);
   if (steps == 0u) return fail(RAINBOW_E_INVALID, "steps must be >= 1");
   std::vector<hipEvent_t> ev(nphase + 1u, nullptr);
@@ -494,7 +494,7 @@ int Engine::phase_ms(uint32_t steps, float* out, uint32_t n) {
     mark();
     if (err == hipSuccess) err = launch_top(args, stream);
     mark();
-    for (uint32_t l = 1; l <= dmax; ++l) {
+    for (uint32_t l = 2; l <= dmax; ++l) {
       if (err == hipSuccess) err = launch_tree(args, l, tree_blocks[l], stream);
       mark();
     }

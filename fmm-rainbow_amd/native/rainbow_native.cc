@@ -80,7 +80,7 @@ int main(int argc, char** argv) {
     if (i < 4)
       std::snprintf(name, sizeof(name), "%s", fixed[i]);
     else if (i < nphase - 1)
-      std::snprintf(name, sizeof(name), "tree_l%d", i - 3);
+      std::snprintf(name, sizeof(name), "tree_l%d", i - 2);
     else
       std::snprintf(name, sizeof(name), "integrate");
     std::printf("  ts %-10s %.4f ms\n", name, static_cast<double>(phase[i]));

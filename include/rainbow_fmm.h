@@ -229,10 +229,10 @@ int rainbow_hip_time_steps(rainbow_hip_t* h, uint32_t steps, float* ms);
 /* Runs `steps` steps without the graph, with a HIP event pair around every
  * kernel launch, and writes the mean duration in ms of each phase:
  * out[0] clear, out[1] deposit, out[2] upward (subtrees), out[3] upward
- * (top levels), out[4 + l - 1] the tree pass of level l (M2L + L2L + near,
- * l = 1..depth), out[4 + depth] integrate.  `n` is the capacity of out (at
- * least 5 + depth).  Returns the number of entries written (>= 0) or a
- * negative status. */
+ * (top levels, with the level-1 fields), out[4 + l - 2] the tree pass of
+ * level l (M2L + L2L + near, l = 2..depth), out[3 + depth] integrate.  `n`
+ * is the capacity of out (at least 4 + depth).  Returns the number of
+ * entries written (>= 0) or a negative status. */
 int rainbow_hip_phase_ms(rainbow_hip_t* h, uint32_t steps, float* out,
                          uint32_t n);
 
