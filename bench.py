@@ -29,7 +29,8 @@ FP32_PEAK_TFLOPS = 157.3
 
 
 def phase_names(depth):
-    return ["clear", "deposit", "upward", "top"] + [f"tree_l{l}" for l in range(2, depth + 1)] + ["integrate"]
+    return (["clear", "deposit", "upward", "top"] + [f"tree_l{l}" for l in range(2, depth + 1)]
+            + ["integrate", "sort"])
 
 
 def algorithmic_bytes(n, depth, stats):
@@ -59,7 +60,33 @@ def algorithmic_bytes(n, depth, stats):
         # P_l >= ceil(T_l / 8): each parent has at most 8 targets.
         out[f"tree_l{l}"] = 16 * t + 48 * t + 48 * ((t + 7) // 8)
     out["integrate"] = 64 * n + 16 * leaves + 48 * leaves
+    # sort: read keys+positions+velocities, write the sorted bodies and perm.
+    out["sort"] = 8 * n + 32 * n + 36 * n
     return out
+
+
+TRAFFIC_TABLE = ROOT / "profiles" / "hbm_traffic_plummer-1m.md"
+
+
+def measured_traffic(phase):
+    """HBM bytes per launch of `phase` from the committed counter summary.
+
+    The summary holds FETCH_SIZE and WRITE_SIZE per dispatch from two
+    `rocprofv3 --pmc` runs of this benchmark, each divided by the factor the
+    same counter reads for a copy kernel of known size; its table rows are
+    `| phase | kernel | fetch KiB | write KiB | bytes |`.  Returns (bytes,
+    source) or (None, None) when the phase is not in the table.
+    """
+    if not TRAFFIC_TABLE.exists():
+        return None, None
+    for line in TRAFFIC_TABLE.read_text().splitlines():
+        cols = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cols) == 5 and cols[0] == phase:
+            try:
+                return int(cols[4]), str(TRAFFIC_TABLE.relative_to(ROOT))
+            except ValueError:
+                return None, None
+    return None, None
 
 
 def cpu_baseline(parts, cfg, budget_s):
@@ -266,8 +293,11 @@ def main():
     names = phase_names(solver.depth)
     per = dict(zip(names, phases))
     bytes_per = algorithmic_bytes(n, solver.depth, stats)
-    dom = max(per, key=per.get)
+    # The dominant kernel: the phase with the largest mean duration (the
+    # sort is several library kernels and is not a candidate).
+    dom = max((k for k in per if k != "sort"), key=per.get)
     achieved = bytes_per[dom] / (per[dom] * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic(dom)
     roofline = {
         "bound": "hbm",
         "kernel": dom,
@@ -275,7 +305,8 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": traffic,
+        "traffic_source": traffic_src,
         "mean_ms": round(per[dom], 5),
         "bytes_per_launch": bytes_per[dom],
     }

@@ -34,6 +34,7 @@
 // adds +0 for them, and a running sum that starts at +0 is never -0, so
 // adding +0 never changes it.
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "rainbow_device.h"
@@ -1052,39 +1053,56 @@ __global__ __launch_bounds__(64) void k_diag_final(const double* partial, uint32
 // launch wrappers
 // ------------------------------------------------------------------------
 
-hipError_t launch_clear(const StepArgs& a, uint32_t nblocks, hipStream_t stream) {
-  if (nblocks == 0u) return hipSuccess;
-  hipLaunchKernelGGL(k_clear, dim3(nblocks), dim3(kClearThreads), 0, stream, a);
+// Launches through hipExtLaunchKernelGGL when an event pair is given, so
+// that the events carry the dispatch's own start and end times.
+#define RB_LAUNCH(t, kernel, grid, block, stream, ...)                                                  \
+  do {                                                                                                \
+    if ((t) != nullptr)                                                                               \
+      hipExtLaunchKernelGGL(kernel, grid, block, 0, stream, (t)->start, (t)->stop, 0, __VA_ARGS__);   \
+    else                                                                                              \
+      hipLaunchKernelGGL(kernel, grid, block, 0, stream, __VA_ARGS__);                                \
+  } while (0)
+
+// A launcher with nothing to launch still records its event pair.
+static hipError_t no_launch(hipStream_t stream, const Timing* t) {
+  if (t == nullptr) return hipSuccess;
+  hipError_t e = hipEventRecord(t->start, stream);
+  return e != hipSuccess ? e : hipEventRecord(t->stop, stream);
+}
+
+hipError_t launch_clear(const StepArgs& a, uint32_t nblocks, hipStream_t stream, const Timing* t) {
+  if (nblocks == 0u) return no_launch(stream, t);
+  RB_LAUNCH(t, k_clear, dim3(nblocks), dim3(kClearThreads), stream, a);
   return hipGetLastError();
 }
 
-hipError_t launch_deposit(const StepArgs& a, uint32_t nblocks, hipStream_t stream) {
-  if (nblocks == 0u) return hipSuccess;
-  hipLaunchKernelGGL(k_deposit, dim3(nblocks), dim3(kDepositThreads), 0, stream, a);
+hipError_t launch_deposit(const StepArgs& a, uint32_t nblocks, hipStream_t stream, const Timing* t) {
+  if (nblocks == 0u) return no_launch(stream, t);
+  RB_LAUNCH(t, k_deposit, dim3(nblocks), dim3(kDepositThreads), stream, a);
   return hipGetLastError();
 }
 
-hipError_t launch_upward(const StepArgs& a, uint32_t nblocks, hipStream_t stream) {
-  if (nblocks == 0u) return hipSuccess;
-  hipLaunchKernelGGL(k_upward, dim3(nblocks), dim3(kUpwardThreads), 0, stream, a);
+hipError_t launch_upward(const StepArgs& a, uint32_t nblocks, hipStream_t stream, const Timing* t) {
+  if (nblocks == 0u) return no_launch(stream, t);
+  RB_LAUNCH(t, k_upward, dim3(nblocks), dim3(kUpwardThreads), stream, a);
   return hipGetLastError();
 }
 
-hipError_t launch_top(const StepArgs& a, hipStream_t stream) {
-  if (a.nsys == 0u) return hipSuccess;
+hipError_t launch_top(const StepArgs& a, hipStream_t stream, const Timing* t) {
+  if (a.nsys == 0u) return no_launch(stream, t);
   if (a.compensated)
-    hipLaunchKernelGGL(k_top<true>, dim3(a.nsys), dim3(kUpwardThreads), 0, stream, a);
+    RB_LAUNCH(t, k_top<true>, dim3(a.nsys), dim3(kUpwardThreads), stream, a);
   else
-    hipLaunchKernelGGL(k_top<false>, dim3(a.nsys), dim3(kUpwardThreads), 0, stream, a);
+    RB_LAUNCH(t, k_top<false>, dim3(a.nsys), dim3(kUpwardThreads), stream, a);
   return hipGetLastError();
 }
 
-hipError_t launch_tree(const StepArgs& a, uint32_t level, uint32_t nblocks, hipStream_t stream) {
-  if (nblocks == 0u) return hipSuccess;
+hipError_t launch_tree(const StepArgs& a, uint32_t level, uint32_t nblocks, hipStream_t stream, const Timing* t) {
+  if (nblocks == 0u) return no_launch(stream, t);
   if (a.compensated)
-    hipLaunchKernelGGL(k_tree<true>, dim3(nblocks), dim3(kTreeThreads), 0, stream, a, level);
+    RB_LAUNCH(t, k_tree<true>, dim3(nblocks), dim3(kTreeThreads), stream, a, level);
   else
-    hipLaunchKernelGGL(k_tree<false>, dim3(nblocks), dim3(kTreeThreads), 0, stream, a, level);
+    RB_LAUNCH(t, k_tree<false>, dim3(nblocks), dim3(kTreeThreads), stream, a, level);
   return hipGetLastError();
 }
 
@@ -1105,12 +1123,12 @@ hipError_t launch_reorder(const StepArgs& a, uint32_t nblocks, const uint32_t* v
   return hipGetLastError();
 }
 
-hipError_t launch_integrate(const StepArgs& a, uint32_t nblocks, hipStream_t stream) {
-  if (nblocks == 0u) return hipSuccess;
+hipError_t launch_integrate(const StepArgs& a, uint32_t nblocks, hipStream_t stream, const Timing* t) {
+  if (nblocks == 0u) return no_launch(stream, t);
   if (a.compensated)
-    hipLaunchKernelGGL(k_integrate<true>, dim3(nblocks), dim3(kIntegrateThreads), 0, stream, a);
+    RB_LAUNCH(t, k_integrate<true>, dim3(nblocks), dim3(kIntegrateThreads), stream, a);
   else
-    hipLaunchKernelGGL(k_integrate<false>, dim3(nblocks), dim3(kIntegrateThreads), 0, stream, a);
+    RB_LAUNCH(t, k_integrate<false>, dim3(nblocks), dim3(kIntegrateThreads), stream, a);
   return hipGetLastError();
 }
 

@@ -7,12 +7,20 @@
 
 namespace rainbow {
 
-hipError_t launch_clear(const StepArgs& a, uint32_t nblocks, hipStream_t stream);
-hipError_t launch_deposit(const StepArgs& a, uint32_t nblocks, hipStream_t stream);
-hipError_t launch_upward(const StepArgs& a, uint32_t nblocks, hipStream_t stream);
-hipError_t launch_top(const StepArgs& a, hipStream_t stream);
-hipError_t launch_tree(const StepArgs& a, uint32_t level, uint32_t nblocks, hipStream_t stream);
-hipError_t launch_integrate(const StepArgs& a, uint32_t nblocks, hipStream_t stream);
+// Optional event pair recorded by the kernel dispatch itself (start and end
+// of the kernel's execution, without the gap before it starts).
+struct Timing {
+  hipEvent_t start;
+  hipEvent_t stop;
+};
+
+hipError_t launch_clear(const StepArgs& a, uint32_t nblocks, hipStream_t stream, const Timing* t = nullptr);
+hipError_t launch_deposit(const StepArgs& a, uint32_t nblocks, hipStream_t stream, const Timing* t = nullptr);
+hipError_t launch_upward(const StepArgs& a, uint32_t nblocks, hipStream_t stream, const Timing* t = nullptr);
+hipError_t launch_top(const StepArgs& a, hipStream_t stream, const Timing* t = nullptr);
+hipError_t launch_tree(const StepArgs& a, uint32_t level, uint32_t nblocks, hipStream_t stream,
+                       const Timing* t = nullptr);
+hipError_t launch_integrate(const StepArgs& a, uint32_t nblocks, hipStream_t stream, const Timing* t = nullptr);
 hipError_t launch_sort_keys(const StepArgs& a, uint32_t nblocks, uint32_t* keys, uint32_t* vals,
                             hipStream_t stream);
 hipError_t launch_reorder(const StepArgs& a, uint32_t nblocks, const uint32_t* vals, float4* tpos, float4* tvel,

@@ -473,41 +473,49 @@ int Engine::time_steps(uint32_t steps, float* ms) {
 }
 
 int Engine::phase_ms(uint32_t steps, float* out, uint32_t n) {
-  const uint32_t nphase = 4u + dmax;  // clear, deposit, upward, top, tree 2..dmax, integrate
+  // clear, deposit, upward, top, tree 2..dmax, integrate, then the body
+  // sort (its mean per step: the steps sort when due, as in step()).
+  const uint32_t nkern = 4u + dmax;
+  const uint32_t nphase = nkern + 1u;
   if (out == nullptr || n < nphase) return fail(RAINBOW_E_INVALID, "phase_ms needs %u entries", nphase);
   if (steps == 0u) return fail(RAINBOW_E_INVALID, "steps must be >= 1");
-  std::vector<hipEvent_t> ev(nphase + 1u, nullptr);
+  std::vector<hipEvent_t> ev(2u * nkern, nullptr);
   for (auto& e : ev) RB_HIP(hipEventCreate(&e));
+  std::vector<Timing> tm(nkern);
+  for (uint32_t i = 0; i < nkern; ++i) tm[i] = Timing{ev[2u * i], ev[2u * i + 1u]};
+  hipEvent_t sort_ev[2] = {nullptr, nullptr};
+  for (auto& e : sort_ev) RB_HIP(hipEventCreate(&e));
   std::vector<double> acc(nphase, 0.0);
   hipError_t err = hipSuccess;
   for (uint32_t s = 0; s < steps && err == hipSuccess; ++s) {
-    uint32_t k = 0;
-    auto mark = [&]() {
-      if (err == hipSuccess) err = hipEventRecord(ev[k++], stream);
-    };
-    mark();
-    if (err == hipSuccess) err = launch_clear(args, n_clear, stream);
-    mark();
-    if (err == hipSuccess) err = launch_deposit(args, n_deposit, stream);
-    mark();
-    if (err == hipSuccess) err = launch_upward(args, n_subtree, stream);
-    mark();
-    if (err == hipSuccess) err = launch_top(args, stream);
-    mark();
-    for (uint32_t l = 2; l <= dmax; ++l) {
-      if (err == hipSuccess) err = launch_tree(args, l, tree_blocks[l], stream);
-      mark();
-    }
-    if (err == hipSuccess) err = launch_integrate(args, n_integrate, stream);
-    mark();
-    if (err == hipSuccess) err = hipEventSynchronize(ev[nphase]);
-    for (uint32_t i = 0; i < nphase && err == hipSuccess; ++i) {
+    if (sort_interval != 0u && since_sort >= sort_interval) {
+      err = hipEventRecord(sort_ev[0], stream);
+      if (err == hipSuccess && sort_bodies() != 0) err = hipErrorUnknown;
+      if (err == hipSuccess) err = hipEventRecord(sort_ev[1], stream);
+      if (err == hipSuccess) err = hipEventSynchronize(sort_ev[1]);
       float ms = 0.0f;
-      err = hipEventElapsedTime(&ms, ev[i], ev[i + 1u]);
+      if (err == hipSuccess) err = hipEventElapsedTime(&ms, sort_ev[0], sort_ev[1]);
+      acc[nkern] += ms;
+      since_sort = 0;
+    }
+    since_sort += 1u;
+    uint32_t k = 0;
+    if (err == hipSuccess) err = launch_clear(args, n_clear, stream, &tm[k++]);
+    if (err == hipSuccess) err = launch_deposit(args, n_deposit, stream, &tm[k++]);
+    if (err == hipSuccess) err = launch_upward(args, n_subtree, stream, &tm[k++]);
+    if (err == hipSuccess) err = launch_top(args, stream, &tm[k++]);
+    for (uint32_t l = 2; l <= dmax; ++l)
+      if (err == hipSuccess) err = launch_tree(args, l, tree_blocks[l], stream, &tm[k++]);
+    if (err == hipSuccess) err = launch_integrate(args, n_integrate, stream, &tm[k++]);
+    if (err == hipSuccess) err = hipStreamSynchronize(stream);
+    for (uint32_t i = 0; i < nkern && err == hipSuccess; ++i) {
+      float ms = 0.0f;
+      err = hipEventElapsedTime(&ms, tm[i].start, tm[i].stop);
       acc[i] += ms;
     }
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
+  for (auto& e : sort_ev) (void)hipEventDestroy(e);
   if (err != hipSuccess) return fail(RAINBOW_E_HIP, "phase timing: %s", hipGetErrorString(err));
   for (uint32_t i = 0; i < nphase; ++i) out[i] = static_cast<float>(acc[i] / steps);
   return static_cast<int>(nphase);

@@ -79,10 +79,12 @@ int main(int argc, char** argv) {
     char name[32];
     if (i < 4)
       std::snprintf(name, sizeof(name), "%s", fixed[i]);
-    else if (i < nphase - 1)
+    else if (i < nphase - 2)
       std::snprintf(name, sizeof(name), "tree_l%d", i - 2);
-    else
+    else if (i == nphase - 2)
       std::snprintf(name, sizeof(name), "integrate");
+    else
+      std::snprintf(name, sizeof(name), "sort");
     std::printf("  ts %-10s %.4f ms\n", name, static_cast<double>(phase[i]));
     total += phase[i];
   }

@@ -230,9 +230,11 @@ int rainbow_hip_time_steps(rainbow_hip_t* h, uint32_t steps, float* ms);
  * kernel launch, and writes the mean duration in ms of each phase:
  * out[0] clear, out[1] deposit, out[2] upward (subtrees), out[3] upward
  * (top levels, with the level-1 fields), out[4 + l - 2] the tree pass of
- * level l (M2L + L2L + near, l = 2..depth), out[3 + depth] integrate.  `n`
- * is the capacity of out (at least 4 + depth).  Returns the number of
- * entries written (>= 0) or a negative status. */
+ * level l (M2L + L2L + near, l = 2..depth), out[3 + depth] integrate,
+ * out[4 + depth] the body sort (the steps sort when due, as rainbow_hip_step
+ * does; the entry is the sort time divided by `steps`).  `n` is the
+ * capacity of out (at least 5 + depth).  Returns the number of entries
+ * written (>= 0) or a negative status. */
 int rainbow_hip_phase_ms(rainbow_hip_t* h, uint32_t steps, float* out,
                          uint32_t n);
 
