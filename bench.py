@@ -28,11 +28,6 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 
 
-def phase_names(depth):
-    return (["clear", "deposit", "upward", "top"] + [f"tree_l{l}" for l in range(2, depth + 1)]
-            + ["integrate", "sort"])
-
-
 def algorithmic_bytes(n, depth, stats):
     """Compulsory bytes per launch of each kernel (each unique address once).
 
@@ -59,6 +54,10 @@ def algorithmic_bytes(n, depth, stats):
         t = stats[l]
         # P_l >= ceil(T_l / 8): each parent has at most 8 targets.
         out[f"tree_l{l}"] = 16 * t + 48 * t + 48 * ((t + 7) // 8)
+    # A launch over levels lo..hi moves the sum of its levels' bytes.
+    for lo in range(2, depth + 1):
+        for hi in range(lo + 1, depth + 1):
+            out[f"tree_l{lo}-{hi}"] = sum(out[f"tree_l{l}"] for l in range(lo, hi + 1))
     out["integrate"] = 64 * n + 16 * leaves + 48 * leaves
     # sort: read keys+positions+velocities, write the sorted bodies and perm.
     out["sort"] = 8 * n + 32 * n + 36 * n
@@ -290,7 +289,7 @@ def main():
     solver.solve_fields()
     stats = solver.tree_stats()
     phases = solver.phase_ms(args.phase_steps)
-    names = phase_names(solver.depth)
+    names = solver.phase_names()
     per = dict(zip(names, phases))
     bytes_per = algorithmic_bytes(n, solver.depth, stats)
     # The dominant kernel: the phase with the largest mean duration (the

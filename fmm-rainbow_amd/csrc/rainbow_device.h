@@ -9,6 +9,8 @@
 //   fields  float4[3C]  {a, phi}{Jxx,Jxy,Jxz,Jyy}{Jyz,Jzz,0,0}  reference Field
 //   source  float4[C]   {com.xyz, G m} per cell, G m = 0 for an empty cell
 //   active  uint8[C]    cell was an M2L target in the last solve
+//   extra   int4[8^D]   finest-level moments of bodies deposited by a block
+//                       that does not own their cell (merged by upward)
 // with the reference's cell numbering level_offset(l) + x + 2^l (y + 2^l z).
 // A step works on a set of systems (a plain solve is a set of one); each
 // system's descriptor carries its own buffer pointers, so systems can join
@@ -29,6 +31,7 @@ struct alignas(16) SysDesc {
   float4* fields;
   float4* source;
   uint8_t* active;
+  int4* extra;
   uint32_t n;          // bodies (particle_count)
   uint32_t depth;      // octree depth
   uint32_t body_base;  // first index of the system in the batch's body space
@@ -80,6 +83,10 @@ struct StepArgs {
   const BodyBlock* deposit_blocks;
   const BodyBlock* integrate_blocks;
   const SubtreeBlock* subtree_blocks;
+  // Per deposit block: the Morton range [lo, hi) of finest cells it owns
+  // (stores without atomics); bodies of other cells go to `extra`.  Ranges
+  // of one system's blocks are disjoint; (0, 0) owns nothing.
+  const uint2* own;
   uint32_t* lists;             // internal cells (count > 16)
   uint32_t* list_count;        // [kLevels][kStripes] items per stripe
   const uint32_t* stripe_off;  // [kLevels][kStripes] first slot of a stripe

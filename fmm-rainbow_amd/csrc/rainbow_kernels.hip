@@ -104,10 +104,24 @@ __global__ __launch_bounds__(kClearThreads) void k_clear(StepArgs a) {
   const CellBlock blk = a.clear_blocks[blockIdx.x];
   const SysDesc d = a.sys[blk.sys];
   if (!d.live) return;
-  const uint32_t ncell = level_off(d.depth + 1u);
+  // Only the finest level: upward and top rewrite every coarser cell.
+  const uint32_t ncell = 1u << (3u * d.depth);
   const uint32_t end = min(blk.first + kClearCells, ncell);
-  for (uint32_t i = blk.first + threadIdx.x; i < end; i += kClearThreads)
-    d.moments[i] = make_int4(0, 0, 0, 0);
+  int4* finest = d.moments + level_off(d.depth);
+  for (uint32_t i = blk.first + threadIdx.x; i < end; i += kClearThreads) {
+    finest[i] = make_int4(0, 0, 0, 0);
+    d.extra[i] = make_int4(0, 0, 0, 0);
+  }
+}
+
+// Interleave the low 10 bits of v with two zero bits between each.
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+  v &= 0x3FFu;
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
 }
 
 // ------------------------------------------------------------------------
@@ -176,33 +190,50 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(StepArgs a) {
     }
   }
   __syncthreads();
+  // Cells in this block's Morton range belong to it alone: their sums are
+  // stored.  Any other cell's sums go to `extra` with atomics (bodies that
+  // moved since the last sort, and cells shared with other blocks);
+  // upward adds `extra` into the finest level.
+  const uint2 own = a.own[blockIdx.x];
   int4* finest = d.moments + level_off(d.depth);
   for (uint32_t s = threadIdx.x; s < kDepositSlots; s += kDepositThreads) {
     const uint32_t key = keys[s];
     if (key == kEmptyKey) continue;
+    const uint32_t z = key / (dim * dim), y = (key / dim) % dim, x = key % dim;
+    const uint32_t morton = spread3(x) | (spread3(y) << 1) | (spread3(z) << 2);
+    if (morton >= own.x && morton < own.y) {
+      finest[key] = make_int4(vals[0][s], vals[1][s], vals[2][s], vals[3][s]);
+      continue;
+    }
     // count and sum_x share one 64-bit atomic: the count never carries out
     // of its 32 bits, and the high word wraps as an i32 sum.
     const unsigned long long packed = static_cast<unsigned long long>(static_cast<uint32_t>(vals[0][s])) |
                                       (static_cast<unsigned long long>(static_cast<uint32_t>(vals[1][s])) << 32);
-    atomicAdd(reinterpret_cast<unsigned long long*>(finest + key), packed);
-    atomicAdd(reinterpret_cast<int*>(finest + key) + 2, vals[2][s]);
-    atomicAdd(reinterpret_cast<int*>(finest + key) + 3, vals[3][s]);
+    atomicAdd(reinterpret_cast<unsigned long long*>(d.extra + key), packed);
+    atomicAdd(reinterpret_cast<int*>(d.extra + key) + 2, vals[2][s]);
+    atomicAdd(reinterpret_cast<int*>(d.extra + key) + 3, vals[3][s]);
   }
+}
+
+// Owned Morton ranges of the deposit blocks from the sorted keys (system <<
+// 18 | Morton): block b owns [key of its first body, key of the next
+// block's first body); a system's first block starts at 0 and its last
+// ends at 2^18.
+__global__ __launch_bounds__(256) void k_own(StepArgs a, uint32_t nblocks, const uint32_t* sorted, uint2* own) {
+  const uint32_t b = blockIdx.x * 256u + threadIdx.x;
+  if (b >= nblocks) return;
+  const BodyBlock blk = a.deposit_blocks[b];
+  const SysDesc d = a.sys[blk.sys];
+  const uint32_t mask = (1u << 18) - 1u;
+  const uint32_t lo = blk.first == 0u ? 0u : (sorted[d.body_base + blk.first] & mask);
+  const uint32_t next = blk.first + kDepositBodies;
+  const uint32_t hi = next >= d.n ? (1u << 18) : (sorted[d.body_base + next] & mask);
+  own[b] = make_uint2(lo, hi);
 }
 
 // ------------------------------------------------------------------------
 // body order: Morton keys, gather and copy-back around a radix sort
 // ------------------------------------------------------------------------
-
-// Interleave the low 10 bits of v with two zero bits between each.
-__device__ __forceinline__ uint32_t spread3(uint32_t v) {
-  v &= 0x3FFu;
-  v = (v * 0x00010001u) & 0xFF0000FFu;
-  v = (v * 0x00000101u) & 0x0F00F00Fu;
-  v = (v * 0x00000011u) & 0xC30C30C3u;
-  v = (v * 0x00000005u) & 0x49249249u;
-  return v;
-}
 
 // key = system << 18 | Morton code of the finest cell; value = slot.
 __global__ __launch_bounds__(kIntegrateThreads) void k_sort_keys(StepArgs a, uint32_t* keys, uint32_t* vals) {
@@ -400,10 +431,22 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
   int4* M = d.moments;
   if (t < 4u) ps.n[t] = 0u;
 
-  // Finest level: 8x8x8 cells, x fastest (rows of 8 contiguous moments).
+  // Finest level: 8x8x8 cells, x fastest (rows of 8 contiguous moments),
+  // plus the sums deposited into `extra` (wrapping i32 adds, exact in any
+  // order), written back when non-zero.
   for (uint32_t i = t; i < 512u; i += kUpwardThreads) {
     const uint32_t ix = i & 7u, iy = (i >> 3) & 7u, iz = i >> 6;
-    lvl0[i] = M[level_off(D) + lin(8u * rx + ix, 8u * ry + iy, 8u * rz + iz, 1u << D)];
+    const uint32_t c = lin(8u * rx + ix, 8u * ry + iy, 8u * rz + iz, 1u << D);
+    int4 m = M[level_off(D) + c];
+    const int4 e = d.extra[c];
+    if (e.x != 0) {
+      m.x = static_cast<int>(static_cast<uint32_t>(m.x) + static_cast<uint32_t>(e.x));
+      m.y = static_cast<int>(static_cast<uint32_t>(m.y) + static_cast<uint32_t>(e.y));
+      m.z = static_cast<int>(static_cast<uint32_t>(m.z) + static_cast<uint32_t>(e.z));
+      m.w = static_cast<int>(static_cast<uint32_t>(m.w) + static_cast<uint32_t>(e.w));
+      M[level_off(D) + c] = m;
+    }
+    lvl0[i] = m;
   }
   __syncthreads();
   if (t < 64u) {
@@ -599,46 +642,198 @@ __global__ __launch_bounds__(kUpwardThreads) void k_top(StepArgs a) {
 }
 
 // ------------------------------------------------------------------------
-// tree(level): M2L (src/rainbow_fmm.wgsl:457-519), L2L (:585-611) and near
-// (:521-572) of the targets of one level, one 64-lane block per internal
-// parent.  The parent's 6x6x6 source window is staged in LDS with one
-// round of loads; the window is walked one source-parent block (8 sources)
-// at a time: the 64 lanes evaluate the 8 x 8 (target, source) pair fields
-// in parallel, then one lane per (target, component) chain adds the 8
-// terms in the reference order.
+// tree(lo..hi): M2L (src/rainbow_fmm.wgsl:457-519), L2L (:585-611) and near
+// (:521-572) of the targets of levels lo..hi, one 256-thread block per
+// internal parent.  The parent's 6x6x6 source window is staged in LDS with
+// one round of loads; the window is walked one source-parent block (8
+// sources) at a time: each of the four waves evaluates one block's 8 x 8
+// (target, source) pair fields, then one lane per (target, component)
+// chain adds the terms in the reference order.
+//
+// A launch over several levels has no dependency between its blocks: an
+// item of level l > lo needs its parent's field, which another block of the
+// same launch writes, so the block recomputes the fields of the parent and
+// of its ancestors down to level lo (single targets, the same operations in
+// the same order, so the values are identical) from the field of the
+// ancestor at level lo - 1, written by an earlier launch.
 // ------------------------------------------------------------------------
 
+// The L2L term of one field component (downward, src/rainbow_fmm.wgsl:
+// 598-610): the parent's field {a, phi, J} shifted by (dx, dy, dz) from the
+// parent's centre to the child's.
+__device__ __forceinline__ float l2l_term(uint32_t k, float4 pa, float4 pj0, float4 pj1, float dx, float dy,
+                                          float dz) {
+  switch (k) {
+    case 0: return pa.x + ((pj0.x * dx + pj0.y * dy) + pj0.z * dz);
+    case 1: return pa.y + ((pj0.y * dx + pj0.w * dy) + pj1.x * dz);
+    case 2: return pa.z + ((pj0.z * dx + pj1.x * dy) + pj1.y * dz);
+    case 3: return pa.w - ((pa.x * dx + pa.y * dy) + pa.z * dz);
+    case 4: return pj0.x;
+    case 5: return pj0.y;
+    case 6: return pj0.z;
+    case 7: return pj0.w;
+    case 8: return pj1.x;
+    default: return pj1.y;
+  }
+}
+
+// Stages the source records of cells 2P-2 .. 2P+3 per axis of level `level`
+// (P = the parent's coordinates); cells outside the level read as empty.
+__device__ __forceinline__ void stage_window(float4* win, const SysDesc& d, uint32_t level, uint32_t px,
+                                             uint32_t py, uint32_t pz, uint32_t tid) {
+  const uint32_t dim = 1u << level;
+  const int dim_i = static_cast<int>(dim);
+  const float4* src = d.source + level_off(level);
+  for (uint32_t w = tid; w < 216u; w += kTreeThreads) {
+    const uint32_t wz = w / 36u, wy = (w / 6u) % 6u, wx = w % 6u;
+    const int qx = 2 * static_cast<int>(px) - 2 + static_cast<int>(wx);
+    const int qy = 2 * static_cast<int>(py) - 2 + static_cast<int>(wy);
+    const int qz = 2 * static_cast<int>(pz) - 2 + static_cast<int>(wz);
+    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (qx >= 0 && qy >= 0 && qz >= 0 && qx < dim_i && qy < dim_i && qz < dim_i)
+      v = src[lin(static_cast<uint32_t>(qx), static_cast<uint32_t>(qy), static_cast<uint32_t>(qz), dim)];
+    win[w] = v;
+  }
+}
+
+// LDS of the ancestor path per ancestor: its 6^3 window and its 10 x 216
+// pair terms.
+constexpr uint32_t kAncestorBytes = 216u * 16u + 10u * 216u * 4u;
+// LDS of one item: the window and the four waves' 10 x 64 chunk terms.
+constexpr uint32_t kItemBytes = 216u * 16u + 4u * 10u * 64u * 4u;
+
+// Fields of the internal ancestors A_lo .. A_{level-1} of an item of
+// `level` whose parent P = A_{level-1} has coordinates (px, py, pz), from
+// the field of A_{lo-1} in memory.  The field of A_j is its M2L sum over
+// its parent's neighbours' children in the reference order (source parent
+// pz, py, px, then child cz, cy, cx), then + the L2L term of A_{j-1}'s
+// field.  The M2L sums of all ancestors are independent and run in
+// parallel; only the L2L additions run in order.  Terms of cells adjacent
+// to the target, empty cells and cells outside the level are zero and
+// leave the sums unchanged (a running sum that starts at +0 never becomes
+// -0, so adding +0 is the identity).  Writes the parent's field to pf[0..9].
+// All threads of the block call it; lds holds (level - lo) * kAncestorBytes.
 template <bool kComp>
-__global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t level) {
+__device__ void ancestor_fields(const SysDesc& d, uint32_t lo, uint32_t level, uint32_t px, uint32_t py,
+                                uint32_t pz, unsigned char* lds, float* pf, uint32_t tid) {
+  const uint32_t na = level - lo;
+  float4* awin = reinterpret_cast<float4*>(lds);               // [na][216]
+  float* st = reinterpret_cast<float*>(lds + na * 216u * 16u);  // [na][10][216]
+  // One round of loads: every ancestor's window, and the field of A_{lo-1}.
+  for (uint32_t w = tid; w < na * 216u; w += kTreeThreads) {
+    const uint32_t i = w / 216u, ww = w - i * 216u;
+    const uint32_t sh = level - 1u - (lo + i);  // A_{lo+i} = P >> sh
+    const uint32_t al = lo + i;
+    const uint32_t dim = 1u << al;
+    const int dim_i = static_cast<int>(dim);
+    const uint32_t qpx = (px >> sh) >> 1, qpy = (py >> sh) >> 1, qpz = (pz >> sh) >> 1;
+    const uint32_t wz = ww / 36u, wy = (ww / 6u) % 6u, wx = ww % 6u;
+    const int qx = 2 * static_cast<int>(qpx) - 2 + static_cast<int>(wx);
+    const int qy = 2 * static_cast<int>(qpy) - 2 + static_cast<int>(wy);
+    const int qz = 2 * static_cast<int>(qpz) - 2 + static_cast<int>(wz);
+    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (qx >= 0 && qy >= 0 && qz >= 0 && qx < dim_i && qy < dim_i && qz < dim_i)
+      v = d.source[level_off(al) +
+                   lin(static_cast<uint32_t>(qx), static_cast<uint32_t>(qy), static_cast<uint32_t>(qz), dim)];
+    awin[w] = v;
+  }
+  if (tid < 10u) {
+    const uint32_t up = level - lo;  // P >> up = A_{lo-1}
+    const uint32_t adim = 1u << (lo - 1u);
+    pf[tid] = reinterpret_cast<const float*>(
+        d.fields + 3u * (level_off(lo - 1u) + lin(px >> up, py >> up, pz >> up, adim)))[tid];
+  }
+  __syncthreads();
+  for (uint32_t w = tid; w < na * 216u; w += kTreeThreads) {
+    const uint32_t i = w / 216u, u = w - i * 216u;
+    const uint32_t sh = level - 1u - (lo + i);
+    const uint32_t dim = 1u << (lo + i);
+    const uint32_t cx = px >> sh, cy = py >> sh, cz = pz >> sh;
+    const float cell_epsilon = (0.06f * kWorldHalfExtent) / static_cast<float>(dim);
+    const float eps2 = cell_epsilon * cell_epsilon;
+    const uint32_t q = u >> 3, c = u & 7u;
+    const uint32_t wsx = 2u * (q % 3u) + (c & 1u);
+    const uint32_t wsy = 2u * ((q / 3u) % 3u) + ((c >> 1) & 1u);
+    const uint32_t wsz = 2u * (q / 9u) + (c >> 2);
+    const uint32_t wtx = 2u + (cx & 1u), wty = 2u + (cy & 1u), wtz = 2u + (cz & 1u);
+    const bool near = (wsx + 1u - wtx) <= 2u && (wsy + 1u - wty) <= 2u && (wsz + 1u - wtz) <= 2u;
+    float t[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (!near) {
+      const float4 sr = awin[i * 216u + wsx + 6u * (wsy + 6u * wsz)];
+      if (sr.w != 0.0f) pair_terms(sr, center1(cx, dim), center1(cy, dim), center1(cz, dim), eps2, t);
+    }
+#pragma unroll
+    for (int k = 0; k < 10; ++k) st[(i * 10u + k) * 216u + u] = t[k];
+  }
+  __syncthreads();
+  // Chains: thread i * 10 + k sums component k of ancestor i.
+  const uint32_t ci = tid / 10u, ck = tid - 10u * ci;
+  Chain<kComp> ch;
+  if (ci < na) {
+    const float* row = st + (ci * 10u + ck) * 216u;
+#pragma unroll 8
+    for (uint32_t u = 0; u < 216u; ++u) ch.add(row[u]);
+  }
+  // L2L in order, from A_lo up to the parent.
+  for (uint32_t i = 0; i < na; ++i) {
+    float v = 0.0f;
+    if (ci == i) {
+      const uint32_t sh = level - 1u - (lo + i);
+      const uint32_t dim = 1u << (lo + i);
+      const uint32_t cx = px >> sh, cy = py >> sh, cz = pz >> sh;
+      const float4 pa = make_float4(pf[0], pf[1], pf[2], pf[3]);
+      const float4 pj0 = make_float4(pf[4], pf[5], pf[6], pf[7]);
+      const float4 pj1 = make_float4(pf[8], pf[9], 0.0f, 0.0f);
+      const float dx = center1(cx, dim) - center1(cx >> 1, dim >> 1);
+      const float dy = center1(cy, dim) - center1(cy >> 1, dim >> 1);
+      const float dz = center1(cz, dim) - center1(cz >> 1, dim >> 1);
+      ch.reset(ch.value() + l2l_term(ck, pa, pj0, pj1, dx, dy, dz));
+      v = ch.value();
+    }
+    __syncthreads();
+    if (ci == i) pf[ck] = v;
+    __syncthreads();
+  }
+}
+
+// kFused: the launch covers several levels (lo < hi); the single-level
+// variant compiles without the ancestor path and keeps fewer registers.
+template <bool kComp, bool kFused>
+__global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t lo, uint32_t hi) {
   constexpr uint32_t kWaves = kTreeThreads / 64u;
-  __shared__ float4 win[216];
-  __shared__ float terms[kWaves][10][64];
+  // Single level: static LDS.  Fused: one dynamic region of
+  // max(kItemBytes, (hi - lo) * kAncestorBytes) bytes, used by the
+  // ancestor path and then by the item.
+  __shared__ float4 win_s[kFused ? 1 : 216];
+  __shared__ float terms_s[kFused ? 1 : kWaves * 10 * 64];
+  extern __shared__ float4 dyn[];
+  __shared__ float pf[12];
+  float4* win = kFused ? dyn : win_s;
+  float* terms = kFused ? reinterpret_cast<float*>(dyn + 216) : terms_s;
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63u;
   const uint32_t wave = tid >> 6;
-  // Items of this level: stripe counts and their prefix across lanes 0..31
-  // (every wave computes the same values).
-  const uint32_t lv = level - 1u;
-  uint32_t cnt = 0u;
-  if (lane < kStripes) cnt = min(a.list_count[lv * kStripes + lane], a.stripe_cap[lv * kStripes + lane]);
-  uint32_t incl = cnt;
-#pragma unroll
-  for (uint32_t o = 1u; o < kStripes; o <<= 1) {
-    const uint32_t v = __shfl_up(incl, o);
-    if (lane >= o) incl += v;
-  }
-  const uint32_t total = __shfl(incl, kStripes - 1u);
-  const uint32_t excl = incl - cnt;
 
-  const uint32_t dim = 1u << level;
-  const uint32_t pdim = dim >> 1u;
-  const int pdim_i = static_cast<int>(pdim);
-  const int dim_i = static_cast<int>(dim);
-  const uint32_t off = level_off(level);
-  const uint32_t poff = level_off(level - 1u);
-  // cell_epsilon, src/rainbow_fmm.wgsl:507 (0.06 * WORLD_HALF_EXTENT / dim).
-  const float cell_epsilon = (0.06f * kWorldHalfExtent) / static_cast<float>(dim);
-  const float eps2_m2l = cell_epsilon * cell_epsilon;
+  // Items of levels hi, hi-1, ..., lo in that order; per level the stripe
+  // counts and their prefix across lanes 0..31 (every wave computes the
+  // same values).
+  auto stripe_count = [&](uint32_t level) -> uint32_t {
+    const uint32_t lv = level - 1u;
+    return lane < kStripes ? min(a.list_count[lv * kStripes + lane], a.stripe_cap[lv * kStripes + lane]) : 0u;
+  };
+  auto prefix = [&](uint32_t cnt) -> uint32_t {
+    uint32_t incl = cnt;
+#pragma unroll
+    for (uint32_t o = 1u; o < kStripes; o <<= 1) {
+      const uint32_t v = __shfl_up(incl, o);
+      if (lane >= o) incl += v;
+    }
+    return incl;
+  };
+  if (!kFused) lo = hi;
+  uint32_t total = 0u;
+  for (uint32_t level = hi; level >= lo; --level) total += __shfl(prefix(stripe_count(level)), kStripes - 1u);
+
   // Pair lane: target tt (child of the parent), source cc of the chunk.
   const uint32_t tt = lane >> 3, cc = lane & 7u;
   const uint32_t wtx = 2u + (tt & 1u), wty = 2u + ((tt >> 1) & 1u), wtz = 2u + (tt >> 2);
@@ -647,32 +842,39 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t leve
   const uint32_t ct = tid / 10u, ck = tid % 10u;
 
   for (uint32_t it = blockIdx.x; it < total; it += gridDim.x) {
-    const uint32_t j = static_cast<uint32_t>(__popcll(__ballot(lane < kStripes && incl <= it)));
+    // Level and list position of item `it`.
+    uint32_t level = hi, rem = it, incl = 0u, cnt = 0u;
+    for (;; --level) {
+      cnt = stripe_count(level);
+      incl = prefix(cnt);
+      const uint32_t tl = __shfl(incl, kStripes - 1u);
+      if (rem < tl || level == lo) break;
+      rem -= tl;
+    }
+    const uint32_t lv = level - 1u;
+    const uint32_t excl = incl - cnt;
+    const uint32_t j = static_cast<uint32_t>(__popcll(__ballot(lane < kStripes && incl <= rem)));
     const uint32_t base = __shfl(excl, j);
-    const uint32_t item = a.lists[a.stripe_off[lv * kStripes + j] + (it - base)];
+    const uint32_t item = a.lists[a.stripe_off[lv * kStripes + j] + (rem - base)];
     const uint32_t s = item >> 16;
     if (s >= a.nsys) continue;
     const uint32_t pidx = item & 0xFFFFu;
     const SysDesc d = a.sys[s];
+    const uint32_t dim = 1u << level;
+    const uint32_t pdim = dim >> 1u;
+    const uint32_t off = level_off(level);
     const uint32_t pz = pidx / (pdim * pdim);
     const uint32_t prest = pidx - pz * pdim * pdim;
     const uint32_t py = prest / pdim;
     const uint32_t px = prest - py * pdim;
     const int* counts = reinterpret_cast<const int*>(d.moments);
 
-    // Stage the window of cells 2P-2 .. 2P+3 per axis; cells outside the
-    // level read as empty.
-    const float4* src = d.source + off;
-    for (uint32_t w = tid; w < 216u; w += kTreeThreads) {
-      const uint32_t wz = w / 36u, wy = (w / 6u) % 6u, wx = w % 6u;
-      const int qx = 2 * static_cast<int>(px) - 2 + static_cast<int>(wx);
-      const int qy = 2 * static_cast<int>(py) - 2 + static_cast<int>(wy);
-      const int qz = 2 * static_cast<int>(pz) - 2 + static_cast<int>(wz);
-      float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      if (qx >= 0 && qy >= 0 && qz >= 0 && qx < dim_i && qy < dim_i && qz < dim_i)
-        v = src[lin(static_cast<uint32_t>(qx), static_cast<uint32_t>(qy), static_cast<uint32_t>(qz), dim)];
-      win[w] = v;
-    }
+    // The parent's field: from memory when its level is below lo, else
+    // recomputed along the chain of ancestors from level lo - 1.
+    const bool recompute = kFused && level - 1u >= lo;
+    if (kFused && recompute) ancestor_fields<kComp>(d, lo, level, px, py, pz, reinterpret_cast<unsigned char*>(dyn), pf, tid);
+
+    stage_window(win, d, level, px, py, pz, tid);
     const uint32_t tx = 2u * px + (tt & 1u), ty = 2u * py + ((tt >> 1) & 1u), tz = 2u * pz + (tt >> 2);
     const int tcount = counts[4u * (off + lin(tx, ty, tz, dim))];
     const float tcx = center1(tx, dim), tcy = center1(ty, dim), tcz = center1(tz, dim);
@@ -686,6 +888,9 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t leve
     const int nz = (pz + 1u < pdim ? 1 : 0) - oz0 + 1;
     const uint32_t m2l_chunks = static_cast<uint32_t>(nx * ny * nz);
     const uint32_t chunks = m2l_chunks + (__syncthreads_or(leaf) ? 4u : 0u);
+    // cell_epsilon, src/rainbow_fmm.wgsl:507 (0.06 * WORLD_HALF_EXTENT / dim).
+    const float cell_epsilon = (0.06f * kWorldHalfExtent) / static_cast<float>(dim);
+    const float eps2_m2l = cell_epsilon * cell_epsilon;
     const float near_epsilon =
         fmaxf(d.softening, (0.72f * kWorldHalfExtent) / static_cast<float>(dim));
     const float eps2_near = near_epsilon * near_epsilon;
@@ -693,25 +898,20 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t leve
     Chain<kComp> ch;
     float inh = 0.0f;
     if (chain) {
-      // L2L (downward, src/rainbow_fmm.wgsl:598-610): this chain's
-      // component of the parent's field shifted to its target's centre.
+      // L2L: this chain's component of the parent's field shifted to its
+      // target's centre.
       float4 pa, pj0, pj1;
-      load_field(d.fields, poff + pidx, pa, pj0, pj1);
+      if (kFused && recompute) {
+        pa = make_float4(pf[0], pf[1], pf[2], pf[3]);
+        pj0 = make_float4(pf[4], pf[5], pf[6], pf[7]);
+        pj1 = make_float4(pf[8], pf[9], 0.0f, 0.0f);
+      } else {
+        load_field(d.fields, level_off(level - 1u) + pidx, pa, pj0, pj1);
+      }
       const float dx = center1(2u * px + (ct & 1u), dim) - center1(px, pdim);
       const float dy = center1(2u * py + ((ct >> 1) & 1u), dim) - center1(py, pdim);
       const float dz = center1(2u * pz + (ct >> 2), dim) - center1(pz, pdim);
-      switch (ck) {
-        case 0: inh = pa.x + ((pj0.x * dx + pj0.y * dy) + pj0.z * dz); break;
-        case 1: inh = pa.y + ((pj0.y * dx + pj0.w * dy) + pj1.x * dz); break;
-        case 2: inh = pa.z + ((pj0.z * dx + pj1.x * dy) + pj1.y * dz); break;
-        case 3: inh = pa.w - ((pa.x * dx + pa.y * dy) + pa.z * dz); break;
-        case 4: inh = pj0.x; break;
-        case 5: inh = pj0.y; break;
-        case 6: inh = pj0.z; break;
-        case 7: inh = pj0.w; break;
-        case 8: inh = pj1.x; break;
-        default: inh = pj1.y; break;
-      }
+      inh = l2l_term(ck, pa, pj0, pj1, dx, dy, dz);
     }
     __syncthreads();
 
@@ -745,7 +945,7 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t leve
         }
       }
 #pragma unroll
-      for (int k = 0; k < 10; ++k) terms[wave][k][lane] = t[k];
+      for (int k = 0; k < 10; ++k) terms[(wave * 10u + k) * 64u + lane] = t[k];
       __syncthreads();
       // The chains add the chunks of this round in order; the L2L term goes
       // in between the last M2L chunk and the first near chunk.
@@ -755,7 +955,7 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t leve
           if (qq >= chunks) break;
           if (qq == m2l_chunks) ch.reset(ch.value() + inh);
 #pragma unroll
-          for (uint32_t c = 0; c < 8u; ++c) ch.add(terms[w][ck][ct * 8u + c]);
+          for (uint32_t c = 0; c < 8u; ++c) ch.add(terms[(w * 10u + ck) * 64u + ct * 8u + c]);
         }
       }
       __syncthreads();
@@ -1063,6 +1263,14 @@ __global__ __launch_bounds__(64) void k_diag_final(const double* partial, uint32
       hipLaunchKernelGGL(kernel, grid, block, 0, stream, __VA_ARGS__);                                \
   } while (0)
 
+#define RB_LAUNCH_LDS(t, kernel, grid, block, lds, stream, ...)                                         \
+  do {                                                                                                \
+    if ((t) != nullptr)                                                                               \
+      hipExtLaunchKernelGGL(kernel, grid, block, lds, stream, (t)->start, (t)->stop, 0, __VA_ARGS__); \
+    else                                                                                              \
+      hipLaunchKernelGGL(kernel, grid, block, lds, stream, __VA_ARGS__);                              \
+  } while (0)
+
 // A launcher with nothing to launch still records its event pair.
 static hipError_t no_launch(hipStream_t stream, const Timing* t) {
   if (t == nullptr) return hipSuccess;
@@ -1097,12 +1305,25 @@ hipError_t launch_top(const StepArgs& a, hipStream_t stream, const Timing* t) {
   return hipGetLastError();
 }
 
-hipError_t launch_tree(const StepArgs& a, uint32_t level, uint32_t nblocks, hipStream_t stream, const Timing* t) {
-  if (nblocks == 0u) return no_launch(stream, t);
-  if (a.compensated)
-    RB_LAUNCH(t, k_tree<true>, dim3(nblocks), dim3(kTreeThreads), stream, a, level);
+hipError_t launch_tree(const StepArgs& a, uint32_t lo, uint32_t hi, uint32_t nblocks, hipStream_t stream,
+                       const Timing* t) {
+  if (nblocks == 0u || lo < 2u || hi < lo || hi >= kLevels) return no_launch(stream, t);
+  const bool fused = lo < hi;
+  const uint32_t lds = fused ? std::max(kItemBytes, (hi - lo) * kAncestorBytes) : 0u;
+  if (a.compensated && fused)
+    RB_LAUNCH_LDS(t, (k_tree<true, true>), dim3(nblocks), dim3(kTreeThreads), lds, stream, a, lo, hi);
+  else if (a.compensated)
+    RB_LAUNCH(t, (k_tree<true, false>), dim3(nblocks), dim3(kTreeThreads), stream, a, lo, hi);
+  else if (fused)
+    RB_LAUNCH_LDS(t, (k_tree<false, true>), dim3(nblocks), dim3(kTreeThreads), lds, stream, a, lo, hi);
   else
-    RB_LAUNCH(t, k_tree<false>, dim3(nblocks), dim3(kTreeThreads), stream, a, level);
+    RB_LAUNCH(t, (k_tree<false, false>), dim3(nblocks), dim3(kTreeThreads), stream, a, lo, hi);
+  return hipGetLastError();
+}
+
+hipError_t launch_own(const StepArgs& a, uint32_t ndeposit, const uint32_t* sorted, uint2* own, hipStream_t stream) {
+  if (ndeposit == 0u) return hipSuccess;
+  hipLaunchKernelGGL(k_own, dim3((ndeposit + 255u) / 256u), dim3(256), 0, stream, a, ndeposit, sorted, own);
   return hipGetLastError();
 }
 

@@ -18,9 +18,12 @@ hipError_t launch_clear(const StepArgs& a, uint32_t nblocks, hipStream_t stream,
 hipError_t launch_deposit(const StepArgs& a, uint32_t nblocks, hipStream_t stream, const Timing* t = nullptr);
 hipError_t launch_upward(const StepArgs& a, uint32_t nblocks, hipStream_t stream, const Timing* t = nullptr);
 hipError_t launch_top(const StepArgs& a, hipStream_t stream, const Timing* t = nullptr);
-hipError_t launch_tree(const StepArgs& a, uint32_t level, uint32_t nblocks, hipStream_t stream,
+// Tree pass of levels lo..hi (2 <= lo <= hi <= 6) in one launch.
+hipError_t launch_tree(const StepArgs& a, uint32_t lo, uint32_t hi, uint32_t nblocks, hipStream_t stream,
                        const Timing* t = nullptr);
 hipError_t launch_integrate(const StepArgs& a, uint32_t nblocks, hipStream_t stream, const Timing* t = nullptr);
+// Owned Morton ranges of the deposit blocks, from the sorted keys.
+hipError_t launch_own(const StepArgs& a, uint32_t ndeposit, const uint32_t* sorted, uint2* own, hipStream_t stream);
 hipError_t launch_sort_keys(const StepArgs& a, uint32_t nblocks, uint32_t* keys, uint32_t* vals,
                             hipStream_t stream);
 hipError_t launch_reorder(const StepArgs& a, uint32_t nblocks, const uint32_t* vals, float4* tpos, float4* tvel,

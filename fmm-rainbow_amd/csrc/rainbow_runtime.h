@@ -38,6 +38,14 @@ constexpr uint32_t kGraphSteps = 16;
 // Bodies are re-sorted into Morton order before every this many steps by
 // default (tunable per engine; 0 = never).
 constexpr uint32_t kDefaultSortInterval = 8;
+// Tree levels 2..kDefaultTreeFusion run in one launch by default (tunable
+// per engine; each deeper level has a launch of its own).
+constexpr uint32_t kDefaultTreeFusion = 5;
+
+// One launch of the tree pass: levels lo..hi.
+struct TreeLaunch {
+  uint32_t lo, hi, blocks;
+};
 
 // One device allocation holding a system's bodies and cells, and its
 // descriptor pointing into it.
@@ -89,6 +97,7 @@ class Engine {
   int diagnostics(uint32_t sys, double out[8]);
   int sort_bodies();
   int set_sort_interval(uint32_t interval);
+  int set_tree_fusion(uint32_t hi);
   int autotune(uint32_t steps, uint32_t* chosen);
   int snapshot(bool restore);
 
@@ -103,15 +112,18 @@ class Engine {
   uint32_t n_integrate = 0;
   uint32_t n_subtree = 0;
   uint32_t tree_blocks[kLevels + 1] = {};
+  std::vector<TreeLaunch> tree_launches;
+  uint32_t tree_fusion = kDefaultTreeFusion;
 
  private:
   int capture(uint32_t steps, hipGraphExec_t* out);
   int prepare_graphs(uint32_t steps);
   uint32_t chunk_steps() const;
+  void plan_tree();
   void drop_graphs();
 
   DevBuf sys_buf, clear_buf, deposit_buf, integrate_buf, subtree_buf, list_buf, count_buf,
-      stripe_buf, diag_buf, key_buf, body_buf, sort_tmp, snap_buf;
+      stripe_buf, diag_buf, key_buf, body_buf, sort_tmp, snap_buf, own_buf;
   uint32_t sort_interval = kDefaultSortInterval;
   uint32_t since_sort = kDefaultSortInterval;
   hipGraphExec_t graph_one = nullptr;

@@ -95,8 +95,10 @@ void apply_physics(SysDesc& s, const Physics& ph) {
 
 size_t system_storage_bytes(uint32_t n) {
   const size_t cells = total_cells(depth_for_particles(n));
+  const size_t finest = level_cells(depth_for_particles(n));
   return 2u * align_up(16u * static_cast<size_t>(n)) + align_up(4u * static_cast<size_t>(n)) +
-         align_up(16u * cells) + align_up(48u * cells) + align_up(16u * cells) + align_up(cells);
+         align_up(16u * cells) + align_up(48u * cells) + align_up(16u * cells) + align_up(cells) +
+         align_up(16u * finest);
 }
 
 int alloc_system(uint32_t n, const Physics& ph, float scale, hipStream_t stream, SystemStorage* out) {
@@ -130,6 +132,8 @@ int alloc_system(uint32_t n, const Physics& ph, float scale, hipStream_t stream,
   d.source = reinterpret_cast<float4*>(p);
   p += align_up(16u * cells);
   d.active = reinterpret_cast<uint8_t*>(p);
+  p += align_up(cells);
+  d.extra = reinterpret_cast<int4*>(p);
   d.n = n;
   d.depth = plan.depth;
   d.scale = scale != 0.0f ? scale : plan.scale;
@@ -217,9 +221,9 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
   for (uint32_t s = 0; s < nsys; ++s) {
     sys[s].body_base = static_cast<uint32_t>(total_bodies);
     const SysDesc& d = sys[s];
-    const uint32_t cells = total_cells(d.depth);
+    const uint32_t finest = level_cells(d.depth);
     const uint32_t ib = d.n / (kAdaptiveLeafCapacity + 1u);
-    for (uint32_t f = 0; f < cells; f += kClearCells) clr.push_back(CellBlock{s, f});
+    for (uint32_t f = 0; f < finest; f += kClearCells) clr.push_back(CellBlock{s, f});
     for (uint32_t f = 0; f < d.n; f += kDepositBodies) dep.push_back(BodyBlock{s, f});
     for (uint32_t f = 0; f < d.n; f += kIntegrateThreads) integ.push_back(BodyBlock{s, f});
     for (uint32_t r = 0; r < level_cells(d.depth - 3u); ++r) {
@@ -252,10 +256,12 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
       (rc = list_buf.ensure(sizeof(uint32_t) * list_total)) ||
       (rc = count_buf.ensure(sizeof(uint32_t) * kLevels * kStripes)) ||
       (rc = stripe_buf.ensure(sizeof(uint32_t) * stripes.size())) ||
-      (rc = diag_buf.ensure(sizeof(double) * (8u * static_cast<size_t>(n_integrate) + 8u))))
+      (rc = diag_buf.ensure(sizeof(double) * (8u * static_cast<size_t>(n_integrate) + 8u))) ||
+      (rc = own_buf.ensure(sizeof(uint2) * n_deposit)))
     return rc;
   args.sys = static_cast<const SysDesc*>(sys_buf.p);
   args.clear_blocks = static_cast<const CellBlock*>(clear_buf.p);
+  args.own = static_cast<const uint2*>(own_buf.p);
   args.deposit_blocks = static_cast<const BodyBlock*>(deposit_buf.p);
   args.integrate_blocks = static_cast<const BodyBlock*>(integrate_buf.p);
   args.subtree_blocks = static_cast<const SubtreeBlock*>(subtree_buf.p);
@@ -270,6 +276,7 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
     for (uint32_t j = 0; j < kStripes; ++j) items += caps[(l - 1u) * kStripes + j];
     tree_blocks[l] = static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(items, 1u), kTreeMaxBlocks));
   }
+  plan_tree();
   if (nsys) {
     RB_HIP(hipMemcpyAsync(sys_buf.p, sys.data(), sizeof(SysDesc) * nsys, hipMemcpyHostToDevice, stream));
     RB_HIP(hipMemcpyAsync(clear_buf.p, clr.data(), sizeof(CellBlock) * n_clear, hipMemcpyHostToDevice, stream));
@@ -278,6 +285,8 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
                           stream));
     RB_HIP(hipMemcpyAsync(subtree_buf.p, sub.data(), sizeof(SubtreeBlock) * n_subtree, hipMemcpyHostToDevice,
                           stream));
+    // No block owns a cell until the first sort.
+    RB_HIP(hipMemsetAsync(own_buf.p, 0, sizeof(uint2) * n_deposit, stream));
   }
   RB_HIP(hipMemcpyAsync(stripe_buf.p, stripes.data(), sizeof(uint32_t) * stripes.size(), hipMemcpyHostToDevice,
                         stream));
@@ -312,6 +321,7 @@ int Engine::sort_bodies() {
   RB_HIP(launch_sort_keys(args, n_integrate, keys0, vals0, stream));
   RB_HIP(sort_pairs(sort_tmp.p, temp_bytes, keys0, keys1, vals0, vals1, n, end_bit, stream));
   RB_HIP(launch_reorder(args, n_integrate, vals1, tpos, tvel, tperm, stream));
+  RB_HIP(launch_own(args, n_deposit, keys1, static_cast<uint2*>(own_buf.p), stream));
   return RAINBOW_OK;
 }
 
@@ -323,13 +333,37 @@ int Engine::push_descriptors() {
   return RAINBOW_OK;
 }
 
+// Tree launches: levels 2..min(tree_fusion, dmax) together, then one launch
+// per deeper level.
+void Engine::plan_tree() {
+  tree_launches.clear();
+  uint32_t l = 2u;
+  while (l <= dmax) {
+    const uint32_t hi = l == 2u ? std::max(2u, std::min(tree_fusion, dmax)) : l;
+    uint64_t blocks = 0;
+    for (uint32_t k = l; k <= hi; ++k) blocks += tree_blocks[k];
+    tree_launches.push_back(
+        TreeLaunch{l, hi, static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(blocks, 1u), kTreeMaxBlocks))});
+    l = hi + 1u;
+  }
+}
+
+int Engine::set_tree_fusion(uint32_t hi) {
+  if (hi < 2u || hi > kLevels - 1u) return fail(RAINBOW_E_INVALID, "tree fusion level must be 2..%u", kLevels - 1u);
+  RB_HIP(hipStreamSynchronize(stream));
+  if (hi != tree_fusion) drop_graphs();
+  tree_fusion = hi;
+  plan_tree();
+  return RAINBOW_OK;
+}
+
 int Engine::enqueue(bool integrate) {
   if (sys.empty()) return RAINBOW_OK;
   RB_HIP(launch_clear(args, n_clear, stream));
   RB_HIP(launch_deposit(args, n_deposit, stream));
   RB_HIP(launch_upward(args, n_subtree, stream));
   RB_HIP(launch_top(args, stream));
-  for (uint32_t l = 2; l <= dmax; ++l) RB_HIP(launch_tree(args, l, tree_blocks[l], stream));
+  for (const TreeLaunch& t : tree_launches) RB_HIP(launch_tree(args, t.lo, t.hi, t.blocks, stream));
   if (integrate) RB_HIP(launch_integrate(args, n_integrate, stream));
   return RAINBOW_OK;
 }
@@ -433,6 +467,24 @@ int Engine::autotune(uint32_t steps, uint32_t* chosen) {
   if (steps == 0u) return fail(RAINBOW_E_INVALID, "steps must be >= 1");
   int rc = snapshot(false);
   if (rc) return rc;
+  // Tree fusion: levels 2..hi in one launch, hi = 2..dmax.
+  {
+    float best = 0.0f;
+    uint32_t best_hi = tree_fusion;
+    for (uint32_t hi = 2u; hi <= std::max(2u, dmax); ++hi) {
+      if ((rc = set_tree_fusion(hi)) != 0) return rc;
+      float ms = 0.0f;
+      if ((rc = time_steps(std::min<uint32_t>(steps, 4u), &ms)) != 0) return rc;  // warm graphs
+      if ((rc = snapshot(true)) != 0) return rc;
+      if ((rc = time_steps(steps, &ms)) != 0) return rc;
+      if ((rc = snapshot(true)) != 0) return rc;
+      if (best == 0.0f || ms < best) {
+        best = ms;
+        best_hi = hi;
+      }
+    }
+    if ((rc = set_tree_fusion(best_hi)) != 0) return rc;
+  }
   float best = 0.0f;
   uint32_t best_interval = sort_interval;
   for (uint32_t c : kCandidates) {
@@ -473,9 +525,9 @@ int Engine::time_steps(uint32_t steps, float* ms) {
 }
 
 int Engine::phase_ms(uint32_t steps, float* out, uint32_t n) {
-  // clear, deposit, upward, top, tree 2..dmax, integrate, then the body
-  // sort (its mean per step: the steps sort when due, as in step()).
-  const uint32_t nkern = 4u + dmax;
+  // clear, deposit, upward, top, the tree launches, integrate, then the
+  // body sort (its mean per step: the steps sort when due, as in step()).
+  const uint32_t nkern = 5u + static_cast<uint32_t>(tree_launches.size());
   const uint32_t nphase = nkern + 1u;
   if (out == nullptr || n < nphase) return fail(RAINBOW_E_INVALID, "phase_ms needs %u entries", nphase);
   if (steps == 0u) return fail(RAINBOW_E_INVALID, "steps must be >= 1");
@@ -504,8 +556,8 @@ int Engine::phase_ms(uint32_t steps, float* out, uint32_t n) {
     if (err == hipSuccess) err = launch_deposit(args, n_deposit, stream, &tm[k++]);
     if (err == hipSuccess) err = launch_upward(args, n_subtree, stream, &tm[k++]);
     if (err == hipSuccess) err = launch_top(args, stream, &tm[k++]);
-    for (uint32_t l = 2; l <= dmax; ++l)
-      if (err == hipSuccess) err = launch_tree(args, l, tree_blocks[l], stream, &tm[k++]);
+    for (const TreeLaunch& t : tree_launches)
+      if (err == hipSuccess) err = launch_tree(args, t.lo, t.hi, t.blocks, stream, &tm[k++]);
     if (err == hipSuccess) err = launch_integrate(args, n_integrate, stream, &tm[k++]);
     if (err == hipSuccess) err = hipStreamSynchronize(stream);
     for (uint32_t i = 0; i < nkern && err == hipSuccess; ++i) {
@@ -776,6 +828,22 @@ int rainbow_hip_set_sort_interval(rainbow_hip_t* h, uint32_t interval) {
   return h->batch.engine.set_sort_interval(interval);
 }
 
+int rainbow_hip_set_tree_fusion(rainbow_hip_t* h, uint32_t hi) {
+  RB_CHECK_HANDLE(h);
+  return h->batch.engine.set_tree_fusion(hi);
+}
+
+int rainbow_hip_tree_launches(rainbow_hip_t* h, uint32_t* lo_hi, uint32_t n) {
+  RB_CHECK_HANDLE(h);
+  const auto& tl = h->batch.engine.tree_launches;
+  if (lo_hi == nullptr || n < 2u * tl.size()) return fail(RAINBOW_E_INVALID, "tree_launches needs %zu entries", 2u * tl.size());
+  for (size_t i = 0; i < tl.size(); ++i) {
+    lo_hi[2u * i] = tl[i].lo;
+    lo_hi[2u * i + 1u] = tl[i].hi;
+  }
+  return static_cast<int>(tl.size());
+}
+
 int rainbow_hip_autotune(rainbow_hip_t* h, uint32_t steps, uint32_t* interval) {
   RB_CHECK_HANDLE(h);
   return h->batch.engine.autotune(steps, interval);
@@ -876,6 +944,11 @@ uint64_t rainbow_batch_particles(rainbow_batch_t* b) { return b ? b->batch.engin
 int rainbow_batch_set_sort_interval(rainbow_batch_t* b, uint32_t interval) {
   RB_CHECK_HANDLE(b);
   return b->batch.engine.set_sort_interval(interval);
+}
+
+int rainbow_batch_set_tree_fusion(rainbow_batch_t* b, uint32_t hi) {
+  RB_CHECK_HANDLE(b);
+  return b->batch.engine.set_tree_fusion(hi);
 }
 
 int rainbow_batch_autotune(rainbow_batch_t* b, uint32_t steps, uint32_t* interval) {

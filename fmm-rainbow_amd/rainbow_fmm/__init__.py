@@ -130,6 +130,9 @@ _SIGNATURES = {
     "rainbow_hip_digest": (_i32, [_vp, _i32, _P(_u32)]),
     "rainbow_hip_time_steps": (_i32, [_vp, _u32, _P(_f32)]),
     "rainbow_hip_phase_ms": (_i32, [_vp, _u32, _P(_f32), _u32]),
+    "rainbow_hip_set_tree_fusion": (_i32, [_vp, _u32]),
+    "rainbow_hip_tree_launches": (_i32, [_vp, _P(_u32), _u32]),
+    "rainbow_batch_set_tree_fusion": (_i32, [_vp, _u32]),
     "rainbow_hip_tree_stats": (_i32, [_vp, _P(_u32)]),
     "rainbow_hip_set_sort_interval": (_i32, [_vp, _u32]),
     "rainbow_hip_autotune": (_i32, [_vp, _u32, _P(_u32)]),
@@ -406,6 +409,21 @@ class Solver:
     def set_sort_interval(self, interval: int):
         _check(lib().rainbow_hip_set_sort_interval(self._h, interval))
 
+    def set_tree_fusion(self, hi: int):
+        """Tree levels 2..hi in one launch; each deeper level alone."""
+        _check(lib().rainbow_hip_set_tree_fusion(self._h, hi))
+
+    def tree_launches(self) -> list[tuple[int, int]]:
+        """(lo, hi) levels of each tree launch of a step, in order."""
+        buf = (_u32 * 16)()
+        n = _check(lib().rainbow_hip_tree_launches(self._h, buf, 16))
+        return [(buf[2 * i], buf[2 * i + 1]) for i in range(n)]
+
+    def phase_names(self) -> list[str]:
+        """Names of the entries phase_ms returns."""
+        tree = [f"tree_l{lo}" if lo == hi else f"tree_l{lo}-{hi}" for lo, hi in self.tree_launches()]
+        return ["clear", "deposit", "upward", "top"] + tree + ["integrate", "sort"]
+
     def autotune(self, steps: int = 32) -> int:
         v = _u32()
         _check(lib().rainbow_hip_autotune(self._h, steps, ctypes.byref(v)))
@@ -481,6 +499,9 @@ class Batch:
 
     def set_sort_interval(self, interval: int):
         _check(lib().rainbow_batch_set_sort_interval(self._h, interval))
+
+    def set_tree_fusion(self, hi: int):
+        _check(lib().rainbow_batch_set_tree_fusion(self._h, hi))
 
     def autotune(self, steps: int = 32) -> int:
         v = _u32()

@@ -368,3 +368,49 @@ def test_autotune_restores_state():
     assert chosen in (0, 2, 4, 8, 16, 32)
     assert_bits(s.particles(), before, "bodies after autotune")
     s.close()
+
+
+@pytest.fixture(scope="module")
+def cfg2_reference():
+    c = wl.CFG2
+    parts = wl.cfg2_particles()
+    slots = slots_for(c["n"], c["dt"], c["gravity"], c["softening"])
+    om, of = orc.solve_fields(slots, parts)
+    op, _, _ = orc.step(slots, parts, 2)
+    return parts, om, of, op
+
+
+@pytest.mark.parametrize("hi", [2, 3, 4, 6])
+def test_tree_fusion_does_not_change_results(hi, cfg2_reference):
+    say(f"tree fusion levels 2..{hi}, 1M Plummer")
+    parts, om, of, op = cfg2_reference
+    c = wl.CFG2
+    s = rf.Solver(c["n"], wl.params_for(c))
+    s.set_tree_fusion(hi)
+    launches = s.tree_launches()
+    assert launches[0] == (2, hi) and launches[-1][1] == s.depth
+    assert len(launches) == 1 + (s.depth - hi)
+    s.upload(parts)
+    s.solve_fields()
+    assert_bits(s.moments(), om, "moments")
+    assert_bits(s.fields(), of, "fields")
+    s.upload(parts)
+    s.step(2)
+    assert_bits(s.particles(), op, "particles")
+    s.close()
+
+
+@pytest.mark.parametrize("compensated", [False, True])
+def test_tree_fusion_ball_all_levels(compensated):
+    say(f"tree fusion 2..6, uniform ball 400k, compensated={compensated}")
+    n = 400000
+    parts = rf.ic_uniform_ball(n, 17, 1.0)
+    dt, g, soft = 1e-3, 1.0, 1e-3
+    s = rf.Solver(n, rf.physics(dt=dt, gravity=g, softening=soft), compensated=compensated)
+    s.set_tree_fusion(6)
+    s.upload(parts)
+    s.solve_fields()
+    om, of = orc.solve_fields(slots_for(n, dt, g, soft), parts, compensated=compensated)
+    assert_bits(s.moments(), om, "moments")
+    assert_bits(s.fields(), of, "fields")
+    s.close()
