@@ -699,8 +699,10 @@ __device__ __forceinline__ void stage_window(float4* win, const SysDesc& d, uint
 // LDS of the ancestor path per ancestor: its 6^3 window and its 10 x 216
 // pair terms.
 constexpr uint32_t kAncestorBytes = 216u * 16u + 10u * 216u * 4u;
-// LDS of one item: the window and the four waves' 10 x 64 chunk terms.
-constexpr uint32_t kItemBytes = 216u * 16u + 4u * 10u * 64u * 4u;
+// Chunks of 8 x 8 pairs per round of the tree item loop (two per wave).
+constexpr uint32_t kTreeChunksPerRound = 8u;
+// LDS of one item: the window and a round's 10 x 64 chunk terms.
+constexpr uint32_t kItemBytes = 216u * 16u + kTreeChunksPerRound * 10u * 64u * 4u;
 
 // Fields of the internal ancestors A_lo .. A_{level-1} of an item of
 // `level` whose parent P = A_{level-1} has coordinates (px, py, pz), from
@@ -801,11 +803,12 @@ __device__ void ancestor_fields(const SysDesc& d, uint32_t lo, uint32_t level, u
 template <bool kComp, bool kFused>
 __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t lo, uint32_t hi) {
   constexpr uint32_t kWaves = kTreeThreads / 64u;
+  constexpr uint32_t kChunksPerRound = kTreeChunksPerRound;
   // Single level: static LDS.  Fused: one dynamic region of
   // max(kItemBytes, (hi - lo) * kAncestorBytes) bytes, used by the
   // ancestor path and then by the item.
   __shared__ float4 win_s[kFused ? 1 : 216];
-  __shared__ float terms_s[kFused ? 1 : kWaves * 10 * 64];
+  __shared__ float terms_s[kFused ? 1 : kChunksPerRound * 10 * 64];
   extern __shared__ float4 dyn[];
   __shared__ float pf[12];
   float4* win = kFused ? dyn : win_s;
@@ -879,15 +882,11 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t lo, 
     const int tcount = counts[4u * (off + lin(tx, ty, tz, dim))];
     const float tcx = center1(tx, dim), tcy = center1(ty, dim), tcz = center1(tz, dim);
     const bool leaf = tcount != 0 && (tcount <= kLeafCapacity || level == d.depth);
-    // Chunks: the in-range source-parent blocks in pz, py, px order (8 M2L
-    // sources each), then, if any target is a leaf, 4 blocks of 8 near
-    // offsets.
-    const int ox0 = px > 0u ? -1 : 0, oy0 = py > 0u ? -1 : 0, oz0 = pz > 0u ? -1 : 0;
-    const int nx = (px + 1u < pdim ? 1 : 0) - ox0 + 1;
-    const int ny = (py + 1u < pdim ? 1 : 0) - oy0 + 1;
-    const int nz = (pz + 1u < pdim ? 1 : 0) - oz0 + 1;
-    const uint32_t m2l_chunks = static_cast<uint32_t>(nx * ny * nz);
-    const uint32_t chunks = m2l_chunks + (__syncthreads_or(leaf) ? 4u : 0u);
+    // 32 chunks of 8 sources: the 27 source-parent blocks in pz, py, px
+    // order (M2L; parents outside the level were staged as empty), then 4
+    // blocks of 8 near offsets (leaf targets only), then one empty chunk.
+    // Zero terms leave the sums unchanged, so every item walks all 32.
+    const bool any_leaf = __syncthreads_or(leaf) != 0;
     // cell_epsilon, src/rainbow_fmm.wgsl:507 (0.06 * WORLD_HALF_EXTENT / dim).
     const float cell_epsilon = (0.06f * kWorldHalfExtent) / static_cast<float>(dim);
     const float eps2_m2l = cell_epsilon * cell_epsilon;
@@ -915,52 +914,59 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t lo, 
     }
     __syncthreads();
 
-    for (uint32_t r0 = 0; r0 < chunks; r0 += kWaves) {
-      // Each wave evaluates one chunk's 8 x 8 pair fields.
-      const uint32_t q = r0 + wave;
-      float t[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-      if (q < m2l_chunks) {
-        const int ox = ox0 + static_cast<int>(q % static_cast<uint32_t>(nx));
-        const int oy = oy0 + static_cast<int>((q / static_cast<uint32_t>(nx)) % static_cast<uint32_t>(ny));
-        const int oz = oz0 + static_cast<int>(q / static_cast<uint32_t>(nx * ny));
-        const uint32_t wsx = static_cast<uint32_t>(2 * (ox + 1)) + (cc & 1u);
-        const uint32_t wsy = static_cast<uint32_t>(2 * (oy + 1)) + ((cc >> 1) & 1u);
-        const uint32_t wsz = static_cast<uint32_t>(2 * (oz + 1)) + (cc >> 2);
-        const bool near = (wsx + 1u - wtx) <= 2u && (wsy + 1u - wty) <= 2u && (wsz + 1u - wtz) <= 2u;
-        if (tcount != 0 && !near) {
-          const float4 sr = win[wsx + 6u * (wsy + 6u * wsz)];
-          if (sr.w != 0.0f) pair_terms(sr, tcx, tcy, tcz, eps2_m2l, t);
+    for (uint32_t r0 = 0; r0 < 32u; r0 += kChunksPerRound) {
+      // Each wave evaluates two chunks' 8 x 8 pair fields.
+#pragma unroll
+      for (uint32_t j = 0; j < kChunksPerRound / kWaves; ++j) {
+        const uint32_t slot = wave * (kChunksPerRound / kWaves) + j;
+        const uint32_t q = r0 + slot;
+        float t[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (q < 27u) {
+          const uint32_t wsx = 2u * (q % 3u) + (cc & 1u);
+          const uint32_t wsy = 2u * ((q / 3u) % 3u) + ((cc >> 1) & 1u);
+          const uint32_t wsz = 2u * (q / 9u) + (cc >> 2);
+          const bool near = (wsx + 1u - wtx) <= 2u && (wsy + 1u - wty) <= 2u && (wsz + 1u - wtz) <= 2u;
+          if (tcount != 0 && !near) {
+            const float4 sr = win[wsx + 6u * (wsy + 6u * wsz)];
+            if (sr.w != 0.0f) pair_terms(sr, tcx, tcy, tcz, eps2_m2l, t);
+          }
+        } else if (any_leaf) {
+          // Near field of leaf targets (near_all, src/rainbow_fmm.wgsl:
+          // 533-571): offsets dz, dy, dx; the centre is skipped; neighbours
+          // outside the level were staged as empty.
+          const uint32_t idx = (q - 27u) * 8u + cc;
+          if (leaf && idx < 27u && idx != 13u) {
+            const uint32_t wx = wtx + idx % 3u - 1u;
+            const uint32_t wy = wty + (idx / 3u) % 3u - 1u;
+            const uint32_t wz = wtz + idx / 9u - 1u;
+            const float4 sr = win[wx + 6u * (wy + 6u * wz)];
+            if (sr.w != 0.0f) pair_terms(sr, tcx, tcy, tcz, eps2_near, t);
+          }
         }
-      } else if (q < chunks) {
-        // Near field of leaf targets (near_all, src/rainbow_fmm.wgsl:533-571):
-        // offsets dz, dy, dx; the centre is skipped; neighbours outside the
-        // level were staged as empty.
-        const uint32_t idx = (q - m2l_chunks) * 8u + cc;
-        if (leaf && idx < 27u && idx != 13u) {
-          const uint32_t wx = wtx + idx % 3u - 1u;
-          const uint32_t wy = wty + (idx / 3u) % 3u - 1u;
-          const uint32_t wz = wtz + idx / 9u - 1u;
-          const float4 sr = win[wx + 6u * (wy + 6u * wz)];
-          if (sr.w != 0.0f) pair_terms(sr, tcx, tcy, tcz, eps2_near, t);
-        }
+#pragma unroll
+        for (int k = 0; k < 10; ++k) terms[(slot * 10u + k) * 64u + lane] = t[k];
       }
-#pragma unroll
-      for (int k = 0; k < 10; ++k) terms[(wave * 10u + k) * 64u + lane] = t[k];
       __syncthreads();
-      // The chains add the chunks of this round in order; the L2L term goes
-      // in between the last M2L chunk and the first near chunk.
+      // The chains add the round's chunks in order; the L2L term goes in
+      // between the last M2L chunk (26) and the first near chunk (27).
       if (chain) {
-        for (uint32_t w = 0; w < kWaves; ++w) {
-          const uint32_t qq = r0 + w;
-          if (qq >= chunks) break;
-          if (qq == m2l_chunks) ch.reset(ch.value() + inh);
 #pragma unroll
-          for (uint32_t c = 0; c < 8u; ++c) ch.add(terms[(w * 10u + ck) * 64u + ct * 8u + c]);
+        for (uint32_t u = 0; u < kChunksPerRound; ++u) {
+          if (r0 + u == 27u) ch.reset(ch.value() + inh);
+          const float4 v0 = *reinterpret_cast<const float4*>(&terms[(u * 10u + ck) * 64u + ct * 8u]);
+          const float4 v1 = *reinterpret_cast<const float4*>(&terms[(u * 10u + ck) * 64u + ct * 8u + 4u]);
+          ch.add(v0.x);
+          ch.add(v0.y);
+          ch.add(v0.z);
+          ch.add(v0.w);
+          ch.add(v1.x);
+          ch.add(v1.y);
+          ch.add(v1.z);
+          ch.add(v1.w);
         }
       }
       __syncthreads();
     }
-    if (chain && chunks == m2l_chunks) ch.reset(ch.value() + inh);
 
     // Store each chain's component of its (non-empty) target's field.
     if (chain) {
