@@ -35,7 +35,7 @@ struct alignas(16) SysDesc {
   uint32_t n;          // bodies (particle_count)
   uint32_t depth;      // octree depth
   uint32_t body_base;  // first index of the system in the batch's body space
-  uint32_t pad;
+  uint32_t cell_base;  // first finest cell of the system in the batch's cell space
   float scale;        // fixed-point scale (control_settings.z)
   float dt;
   float gravity;
@@ -86,13 +86,27 @@ struct StepArgs {
   // Per deposit block: the Morton range [lo, hi) of finest cells it owns
   // (stores without atomics); bodies of other cells go to `extra`.  Ranges
   // of one system's blocks are disjoint; (0, 0) owns nothing.
-  const uint2* own;
+  uint2* own;
+  // Counting sort of the bodies by the Morton code of their finest cell,
+  // run within a step when `sorting` is set: the deposit writes each body's
+  // index within its cell to rank (bit 31 set: after the cell's owned
+  // bodies), upward writes each cell's offset within its subtree to cstart
+  // and top each subtree's offset within its system to sstart (both in
+  // Morton order, indexed from the system's cell_base), and the scatter
+  // moves the bodies to tpos/tvel/tperm, copied back before integrate.
+  uint32_t* rank;
+  uint32_t* cstart;
+  uint32_t* sstart;
+  float4* tpos;
+  float4* tvel;
+  uint32_t* tperm;
   uint32_t* lists;             // internal cells (count > 16)
   uint32_t* list_count;        // [kLevels][kStripes] items per stripe
   const uint32_t* stripe_off;  // [kLevels][kStripes] first slot of a stripe
   const uint32_t* stripe_cap;  // [kLevels][kStripes] slots of a stripe
   uint32_t nsys;
   uint32_t compensated;
+  uint32_t sorting;
 };
 
 constexpr uint32_t kClearThreads = 256;

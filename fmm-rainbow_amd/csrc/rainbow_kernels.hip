@@ -46,6 +46,9 @@ namespace {
 constexpr float kWorldHalfExtent = 1.60f;
 constexpr int kLeafCapacity = 16;
 constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
+// Rank flag: the body's index counts from the end of its cell's bodies
+// deposited by the owning block.
+constexpr uint32_t kRankExtra = 0x80000000u;
 
 __device__ __forceinline__ uint32_t level_off(uint32_t level) {
   return ((1u << (3u * level)) - 1u) / 7u;
@@ -129,7 +132,7 @@ __device__ __forceinline__ uint32_t spread3(uint32_t v) {
 // ------------------------------------------------------------------------
 
 __global__ __launch_bounds__(kDepositThreads) void k_deposit(StepArgs a) {
-  __shared__ uint32_t keys[kDepositSlots];
+  __shared__ uint32_t keys[kDepositSlots];  // after the flush: rank base per slot
   __shared__ int vals[4][kDepositSlots];
   const BodyBlock blk = a.deposit_blocks[blockIdx.x];
   const SysDesc d = a.sys[blk.sys];
@@ -144,6 +147,9 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(StepArgs a) {
   __syncthreads();
   const uint32_t dim = 1u << d.depth;
   const uint32_t lane = threadIdx.x & 63u;
+  // Per body of this thread: its hash slot and its index among the block's
+  // bodies of that cell (used when the step sorts).
+  uint32_t my_slot[kDepositPerThread], my_rank[kDepositPerThread];
 #pragma unroll
   for (uint32_t k = 0; k < kDepositPerThread; ++k) {
     const uint32_t i = blk.first + k * kDepositThreads + threadIdx.x;
@@ -175,19 +181,27 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(StepArgs a) {
           v[c] = static_cast<int>(static_cast<uint32_t>(v[c]) + static_cast<uint32_t>(t[c]));
       }
     }
-    const bool tail = lane == 63u || ((heads >> (lane + 1u)) & 1ull) != 0ull;
+    // Heads after this lane; the lane before the next head ends the run.
+    const unsigned long long later = lane == 63u ? 0ull : (heads >> (lane + 1u));
+    const bool tail = lane == 63u || (later & 1ull) != 0ull;
+    uint32_t h = 0u, run_base = 0u;
     if (valid && tail) {
       // Open addressing; at most kDepositBodies distinct keys in a table of
       // 2 kDepositBodies slots, so an empty slot always exists.
-      uint32_t h = (key * 2654435761u) >> (32 - 11);
+      h = (key * 2654435761u) >> (32 - 11);
       for (;;) {
         const uint32_t prev = atomicCAS(&keys[h], kEmptyKey, key);
         if (prev == kEmptyKey || prev == key) break;
         h = (h + 1u) & (kDepositSlots - 1u);
       }
+      run_base = static_cast<uint32_t>(atomicAdd(&vals[0][h], v[0]));
 #pragma unroll
-      for (int c = 0; c < 4; ++c) atomicAdd(&vals[c][h], v[c]);
+      for (int c = 1; c < 4; ++c) atomicAdd(&vals[c][h], v[c]);
     }
+    // The run's last lane holds its slot and its first index in the slot.
+    const uint32_t tail_lane = later == 0ull ? 63u : lane + static_cast<uint32_t>(__ffsll(static_cast<long long>(later))) - 1u;
+    my_slot[k] = static_cast<uint32_t>(__shfl(static_cast<int>(h), static_cast<int>(tail_lane)));
+    my_rank[k] = static_cast<uint32_t>(__shfl(static_cast<int>(run_base), static_cast<int>(tail_lane))) + (lane - start);
   }
   __syncthreads();
   // Cells in this block's Morton range belong to it alone: their sums are
@@ -203,76 +217,121 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(StepArgs a) {
     const uint32_t morton = spread3(x) | (spread3(y) << 1) | (spread3(z) << 2);
     if (morton >= own.x && morton < own.y) {
       finest[key] = make_int4(vals[0][s], vals[1][s], vals[2][s], vals[3][s]);
+      keys[s] = 0u;
       continue;
     }
     // count and sum_x share one 64-bit atomic: the count never carries out
-    // of its 32 bits, and the high word wraps as an i32 sum.
+    // of its 32 bits, and the high word wraps as an i32 sum.  The old count
+    // places this block's bodies after those already in `extra`.
     const unsigned long long packed = static_cast<unsigned long long>(static_cast<uint32_t>(vals[0][s])) |
                                       (static_cast<unsigned long long>(static_cast<uint32_t>(vals[1][s])) << 32);
-    atomicAdd(reinterpret_cast<unsigned long long*>(d.extra + key), packed);
+    const unsigned long long old = atomicAdd(reinterpret_cast<unsigned long long*>(d.extra + key), packed);
     atomicAdd(reinterpret_cast<int*>(d.extra + key) + 2, vals[2][s]);
     atomicAdd(reinterpret_cast<int*>(d.extra + key) + 3, vals[3][s]);
+    keys[s] = kRankExtra | static_cast<uint32_t>(old);
+  }
+  if (!a.sorting) return;
+  __syncthreads();
+#pragma unroll
+  for (uint32_t k = 0; k < kDepositPerThread; ++k) {
+    const uint32_t i = blk.first + k * kDepositThreads + threadIdx.x;
+    if (i < d.n) a.rank[d.body_base + i] = keys[my_slot[k]] + my_rank[k];
   }
 }
 
-// Owned Morton ranges of the deposit blocks from the sorted keys (system <<
-// 18 | Morton): block b owns [key of its first body, key of the next
-// block's first body); a system's first block starts at 0 and its last
-// ends at 2^18.
-__global__ __launch_bounds__(256) void k_own(StepArgs a, uint32_t nblocks, const uint32_t* sorted, uint2* own) {
-  const uint32_t b = blockIdx.x * 256u + threadIdx.x;
-  if (b >= nblocks) return;
-  const BodyBlock blk = a.deposit_blocks[b];
-  const SysDesc d = a.sys[blk.sys];
-  const uint32_t mask = (1u << 18) - 1u;
-  const uint32_t lo = blk.first == 0u ? 0u : (sorted[d.body_base + blk.first] & mask);
-  const uint32_t next = blk.first + kDepositBodies;
-  const uint32_t hi = next >= d.n ? (1u << 18) : (sorted[d.body_base + next] & mask);
-  own[b] = make_uint2(lo, hi);
+// Morton code of the finest cell of a position.
+__device__ __forceinline__ uint32_t morton_of(const float4 p, uint32_t dim) {
+  return spread3(cell1(unit_coord(p.x), dim)) | (spread3(cell1(unit_coord(p.y), dim)) << 1) |
+         (spread3(cell1(unit_coord(p.z), dim)) << 2);
 }
 
-// ------------------------------------------------------------------------
-// body order: Morton keys, gather and copy-back around a radix sort
-// ------------------------------------------------------------------------
+// Inverse of spread3 on 3k-bit codes: every third bit of v.
+__device__ __forceinline__ uint32_t compact3(uint32_t v) {
+  v &= 0x49249249u;
+  v = (v ^ (v >> 2)) & 0xC30C30C3u;
+  v = (v ^ (v >> 4)) & 0x0F00F00Fu;
+  v = (v ^ (v >> 8)) & 0xFF0000FFu;
+  v = (v ^ (v >> 16)) & 0x000003FFu;
+  return v;
+}
 
-// key = system << 18 | Morton code of the finest cell; value = slot.
-__global__ __launch_bounds__(kIntegrateThreads) void k_sort_keys(StepArgs a, uint32_t* keys, uint32_t* vals) {
+// Exclusive prefix sum of v[0..511] in LDS, in place, two values per
+// thread of a 256-thread block (wsum: 4 words of LDS).
+__device__ void scan512(uint32_t* v, uint32_t* wsum, uint32_t t) {
+  const uint32_t a0 = v[2u * t], a1 = v[2u * t + 1u];
+  const uint32_t lane = t & 63u, w = t >> 6;
+  uint32_t x = a0 + a1;
+#pragma unroll
+  for (uint32_t o = 1u; o < 64u; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63u) wsum[w] = x;
+  __syncthreads();
+  uint32_t base = 0u;
+  for (uint32_t j = 0; j < w; ++j) base += wsum[j];
+  const uint32_t excl = base + x - (a0 + a1);
+  v[2u * t] = excl;
+  v[2u * t + 1u] = excl + a0;
+  __syncthreads();
+}
+
+// Scatter of the counting sort (bodies in Morton order of their finest
+// cell): body i goes to slot sstart[subtree] + cstart[cell] + its index in
+// the cell, in the staging buffers.
+__global__ __launch_bounds__(kIntegrateThreads) void k_scatter(StepArgs a) {
   const BodyBlock blk = a.integrate_blocks[blockIdx.x];
   const SysDesc d = a.sys[blk.sys];
+  if (!d.live) return;
   const uint32_t i = blk.first + threadIdx.x;
   if (i >= d.n) return;
   const float4 p = d.pos[i];
   const uint32_t dim = 1u << d.depth;
-  const uint32_t m = spread3(cell1(unit_coord(p.x), dim)) | (spread3(cell1(unit_coord(p.y), dim)) << 1) |
-                     (spread3(cell1(unit_coord(p.z), dim)) << 2);
-  keys[d.body_base + i] = (blk.sys << 18) | m;
-  vals[d.body_base + i] = i;
+  const uint32_t cx = cell1(unit_coord(p.x), dim), cy = cell1(unit_coord(p.y), dim), cz = cell1(unit_coord(p.z), dim);
+  const uint32_t m = spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2);
+  uint32_t r = a.rank[d.body_base + i];
+  if (r & kRankExtra) {
+    // After the bodies the owning block stored: the cell's merged count
+    // minus what the other blocks added through `extra`.
+    const uint32_t c = lin(cx, cy, cz, dim);
+    const uint32_t owned = static_cast<uint32_t>(d.moments[level_off(d.depth) + c].x) -
+                           static_cast<uint32_t>(d.extra[c].x);
+    r = owned + (r & ~kRankExtra);
+  }
+  const uint32_t dst = a.sstart[(d.cell_base >> 9) + (m >> 9)] + a.cstart[d.cell_base + m] + r;
+  if (dst >= d.n) return;  // unreachable when the counts are consistent
+  const uint32_t g = d.body_base + dst;
+  a.tpos[g] = p;
+  a.tvel[g] = d.vel[i];
+  a.tperm[g] = d.perm[i];
 }
 
-__global__ __launch_bounds__(kIntegrateThreads) void k_gather(StepArgs a, const uint32_t* vals, float4* tpos,
-                                                               float4* tvel, uint32_t* tperm) {
+__global__ __launch_bounds__(kIntegrateThreads) void k_copy_back(StepArgs a) {
   const BodyBlock blk = a.integrate_blocks[blockIdx.x];
   const SysDesc d = a.sys[blk.sys];
+  if (!d.live) return;
   const uint32_t i = blk.first + threadIdx.x;
   if (i >= d.n) return;
   const uint32_t g = d.body_base + i;
-  const uint32_t j = vals[g];  // slot of the body that goes to position i
-  if (j >= d.n) return;
-  tpos[g] = d.pos[j];
-  tvel[g] = d.vel[j];
-  tperm[g] = d.perm[j];
+  d.pos[i] = a.tpos[g];
+  d.vel[i] = a.tvel[g];
+  d.perm[i] = a.tperm[g];
 }
 
-__global__ __launch_bounds__(kIntegrateThreads) void k_copy_back(StepArgs a, const float4* tpos,
-                                                                  const float4* tvel, const uint32_t* tperm) {
-  const BodyBlock blk = a.integrate_blocks[blockIdx.x];
+// Owned Morton ranges of the deposit blocks from the sorted bodies: block
+// b owns [cell of its first body, cell of the next block's first body); a
+// system's first block starts at 0 and its last ends at 2^18.
+__global__ __launch_bounds__(256) void k_own(StepArgs a, uint32_t nblocks) {
+  const uint32_t b = blockIdx.x * 256u + threadIdx.x;
+  if (b >= nblocks) return;
+  const BodyBlock blk = a.deposit_blocks[b];
   const SysDesc d = a.sys[blk.sys];
-  const uint32_t i = blk.first + threadIdx.x;
-  if (i >= d.n) return;
-  const uint32_t g = d.body_base + i;
-  d.pos[i] = tpos[g];
-  d.vel[i] = tvel[g];
-  d.perm[i] = tperm[g];
+  if (!d.live) return;
+  const uint32_t dim = 1u << d.depth;
+  const uint32_t lo = blk.first == 0u ? 0u : morton_of(d.pos[blk.first], dim);
+  const uint32_t next = blk.first + kDepositBodies;
+  const uint32_t hi = next >= d.n ? (1u << 18) : morton_of(d.pos[next], dim);
+  a.own[b] = make_uint2(lo, hi);
 }
 
 // ------------------------------------------------------------------------
@@ -449,6 +508,18 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
     lvl0[i] = m;
   }
   __syncthreads();
+  if (a.sorting) {
+    // Offsets of the subtree's cells in Morton order (the subtree's 512
+    // finest cells are the contiguous Morton range root << 9 ..).
+    __shared__ uint32_t sc[512];
+    __shared__ uint32_t wsum[4];
+    for (uint32_t u = t; u < 512u; u += kUpwardThreads)
+      sc[u] = static_cast<uint32_t>(lvl0[compact3(u) + 8u * (compact3(u >> 1) + 8u * compact3(u >> 2))].x);
+    __syncthreads();
+    scan512(sc, wsum, t);
+    const uint32_t root_m = spread3(rx) | (spread3(ry) << 1) | (spread3(rz) << 2);
+    for (uint32_t u = t; u < 512u; u += kUpwardThreads) a.cstart[d.cell_base + (root_m << 9) + u] = sc[u];
+  }
   if (t < 64u) {
     const uint32_t cx = t & 3u, cy = (t >> 2) & 3u, cz = t >> 4;
     const int4 s = sum8(lvl0, cx, cy, cz, 8u);
@@ -567,6 +638,17 @@ __global__ __launch_bounds__(kUpwardThreads) void k_top(StepArgs a) {
       M[level_off(k) + t] = v;
     }
     __syncthreads();
+  }
+  if (a.sorting) {
+    // Offsets of the subtrees (level r cells, Morton order) in the system.
+    __shared__ uint32_t sc[512];
+    __shared__ uint32_t wsum[4];
+    const uint32_t dr = 1u << r;
+    for (uint32_t u = t; u < 512u; u += kUpwardThreads)
+      sc[u] = u < nr ? static_cast<uint32_t>(buf(0)[lin(compact3(u), compact3(u >> 1), compact3(u >> 2), dr)].x) : 0u;
+    __syncthreads();
+    scan512(sc, wsum, t);
+    for (uint32_t u = t; u < nr; u += kUpwardThreads) a.sstart[(d.cell_base >> 9) + u] = sc[u];
   }
   // Levels 0..r-1: source records and lists; levels 1..r: flags.
   uint32_t items[4], slots[4], levs[4];
@@ -1327,26 +1409,14 @@ hipError_t launch_tree(const StepArgs& a, uint32_t lo, uint32_t hi, uint32_t nbl
   return hipGetLastError();
 }
 
-hipError_t launch_own(const StepArgs& a, uint32_t ndeposit, const uint32_t* sorted, uint2* own, hipStream_t stream) {
-  if (ndeposit == 0u) return hipSuccess;
-  hipLaunchKernelGGL(k_own, dim3((ndeposit + 255u) / 256u), dim3(256), 0, stream, a, ndeposit, sorted, own);
-  return hipGetLastError();
-}
-
-hipError_t launch_sort_keys(const StepArgs& a, uint32_t nblocks, uint32_t* keys, uint32_t* vals,
-                            hipStream_t stream) {
-  if (nblocks == 0u) return hipSuccess;
-  hipLaunchKernelGGL(k_sort_keys, dim3(nblocks), dim3(kIntegrateThreads), 0, stream, a, keys, vals);
-  return hipGetLastError();
-}
-
-hipError_t launch_reorder(const StepArgs& a, uint32_t nblocks, const uint32_t* vals, float4* tpos, float4* tvel,
-                          uint32_t* tperm, hipStream_t stream) {
-  if (nblocks == 0u) return hipSuccess;
-  hipLaunchKernelGGL(k_gather, dim3(nblocks), dim3(kIntegrateThreads), 0, stream, a, vals, tpos, tvel, tperm);
+hipError_t launch_sort(const StepArgs& a, uint32_t nintegrate, uint32_t ndeposit, hipStream_t stream) {
+  if (nintegrate == 0u) return hipSuccess;
+  hipLaunchKernelGGL(k_scatter, dim3(nintegrate), dim3(kIntegrateThreads), 0, stream, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_copy_back, dim3(nblocks), dim3(kIntegrateThreads), 0, stream, a, tpos, tvel, tperm);
+  hipLaunchKernelGGL(k_copy_back, dim3(nintegrate), dim3(kIntegrateThreads), 0, stream, a);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_own, dim3((ndeposit + 255u) / 256u), dim3(256), 0, stream, a, ndeposit);
   return hipGetLastError();
 }
 

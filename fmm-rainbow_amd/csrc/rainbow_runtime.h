@@ -35,7 +35,7 @@ const char* last_error();
 
 // Steps captured in the multi-step graph.
 constexpr uint32_t kGraphSteps = 16;
-// Bodies are re-sorted into Morton order before every this many steps by
+// Bodies are re-sorted into Morton order in every this many-th step by
 // default (tunable per engine; 0 = never).
 constexpr uint32_t kDefaultSortInterval = 8;
 // Tree levels 2..kDefaultTreeFusion run in one launch by default (tunable
@@ -85,7 +85,7 @@ class Engine {
   // work tables and drops captured graphs.  Waits for the stream first.
   int configure(const std::vector<SysDesc>& systems);
   int push_descriptors();
-  int enqueue(bool integrate);
+  int enqueue(bool integrate, bool sorting);
   int solve_fields();
   int step(uint32_t steps);
   int sync();
@@ -95,7 +95,6 @@ class Engine {
   int download(uint32_t sys, int which, void* dst, size_t bytes);
   int accelerations(uint32_t sys, void* dst, size_t bytes);
   int diagnostics(uint32_t sys, double out[8]);
-  int sort_bodies();
   int set_sort_interval(uint32_t interval);
   int set_tree_fusion(uint32_t hi);
   int autotune(uint32_t steps, uint32_t* chosen);
@@ -116,18 +115,23 @@ class Engine {
   uint32_t tree_fusion = kDefaultTreeFusion;
 
  private:
-  int capture(uint32_t steps, hipGraphExec_t* out);
+  int capture(uint32_t steps, uint32_t sort_every, hipGraphExec_t* out);
   int prepare_graphs(uint32_t steps);
   uint32_t chunk_steps() const;
   void plan_tree();
   void drop_graphs();
 
   DevBuf sys_buf, clear_buf, deposit_buf, integrate_buf, subtree_buf, list_buf, count_buf,
-      stripe_buf, diag_buf, key_buf, body_buf, sort_tmp, snap_buf, own_buf;
+      stripe_buf, diag_buf, body_buf, snap_buf, own_buf, rank_buf, cstart_buf, sstart_buf;
   uint32_t sort_interval = kDefaultSortInterval;
   uint32_t since_sort = kDefaultSortInterval;
   hipGraphExec_t graph_one = nullptr;
-  hipGraphExec_t graph_many = nullptr;
+  hipGraphExec_t graph_sort = nullptr;
+  hipGraphExec_t graph_chunk = nullptr;
+  hipGraphExec_t graph_chunk_sort = nullptr;
+  hipStream_t side = nullptr;  // the sort's scatter beside the tree pass
+  hipEvent_t ev_fork = nullptr;
+  hipEvent_t ev_join = nullptr;
   hipEvent_t ev_a = nullptr;
   hipEvent_t ev_b = nullptr;
 };

@@ -178,26 +178,32 @@ void DevBuf::release() {
 
 Engine::~Engine() {
   if (stream) (void)hipStreamSynchronize(stream);
+  if (side) (void)hipStreamSynchronize(side);
   drop_graphs();
-  for (DevBuf* b : {&sys_buf, &clear_buf, &deposit_buf, &integrate_buf, &subtree_buf, &list_buf,
-                    &count_buf, &stripe_buf, &diag_buf, &key_buf, &body_buf, &sort_tmp, &snap_buf})
+  for (DevBuf* b : {&sys_buf, &clear_buf, &deposit_buf, &integrate_buf, &subtree_buf, &list_buf, &count_buf,
+                    &stripe_buf, &diag_buf, &body_buf, &snap_buf, &own_buf, &rank_buf, &cstart_buf, &sstart_buf})
     b->release();
-  if (ev_a) (void)hipEventDestroy(ev_a);
-  if (ev_b) (void)hipEventDestroy(ev_b);
+  for (hipEvent_t e : {ev_a, ev_b, ev_fork, ev_join})
+    if (e) (void)hipEventDestroy(e);
+  if (side) (void)hipStreamDestroy(side);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
 void Engine::drop_graphs() {
-  if (graph_one) (void)hipGraphExecDestroy(graph_one);
-  if (graph_many) (void)hipGraphExecDestroy(graph_many);
-  graph_one = graph_many = nullptr;
+  for (hipGraphExec_t* g : {&graph_one, &graph_sort, &graph_chunk, &graph_chunk_sort}) {
+    if (*g) (void)hipGraphExecDestroy(*g);
+    *g = nullptr;
+  }
 }
 
 int Engine::open(uint32_t opts) {
   options = opts;
   RB_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  RB_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
   RB_HIP(hipEventCreate(&ev_a));
   RB_HIP(hipEventCreate(&ev_b));
+  RB_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+  RB_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
   return RAINBOW_OK;
 }
 
@@ -218,8 +224,11 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
   std::vector<uint32_t> caps(kLevels * kStripes, 0u);
   dmax = 0;
   total_bodies = 0;
+  uint64_t total_finest = 0;
   for (uint32_t s = 0; s < nsys; ++s) {
     sys[s].body_base = static_cast<uint32_t>(total_bodies);
+    sys[s].cell_base = static_cast<uint32_t>(total_finest);
+    total_finest += level_cells(sys[s].depth);
     const SysDesc& d = sys[s];
     const uint32_t finest = level_cells(d.depth);
     const uint32_t ib = d.n / (kAdaptiveLeafCapacity + 1u);
@@ -257,11 +266,22 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
       (rc = count_buf.ensure(sizeof(uint32_t) * kLevels * kStripes)) ||
       (rc = stripe_buf.ensure(sizeof(uint32_t) * stripes.size())) ||
       (rc = diag_buf.ensure(sizeof(double) * (8u * static_cast<size_t>(n_integrate) + 8u))) ||
-      (rc = own_buf.ensure(sizeof(uint2) * n_deposit)))
+      (rc = own_buf.ensure(sizeof(uint2) * n_deposit)) || (rc = rank_buf.ensure(4ull * total_bodies)) ||
+      (rc = body_buf.ensure(36ull * total_bodies)) || (rc = cstart_buf.ensure(4ull * total_finest)) ||
+      (rc = sstart_buf.ensure(4ull * (total_finest / 512u + 1u))))
     return rc;
+  if (total_bodies > 0xFFFFFFFFull || total_finest > 0xFFFFFFFFull)
+    return fail(RAINBOW_E_INVALID, "at most 2^32 - 1 bodies and finest cells per batch");
   args.sys = static_cast<const SysDesc*>(sys_buf.p);
   args.clear_blocks = static_cast<const CellBlock*>(clear_buf.p);
-  args.own = static_cast<const uint2*>(own_buf.p);
+  args.own = static_cast<uint2*>(own_buf.p);
+  args.rank = static_cast<uint32_t*>(rank_buf.p);
+  args.cstart = static_cast<uint32_t*>(cstart_buf.p);
+  args.sstart = static_cast<uint32_t*>(sstart_buf.p);
+  args.tpos = static_cast<float4*>(body_buf.p);
+  args.tvel = args.tpos + total_bodies;
+  args.tperm = reinterpret_cast<uint32_t*>(args.tvel + total_bodies);
+  args.sorting = 0u;
   args.deposit_blocks = static_cast<const BodyBlock*>(deposit_buf.p);
   args.integrate_blocks = static_cast<const BodyBlock*>(integrate_buf.p);
   args.subtree_blocks = static_cast<const SubtreeBlock*>(subtree_buf.p);
@@ -293,35 +313,6 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
   RB_HIP(hipMemsetAsync(count_buf.p, 0, sizeof(uint32_t) * kLevels * kStripes, stream));
   RB_HIP(hipStreamSynchronize(stream));
   since_sort = sort_interval;
-  return RAINBOW_OK;
-}
-
-// Reorders every system's bodies by the Morton code of their finest cell
-// (keys carry the system index above 18 code bits).  The order is internal:
-// perm maps slots back to caller indices, and no result depends on it.
-int Engine::sort_bodies() {
-  if (sys.empty() || sys.size() > (1u << 14) || total_bodies > 0xFFFFFFFFull) return RAINBOW_OK;
-  const uint32_t n = static_cast<uint32_t>(total_bodies);
-  uint32_t sys_bits = 0;
-  while ((1u << sys_bits) < sys.size()) ++sys_bits;
-  const uint32_t end_bit = 18u + sys_bits;
-  size_t temp_bytes = 0;
-  RB_HIP(sort_pairs(nullptr, temp_bytes, nullptr, nullptr, nullptr, nullptr, n, end_bit, stream));
-  int rc;
-  if ((rc = key_buf.ensure(4ull * 4u * n)) || (rc = body_buf.ensure(36ull * n)) ||
-      (rc = sort_tmp.ensure(temp_bytes)))
-    return rc;
-  uint32_t* keys0 = static_cast<uint32_t*>(key_buf.p);
-  uint32_t* keys1 = keys0 + n;
-  uint32_t* vals0 = keys1 + n;
-  uint32_t* vals1 = vals0 + n;
-  float4* tpos = static_cast<float4*>(body_buf.p);
-  float4* tvel = tpos + n;
-  uint32_t* tperm = reinterpret_cast<uint32_t*>(tvel + n);
-  RB_HIP(launch_sort_keys(args, n_integrate, keys0, vals0, stream));
-  RB_HIP(sort_pairs(sort_tmp.p, temp_bytes, keys0, keys1, vals0, vals1, n, end_bit, stream));
-  RB_HIP(launch_reorder(args, n_integrate, vals1, tpos, tvel, tperm, stream));
-  RB_HIP(launch_own(args, n_deposit, keys1, static_cast<uint2*>(own_buf.p), stream));
   return RAINBOW_OK;
 }
 
@@ -357,22 +348,38 @@ int Engine::set_tree_fusion(uint32_t hi) {
   return RAINBOW_OK;
 }
 
-int Engine::enqueue(bool integrate) {
+// One step (or, without integrate, one field solve).  A sorting step
+// reorders the bodies by the Morton code of their finest cell: its deposit,
+// upward and top also produce the ranks and offsets, and the scatter, copy-
+// back and owned ranges run on the side stream while the tree pass runs
+// (the tree pass does not read the bodies), joined before integrate.
+int Engine::enqueue(bool integrate, bool sorting) {
   if (sys.empty()) return RAINBOW_OK;
-  RB_HIP(launch_clear(args, n_clear, stream));
-  RB_HIP(launch_deposit(args, n_deposit, stream));
-  RB_HIP(launch_upward(args, n_subtree, stream));
-  RB_HIP(launch_top(args, stream));
-  for (const TreeLaunch& t : tree_launches) RB_HIP(launch_tree(args, t.lo, t.hi, t.blocks, stream));
-  if (integrate) RB_HIP(launch_integrate(args, n_integrate, stream));
+  StepArgs a = args;
+  a.sorting = sorting ? 1u : 0u;
+  RB_HIP(launch_clear(a, n_clear, stream));
+  RB_HIP(launch_deposit(a, n_deposit, stream));
+  RB_HIP(launch_upward(a, n_subtree, stream));
+  RB_HIP(launch_top(a, stream));
+  if (sorting) {
+    RB_HIP(hipEventRecord(ev_fork, stream));
+    RB_HIP(hipStreamWaitEvent(side, ev_fork, 0));
+    RB_HIP(launch_sort(a, n_integrate, n_deposit, side));
+    RB_HIP(hipEventRecord(ev_join, side));
+  }
+  for (const TreeLaunch& t : tree_launches) RB_HIP(launch_tree(a, t.lo, t.hi, t.blocks, stream));
+  if (sorting) RB_HIP(hipStreamWaitEvent(stream, ev_join, 0));
+  if (integrate) RB_HIP(launch_integrate(a, n_integrate, stream));
   return RAINBOW_OK;
 }
 
-int Engine::capture(uint32_t steps, hipGraphExec_t* out) {
+// Captures `steps` steps; step j sorts when sort_every != 0 and j is a
+// multiple of it.
+int Engine::capture(uint32_t steps, uint32_t sort_every, hipGraphExec_t* out) {
   hipGraph_t graph = nullptr;
   RB_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
   int rc = RAINBOW_OK;
-  for (uint32_t s = 0; s < steps && rc == RAINBOW_OK; ++s) rc = enqueue(true);
+  for (uint32_t s = 0; s < steps && rc == RAINBOW_OK; ++s) rc = enqueue(true, sort_every != 0u && s % sort_every == 0u);
   const hipError_t e = hipStreamEndCapture(stream, &graph);
   if (rc != RAINBOW_OK) {
     if (graph) (void)hipGraphDestroy(graph);
@@ -385,42 +392,61 @@ int Engine::capture(uint32_t steps, hipGraphExec_t* out) {
   return RAINBOW_OK;
 }
 
-// Steps per captured multi-step graph: the sort interval, at most
-// kGraphSteps (kGraphSteps when bodies are never re-sorted).
+// Steps per captured multi-step graph: kGraphSteps, or for a sort interval
+// of at most kGraphSteps the largest multiple of it up to kGraphSteps.
 uint32_t Engine::chunk_steps() const {
-  return sort_interval == 0u ? kGraphSteps : std::min(sort_interval, kGraphSteps);
+  if (sort_interval == 0u || sort_interval > kGraphSteps) return kGraphSteps;
+  return sort_interval * (kGraphSteps / sort_interval);
 }
 
+// Graphs: one step, one sorting step, and chunks of chunk_steps() steps
+// without sorts and with sorts due from their first step.
 int Engine::prepare_graphs(uint32_t steps) {
   int rc;
   const uint32_t chunk = chunk_steps();
-  if (steps >= chunk && graph_many == nullptr && (rc = capture(chunk, &graph_many)) != 0) return rc;
-  if (graph_one == nullptr && (rc = capture(1u, &graph_one)) != 0) return rc;
+  const uint32_t every = sort_interval <= kGraphSteps ? sort_interval : chunk;
+  if (graph_one == nullptr && sort_interval != 1u && (rc = capture(1u, 0u, &graph_one)) != 0) return rc;
+  if (sort_interval != 0u && graph_sort == nullptr && (rc = capture(1u, 1u, &graph_sort)) != 0) return rc;
+  if (steps >= chunk && chunk > 1u) {
+    if (sort_interval != 1u && graph_chunk == nullptr && (rc = capture(chunk, 0u, &graph_chunk)) != 0) return rc;
+    if (sort_interval != 0u && graph_chunk_sort == nullptr && (rc = capture(chunk, every, &graph_chunk_sort)) != 0)
+      return rc;
+  }
   return RAINBOW_OK;
 }
 
+// since_sort counts the steps since the last sorting step (that step
+// included); a step sorts when since_sort has reached the interval.
 int Engine::step(uint32_t steps) {
   if (sys.empty() || steps == 0u) return RAINBOW_OK;
   int rc;
   if ((rc = prepare_graphs(steps)) != 0) return rc;
-  // Bodies are re-sorted before every sort_interval steps; the graphs hold
-  // the step kernels only.
   const uint32_t chunk = chunk_steps();
   uint32_t done = 0;
   while (done < steps) {
-    if (sort_interval != 0u && since_sort >= sort_interval) {
-      if ((rc = sort_bodies()) != 0) return rc;
-      since_sort = 0;
-    }
-    const bool aligned = sort_interval == 0u || since_sort % chunk == 0u;
-    if (steps - done >= chunk && aligned && graph_many != nullptr) {
-      RB_HIP(hipGraphLaunch(graph_many, stream));
-      done += chunk;
-      since_sort += chunk;
+    const bool sort_due = sort_interval != 0u && since_sort >= sort_interval;
+    const bool whole = chunk > 1u && steps - done >= chunk;
+    if (sort_due) {
+      if (whole && graph_chunk_sort != nullptr) {
+        RB_HIP(hipGraphLaunch(graph_chunk_sort, stream));
+        done += chunk;
+        since_sort = sort_interval <= kGraphSteps ? sort_interval : chunk;
+      } else {
+        RB_HIP(hipGraphLaunch(graph_sort, stream));
+        done += 1u;
+        since_sort = 1u;
+      }
     } else {
-      RB_HIP(hipGraphLaunch(graph_one, stream));
-      done += 1u;
-      since_sort += 1u;
+      const uint64_t free_steps = sort_interval == 0u ? ~0ull : sort_interval - since_sort;
+      if (whole && free_steps >= chunk && graph_chunk != nullptr) {
+        RB_HIP(hipGraphLaunch(graph_chunk, stream));
+        done += chunk;
+        since_sort += chunk;
+      } else {
+        RB_HIP(hipGraphLaunch(graph_one, stream));
+        done += 1u;
+        since_sort += 1u;
+      }
     }
   }
   return RAINBOW_OK;
@@ -463,7 +489,7 @@ int Engine::snapshot(bool restore) {
 // Times `steps` steps at each candidate sort interval from the same
 // starting state and keeps the fastest; the bodies are restored afterwards.
 int Engine::autotune(uint32_t steps, uint32_t* chosen) {
-  static const uint32_t kCandidates[] = {2u, 4u, 8u, 16u, 32u, 0u};
+  static const uint32_t kCandidates[] = {1u, 2u, 4u, 8u, 16u, 32u, 0u};
   if (steps == 0u) return fail(RAINBOW_E_INVALID, "steps must be >= 1");
   int rc = snapshot(false);
   if (rc) return rc;
@@ -504,7 +530,7 @@ int Engine::autotune(uint32_t steps, uint32_t* chosen) {
   return RAINBOW_OK;
 }
 
-int Engine::solve_fields() { return enqueue(false); }
+int Engine::solve_fields() { return enqueue(false, false); }
 
 int Engine::sync() {
   RB_HIP(hipStreamSynchronize(stream));
@@ -540,25 +566,32 @@ int Engine::phase_ms(uint32_t steps, float* out, uint32_t n) {
   std::vector<double> acc(nphase, 0.0);
   hipError_t err = hipSuccess;
   for (uint32_t s = 0; s < steps && err == hipSuccess; ++s) {
-    if (sort_interval != 0u && since_sort >= sort_interval) {
-      err = hipEventRecord(sort_ev[0], stream);
-      if (err == hipSuccess && sort_bodies() != 0) err = hipErrorUnknown;
-      if (err == hipSuccess) err = hipEventRecord(sort_ev[1], stream);
-      if (err == hipSuccess) err = hipEventSynchronize(sort_ev[1]);
+    const bool sorting = sort_interval != 0u && since_sort >= sort_interval;
+    since_sort = sorting ? 1u : since_sort + 1u;
+    StepArgs a = args;
+    a.sorting = sorting ? 1u : 0u;
+    uint32_t k = 0;
+    if (err == hipSuccess) err = launch_clear(a, n_clear, stream, &tm[k++]);
+    if (err == hipSuccess) err = launch_deposit(a, n_deposit, stream, &tm[k++]);
+    if (err == hipSuccess) err = launch_upward(a, n_subtree, stream, &tm[k++]);
+    if (err == hipSuccess) err = launch_top(a, stream, &tm[k++]);
+    if (sorting) {
+      // The sort kernels run on the side stream beside the tree pass.
+      if (err == hipSuccess) err = hipEventRecord(sort_ev[0], stream);
+      if (err == hipSuccess) err = hipStreamWaitEvent(side, sort_ev[0], 0);
+      if (err == hipSuccess) err = launch_sort(a, n_integrate, n_deposit, side);
+      if (err == hipSuccess) err = hipEventRecord(sort_ev[1], side);
+    }
+    for (const TreeLaunch& t : tree_launches)
+      if (err == hipSuccess) err = launch_tree(a, t.lo, t.hi, t.blocks, stream, &tm[k++]);
+    if (sorting && err == hipSuccess) err = hipStreamWaitEvent(stream, sort_ev[1], 0);
+    if (err == hipSuccess) err = launch_integrate(a, n_integrate, stream, &tm[k++]);
+    if (sorting && err == hipSuccess) {
+      err = hipEventSynchronize(sort_ev[1]);
       float ms = 0.0f;
       if (err == hipSuccess) err = hipEventElapsedTime(&ms, sort_ev[0], sort_ev[1]);
       acc[nkern] += ms;
-      since_sort = 0;
     }
-    since_sort += 1u;
-    uint32_t k = 0;
-    if (err == hipSuccess) err = launch_clear(args, n_clear, stream, &tm[k++]);
-    if (err == hipSuccess) err = launch_deposit(args, n_deposit, stream, &tm[k++]);
-    if (err == hipSuccess) err = launch_upward(args, n_subtree, stream, &tm[k++]);
-    if (err == hipSuccess) err = launch_top(args, stream, &tm[k++]);
-    for (const TreeLaunch& t : tree_launches)
-      if (err == hipSuccess) err = launch_tree(args, t.lo, t.hi, t.blocks, stream, &tm[k++]);
-    if (err == hipSuccess) err = launch_integrate(args, n_integrate, stream, &tm[k++]);
     if (err == hipSuccess) err = hipStreamSynchronize(stream);
     for (uint32_t i = 0; i < nkern && err == hipSuccess; ++i) {
       float ms = 0.0f;

@@ -239,9 +239,10 @@ int rainbow_hip_time_steps(rainbow_hip_t* h, uint32_t steps, float* ms);
 int rainbow_hip_phase_ms(rainbow_hip_t* h, uint32_t steps, float* out,
                          uint32_t n);
 
-/* Bodies are stored in Morton order of their finest cell and re-sorted
- * before every `interval` steps (0 = never; default 8).  The order is
- * internal: downloads return caller order and no result depends on it. */
+/* Bodies are stored in Morton order of their finest cell: every
+ * `interval`-th step (0 = never; default 8) counting-sorts them by the cell
+ * its deposit found, beside its tree pass.  The order is internal:
+ * downloads return caller order and no result depends on it. */
 int rainbow_hip_set_sort_interval(rainbow_hip_t* h, uint32_t interval);
 
 /* The tree pass runs levels 2..hi in one launch (2 <= hi <= 6; default 5):
@@ -257,8 +258,8 @@ int rainbow_hip_tree_launches(rainbow_hip_t* h, uint32_t* lo_hi, uint32_t n);
 
 /* Autotuner: from the same starting state, times `steps` steps at each tree
  * fusion level (2..depth) and keeps the fastest, then at each candidate
- * sort interval (2, 4, 8, 16, 32, never), keeps the fastest and writes it
- * to *interval.  The bodies are restored afterwards. */
+ * sort interval (1, 2, 4, 8, 16, 32, never), keeps the fastest and writes
+ * it to *interval.  The bodies are restored afterwards. */
 int rainbow_hip_autotune(rainbow_hip_t* h, uint32_t steps, uint32_t* interval);
 
 /* Number of cells active in the last solve at each level (M2L targets:

@@ -339,7 +339,7 @@ def test_diagnostics_fp64():
     s.close()
 
 
-@pytest.mark.parametrize("interval", [0, 1, 3, 32])
+@pytest.mark.parametrize("interval", [0, 1, 3, 20, 32])
 def test_sort_interval_does_not_change_results(interval):
     say(f"sort interval {interval}")
     n = 150000
@@ -365,7 +365,7 @@ def test_autotune_restores_state():
     s.step(5)
     before = s.particles()
     chosen = s.autotune(8)
-    assert chosen in (0, 2, 4, 8, 16, 32)
+    assert chosen in (0, 1, 2, 4, 8, 16, 32)
     assert_bits(s.particles(), before, "bodies after autotune")
     s.close()
 
