@@ -223,6 +223,7 @@ def serve_form(args):
             time.sleep(max(0.0, min(0.001, arrivals[nxt] - (time.monotonic() - t0))))
     wall = time.monotonic() - t0
     lat = np.array([done[j].finish_s - (t0 + arrivals[j]) for j in range(nreq)])
+    host = srv.stats()
     srv.close()
     return {
         "metric": "serving: p50/p99 request latency and aggregate particle-steps/s",
@@ -231,6 +232,8 @@ def serve_form(args):
         "n_gpus": 1,
         "p50_ms": float(np.percentile(lat, 50) * 1e3),
         "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "wall_s": wall,
+        "server_host_s": {k: round(v, 4) if isinstance(v, float) else v for k, v in host.items()},
         "higher_is_better": True,
         "dtype": "f32",
         "data": "synthetic",

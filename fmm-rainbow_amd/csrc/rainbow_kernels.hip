@@ -1335,11 +1335,44 @@ __global__ __launch_bounds__(64) void k_diag_final(const double* partial, uint32
   }
 }
 
+// ------------------------------------------------------------------------
+// particle transfer: the reference's interleaved Particle {pos_hue, vel_hue}
+// (32 B) to and from the pos/vel streams, in caller order
+// ------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_split_particles(SysDesc d, const float4* in) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= d.n) return;
+  d.pos[i] = in[2u * i];
+  d.vel[i] = in[2u * i + 1u];
+  d.perm[i] = i;
+}
+
+__global__ __launch_bounds__(256) void k_join_particles(SysDesc d, float4* out) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= d.n) return;
+  const uint32_t j = d.perm[i] < d.n ? d.perm[i] : i;
+  out[2u * j] = d.pos[i];
+  out[2u * j + 1u] = d.vel[i];
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------------
 // launch wrappers
 // ------------------------------------------------------------------------
+
+hipError_t launch_split_particles(const SysDesc& d, const float4* in, hipStream_t stream) {
+  if (d.n == 0u) return hipSuccess;
+  hipLaunchKernelGGL(k_split_particles, dim3((d.n + 255u) / 256u), dim3(256), 0, stream, d, in);
+  return hipGetLastError();
+}
+
+hipError_t launch_join_particles(const SysDesc& d, float4* out, hipStream_t stream) {
+  if (d.n == 0u) return hipSuccess;
+  hipLaunchKernelGGL(k_join_particles, dim3((d.n + 255u) / 256u), dim3(256), 0, stream, d, out);
+  return hipGetLastError();
+}
 
 // Launches through hipExtLaunchKernelGGL when an event pair is given, so
 // that the events carry the dispatch's own start and end times.

@@ -25,6 +25,10 @@ hipError_t launch_integrate(const StepArgs& a, uint32_t nblocks, hipStream_t str
 // The counting sort's scatter, copy-back and owned ranges (after upward and
 // top of a step run with a.sorting set).
 hipError_t launch_sort(const StepArgs& a, uint32_t nintegrate, uint32_t ndeposit, hipStream_t stream);
+// Interleaved particles (32 B each, caller order) to the pos/vel/perm
+// streams of a system, and back.
+hipError_t launch_split_particles(const SysDesc& d, const float4* in, hipStream_t stream);
+hipError_t launch_join_particles(const SysDesc& d, float4* out, hipStream_t stream);
 hipError_t launch_accel(const StepArgs& a, uint32_t nblocks, uint32_t sys, float4* out,
                         hipStream_t stream);
 hipError_t launch_diag(const StepArgs& a, uint32_t nblocks, uint32_t sys, double* partial,

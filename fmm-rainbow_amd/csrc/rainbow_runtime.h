@@ -61,8 +61,12 @@ size_t system_storage_bytes(uint32_t n);
 int alloc_system(uint32_t n, const Physics& ph, float scale, hipStream_t stream, SystemStorage* out);
 void free_system(SystemStorage* s);
 // Copies reference-layout particles in / a buffer out (synchronous on stream).
-int upload_system(const SysDesc& d, const void* data, size_t bytes, hipStream_t stream);
-int download_system(const SysDesc& d, int which, void* dst, size_t bytes, hipStream_t stream);
+struct DevBuf;
+// Particles cross the bus in the reference's interleaved layout through the
+// device staging buffer `io` (grown as needed); kernels split them into the
+// pos/vel streams and restore caller order on the way out.
+int upload_system(const SysDesc& d, const void* data, size_t bytes, hipStream_t stream, DevBuf* io);
+int download_system(const SysDesc& d, int which, void* dst, size_t bytes, hipStream_t stream, DevBuf* io);
 
 // A device buffer that only grows.
 struct DevBuf {
@@ -88,6 +92,7 @@ class Engine {
   int enqueue(bool integrate, bool sorting);
   int solve_fields();
   int step(uint32_t steps);
+  int step_eager(uint32_t steps);
   int sync();
   int time_steps(uint32_t steps, float* ms);
   int phase_ms(uint32_t steps, float* out, uint32_t n);
@@ -123,6 +128,11 @@ class Engine {
 
   DevBuf sys_buf, clear_buf, deposit_buf, integrate_buf, subtree_buf, list_buf, count_buf,
       stripe_buf, diag_buf, body_buf, snap_buf, own_buf, rank_buf, cstart_buf, sstart_buf;
+
+ public:
+  DevBuf io_buf;  // particle staging for uploads and downloads
+
+ private:
   uint32_t sort_interval = kDefaultSortInterval;
   uint32_t since_sort = kDefaultSortInterval;
   hipGraphExec_t graph_one = nullptr;

@@ -181,7 +181,8 @@ Engine::~Engine() {
   if (side) (void)hipStreamSynchronize(side);
   drop_graphs();
   for (DevBuf* b : {&sys_buf, &clear_buf, &deposit_buf, &integrate_buf, &subtree_buf, &list_buf, &count_buf,
-                    &stripe_buf, &diag_buf, &body_buf, &snap_buf, &own_buf, &rank_buf, &cstart_buf, &sstart_buf})
+                    &stripe_buf, &diag_buf, &body_buf, &snap_buf, &own_buf, &rank_buf, &cstart_buf, &sstart_buf,
+                    &io_buf})
     b->release();
   for (hipEvent_t e : {ev_a, ev_b, ev_fork, ev_join})
     if (e) (void)hipEventDestroy(e);
@@ -210,7 +211,23 @@ int Engine::open(uint32_t opts) {
 int Engine::configure(const std::vector<SysDesc>& systems) {
   if (systems.size() > 65535u) return fail(RAINBOW_E_INVALID, "at most 65535 systems");
   RB_HIP(hipStreamSynchronize(stream));
+  RB_HIP(hipStreamSynchronize(side));
   drop_graphs();
+  // Owned deposit ranges of the systems that stay (same body buffer and
+  // count) carry over; the others start owning nothing until their sort.
+  std::vector<uint2> old_own(n_deposit);
+  if (n_deposit > 0u)
+    RB_HIP(hipMemcpy(old_own.data(), own_buf.p, sizeof(uint2) * n_deposit, hipMemcpyDeviceToHost));
+  std::vector<std::pair<const float4*, uint32_t>> old_first;  // (bodies, first deposit block)
+  {
+    uint32_t f = 0;
+    for (const SysDesc& d : sys) {
+      old_first.push_back({d.pos, f});
+      f += (d.n + kDepositBodies - 1u) / kDepositBodies;
+    }
+  }
+  const std::vector<SysDesc> old_sys = sys;
+  std::vector<uint2> own;  // uploaded below; lives until the final synchronize
   sys = systems;
   const uint32_t nsys = static_cast<uint32_t>(sys.size());
   std::vector<CellBlock> clr;
@@ -305,8 +322,18 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
                           stream));
     RB_HIP(hipMemcpyAsync(subtree_buf.p, sub.data(), sizeof(SubtreeBlock) * n_subtree, hipMemcpyHostToDevice,
                           stream));
-    // No block owns a cell until the first sort.
-    RB_HIP(hipMemsetAsync(own_buf.p, 0, sizeof(uint2) * n_deposit, stream));
+    own.assign(n_deposit, make_uint2(0u, 0u));
+    uint32_t f = 0;
+    for (const SysDesc& d : sys) {
+      const uint32_t nb = (d.n + kDepositBodies - 1u) / kDepositBodies;
+      for (size_t o = 0; o < old_sys.size(); ++o)
+        if (old_sys[o].pos == d.pos && old_sys[o].n == d.n) {
+          for (uint32_t b = 0; b < nb; ++b) own[f + b] = old_own[old_first[o].second + b];
+          break;
+        }
+      f += nb;
+    }
+    RB_HIP(hipMemcpyAsync(own_buf.p, own.data(), sizeof(uint2) * n_deposit, hipMemcpyHostToDevice, stream));
   }
   RB_HIP(hipMemcpyAsync(stripe_buf.p, stripes.data(), sizeof(uint32_t) * stripes.size(), hipMemcpyHostToDevice,
                         stream));
@@ -448,6 +475,18 @@ int Engine::step(uint32_t steps) {
         since_sort += 1u;
       }
     }
+  }
+  return RAINBOW_OK;
+}
+
+// Steps launched directly, without graphs (for a set of systems that
+// changes too often to amortise a capture).
+int Engine::step_eager(uint32_t steps) {
+  for (uint32_t k = 0; k < steps; ++k) {
+    const bool sorting = sort_interval != 0u && since_sort >= sort_interval;
+    int rc = enqueue(true, sorting);
+    if (rc) return rc;
+    since_sort = sorting ? 1u : since_sort + 1u;
   }
   return RAINBOW_OK;
 }
@@ -606,49 +645,29 @@ int Engine::phase_ms(uint32_t steps, float* out, uint32_t n) {
   return static_cast<int>(nphase);
 }
 
-int upload_system(const SysDesc& d, const void* data, size_t bytes, hipStream_t stream) {
+int upload_system(const SysDesc& d, const void* data, size_t bytes, hipStream_t stream, DevBuf* io) {
   if (data == nullptr || bytes != 32ull * d.n)
     return fail(RAINBOW_E_INVALID, "upload expects %llu bytes, got %zu",
                 static_cast<unsigned long long>(32ull * d.n), bytes);
-  std::vector<float4> pos(d.n), vel(d.n);
-  std::vector<uint32_t> perm(d.n);
-  const float* f = static_cast<const float*>(data);
-  for (uint32_t i = 0; i < d.n; ++i) {
-    pos[i] = make_float4(f[8u * i + 0u], f[8u * i + 1u], f[8u * i + 2u], f[8u * i + 3u]);
-    vel[i] = make_float4(f[8u * i + 4u], f[8u * i + 5u], f[8u * i + 6u], f[8u * i + 7u]);
-    perm[i] = i;
-  }
-  RB_HIP(hipMemcpyAsync(d.pos, pos.data(), 16ull * d.n, hipMemcpyHostToDevice, stream));
-  RB_HIP(hipMemcpyAsync(d.vel, vel.data(), 16ull * d.n, hipMemcpyHostToDevice, stream));
-  RB_HIP(hipMemcpyAsync(d.perm, perm.data(), 4ull * d.n, hipMemcpyHostToDevice, stream));
+  int rc = io->ensure(bytes);
+  if (rc) return rc;
+  RB_HIP(hipMemcpyAsync(io->p, data, bytes, hipMemcpyHostToDevice, stream));
+  RB_HIP(launch_split_particles(d, static_cast<const float4*>(io->p), stream));
   RB_HIP(hipStreamSynchronize(stream));
   return RAINBOW_OK;
 }
 
-int download_system(const SysDesc& d, int which, void* dst, size_t bytes, hipStream_t stream) {
+int download_system(const SysDesc& d, int which, void* dst, size_t bytes, hipStream_t stream, DevBuf* io) {
   const uint64_t cells = total_cells(d.depth);
   if (dst == nullptr) return fail(RAINBOW_E_INVALID, "null destination");
   if (which == RAINBOW_PARTICLES) {
     if (bytes != 32ull * d.n)
       return fail(RAINBOW_E_INVALID, "particles need %llu bytes", static_cast<unsigned long long>(32ull * d.n));
-    std::vector<float4> pos(d.n), vel(d.n);
-    std::vector<uint32_t> perm(d.n);
-    RB_HIP(hipMemcpyAsync(pos.data(), d.pos, 16ull * d.n, hipMemcpyDeviceToHost, stream));
-    RB_HIP(hipMemcpyAsync(vel.data(), d.vel, 16ull * d.n, hipMemcpyDeviceToHost, stream));
-    RB_HIP(hipMemcpyAsync(perm.data(), d.perm, 4ull * d.n, hipMemcpyDeviceToHost, stream));
+    int rc = io->ensure(bytes);
+    if (rc) return rc;
+    RB_HIP(launch_join_particles(d, static_cast<float4*>(io->p), stream));
+    RB_HIP(hipMemcpyAsync(dst, io->p, bytes, hipMemcpyDeviceToHost, stream));
     RB_HIP(hipStreamSynchronize(stream));
-    float* f = static_cast<float*>(dst);
-    for (uint32_t i = 0; i < d.n; ++i) {
-      const size_t o = 8u * static_cast<size_t>(perm[i] < d.n ? perm[i] : i);
-      f[o + 0u] = pos[i].x;
-      f[o + 1u] = pos[i].y;
-      f[o + 2u] = pos[i].z;
-      f[o + 3u] = pos[i].w;
-      f[o + 4u] = vel[i].x;
-      f[o + 5u] = vel[i].y;
-      f[o + 6u] = vel[i].z;
-      f[o + 7u] = vel[i].w;
-    }
     return RAINBOW_OK;
   }
   if (which == RAINBOW_MOMENTS) {
@@ -672,12 +691,12 @@ int Engine::upload(uint32_t s, const void* data, size_t bytes) {
   if (s >= sys.size()) return fail(RAINBOW_E_INVALID, "system %u out of range", s);
   RB_HIP(hipStreamSynchronize(stream));
   since_sort = sort_interval;
-  return upload_system(sys[s], data, bytes, stream);
+  return upload_system(sys[s], data, bytes, stream, &io_buf);
 }
 
 int Engine::download(uint32_t s, int which, void* dst, size_t bytes) {
   if (s >= sys.size()) return fail(RAINBOW_E_INVALID, "system %u out of range", s);
-  return download_system(sys[s], which, dst, bytes, stream);
+  return download_system(sys[s], which, dst, bytes, stream, &io_buf);
 }
 
 int Engine::accelerations(uint32_t s, void* dst, size_t bytes) {

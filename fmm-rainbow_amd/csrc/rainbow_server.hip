@@ -2,18 +2,24 @@
 // batched step.  A request is admitted into a resident slot when a slot and
 // device memory (counted against a byte budget) are free; its trees and
 // particles stay on the device until its last step; every scheduling
-// iteration runs one batched step over all resident systems, so requests
-// that arrive while others run join at the next step boundary and their
-// work overlaps on the device.  Each system's arithmetic depends only on its
-// own data, so a request's result is the same alone and in any mix.
+// iteration runs batched steps over all resident systems (up to
+// kServeSteps, and no further than the next request's last step), so
+// requests that arrive while others run join at the next iteration and
+// their work overlaps on the device.  Each system's arithmetic depends only
+// on its own data, so a request's result is the same alone and in any mix.
+// The set of systems changes at most iterations, so steps are launched
+// directly rather than captured.
 //
 // The reference has no serving loop; it advances one system per frame
 // (app.js:712-741).
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <deque>
+#include <map>
+#include <cstring>
 #include <new>
 #include <unordered_map>
 #include <vector>
@@ -24,17 +30,60 @@
 namespace rainbow {
 namespace {
 
+constexpr uint32_t kServeSteps = 8;
+
 double now_s() {
   using namespace std::chrono;
   return duration_cast<duration<double>>(steady_clock::now().time_since_epoch()).count();
 }
+
+// Page-locked host buffers, reused: requests are copied into one at submit
+// and results land in one, so transfers run as DMA at full bus speed.
+class PinnedPool {
+ public:
+  ~PinnedPool() {
+    for (auto& kv : free_) (void)hipHostFree(kv.second);
+  }
+  void* get(size_t bytes) {
+    auto it = free_.lower_bound(bytes);
+    if (it != free_.end() && it->first <= 2u * bytes) {
+      void* p = it->second;
+      size_[p] = it->first;
+      free_.erase(it);
+      return p;
+    }
+    void* p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    size_[p] = bytes;
+    return p;
+  }
+  void put(void* p) {
+    if (p == nullptr) return;
+    auto it = size_.find(p);
+    if (it == size_.end()) return;
+    free_.emplace(it->second, p);
+    size_.erase(it);
+  }
+
+ private:
+  std::multimap<size_t, void*> free_;
+  std::unordered_map<void*, size_t> size_;  // buffers in use
+};
 
 struct Pending {
   uint64_t id;
   uint32_t n;
   uint32_t steps;
   double submit_s;
-  std::vector<uint8_t> data;
+  void* data;  // pinned, 32 n bytes
+};
+
+struct Result {
+  void* data;  // pinned, 32 n bytes
+  size_t bytes;
 };
 
 struct Slot {
@@ -51,15 +100,19 @@ struct Slot {
 }  // namespace
 
 // Device bytes a resident request holds: its storage plus its share of the
-// work tables and lists.
+// engine's work tables, lists and sort buffers (ranks and staging: 40 bytes
+// per body; cell offsets: 4 per finest cell; owned ranges: 8 per deposit
+// block).
 uint64_t request_bytes(uint32_t n) {
   const Plan p = make_plan(n);
   uint64_t lists = 0;
   for (uint32_t k = 0; k < p.depth; ++k) lists += std::min<uint64_t>(level_cells(k), n / 17u);
-  const uint64_t blocks = (n + kDepositBodies - 1u) / kDepositBodies + (n + kIntegrateThreads - 1u) / kIntegrateThreads +
-                          level_cells(p.depth - 3u) + (p.cells + kClearCells - 1u) / kClearCells;
+  const uint64_t deposit_blocks = (n + kDepositBodies - 1u) / kDepositBodies;
+  const uint64_t blocks = deposit_blocks + (n + kIntegrateThreads - 1u) / kIntegrateThreads +
+                          level_cells(p.depth - 3u) + (level_cells(p.depth) + kClearCells - 1u) / kClearCells;
   return system_storage_bytes(n) + 4u * lists + 8u * blocks +
-         64u * ((n + kIntegrateThreads - 1u) / kIntegrateThreads) + sizeof(SysDesc) + 4u;
+         64u * ((n + kIntegrateThreads - 1u) / kIntegrateThreads) + sizeof(SysDesc) + 4u + 40ull * n +
+         4ull * level_cells(p.depth) + 4ull * level_cells(p.depth - 3u) + 8ull * deposit_blocks;
 }
 
 }  // namespace rainbow
@@ -76,11 +129,17 @@ struct rainbow_server {
   std::deque<rainbow::Pending> queue;
   std::vector<rainbow::Slot> slots;
   std::deque<rainbow_completion_t> done;
-  std::unordered_map<uint64_t, std::vector<uint8_t>> results;
+  rainbow::PinnedPool pinned;
+  std::unordered_map<uint64_t, rainbow::Result> results;
+  // Host seconds spent admitting (allocation + upload), reconfiguring,
+  // stepping (launch + wait) and retiring (download + free); steps; polls.
+  double stats[6] = {0, 0, 0, 0, 0, 0};
 
   ~rainbow_server() {
     if (engine.stream) (void)hipStreamSynchronize(engine.stream);
     for (auto& s : slots) rainbow::free_system(&s.storage);
+    for (auto& p : queue) pinned.put(p.data);
+    for (auto& r : results) pinned.put(r.second.data);
   }
 
   int reconfigure() {
@@ -93,10 +152,15 @@ struct rainbow_server {
 
   int retire(size_t i, double t) {
     rainbow::Slot& s = slots[i];
-    std::vector<uint8_t> out(32ull * s.n);
-    const int rc = rainbow::download_system(s.storage.desc, RAINBOW_PARTICLES, out.data(), out.size(),
-                                            engine.stream);
-    if (rc) return rc;
+    const size_t bytes = 32ull * s.n;
+    void* out = pinned.get(bytes);
+    if (out == nullptr) return fail(RAINBOW_E_NOMEM, "no pinned host memory for a %zu-byte result", bytes);
+    const int rc = rainbow::download_system(s.storage.desc, RAINBOW_PARTICLES, out, bytes, engine.stream,
+                                            &engine.io_buf);
+    if (rc) {
+      pinned.put(out);
+      return rc;
+    }
     rainbow_completion_t c;
     c.id = s.id;
     c.particles = s.n;
@@ -105,7 +169,9 @@ struct rainbow_server {
     c.admit_s = s.admit_s;
     c.finish_s = t;
     done.push_back(c);
-    results[s.id] = std::move(out);
+    auto old = results.find(s.id);
+    if (old != results.end()) pinned.put(old->second.data);
+    results[s.id] = rainbow::Result{out, bytes};
     rainbow::free_system(&s.storage);
     used -= s.bytes;
     slots.erase(slots.begin() + static_cast<std::ptrdiff_t>(i));
@@ -127,11 +193,12 @@ struct rainbow_server {
       s.bytes = need;
       int rc = rainbow::alloc_system(p.n, physics, 0.0f, engine.stream, &s.storage);
       if (rc) return rc;
-      rc = rainbow::upload_system(s.storage.desc, p.data.data(), p.data.size(), engine.stream);
+      rc = rainbow::upload_system(s.storage.desc, p.data, 32ull * p.n, engine.stream, &engine.io_buf);
       if (rc) {
         rainbow::free_system(&s.storage);
         return rc;
       }
+      pinned.put(p.data);
       s.admit_s = rainbow::now_s();
       used += need;
       queue.pop_front();
@@ -186,29 +253,49 @@ int rainbow_server_submit(rainbow_server_t* s, const rainbow_request_t* r) {
   p.n = r->particles;
   p.steps = r->steps;
   p.submit_s = rainbow::now_s();
-  const uint8_t* src = static_cast<const uint8_t*>(r->data);
-  p.data.assign(src, src + 32ull * r->particles);
-  s->queue.push_back(std::move(p));
+  p.data = s->pinned.get(32ull * r->particles);
+  if (p.data == nullptr) return fail(RAINBOW_E_NOMEM, "no pinned host memory for the request");
+  std::memcpy(p.data, r->data, 32ull * r->particles);
+  s->queue.push_back(p);
   return RAINBOW_OK;
 }
 
 int rainbow_server_poll(rainbow_server_t* s, uint32_t* stepped) {
   if (s == nullptr) return fail(RAINBOW_E_INVALID, "null handle");
   if (stepped) *stepped = 0u;
+  s->stats[5] += 1.0;
+  double t0 = rainbow::now_s();
   int rc = s->admit();
   if (rc) return rc;
+  double t1 = rainbow::now_s();
+  s->stats[0] += t1 - t0;
   // Requests of zero steps complete as soon as they are resident.
   for (size_t i = s->slots.size(); i-- > 0;)
-    if (s->slots[i].left == 0u && (rc = s->retire(i, rainbow::now_s())) != 0) return rc;
+    if (s->slots[i].left == 0u && (rc = s->retire(i, t1)) != 0) return rc;
+  t0 = rainbow::now_s();
+  s->stats[3] += t0 - t1;
   if (s->dirty && (rc = s->reconfigure()) != 0) return rc;
+  t1 = rainbow::now_s();
+  s->stats[1] += t1 - t0;
   if (s->slots.empty()) return RAINBOW_OK;
-  if ((rc = s->engine.step(1u)) != 0) return rc;
+  uint32_t k = rainbow::kServeSteps;
+  for (const auto& slot : s->slots) k = std::min(k, slot.left);
+  if ((rc = s->engine.step_eager(k)) != 0) return rc;
   if ((rc = s->engine.sync()) != 0) return rc;
   const double t = rainbow::now_s();
+  s->stats[2] += t - t1;
+  s->stats[4] += k;
   if (stepped) *stepped = static_cast<uint32_t>(s->slots.size());
-  for (auto& slot : s->slots) slot.left -= 1u;
+  for (auto& slot : s->slots) slot.left -= k;
   for (size_t i = s->slots.size(); i-- > 0;)
     if (s->slots[i].left == 0u && (rc = s->retire(i, t)) != 0) return rc;
+  s->stats[3] += rainbow::now_s() - t;
+  return RAINBOW_OK;
+}
+
+int rainbow_server_stats(rainbow_server_t* s, double out[6]) {
+  if (s == nullptr || out == nullptr) return fail(RAINBOW_E_INVALID, "null argument");
+  for (int k = 0; k < 6; ++k) out[k] = s->stats[k];
   return RAINBOW_OK;
 }
 
@@ -227,10 +314,11 @@ int rainbow_server_result(rainbow_server_t* s, uint64_t id, void* dst, size_t by
   if (s == nullptr || dst == nullptr) return fail(RAINBOW_E_INVALID, "null argument");
   auto it = s->results.find(id);
   if (it == s->results.end()) return fail(RAINBOW_E_STATE, "no result for request %llu", static_cast<unsigned long long>(id));
-  if (bytes != it->second.size())
+  if (bytes != it->second.bytes)
     return fail(RAINBOW_E_INVALID, "result of request %llu is %zu bytes", static_cast<unsigned long long>(id),
-                it->second.size());
-  std::copy(it->second.begin(), it->second.end(), static_cast<uint8_t*>(dst));
+                it->second.bytes);
+  std::memcpy(dst, it->second.data, bytes);
+  s->pinned.put(it->second.data);
   s->results.erase(it);
   return RAINBOW_OK;
 }

@@ -155,6 +155,7 @@ _SIGNATURES = {
     "rainbow_server_request_bytes": (_u64, [_u32]),
     "rainbow_server_submit": (_i32, [_vp, _P(Request)]),
     "rainbow_server_poll": (_i32, [_vp, _P(_u32)]),
+    "rainbow_server_stats": (_i32, [_vp, _P(_f64)]),
     "rainbow_server_completions": (_i32, [_vp, _P(Completion), _u32, _P(_u32)]),
     "rainbow_server_result": (_i32, [_vp, _u64, _vp, _sz]),
     "rainbow_server_state": (_i32, [_vp, _P(_u32), _P(_u32), _P(_u64)]),
@@ -558,6 +559,13 @@ class Server:
         out = np.empty((n, 8), dtype=np.float32)
         _check(lib().rainbow_server_result(self._h, rid, _ptr(out), out.nbytes))
         return out
+
+    def stats(self) -> dict:
+        """Cumulative host seconds per scheduling activity, steps and polls."""
+        buf = (_f64 * 6)()
+        _check(lib().rainbow_server_stats(self._h, buf))
+        return {"admit_s": buf[0], "configure_s": buf[1], "step_s": buf[2], "retire_s": buf[3],
+                "steps": int(buf[4]), "polls": int(buf[5])}
 
     def state(self):
         q, r, b = _u32(), _u32(), _u64()

@@ -350,9 +350,15 @@ uint64_t rainbow_server_request_bytes(uint32_t particles);
  * never fit the budget. */
 int rainbow_server_submit(rainbow_server_t* s, const rainbow_request_t* r);
 /* One scheduling iteration: retire finished systems, admit queued requests
- * while budget and slots allow, run one batched step over every resident
- * system.  Writes the number of systems stepped. */
+ * while budget and slots allow, run batched steps over every resident
+ * system (up to 8, and no further than the earliest last step of a
+ * resident request).  Writes the number of systems stepped. */
 int rainbow_server_poll(rainbow_server_t* s, uint32_t* stepped);
+/* Cumulative host seconds of the scheduling iterations: out[0] admitting
+ * (allocation and upload), out[1] reconfiguring the batch, out[2] stepping
+ * (launch and wait), out[3] retiring (download and free); out[4] batched
+ * steps run, out[5] iterations. */
+int rainbow_server_stats(rainbow_server_t* s, double out[6]);
 /* Pops up to `cap` completions; writes how many. */
 int rainbow_server_completions(rainbow_server_t* s, rainbow_completion_t* out,
                                uint32_t cap, uint32_t* n);
