@@ -244,6 +244,74 @@ def serve_form(args):
     }
 
 
+def precision_form(args):
+    """5,242,880 bodies in 64 Plummer clumps (a = 0.02, cut at 10 a; masses
+    log-uniform over two decades; centres uniform in the unit cube), FP32
+    with Neumaier-compensated sums, 20 steps.  Force error of the FMM
+    acceleration against the FP64 direct sum at 4096 sampled bodies (both
+    the compensated and the plain FP32 variant), FP64 energy and momentum
+    drift over the steps."""
+    import numpy as np
+
+    import rainbow_fmm as rf
+
+    n = args.precision_n
+    steps = 20
+    parts = rf.ic_clumps(n, 7, 64, 0.02, 10.0, 1.0)
+    prm = rf.physics(dt=5e-4, gravity=1.0, softening=1e-3)
+    rng = np.random.default_rng(8)
+    targets = np.sort(rng.choice(n, size=4096, replace=False)).astype(np.uint32)
+
+    def force_error(compensated):
+        s = rf.Solver(n, prm, compensated=compensated)
+        s.upload(parts)
+        s.solve_fields()
+        acc = s.accelerations()[targets]
+        ref = s.direct_sum_f64(targets)
+        s.close()
+        num = np.linalg.norm(acc[:, :3].astype(np.float64) - ref[:, :3], axis=1)
+        den = np.linalg.norm(ref[:, :3], axis=1)
+        rel = num / den
+        return {"median": float(np.median(rel)), "p90": float(np.percentile(rel, 90)),
+                "p99": float(np.percentile(rel, 99)), "max": float(rel.max()),
+                "rms": float(np.sqrt(np.mean(rel ** 2))),
+                "phi_rel_median": float(np.median(np.abs(acc[:, 3] - ref[:, 3]) / np.abs(ref[:, 3])))}
+
+    err_comp = force_error(True)
+    err_plain = force_error(False)
+
+    s = rf.Solver(n, prm, compensated=True)
+    depth = s.depth
+    s.upload(parts)
+    s.solve_fields()
+    d0 = s.diagnostics()
+    ms = s.time_steps(steps)
+    s.solve_fields()
+    d1 = s.diagnostics()
+    s.close()
+    e0, e1 = d0[0] + d0[1], d1[0] + d1[1]
+    p0, p1 = np.linalg.norm(d0[2:5]), np.linalg.norm(d1[2:5])
+    return {
+        "metric": "precision form: force error vs FP64 direct sum, energy drift",
+        "value": n * steps / (ms / 1e3),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "ms_per_step": ms / steps,
+        "higher_is_better": True,
+        "dtype": "f32 (compensated sums) + f64 reductions",
+        "data": "synthetic",
+        "config": {"workload": f"clumps-{n}: 64 Plummer clumps a=0.02 cut at 10a (seed 7), masses 10^U(0,2), "
+                               f"centres in the unit cube, G=1, softening 1e-3, dt 5e-4, depth {depth}"},
+        "force_error_compensated": err_comp,
+        "force_error_plain_f32": err_plain,
+        "force_error_targets": 4096,
+        "energy": {"E0": e0, "E20": e1, "relative_drift": (e1 - e0) / abs(e0),
+                   "kinetic0": d0[0], "potential0": d0[1]},
+        "momentum": {"P0": p0, "P20": p1, "angular0": d0[5:8].tolist(), "angular20": d1[5:8].tolist()},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -253,7 +321,8 @@ def main():
     ap.add_argument("--phase-steps", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--form", choices=["plain", "batch", "serve"], default="plain")
+    ap.add_argument("--form", choices=["plain", "batch", "serve", "precision"], default="plain")
+    ap.add_argument("--precision-n", type=int, default=5242880)
     ap.add_argument("--systems", type=int, default=256)
     ap.add_argument("--requests", type=int, default=96)
     ap.add_argument("--serve-nmax", type=int, default=2097152)
@@ -274,6 +343,9 @@ def main():
         return
     if args.form == "serve":
         print(json.dumps(serve_form(args)), flush=True)
+        return
+    if args.form == "precision":
+        print(json.dumps(precision_form(args)), flush=True)
         return
 
     cfg = dict(wl.CFG2)

@@ -1336,6 +1336,81 @@ __global__ __launch_bounds__(64) void k_diag_final(const double* partial, uint32
 }
 
 // ------------------------------------------------------------------------
+// FP64 direct sum (precision form): the softened Newtonian field of every
+// body (mass 1/n) at target bodies, as the oracle's direct sum defines it.
+// Blocks tile (256 targets) x (source chunk); partial sums per chunk are
+// added in chunk order, so the result does not depend on scheduling.
+// ------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_invert_perm(SysDesc d, uint32_t* inv) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= d.n) return;
+  const uint32_t c = d.perm[i];
+  if (c < d.n) inv[c] = i;
+}
+
+__global__ __launch_bounds__(256) void k_direct_f64(SysDesc d, const uint32_t* inv, const uint32_t* targets,
+                                                    uint32_t nt, uint32_t chunk, double* partial) {
+  __shared__ double sx[256], sy[256], sz[256];
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  const bool valid = t < nt;
+  const uint32_t self = valid ? min(inv[min(targets[t], d.n - 1u)], d.n - 1u) : 0xFFFFFFFFu;
+  double xi = 0.0, yi = 0.0, zi = 0.0;
+  if (valid) {
+    const float4 p = d.pos[self];
+    xi = p.x;
+    yi = p.y;
+    zi = p.z;
+  }
+  const double e2 = static_cast<double>(d.softening) * static_cast<double>(d.softening);
+  const uint32_t j0 = blockIdx.y * chunk;
+  const uint32_t j1 = min(d.n, j0 + chunk);
+  double ax = 0.0, ay = 0.0, az = 0.0, phi = 0.0;
+  for (uint32_t jt = j0; jt < j1; jt += 256u) {
+    const uint32_t j = jt + threadIdx.x;
+    if (j < j1) {
+      const float4 q = d.pos[j];
+      sx[threadIdx.x] = q.x;
+      sy[threadIdx.x] = q.y;
+      sz[threadIdx.x] = q.z;
+    }
+    __syncthreads();
+    const uint32_t m = min(256u, j1 - jt);
+    for (uint32_t k = 0; k < m; ++k) {
+      if (jt + k == self) continue;
+      const double dx = sx[k] - xi, dy = sy[k] - yi, dz = sz[k] - zi;
+      const double r2 = dx * dx + dy * dy + dz * dz + e2;
+      const double inv_r = 1.0 / sqrt(r2);
+      const double inv3 = inv_r * inv_r * inv_r;
+      ax += dx * inv3;
+      ay += dy * inv3;
+      az += dz * inv3;
+      phi -= inv_r;
+    }
+    __syncthreads();
+  }
+  if (valid) {
+    double* o = partial + 4u * (static_cast<size_t>(blockIdx.y) * nt + t);
+    o[0] = ax;
+    o[1] = ay;
+    o[2] = az;
+    o[3] = phi;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_direct_final(SysDesc d, uint32_t nt, uint32_t nchunks,
+                                                      const double* partial, double* out) {
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  if (t >= nt) return;
+  const double gm = static_cast<double>(d.gravity) / static_cast<double>(d.n);
+  for (uint32_t c = 0; c < 4u; ++c) {
+    double s = 0.0;
+    for (uint32_t k = 0; k < nchunks; ++k) s += partial[4u * (static_cast<size_t>(k) * nt + t) + c];
+    out[4u * t + c] = gm * s;
+  }
+}
+
+// ------------------------------------------------------------------------
 // particle transfer: the reference's interleaved Particle {pos_hue, vel_hue}
 // (32 B) to and from the pos/vel streams, in caller order
 // ------------------------------------------------------------------------
@@ -1361,6 +1436,20 @@ __global__ __launch_bounds__(256) void k_join_particles(SysDesc d, float4* out) 
 // ------------------------------------------------------------------------
 // launch wrappers
 // ------------------------------------------------------------------------
+
+hipError_t launch_direct_f64(const SysDesc& d, const uint32_t* targets, uint32_t nt, uint32_t* inv,
+                             double* partial, uint32_t nchunks, double* out, hipStream_t stream) {
+  if (nt == 0u || d.n == 0u) return hipSuccess;
+  hipLaunchKernelGGL(k_invert_perm, dim3((d.n + 255u) / 256u), dim3(256), 0, stream, d, inv);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const uint32_t chunk = ((d.n + nchunks - 1u) / nchunks + 255u) / 256u * 256u;
+  hipLaunchKernelGGL(k_direct_f64, dim3((nt + 255u) / 256u, nchunks), dim3(256), 0, stream, d, inv, targets, nt,
+                     chunk, partial);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_direct_final, dim3((nt + 255u) / 256u), dim3(256), 0, stream, d, nt, nchunks, partial, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_split_particles(const SysDesc& d, const float4* in, hipStream_t stream) {
   if (d.n == 0u) return hipSuccess;

@@ -25,6 +25,11 @@ hipError_t launch_integrate(const StepArgs& a, uint32_t nblocks, hipStream_t str
 // The counting sort's scatter, copy-back and owned ranges (after upward and
 // top of a step run with a.sorting set).
 hipError_t launch_sort(const StepArgs& a, uint32_t nintegrate, uint32_t ndeposit, hipStream_t stream);
+// FP64 direct sum at `nt` target bodies (caller indices, device array):
+// out[4 t ..] = {ax, ay, az, phi}.  inv: n words of scratch; partial:
+// 4 nt nchunks doubles of scratch.
+hipError_t launch_direct_f64(const SysDesc& d, const uint32_t* targets, uint32_t nt, uint32_t* inv,
+                             double* partial, uint32_t nchunks, double* out, hipStream_t stream);
 // Interleaved particles (32 B each, caller order) to the pos/vel/perm
 // streams of a system, and back.
 hipError_t launch_split_particles(const SysDesc& d, const float4* in, hipStream_t stream);

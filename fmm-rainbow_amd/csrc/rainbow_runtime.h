@@ -100,6 +100,7 @@ class Engine {
   int download(uint32_t sys, int which, void* dst, size_t bytes);
   int accelerations(uint32_t sys, void* dst, size_t bytes);
   int diagnostics(uint32_t sys, double out[8]);
+  int direct_sum_f64(uint32_t sys, const uint32_t* targets, uint32_t nt, double* out);
   int set_sort_interval(uint32_t interval);
   int set_tree_fusion(uint32_t hi);
   int autotune(uint32_t steps, uint32_t* chosen);

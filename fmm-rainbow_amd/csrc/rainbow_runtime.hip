@@ -714,6 +714,29 @@ int Engine::accelerations(uint32_t s, void* dst, size_t bytes) {
   return RAINBOW_OK;
 }
 
+int Engine::direct_sum_f64(uint32_t s, const uint32_t* targets, uint32_t nt, double* out) {
+  if (s >= sys.size()) return fail(RAINBOW_E_INVALID, "system %u out of range", s);
+  if (nt == 0u) return RAINBOW_OK;
+  if (targets == nullptr || out == nullptr) return fail(RAINBOW_E_INVALID, "null argument");
+  const SysDesc& d = sys[s];
+  for (uint32_t t = 0; t < nt; ++t)
+    if (targets[t] >= d.n) return fail(RAINBOW_E_INVALID, "target %u out of range", targets[t]);
+  const uint32_t nchunks = std::max(1u, std::min(256u, 4096u / ((nt + 255u) / 256u)));
+  const size_t need = 4ull * d.n + 4ull * nt + 8ull * 4u * nt * (nchunks + 1u) + 64u;
+  int rc = io_buf.ensure(need);
+  if (rc) return rc;
+  char* p = static_cast<char*>(io_buf.p);
+  uint32_t* inv = reinterpret_cast<uint32_t*>(p);
+  uint32_t* dt = inv + d.n;
+  double* partial = reinterpret_cast<double*>(p + ((4ull * d.n + 4ull * nt + 7u) & ~7ull));
+  double* dout = partial + 4ull * nt * nchunks;
+  RB_HIP(hipMemcpyAsync(dt, targets, 4ull * nt, hipMemcpyHostToDevice, stream));
+  RB_HIP(launch_direct_f64(d, dt, nt, inv, partial, nchunks, dout, stream));
+  RB_HIP(hipMemcpyAsync(out, dout, 32ull * nt, hipMemcpyDeviceToHost, stream));
+  RB_HIP(hipStreamSynchronize(stream));
+  return RAINBOW_OK;
+}
+
 int Engine::diagnostics(uint32_t s, double out[8]) {
   if (s >= sys.size()) return fail(RAINBOW_E_INVALID, "system %u out of range", s);
   double* partial = static_cast<double*>(diag_buf.p);
@@ -929,6 +952,11 @@ int rainbow_hip_diagnostics(rainbow_hip_t* h, double out[8]) {
   RB_CHECK_HANDLE(h);
   if (out == nullptr) return fail(RAINBOW_E_INVALID, "null output");
   return h->batch.engine.diagnostics(0u, out);
+}
+
+int rainbow_hip_direct_sum_f64(rainbow_hip_t* h, const uint32_t* targets, uint32_t ntargets, double* out) {
+  RB_CHECK_HANDLE(h);
+  return h->batch.engine.direct_sum_f64(0u, targets, ntargets, out);
 }
 
 int rainbow_hip_accelerations(rainbow_hip_t* h, void* dst, size_t bytes) {

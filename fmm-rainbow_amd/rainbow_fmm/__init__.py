@@ -131,6 +131,7 @@ _SIGNATURES = {
     "rainbow_hip_time_steps": (_i32, [_vp, _u32, _P(_f32)]),
     "rainbow_hip_phase_ms": (_i32, [_vp, _u32, _P(_f32), _u32]),
     "rainbow_hip_set_tree_fusion": (_i32, [_vp, _u32]),
+    "rainbow_hip_direct_sum_f64": (_i32, [_vp, _P(_u32), _u32, _P(_f64)]),
     "rainbow_hip_tree_launches": (_i32, [_vp, _P(_u32), _u32]),
     "rainbow_batch_set_tree_fusion": (_i32, [_vp, _u32]),
     "rainbow_hip_tree_stats": (_i32, [_vp, _P(_u32)]),
@@ -409,6 +410,14 @@ class Solver:
 
     def set_sort_interval(self, interval: int):
         _check(lib().rainbow_hip_set_sort_interval(self._h, interval))
+
+    def direct_sum_f64(self, targets) -> np.ndarray:
+        """FP64 direct-sum {ax, ay, az, phi} at the given bodies (caller indices)."""
+        t = np.ascontiguousarray(targets, dtype=np.uint32)
+        out = np.zeros((t.shape[0], 4), dtype=np.float64)
+        _check(lib().rainbow_hip_direct_sum_f64(self._h, t.ctypes.data_as(_P(_u32)), t.shape[0],
+                                                out.ctypes.data_as(_P(_f64))))
+        return out
 
     def set_tree_fusion(self, hi: int):
         """Tree levels 2..hi in one launch; each deeper level alone."""

@@ -274,6 +274,13 @@ int rainbow_hip_tree_stats(rainbow_hip_t* h, uint32_t out[8]);
  * rainbow_hip_solve_fields for a state that matches the particles. */
 int rainbow_hip_diagnostics(rainbow_hip_t* h, double out[8]);
 
+/* FP64 direct sum over all bodies (mass 1/N each, softening from the
+ * physics) at `ntargets` bodies given by caller index: out[4 t + 0..3] =
+ * {ax, ay, az, phi} = G/N sum_{j != target} {r, -|r|} / (|r|^2 + eps^2)^{3/2
+ * or 1/2}, r = x_j - x_target.  The exact reference for force-error
+ * measurements; O(N ntargets) work on the device. */
+int rainbow_hip_direct_sum_f64(rainbow_hip_t* h, const uint32_t* targets, uint32_t ntargets, double* out);
+
 /* Per-body acceleration the integrator applies (local expansion plus the
  * own-leaf monopole), as float4 {ax, ay, az, phi} per body, for the current
  * particles and the fields of the last solve.  `bytes` = 16 N. */

@@ -414,3 +414,23 @@ def test_tree_fusion_ball_all_levels(compensated):
     assert_bits(s.moments(), om, "moments")
     assert_bits(s.fields(), of, "fields")
     s.close()
+
+
+def test_direct_sum_f64_matches_oracle():
+    say("FP64 direct sum on the device vs the oracle's")
+    n = 40000
+    parts = rf.ic_plummer(n, 21, 0.1, 1.5, 1.0)
+    c = dict(dt=5e-4, gravity=1.0, softening=1e-3)
+    s = rf.Solver(n, rf.physics(**c))
+    s.upload(parts)
+    s.step(3)  # bodies re-ordered internally: targets are caller indices
+    now = s.particles()
+    targets = np.array([0, 1, 7, 999, n - 1] + list(range(100, 40000, 157)), dtype=np.uint32)
+    got = s.direct_sum_f64(targets)
+    # The system holds its softening as an f32.
+    want = orc.direct_sum_f64(now, c["gravity"], float(np.float32(c["softening"])), targets)
+    scale = np.linalg.norm(want[:, :3], axis=1)
+    # FP64 sums in a different order: agreement to ~1e-13 of the field size.
+    np.testing.assert_allclose(got[:, :3], want[:, :3], rtol=0, atol=float(1e-12 * scale.max()))
+    np.testing.assert_allclose(got[:, 3], want[:, 3], rtol=1e-12)
+    s.close()
