@@ -3,15 +3,17 @@
 Workload (default): 1,048,576 equal-mass bodies in a seeded Plummer sphere
 (scale radius 0.1, cut at 1.5, virial velocities, seed 2), G=1, softening
 1e-3, dt 5e-4, planner depth 6; K timed steps after W warm-up steps.  A
-step is clear, deposit, upward, tree(level 1..6), integrate, replayed from a
-captured HIP graph with the bodies already resident in HBM.
+step is deposit, upward, top, tree (levels 2..6 in a few launches),
+integrate, with the body sort every few steps beside the tree pass,
+replayed from captured HIP graphs with the bodies resident in HBM.
 
 Prints ONE JSON line.  `roofline` describes the kernel with the largest
-mean duration (HIP events around each launch on the library's stream,
-outside the timed region): algorithmic bytes per launch / mean duration
-against the 8 TB/s HBM peak.  `cpu_baseline` times the oracle (the C++ CPU
-restatement of the reference step, OpenMP) on the same input for a bounded
-number of steps.
+mean duration (event pairs recorded by each kernel dispatch, over the same
+number of steps run un-graphed after the timed region): algorithmic bytes
+per launch / mean duration against the 8 TB/s HBM peak, or for a tree
+launch algorithmic FP32 operations against the 157.3 TFLOP/s FP32 peak.
+`cpu_baseline` times the oracle (the C++ CPU restatement of the reference
+step, OpenMP) on the same input for a bounded number of steps.
 """
 
 import argparse
@@ -31,10 +33,9 @@ FP32_PEAK_TFLOPS = 157.3
 def algorithmic_bytes(n, depth, stats):
     """Compulsory bytes per launch of each kernel (each unique address once).
 
-    clear      16 C                      zero the moments
     deposit    16 N + 16 C_d             read positions, write finest moments
-    upward     read the finest cells and the levels below depth-3 once,
-               write levels depth-1..depth-3
+    upward     read the finest cells once, zero the next step's finest
+               buffer, write levels depth-1..depth-3
     top        read level depth-3, write the levels above
     tree(l)    16 T_l + 48 T_l + 48 P_l: read the level's target moments,
                write their fields, read their parents' fields (the 6^3
@@ -47,8 +48,8 @@ def algorithmic_bytes(n, depth, stats):
     cd = 8 ** depth
     leaves = stats[7]
     top = (8 ** (depth - 2) - 1) // 7  # cells of levels 0..depth-3
-    out = {"clear": 16 * cells, "deposit": 16 * n + 16 * cd,
-           "upward": 16 * (cells - top) + 16 * (cells - cd - top + 8 ** (depth - 3)),
+    out = {"deposit": 16 * n + 16 * cd,
+           "upward": 16 * cd + 16 * cd + 16 * (cells - cd - top + 8 ** (depth - 3)),
            "top": 16 * (8 ** (depth - 3)) + 16 * (top - 8 ** (depth - 3))}
     for l in range(2, depth + 1):
         t = stats[l]

@@ -10,7 +10,12 @@
 //   source  float4[C]   {com.xyz, G m} per cell, G m = 0 for an empty cell
 //   active  uint8[C]    cell was an M2L target in the last solve
 //   extra   int4[8^D]   finest-level moments of bodies deposited by a block
-//                       that does not own their cell (merged by upward)
+//                       that does not own their cell (merged and zeroed by
+//                       upward)
+//   finest_alt int4[8^D] the finest level's second buffer: steps alternate
+//                       between the moments array's finest level (parity 0)
+//                       and this one (parity 1); a step's upward zeroes the
+//                       buffer the next step deposits into
 // with the reference's cell numbering level_offset(l) + x + 2^l (y + 2^l z).
 // A step works on a set of systems (a plain solve is a set of one); each
 // system's descriptor carries its own buffer pointers, so systems can join
@@ -32,6 +37,7 @@ struct alignas(16) SysDesc {
   float4* source;
   uint8_t* active;
   int4* extra;
+  int4* finest_alt;
   uint32_t n;          // bodies (particle_count)
   uint32_t depth;      // octree depth
   uint32_t body_base;  // first index of the system in the batch's body space
@@ -47,7 +53,7 @@ struct alignas(16) SysDesc {
   float control_point[4];
   float control_axis_mode[4];
 };
-static_assert(sizeof(SysDesc) == 144, "SysDesc layout");
+static_assert(sizeof(SysDesc) == 160, "SysDesc layout");
 
 // Work list of the particle passes: a block covers bodies [first, first +
 // block size) of one system.
@@ -63,13 +69,6 @@ struct alignas(8) SubtreeBlock {
   uint32_t root;  // linear index of the subtree root within level depth-3
 };
 
-// Work list of the clear pass: a block zeroes cells [first, first +
-// kClearCells) of one system.
-struct alignas(8) CellBlock {
-  uint32_t sys;
-  uint32_t first;
-};
-
 constexpr uint32_t kLevels = 7;  // levels 0..6
 
 // Internal cells are listed per level in kStripes stripes; a block appends
@@ -79,7 +78,6 @@ constexpr uint32_t kStripes = 32;
 // Everything a kernel of the step needs, passed by value.
 struct StepArgs {
   const SysDesc* sys;
-  const CellBlock* clear_blocks;
   const BodyBlock* deposit_blocks;
   const BodyBlock* integrate_blocks;
   const SubtreeBlock* subtree_blocks;
@@ -96,6 +94,7 @@ struct StepArgs {
   // moves the bodies to tpos/tvel/tperm, copied back before integrate.
   uint32_t* rank;
   uint32_t* cstart;
+  uint32_t* xstart;  // cstart + the cell's bodies stored by its owning block
   uint32_t* sstart;
   float4* tpos;
   float4* tvel;
@@ -107,10 +106,9 @@ struct StepArgs {
   uint32_t nsys;
   uint32_t compensated;
   uint32_t sorting;
+  uint32_t parity;  // finest-level buffer of this step (see finest_alt)
 };
 
-constexpr uint32_t kClearThreads = 256;
-constexpr uint32_t kClearCells = 4096;
 constexpr uint32_t kDepositThreads = 256;
 constexpr uint32_t kDepositPerThread = 4;
 constexpr uint32_t kDepositBodies = kDepositThreads * kDepositPerThread;  // per block

@@ -1,23 +1,27 @@
 // gfx950 kernels of the p=1 Cartesian uniform-octree FMM step.
 //
-// One step is six passes over a batch of systems (a plain solve is a batch
+// One step is five passes over a batch of systems (a plain solve is a batch
 // of one), launched in order on one stream and captured in a HIP graph:
 //
-//   clear      zero moments and the work-list counters
 //   deposit    P2M: finest-cell binning and fixed-point moment sums,
-//              pre-aggregated per block in an LDS hash table, then global
-//              integer atomics (exact in any order)
-//   upward     M2M: one block per 8x8x8 finest-cell subtree sums three levels
-//              in LDS, writes each cell's source record (centre of mass and
-//              G m), lists the internal cells (count > 16) and zeroes the
-//              fields of cells that stopped being M2L targets
+//              pre-aggregated per block in an LDS hash table; a block stores
+//              the cells of its own Morton range and adds the others into a
+//              straggler accumulator with integer atomics (exact in any
+//              order)
+//   upward     M2M: one block per 8x8x8 finest-cell subtree merges the
+//              stragglers, sums three levels in LDS, writes each cell's
+//              source record (centre of mass and G m), lists the internal
+//              cells (count > 16), zeroes the fields of cells that stopped
+//              being M2L targets and the finest-level buffer of the next step
 //   top        the levels above the subtrees and the fields of level 1,
 //              one block per system
-//   tree(l)    for l = 2..depth: M2L, L2L and the near field of every M2L
-//              target of level l (a non-empty child of an internal cell),
-//              four waves per internal parent
+//   tree       for levels 2..depth, in a few launches over runs of levels:
+//              M2L, L2L and the near field of every M2L target (a non-empty
+//              child of an internal cell), four waves per internal parent
 //   integrate  adaptive descent, local expansion, own-leaf monopole,
 //              symplectic Euler kick-drift and the soft absorbing boundary
+// A sorting step also reorders the bodies by Morton code (scatter, copy-back
+// and owned ranges) on a second stream beside the tree pass.
 //
 // The reference runs the same physics as 2d+5 dispatches
 // (native/main.rs:411-421): clear_cells, deposit, upward per level, m2l_all,
@@ -97,24 +101,14 @@ __device__ __forceinline__ void load_field(const float4* fields, uint32_t cell, 
   j1 = fields[3u * cell + 2u];
 }
 
-// ------------------------------------------------------------------------
-// clear
-// ------------------------------------------------------------------------
+// The finest level of the step's parity (see SysDesc::finest_alt).
+__device__ __forceinline__ int4* finest_of(const SysDesc& d, uint32_t parity) {
+  return parity ? d.finest_alt : d.moments + level_off(d.depth);
+}
 
-__global__ __launch_bounds__(kClearThreads) void k_clear(StepArgs a) {
-  if (blockIdx.x == 0)
-    for (uint32_t i = threadIdx.x; i < kLevels * kStripes; i += kClearThreads) a.list_count[i] = 0u;
-  const CellBlock blk = a.clear_blocks[blockIdx.x];
-  const SysDesc d = a.sys[blk.sys];
-  if (!d.live) return;
-  // Only the finest level: upward and top rewrite every coarser cell.
-  const uint32_t ncell = 1u << (3u * d.depth);
-  const uint32_t end = min(blk.first + kClearCells, ncell);
-  int4* finest = d.moments + level_off(d.depth);
-  for (uint32_t i = blk.first + threadIdx.x; i < end; i += kClearThreads) {
-    finest[i] = make_int4(0, 0, 0, 0);
-    d.extra[i] = make_int4(0, 0, 0, 0);
-  }
+// Body count of a cell (level, index within the level) at a parity.
+__device__ __forceinline__ int cell_count(const SysDesc& d, uint32_t parity, uint32_t level, uint32_t idx) {
+  return level == d.depth ? finest_of(d, parity)[idx].x : d.moments[level_off(level) + idx].x;
 }
 
 // Interleave the low 10 bits of v with two zero bits between each.
@@ -134,6 +128,11 @@ __device__ __forceinline__ uint32_t spread3(uint32_t v) {
 __global__ __launch_bounds__(kDepositThreads) void k_deposit(StepArgs a) {
   __shared__ uint32_t keys[kDepositSlots];  // after the flush: rank base per slot
   __shared__ int vals[4][kDepositSlots];
+  // The step's work-list counters start at zero (the previous step's tree
+  // pass, their last reader, has finished; upward, their first writer,
+  // runs after this kernel).
+  if (blockIdx.x == 0u)
+    for (uint32_t i = threadIdx.x; i < kLevels * kStripes; i += kDepositThreads) a.list_count[i] = 0u;
   const BodyBlock blk = a.deposit_blocks[blockIdx.x];
   const SysDesc d = a.sys[blk.sys];
   if (!d.live) return;
@@ -209,7 +208,7 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(StepArgs a) {
   // moved since the last sort, and cells shared with other blocks);
   // upward adds `extra` into the finest level.
   const uint2 own = a.own[blockIdx.x];
-  int4* finest = d.moments + level_off(d.depth);
+  int4* finest = finest_of(d, a.parity);
   for (uint32_t s = threadIdx.x; s < kDepositSlots; s += kDepositThreads) {
     const uint32_t key = keys[s];
     if (key == kEmptyKey) continue;
@@ -289,16 +288,11 @@ __global__ __launch_bounds__(kIntegrateThreads) void k_scatter(StepArgs a) {
   const uint32_t dim = 1u << d.depth;
   const uint32_t cx = cell1(unit_coord(p.x), dim), cy = cell1(unit_coord(p.y), dim), cz = cell1(unit_coord(p.z), dim);
   const uint32_t m = spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2);
-  uint32_t r = a.rank[d.body_base + i];
-  if (r & kRankExtra) {
-    // After the bodies the owning block stored: the cell's merged count
-    // minus what the other blocks added through `extra`.
-    const uint32_t c = lin(cx, cy, cz, dim);
-    const uint32_t owned = static_cast<uint32_t>(d.moments[level_off(d.depth) + c].x) -
-                           static_cast<uint32_t>(d.extra[c].x);
-    r = owned + (r & ~kRankExtra);
-  }
-  const uint32_t dst = a.sstart[(d.cell_base >> 9) + (m >> 9)] + a.cstart[d.cell_base + m] + r;
+  const uint32_t r = a.rank[d.body_base + i];
+  // Bodies flagged in the rank follow the cell's bodies stored by its
+  // owning block.
+  const uint32_t* cs = (r & kRankExtra) ? a.xstart : a.cstart;
+  const uint32_t dst = a.sstart[(d.cell_base >> 9) + (m >> 9)] + cs[d.cell_base + m] + (r & ~kRankExtra);
   if (dst >= d.n) return;  // unreachable when the counts are consistent
   const uint32_t g = d.body_base + dst;
   a.tpos[g] = p;
@@ -490,21 +484,28 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
   int4* M = d.moments;
   if (t < 4u) ps.n[t] = 0u;
 
-  // Finest level: 8x8x8 cells, x fastest (rows of 8 contiguous moments),
-  // plus the sums deposited into `extra` (wrapping i32 adds, exact in any
-  // order), written back when non-zero.
+  // Finest level (this step's buffer): 8x8x8 cells, x fastest (rows of 8
+  // contiguous moments), plus the sums deposited into `extra` (wrapping i32
+  // adds, exact in any order), written back when non-zero; `extra` and the
+  // other buffer's cells are zeroed for the next step's deposit.
+  __shared__ uint32_t owned[512];
+  int4* const F = finest_of(d, a.parity);
+  int4* const next = finest_of(d, a.parity ^ 1u);
   for (uint32_t i = t; i < 512u; i += kUpwardThreads) {
     const uint32_t ix = i & 7u, iy = (i >> 3) & 7u, iz = i >> 6;
     const uint32_t c = lin(8u * rx + ix, 8u * ry + iy, 8u * rz + iz, 1u << D);
-    int4 m = M[level_off(D) + c];
+    int4 m = F[c];
+    owned[i] = static_cast<uint32_t>(m.x);
     const int4 e = d.extra[c];
     if (e.x != 0) {
       m.x = static_cast<int>(static_cast<uint32_t>(m.x) + static_cast<uint32_t>(e.x));
       m.y = static_cast<int>(static_cast<uint32_t>(m.y) + static_cast<uint32_t>(e.y));
       m.z = static_cast<int>(static_cast<uint32_t>(m.z) + static_cast<uint32_t>(e.z));
       m.w = static_cast<int>(static_cast<uint32_t>(m.w) + static_cast<uint32_t>(e.w));
-      M[level_off(D) + c] = m;
+      F[c] = m;
+      d.extra[c] = make_int4(0, 0, 0, 0);
     }
+    next[c] = make_int4(0, 0, 0, 0);
     lvl0[i] = m;
   }
   __syncthreads();
@@ -518,7 +519,11 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
     __syncthreads();
     scan512(sc, wsum, t);
     const uint32_t root_m = spread3(rx) | (spread3(ry) << 1) | (spread3(rz) << 2);
-    for (uint32_t u = t; u < 512u; u += kUpwardThreads) a.cstart[d.cell_base + (root_m << 9) + u] = sc[u];
+    for (uint32_t u = t; u < 512u; u += kUpwardThreads) {
+      const uint32_t g = d.cell_base + (root_m << 9) + u;
+      a.cstart[g] = sc[u];
+      a.xstart[g] = sc[u] + owned[compact3(u) + 8u * (compact3(u >> 1) + 8u * compact3(u >> 2))];
+    }
   }
   if (t < 64u) {
     const uint32_t cx = t & 3u, cy = (t >> 2) & 3u, cz = t >> 4;
@@ -952,7 +957,6 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t lo, 
     const uint32_t prest = pidx - pz * pdim * pdim;
     const uint32_t py = prest / pdim;
     const uint32_t px = prest - py * pdim;
-    const int* counts = reinterpret_cast<const int*>(d.moments);
 
     // The parent's field: from memory when its level is below lo, else
     // recomputed along the chain of ancestors from level lo - 1.
@@ -961,7 +965,7 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t lo, 
 
     stage_window(win, d, level, px, py, pz, tid);
     const uint32_t tx = 2u * px + (tt & 1u), ty = 2u * py + ((tt >> 1) & 1u), tz = 2u * pz + (tt >> 2);
-    const int tcount = counts[4u * (off + lin(tx, ty, tz, dim))];
+    const int tcount = cell_count(d, a.parity, level, lin(tx, ty, tz, dim));
     const float tcx = center1(tx, dim), tcy = center1(ty, dim), tcz = center1(tz, dim);
     const bool leaf = tcount != 0 && (tcount <= kLeafCapacity || level == d.depth);
     // 32 chunks of 8 sources: the 27 source-parent blocks in pz, py, px
@@ -1054,7 +1058,7 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t lo, 
     if (chain) {
       const uint32_t x = 2u * px + (ct & 1u), y = 2u * py + ((ct >> 1) & 1u), z = 2u * pz + (ct >> 2);
       const uint32_t g = off + lin(x, y, z, dim);
-      if (counts[4u * g] != 0) {
+      if (cell_count(d, a.parity, level, lin(x, y, z, dim)) != 0) {
         float* f = reinterpret_cast<float*>(d.fields + 3u * g);
         f[ck] = ch.value();
         if (ck == 9u) {
@@ -1083,9 +1087,8 @@ struct Leaf {
 // potential phi_c - a_c.d - d.(J d)/2 - G (count-1)/N / r_soft, which the
 // reference does not form (energy diagnostic only).
 template <bool kComp, bool kPhi>
-__device__ __forceinline__ Leaf leaf_field(const SysDesc& d, float px, float py, float pz) {
+__device__ __forceinline__ Leaf leaf_field(const SysDesc& d, uint32_t parity, float px, float py, float pz) {
   const float ux = unit_coord(px), uy = unit_coord(py), uz = unit_coord(pz);
-  const int* counts = reinterpret_cast<const int*>(d.moments);
   Leaf out;
   // The cell of every level depends only on the position, so all counts are
   // fetched at once; the leaf is then the first level with count <= 16, or
@@ -1096,7 +1099,7 @@ __device__ __forceinline__ Leaf leaf_field(const SysDesc& d, float px, float py,
   for (uint32_t level = 1u; level < kLevels; ++level) {
     if (level <= d.depth) {
       const uint32_t sh = d.depth - level;
-      lc[level] = counts[4u * (level_off(level) + lin(fx >> sh, fy >> sh, fz >> sh, 1u << level))];
+      lc[level] = cell_count(d, parity, level, lin(fx >> sh, fy >> sh, fz >> sh, 1u << level));
     }
   }
   uint32_t chosen = d.depth;
@@ -1113,7 +1116,11 @@ __device__ __forceinline__ Leaf leaf_field(const SysDesc& d, float px, float py,
   const float dx = px - center1(c[0], out.dim);
   const float dy = py - center1(c[1], out.dim);
   const float dz = pz - center1(c[2], out.dim);
-  const uint32_t count = static_cast<uint32_t>(counts[4u * cell]);
+  int chosen_count = lc[1];
+#pragma unroll
+  for (uint32_t level = 2u; level < kLevels; ++level)
+    if (level == chosen) chosen_count = lc[level];
+  const uint32_t count = static_cast<uint32_t>(chosen_count);
   out.count = count;
   float self[3] = {0.0f, 0.0f, 0.0f};
   float self_phi = 0.0f;
@@ -1183,7 +1190,7 @@ __global__ __launch_bounds__(kIntegrateThreads) void k_integrate(StepArgs a) {
   const float4 v4 = d.vel[i];
   float pos[3] = {p4.x, p4.y, p4.z};
   float velocity[3] = {v4.x, v4.y, v4.z};
-  const Leaf leaf = leaf_field<kComp, false>(d, p4.x, p4.y, p4.z);
+  const Leaf leaf = leaf_field<kComp, false>(d, a.parity, p4.x, p4.y, p4.z);
   float acceleration[3] = {leaf.ax, leaf.ay, leaf.az};
   const uint32_t count = leaf.count;
 
@@ -1285,7 +1292,7 @@ __global__ __launch_bounds__(kIntegrateThreads) void k_accel(StepArgs a, uint32_
   const uint32_t i = blk.first + threadIdx.x;
   if (i >= d.n) return;
   const float4 p4 = d.pos[i];
-  const Leaf leaf = leaf_field<kComp, true>(d, p4.x, p4.y, p4.z);
+  const Leaf leaf = leaf_field<kComp, true>(d, a.parity, p4.x, p4.y, p4.z);
   out[d.perm[i]] = make_float4(leaf.ax, leaf.ay, leaf.az, leaf.phi);
 }
 
@@ -1302,7 +1309,7 @@ __global__ __launch_bounds__(kIntegrateThreads) void k_diag_partial(StepArgs a, 
   if (blk.sys == sys && i < d.n) {
     const float4 p4 = d.pos[i];
     const float4 v4 = d.vel[i];
-    const Leaf leaf = leaf_field<kComp, true>(d, p4.x, p4.y, p4.z);
+    const Leaf leaf = leaf_field<kComp, true>(d, a.parity, p4.x, p4.y, p4.z);
     const double m = 1.0 / static_cast<double>(d.n);
     const double x = p4.x, y = p4.y, z = p4.z, vx = v4.x, vy = v4.y, vz = v4.z;
     v[0] = 0.5 * m * (vx * vx + vy * vy + vz * vz);
@@ -1486,12 +1493,6 @@ static hipError_t no_launch(hipStream_t stream, const Timing* t) {
   if (t == nullptr) return hipSuccess;
   hipError_t e = hipEventRecord(t->start, stream);
   return e != hipSuccess ? e : hipEventRecord(t->stop, stream);
-}
-
-hipError_t launch_clear(const StepArgs& a, uint32_t nblocks, hipStream_t stream, const Timing* t) {
-  if (nblocks == 0u) return no_launch(stream, t);
-  RB_LAUNCH(t, k_clear, dim3(nblocks), dim3(kClearThreads), stream, a);
-  return hipGetLastError();
 }
 
 hipError_t launch_deposit(const StepArgs& a, uint32_t nblocks, hipStream_t stream, const Timing* t) {

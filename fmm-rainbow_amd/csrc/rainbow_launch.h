@@ -14,7 +14,6 @@ struct Timing {
   hipEvent_t stop;
 };
 
-hipError_t launch_clear(const StepArgs& a, uint32_t nblocks, hipStream_t stream, const Timing* t = nullptr);
 hipError_t launch_deposit(const StepArgs& a, uint32_t nblocks, hipStream_t stream, const Timing* t = nullptr);
 hipError_t launch_upward(const StepArgs& a, uint32_t nblocks, hipStream_t stream, const Timing* t = nullptr);
 hipError_t launch_top(const StepArgs& a, hipStream_t stream, const Timing* t = nullptr);

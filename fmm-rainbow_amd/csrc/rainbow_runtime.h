@@ -38,9 +38,10 @@ constexpr uint32_t kGraphSteps = 16;
 // Bodies are re-sorted into Morton order in every this many-th step by
 // default (tunable per engine; 0 = never).
 constexpr uint32_t kDefaultSortInterval = 8;
-// Tree levels 2..kDefaultTreeFusion run in one launch by default (tunable
-// per engine; each deeper level has a launch of its own).
-constexpr uint32_t kDefaultTreeFusion = 5;
+// Tree launches: bit l of the split mask (2 <= l <= 6) starts a launch at
+// level l; a launch runs its levels up to the next start.  Default: levels
+// 2..5 together and 6 alone (tunable per engine).
+constexpr uint32_t kDefaultTreeSplit = (1u << 2) | (1u << 6);
 
 // One launch of the tree pass: levels lo..hi.
 struct TreeLaunch {
@@ -66,7 +67,9 @@ struct DevBuf;
 // device staging buffer `io` (grown as needed); kernels split them into the
 // pos/vel streams and restore caller order on the way out.
 int upload_system(const SysDesc& d, const void* data, size_t bytes, hipStream_t stream, DevBuf* io);
-int download_system(const SysDesc& d, int which, void* dst, size_t bytes, hipStream_t stream, DevBuf* io);
+// finest_parity: the finest-level buffer holding the last solve's moments.
+int download_system(const SysDesc& d, int which, void* dst, size_t bytes, hipStream_t stream, DevBuf* io,
+                    uint32_t finest_parity = 0u);
 
 // A device buffer that only grows.
 struct DevBuf {
@@ -89,7 +92,7 @@ class Engine {
   // work tables and drops captured graphs.  Waits for the stream first.
   int configure(const std::vector<SysDesc>& systems);
   int push_descriptors();
-  int enqueue(bool integrate, bool sorting);
+  int enqueue(bool integrate, bool sorting, uint32_t step_parity);
   int solve_fields();
   int step(uint32_t steps);
   int step_eager(uint32_t steps);
@@ -103,6 +106,7 @@ class Engine {
   int direct_sum_f64(uint32_t sys, const uint32_t* targets, uint32_t nt, double* out);
   int set_sort_interval(uint32_t interval);
   int set_tree_fusion(uint32_t hi);
+  int set_tree_split(uint32_t starts);
   int autotune(uint32_t steps, uint32_t* chosen);
   int snapshot(bool restore);
 
@@ -112,22 +116,24 @@ class Engine {
   uint64_t total_bodies = 0;
   std::vector<SysDesc> sys;
   StepArgs args{};
-  uint32_t n_clear = 0;
   uint32_t n_deposit = 0;
   uint32_t n_integrate = 0;
   uint32_t n_subtree = 0;
   uint32_t tree_blocks[kLevels + 1] = {};
   std::vector<TreeLaunch> tree_launches;
-  uint32_t tree_fusion = kDefaultTreeFusion;
+  uint32_t tree_split = kDefaultTreeSplit;
+  // Finest-level buffer the next solve deposits into; the last solve used
+  // parity ^ 1.
+  uint32_t parity = 0;
 
  private:
-  int capture(uint32_t steps, uint32_t sort_every, hipGraphExec_t* out);
-  int prepare_graphs(uint32_t steps);
+  int capture(uint32_t steps, uint32_t sort_every, uint32_t start_parity, hipGraphExec_t* out);
+  int prepare_graphs(uint32_t steps, uint32_t start_parity);
   uint32_t chunk_steps() const;
   void plan_tree();
   void drop_graphs();
 
-  DevBuf sys_buf, clear_buf, deposit_buf, integrate_buf, subtree_buf, list_buf, count_buf,
+  DevBuf sys_buf, deposit_buf, integrate_buf, subtree_buf, list_buf, count_buf,
       stripe_buf, diag_buf, body_buf, snap_buf, own_buf, rank_buf, cstart_buf, sstart_buf;
 
  public:
@@ -136,10 +142,11 @@ class Engine {
  private:
   uint32_t sort_interval = kDefaultSortInterval;
   uint32_t since_sort = kDefaultSortInterval;
-  hipGraphExec_t graph_one = nullptr;
-  hipGraphExec_t graph_sort = nullptr;
-  hipGraphExec_t graph_chunk = nullptr;
-  hipGraphExec_t graph_chunk_sort = nullptr;
+  // Graphs per starting parity of the finest-level buffers.
+  hipGraphExec_t graph_one[2] = {nullptr, nullptr};
+  hipGraphExec_t graph_sort[2] = {nullptr, nullptr};
+  hipGraphExec_t graph_chunk[2] = {nullptr, nullptr};
+  hipGraphExec_t graph_chunk_sort[2] = {nullptr, nullptr};
   hipStream_t side = nullptr;  // the sort's scatter beside the tree pass
   hipEvent_t ev_fork = nullptr;
   hipEvent_t ev_join = nullptr;

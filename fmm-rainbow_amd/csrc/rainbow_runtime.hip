@@ -98,7 +98,7 @@ size_t system_storage_bytes(uint32_t n) {
   const size_t finest = level_cells(depth_for_particles(n));
   return 2u * align_up(16u * static_cast<size_t>(n)) + align_up(4u * static_cast<size_t>(n)) +
          align_up(16u * cells) + align_up(48u * cells) + align_up(16u * cells) + align_up(cells) +
-         align_up(16u * finest);
+         2u * align_up(16u * finest);
 }
 
 int alloc_system(uint32_t n, const Physics& ph, float scale, hipStream_t stream, SystemStorage* out) {
@@ -134,6 +134,8 @@ int alloc_system(uint32_t n, const Physics& ph, float scale, hipStream_t stream,
   d.active = reinterpret_cast<uint8_t*>(p);
   p += align_up(cells);
   d.extra = reinterpret_cast<int4*>(p);
+  p += align_up(16u * level_cells(plan.depth));
+  d.finest_alt = reinterpret_cast<int4*>(p);
   d.n = n;
   d.depth = plan.depth;
   d.scale = scale != 0.0f ? scale : plan.scale;
@@ -180,7 +182,7 @@ Engine::~Engine() {
   if (stream) (void)hipStreamSynchronize(stream);
   if (side) (void)hipStreamSynchronize(side);
   drop_graphs();
-  for (DevBuf* b : {&sys_buf, &clear_buf, &deposit_buf, &integrate_buf, &subtree_buf, &list_buf, &count_buf,
+  for (DevBuf* b : {&sys_buf, &deposit_buf, &integrate_buf, &subtree_buf, &list_buf, &count_buf,
                     &stripe_buf, &diag_buf, &body_buf, &snap_buf, &own_buf, &rank_buf, &cstart_buf, &sstart_buf,
                     &io_buf})
     b->release();
@@ -191,10 +193,11 @@ Engine::~Engine() {
 }
 
 void Engine::drop_graphs() {
-  for (hipGraphExec_t* g : {&graph_one, &graph_sort, &graph_chunk, &graph_chunk_sort}) {
-    if (*g) (void)hipGraphExecDestroy(*g);
-    *g = nullptr;
-  }
+  for (hipGraphExec_t* g : {graph_one, graph_sort, graph_chunk, graph_chunk_sort})
+    for (int k = 0; k < 2; ++k) {
+      if (g[k]) (void)hipGraphExecDestroy(g[k]);
+      g[k] = nullptr;
+    }
 }
 
 int Engine::open(uint32_t opts) {
@@ -230,7 +233,6 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
   std::vector<uint2> own;  // uploaded below; lives until the final synchronize
   sys = systems;
   const uint32_t nsys = static_cast<uint32_t>(sys.size());
-  std::vector<CellBlock> clr;
   std::vector<BodyBlock> dep, integ;
   std::vector<SubtreeBlock> sub;
   // Capacity of every (level, stripe) list: a subtree block appends at most
@@ -249,7 +251,6 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
     const SysDesc& d = sys[s];
     const uint32_t finest = level_cells(d.depth);
     const uint32_t ib = d.n / (kAdaptiveLeafCapacity + 1u);
-    for (uint32_t f = 0; f < finest; f += kClearCells) clr.push_back(CellBlock{s, f});
     for (uint32_t f = 0; f < d.n; f += kDepositBodies) dep.push_back(BodyBlock{s, f});
     for (uint32_t f = 0; f < d.n; f += kIntegrateThreads) integ.push_back(BodyBlock{s, f});
     for (uint32_t r = 0; r < level_cells(d.depth - 3u); ++r) {
@@ -263,7 +264,6 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
     dmax = std::max(dmax, d.depth);
     total_bodies += d.n;
   }
-  n_clear = static_cast<uint32_t>(clr.size());
   n_deposit = static_cast<uint32_t>(dep.size());
   n_integrate = static_cast<uint32_t>(integ.size());
   n_subtree = static_cast<uint32_t>(sub.size());
@@ -275,7 +275,7 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
     list_total += caps[i];
   }
   int rc;
-  if ((rc = sys_buf.ensure(sizeof(SysDesc) * nsys)) || (rc = clear_buf.ensure(sizeof(CellBlock) * n_clear)) ||
+  if ((rc = sys_buf.ensure(sizeof(SysDesc) * nsys)) ||
       (rc = deposit_buf.ensure(sizeof(BodyBlock) * n_deposit)) ||
       (rc = integrate_buf.ensure(sizeof(BodyBlock) * n_integrate)) ||
       (rc = subtree_buf.ensure(sizeof(SubtreeBlock) * n_subtree)) ||
@@ -284,16 +284,16 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
       (rc = stripe_buf.ensure(sizeof(uint32_t) * stripes.size())) ||
       (rc = diag_buf.ensure(sizeof(double) * (8u * static_cast<size_t>(n_integrate) + 8u))) ||
       (rc = own_buf.ensure(sizeof(uint2) * n_deposit)) || (rc = rank_buf.ensure(4ull * total_bodies)) ||
-      (rc = body_buf.ensure(36ull * total_bodies)) || (rc = cstart_buf.ensure(4ull * total_finest)) ||
+      (rc = body_buf.ensure(36ull * total_bodies)) || (rc = cstart_buf.ensure(8ull * total_finest)) ||
       (rc = sstart_buf.ensure(4ull * (total_finest / 512u + 1u))))
     return rc;
   if (total_bodies > 0xFFFFFFFFull || total_finest > 0xFFFFFFFFull)
     return fail(RAINBOW_E_INVALID, "at most 2^32 - 1 bodies and finest cells per batch");
   args.sys = static_cast<const SysDesc*>(sys_buf.p);
-  args.clear_blocks = static_cast<const CellBlock*>(clear_buf.p);
   args.own = static_cast<uint2*>(own_buf.p);
   args.rank = static_cast<uint32_t*>(rank_buf.p);
   args.cstart = static_cast<uint32_t*>(cstart_buf.p);
+  args.xstart = args.cstart + total_finest;
   args.sstart = static_cast<uint32_t*>(sstart_buf.p);
   args.tpos = static_cast<float4*>(body_buf.p);
   args.tvel = args.tpos + total_bodies;
@@ -316,7 +316,6 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
   plan_tree();
   if (nsys) {
     RB_HIP(hipMemcpyAsync(sys_buf.p, sys.data(), sizeof(SysDesc) * nsys, hipMemcpyHostToDevice, stream));
-    RB_HIP(hipMemcpyAsync(clear_buf.p, clr.data(), sizeof(CellBlock) * n_clear, hipMemcpyHostToDevice, stream));
     RB_HIP(hipMemcpyAsync(deposit_buf.p, dep.data(), sizeof(BodyBlock) * n_deposit, hipMemcpyHostToDevice, stream));
     RB_HIP(hipMemcpyAsync(integrate_buf.p, integ.data(), sizeof(BodyBlock) * n_integrate, hipMemcpyHostToDevice,
                           stream));
@@ -351,13 +350,14 @@ int Engine::push_descriptors() {
   return RAINBOW_OK;
 }
 
-// Tree launches: levels 2..min(tree_fusion, dmax) together, then one launch
-// per deeper level.
+// Tree launches from the split mask: a launch starts at level 2 and at
+// every level whose bit is set, and runs up to the next start (or dmax).
 void Engine::plan_tree() {
   tree_launches.clear();
   uint32_t l = 2u;
   while (l <= dmax) {
-    const uint32_t hi = l == 2u ? std::max(2u, std::min(tree_fusion, dmax)) : l;
+    uint32_t hi = l;
+    while (hi + 1u <= dmax && !((tree_split >> (hi + 1u)) & 1u)) ++hi;
     uint64_t blocks = 0;
     for (uint32_t k = l; k <= hi; ++k) blocks += tree_blocks[k];
     tree_launches.push_back(
@@ -366,13 +366,22 @@ void Engine::plan_tree() {
   }
 }
 
-int Engine::set_tree_fusion(uint32_t hi) {
-  if (hi < 2u || hi > kLevels - 1u) return fail(RAINBOW_E_INVALID, "tree fusion level must be 2..%u", kLevels - 1u);
+int Engine::set_tree_split(uint32_t starts) {
+  if (starts & ~0x7Cu) return fail(RAINBOW_E_INVALID, "tree split bits must be within levels 2..6");
+  starts |= 1u << 2;
   RB_HIP(hipStreamSynchronize(stream));
-  if (hi != tree_fusion) drop_graphs();
-  tree_fusion = hi;
+  if (starts != tree_split) drop_graphs();
+  tree_split = starts;
   plan_tree();
   return RAINBOW_OK;
+}
+
+// Levels 2..hi in one launch, every deeper level alone.
+int Engine::set_tree_fusion(uint32_t hi) {
+  if (hi < 2u || hi > kLevels - 1u) return fail(RAINBOW_E_INVALID, "tree fusion level must be 2..%u", kLevels - 1u);
+  uint32_t starts = 1u << 2;
+  for (uint32_t l = hi + 1u; l < kLevels; ++l) starts |= 1u << l;
+  return set_tree_split(starts);
 }
 
 // One step (or, without integrate, one field solve).  A sorting step
@@ -380,11 +389,11 @@ int Engine::set_tree_fusion(uint32_t hi) {
 // upward and top also produce the ranks and offsets, and the scatter, copy-
 // back and owned ranges run on the side stream while the tree pass runs
 // (the tree pass does not read the bodies), joined before integrate.
-int Engine::enqueue(bool integrate, bool sorting) {
+int Engine::enqueue(bool integrate, bool sorting, uint32_t step_parity) {
   if (sys.empty()) return RAINBOW_OK;
   StepArgs a = args;
   a.sorting = sorting ? 1u : 0u;
-  RB_HIP(launch_clear(a, n_clear, stream));
+  a.parity = step_parity;
   RB_HIP(launch_deposit(a, n_deposit, stream));
   RB_HIP(launch_upward(a, n_subtree, stream));
   RB_HIP(launch_top(a, stream));
@@ -402,11 +411,12 @@ int Engine::enqueue(bool integrate, bool sorting) {
 
 // Captures `steps` steps; step j sorts when sort_every != 0 and j is a
 // multiple of it.
-int Engine::capture(uint32_t steps, uint32_t sort_every, hipGraphExec_t* out) {
+int Engine::capture(uint32_t steps, uint32_t sort_every, uint32_t start_parity, hipGraphExec_t* out) {
   hipGraph_t graph = nullptr;
   RB_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
   int rc = RAINBOW_OK;
-  for (uint32_t s = 0; s < steps && rc == RAINBOW_OK; ++s) rc = enqueue(true, sort_every != 0u && s % sort_every == 0u);
+  for (uint32_t s = 0; s < steps && rc == RAINBOW_OK; ++s)
+    rc = enqueue(true, sort_every != 0u && s % sort_every == 0u, start_parity ^ (s & 1u));
   const hipError_t e = hipStreamEndCapture(stream, &graph);
   if (rc != RAINBOW_OK) {
     if (graph) (void)hipGraphDestroy(graph);
@@ -427,16 +437,19 @@ uint32_t Engine::chunk_steps() const {
 }
 
 // Graphs: one step, one sorting step, and chunks of chunk_steps() steps
-// without sorts and with sorts due from their first step.
-int Engine::prepare_graphs(uint32_t steps) {
+// without sorts and with sorts due from their first step, each for the
+// finest-level parity it starts at (captured when first needed).
+int Engine::prepare_graphs(uint32_t steps, uint32_t p) {
   int rc;
   const uint32_t chunk = chunk_steps();
   const uint32_t every = sort_interval <= kGraphSteps ? sort_interval : chunk;
-  if (graph_one == nullptr && sort_interval != 1u && (rc = capture(1u, 0u, &graph_one)) != 0) return rc;
-  if (sort_interval != 0u && graph_sort == nullptr && (rc = capture(1u, 1u, &graph_sort)) != 0) return rc;
+  if (graph_one[p] == nullptr && sort_interval != 1u && (rc = capture(1u, 0u, p, &graph_one[p])) != 0) return rc;
+  if (sort_interval != 0u && graph_sort[p] == nullptr && (rc = capture(1u, 1u, p, &graph_sort[p])) != 0) return rc;
   if (steps >= chunk && chunk > 1u) {
-    if (sort_interval != 1u && graph_chunk == nullptr && (rc = capture(chunk, 0u, &graph_chunk)) != 0) return rc;
-    if (sort_interval != 0u && graph_chunk_sort == nullptr && (rc = capture(chunk, every, &graph_chunk_sort)) != 0)
+    if (sort_interval != 1u && graph_chunk[p] == nullptr && (rc = capture(chunk, 0u, p, &graph_chunk[p])) != 0)
+      return rc;
+    if (sort_interval != 0u && graph_chunk_sort[p] == nullptr &&
+        (rc = capture(chunk, every, p, &graph_chunk_sort[p])) != 0)
       return rc;
   }
   return RAINBOW_OK;
@@ -447,34 +460,37 @@ int Engine::prepare_graphs(uint32_t steps) {
 int Engine::step(uint32_t steps) {
   if (sys.empty() || steps == 0u) return RAINBOW_OK;
   int rc;
-  if ((rc = prepare_graphs(steps)) != 0) return rc;
   const uint32_t chunk = chunk_steps();
   uint32_t done = 0;
   while (done < steps) {
+    // Graphs of the current parity (the other parity's are captured when
+    // the parity first reaches it).
+    if ((rc = prepare_graphs(steps - done, parity)) != 0) return rc;
+    const uint32_t p = parity;
     const bool sort_due = sort_interval != 0u && since_sort >= sort_interval;
     const bool whole = chunk > 1u && steps - done >= chunk;
+    uint32_t ran = 1u;
     if (sort_due) {
-      if (whole && graph_chunk_sort != nullptr) {
-        RB_HIP(hipGraphLaunch(graph_chunk_sort, stream));
-        done += chunk;
+      if (whole && graph_chunk_sort[p] != nullptr) {
+        RB_HIP(hipGraphLaunch(graph_chunk_sort[p], stream));
+        ran = chunk;
         since_sort = sort_interval <= kGraphSteps ? sort_interval : chunk;
       } else {
-        RB_HIP(hipGraphLaunch(graph_sort, stream));
-        done += 1u;
+        RB_HIP(hipGraphLaunch(graph_sort[p], stream));
         since_sort = 1u;
       }
     } else {
       const uint64_t free_steps = sort_interval == 0u ? ~0ull : sort_interval - since_sort;
-      if (whole && free_steps >= chunk && graph_chunk != nullptr) {
-        RB_HIP(hipGraphLaunch(graph_chunk, stream));
-        done += chunk;
-        since_sort += chunk;
+      if (whole && free_steps >= chunk && graph_chunk[p] != nullptr) {
+        RB_HIP(hipGraphLaunch(graph_chunk[p], stream));
+        ran = chunk;
       } else {
-        RB_HIP(hipGraphLaunch(graph_one, stream));
-        done += 1u;
-        since_sort += 1u;
+        RB_HIP(hipGraphLaunch(graph_one[p], stream));
       }
+      since_sort += ran;
     }
+    done += ran;
+    parity ^= ran & 1u;
   }
   return RAINBOW_OK;
 }
@@ -484,8 +500,9 @@ int Engine::step(uint32_t steps) {
 int Engine::step_eager(uint32_t steps) {
   for (uint32_t k = 0; k < steps; ++k) {
     const bool sorting = sort_interval != 0u && since_sort >= sort_interval;
-    int rc = enqueue(true, sorting);
+    int rc = enqueue(true, sorting, parity);
     if (rc) return rc;
+    parity ^= 1u;
     since_sort = sorting ? 1u : since_sort + 1u;
   }
   return RAINBOW_OK;
@@ -532,12 +549,13 @@ int Engine::autotune(uint32_t steps, uint32_t* chosen) {
   if (steps == 0u) return fail(RAINBOW_E_INVALID, "steps must be >= 1");
   int rc = snapshot(false);
   if (rc) return rc;
-  // Tree fusion: levels 2..hi in one launch, hi = 2..dmax.
+  // Tree launch split: every partition of levels 2..dmax into runs.
   {
     float best = 0.0f;
-    uint32_t best_hi = tree_fusion;
-    for (uint32_t hi = 2u; hi <= std::max(2u, dmax); ++hi) {
-      if ((rc = set_tree_fusion(hi)) != 0) return rc;
+    uint32_t best_split = tree_split;
+    const uint32_t free_bits = dmax > 2u ? dmax - 2u : 0u;  // levels 3..dmax
+    for (uint32_t m = 0; m < (1u << free_bits); ++m) {
+      if ((rc = set_tree_split((1u << 2) | (m << 3))) != 0) return rc;
       float ms = 0.0f;
       if ((rc = time_steps(std::min<uint32_t>(steps, 4u), &ms)) != 0) return rc;  // warm graphs
       if ((rc = snapshot(true)) != 0) return rc;
@@ -545,10 +563,10 @@ int Engine::autotune(uint32_t steps, uint32_t* chosen) {
       if ((rc = snapshot(true)) != 0) return rc;
       if (best == 0.0f || ms < best) {
         best = ms;
-        best_hi = hi;
+        best_split = tree_split;
       }
     }
-    if ((rc = set_tree_fusion(best_hi)) != 0) return rc;
+    if ((rc = set_tree_split(best_split)) != 0) return rc;
   }
   float best = 0.0f;
   uint32_t best_interval = sort_interval;
@@ -569,7 +587,11 @@ int Engine::autotune(uint32_t steps, uint32_t* chosen) {
   return RAINBOW_OK;
 }
 
-int Engine::solve_fields() { return enqueue(false, false); }
+int Engine::solve_fields() {
+  const int rc = enqueue(false, false, parity);
+  if (rc == RAINBOW_OK) parity ^= 1u;
+  return rc;
+}
 
 int Engine::sync() {
   RB_HIP(hipStreamSynchronize(stream));
@@ -578,8 +600,8 @@ int Engine::sync() {
 
 int Engine::time_steps(uint32_t steps, float* ms) {
   int rc;
-  // Graphs are captured before the timed region.
-  if ((rc = prepare_graphs(steps)) != 0) return rc;
+  // Graphs of both parities are captured before the timed region.
+  if ((rc = prepare_graphs(steps, 0u)) != 0 || (rc = prepare_graphs(steps, 1u)) != 0) return rc;
   RB_HIP(hipStreamSynchronize(stream));
   RB_HIP(hipEventRecord(ev_a, stream));
   if ((rc = step(steps)) != 0) return rc;
@@ -590,9 +612,9 @@ int Engine::time_steps(uint32_t steps, float* ms) {
 }
 
 int Engine::phase_ms(uint32_t steps, float* out, uint32_t n) {
-  // clear, deposit, upward, top, the tree launches, integrate, then the
-  // body sort (its mean per step: the steps sort when due, as in step()).
-  const uint32_t nkern = 5u + static_cast<uint32_t>(tree_launches.size());
+  // deposit, upward, top, the tree launches, integrate, then the body sort
+  // (its mean per step: the steps sort when due, as in step()).
+  const uint32_t nkern = 4u + static_cast<uint32_t>(tree_launches.size());
   const uint32_t nphase = nkern + 1u;
   if (out == nullptr || n < nphase) return fail(RAINBOW_E_INVALID, "phase_ms needs %u entries", nphase);
   if (steps == 0u) return fail(RAINBOW_E_INVALID, "steps must be >= 1");
@@ -609,8 +631,9 @@ int Engine::phase_ms(uint32_t steps, float* out, uint32_t n) {
     since_sort = sorting ? 1u : since_sort + 1u;
     StepArgs a = args;
     a.sorting = sorting ? 1u : 0u;
+    a.parity = parity;
+    parity ^= 1u;
     uint32_t k = 0;
-    if (err == hipSuccess) err = launch_clear(a, n_clear, stream, &tm[k++]);
     if (err == hipSuccess) err = launch_deposit(a, n_deposit, stream, &tm[k++]);
     if (err == hipSuccess) err = launch_upward(a, n_subtree, stream, &tm[k++]);
     if (err == hipSuccess) err = launch_top(a, stream, &tm[k++]);
@@ -653,11 +676,17 @@ int upload_system(const SysDesc& d, const void* data, size_t bytes, hipStream_t 
   if (rc) return rc;
   RB_HIP(hipMemcpyAsync(io->p, data, bytes, hipMemcpyHostToDevice, stream));
   RB_HIP(launch_split_particles(d, static_cast<const float4*>(io->p), stream));
+  // The next deposit expects empty finest-level buffers and stragglers.
+  const size_t fb = 16ull * level_cells(d.depth);
+  RB_HIP(hipMemsetAsync(d.moments + level_offset(d.depth), 0, fb, stream));
+  RB_HIP(hipMemsetAsync(d.finest_alt, 0, fb, stream));
+  RB_HIP(hipMemsetAsync(d.extra, 0, fb, stream));
   RB_HIP(hipStreamSynchronize(stream));
   return RAINBOW_OK;
 }
 
-int download_system(const SysDesc& d, int which, void* dst, size_t bytes, hipStream_t stream, DevBuf* io) {
+int download_system(const SysDesc& d, int which, void* dst, size_t bytes, hipStream_t stream, DevBuf* io,
+                    uint32_t finest_parity) {
   const uint64_t cells = total_cells(d.depth);
   if (dst == nullptr) return fail(RAINBOW_E_INVALID, "null destination");
   if (which == RAINBOW_PARTICLES) {
@@ -673,7 +702,12 @@ int download_system(const SysDesc& d, int which, void* dst, size_t bytes, hipStr
   if (which == RAINBOW_MOMENTS) {
     if (bytes != 16ull * cells)
       return fail(RAINBOW_E_INVALID, "moments need %llu bytes", static_cast<unsigned long long>(16ull * cells));
-    RB_HIP(hipMemcpyAsync(dst, d.moments, bytes, hipMemcpyDeviceToHost, stream));
+    // The finest level of the last solve may sit in the second buffer.
+    const size_t coarse = 16ull * level_offset(d.depth);
+    const void* finest = finest_parity ? static_cast<const void*>(d.finest_alt)
+                                       : static_cast<const void*>(d.moments + level_offset(d.depth));
+    RB_HIP(hipMemcpyAsync(dst, d.moments, coarse, hipMemcpyDeviceToHost, stream));
+    RB_HIP(hipMemcpyAsync(static_cast<char*>(dst) + coarse, finest, bytes - coarse, hipMemcpyDeviceToHost, stream));
     RB_HIP(hipStreamSynchronize(stream));
     return RAINBOW_OK;
   }
@@ -696,7 +730,7 @@ int Engine::upload(uint32_t s, const void* data, size_t bytes) {
 
 int Engine::download(uint32_t s, int which, void* dst, size_t bytes) {
   if (s >= sys.size()) return fail(RAINBOW_E_INVALID, "system %u out of range", s);
-  return download_system(sys[s], which, dst, bytes, stream, &io_buf);
+  return download_system(sys[s], which, dst, bytes, stream, &io_buf, parity ^ 1u);
 }
 
 int Engine::accelerations(uint32_t s, void* dst, size_t bytes) {
@@ -706,7 +740,9 @@ int Engine::accelerations(uint32_t s, void* dst, size_t bytes) {
     return fail(RAINBOW_E_INVALID, "accelerations need %llu bytes", static_cast<unsigned long long>(16ull * d.n));
   float4* out = nullptr;
   RB_HIP(hipMalloc(reinterpret_cast<void**>(&out), 16ull * d.n));
-  hipError_t e = launch_accel(args, n_integrate, s, out, stream);
+  StepArgs a = args;
+  a.parity = parity ^ 1u;  // the last solve's finest level
+  hipError_t e = launch_accel(a, n_integrate, s, out, stream);
   if (e == hipSuccess) e = hipMemcpyAsync(dst, out, bytes, hipMemcpyDeviceToHost, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   (void)hipFree(out);
@@ -741,7 +777,9 @@ int Engine::diagnostics(uint32_t s, double out[8]) {
   if (s >= sys.size()) return fail(RAINBOW_E_INVALID, "system %u out of range", s);
   double* partial = static_cast<double*>(diag_buf.p);
   double* result = partial + 8u * static_cast<size_t>(n_integrate);
-  RB_HIP(launch_diag(args, n_integrate, s, partial, result, stream));
+  StepArgs a = args;
+  a.parity = parity ^ 1u;  // the last solve's finest level
+  RB_HIP(launch_diag(a, n_integrate, s, partial, result, stream));
   RB_HIP(hipMemcpyAsync(out, result, 8u * sizeof(double), hipMemcpyDeviceToHost, stream));
   RB_HIP(hipStreamSynchronize(stream));
   return RAINBOW_OK;
@@ -908,6 +946,11 @@ int rainbow_hip_set_tree_fusion(rainbow_hip_t* h, uint32_t hi) {
   return h->batch.engine.set_tree_fusion(hi);
 }
 
+int rainbow_hip_set_tree_split(rainbow_hip_t* h, uint32_t starts) {
+  RB_CHECK_HANDLE(h);
+  return h->batch.engine.set_tree_split(starts);
+}
+
 int rainbow_hip_tree_launches(rainbow_hip_t* h, uint32_t* lo_hi, uint32_t n) {
   RB_CHECK_HANDLE(h);
   const auto& tl = h->batch.engine.tree_launches;
@@ -1029,6 +1072,11 @@ int rainbow_batch_set_sort_interval(rainbow_batch_t* b, uint32_t interval) {
 int rainbow_batch_set_tree_fusion(rainbow_batch_t* b, uint32_t hi) {
   RB_CHECK_HANDLE(b);
   return b->batch.engine.set_tree_fusion(hi);
+}
+
+int rainbow_batch_set_tree_split(rainbow_batch_t* b, uint32_t starts) {
+  RB_CHECK_HANDLE(b);
+  return b->batch.engine.set_tree_split(starts);
 }
 
 int rainbow_batch_autotune(rainbow_batch_t* b, uint32_t steps, uint32_t* interval) {

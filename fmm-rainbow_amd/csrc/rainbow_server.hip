@@ -101,18 +101,17 @@ struct Slot {
 
 // Device bytes a resident request holds: its storage plus its share of the
 // engine's work tables, lists and sort buffers (ranks and staging: 40 bytes
-// per body; cell offsets: 4 per finest cell; owned ranges: 8 per deposit
+// per body; cell offsets: 8 per finest cell; owned ranges: 8 per deposit
 // block).
 uint64_t request_bytes(uint32_t n) {
   const Plan p = make_plan(n);
   uint64_t lists = 0;
   for (uint32_t k = 0; k < p.depth; ++k) lists += std::min<uint64_t>(level_cells(k), n / 17u);
   const uint64_t deposit_blocks = (n + kDepositBodies - 1u) / kDepositBodies;
-  const uint64_t blocks = deposit_blocks + (n + kIntegrateThreads - 1u) / kIntegrateThreads +
-                          level_cells(p.depth - 3u) + (level_cells(p.depth) + kClearCells - 1u) / kClearCells;
+  const uint64_t blocks = deposit_blocks + (n + kIntegrateThreads - 1u) / kIntegrateThreads + level_cells(p.depth - 3u);
   return system_storage_bytes(n) + 4u * lists + 8u * blocks +
          64u * ((n + kIntegrateThreads - 1u) / kIntegrateThreads) + sizeof(SysDesc) + 4u + 40ull * n +
-         4ull * level_cells(p.depth) + 4ull * level_cells(p.depth - 3u) + 8ull * deposit_blocks;
+         8ull * level_cells(p.depth) + 4ull * level_cells(p.depth - 3u) + 8ull * deposit_blocks;
 }
 
 }  // namespace rainbow

@@ -73,19 +73,19 @@ int main(int argc, char** argv) {
   float phase[16];
   const int nphase = rainbow_hip_phase_ms(h, 5u, phase, 16u);
   if (nphase < 0) check(nphase, "phase timing");
-  const char* fixed[4] = {"clear", "deposit", "upward", "top"};
+  const char* fixed[3] = {"deposit", "upward", "top"};
   uint32_t lo_hi[16];
   const int ntree = rainbow_hip_tree_launches(h, lo_hi, 16u);
   if (ntree < 0) check(ntree, "tree launches");
   double total = 0.0;
   for (int i = 0; i < nphase; ++i) {
     char name[32];
-    if (i < 4)
+    if (i < 3)
       std::snprintf(name, sizeof(name), "%s", fixed[i]);
-    else if (i < 4 + ntree && lo_hi[2 * (i - 4)] == lo_hi[2 * (i - 4) + 1])
-      std::snprintf(name, sizeof(name), "tree_l%u", lo_hi[2 * (i - 4)]);
-    else if (i < 4 + ntree)
-      std::snprintf(name, sizeof(name), "tree_l%u-%u", lo_hi[2 * (i - 4)], lo_hi[2 * (i - 4) + 1]);
+    else if (i < 3 + ntree && lo_hi[2 * (i - 3)] == lo_hi[2 * (i - 3) + 1])
+      std::snprintf(name, sizeof(name), "tree_l%u", lo_hi[2 * (i - 3)]);
+    else if (i < 3 + ntree)
+      std::snprintf(name, sizeof(name), "tree_l%u-%u", lo_hi[2 * (i - 3)], lo_hi[2 * (i - 3) + 1]);
     else if (i == nphase - 2)
       std::snprintf(name, sizeof(name), "integrate");
     else

@@ -131,6 +131,8 @@ _SIGNATURES = {
     "rainbow_hip_time_steps": (_i32, [_vp, _u32, _P(_f32)]),
     "rainbow_hip_phase_ms": (_i32, [_vp, _u32, _P(_f32), _u32]),
     "rainbow_hip_set_tree_fusion": (_i32, [_vp, _u32]),
+    "rainbow_hip_set_tree_split": (_i32, [_vp, _u32]),
+    "rainbow_batch_set_tree_split": (_i32, [_vp, _u32]),
     "rainbow_hip_direct_sum_f64": (_i32, [_vp, _P(_u32), _u32, _P(_f64)]),
     "rainbow_hip_tree_launches": (_i32, [_vp, _P(_u32), _u32]),
     "rainbow_batch_set_tree_fusion": (_i32, [_vp, _u32]),
@@ -423,6 +425,10 @@ class Solver:
         """Tree levels 2..hi in one launch; each deeper level alone."""
         _check(lib().rainbow_hip_set_tree_fusion(self._h, hi))
 
+    def set_tree_split(self, starts: int):
+        """Bit l of `starts` begins a tree launch at level l (bit 2 implied)."""
+        _check(lib().rainbow_hip_set_tree_split(self._h, starts))
+
     def tree_launches(self) -> list[tuple[int, int]]:
         """(lo, hi) levels of each tree launch of a step, in order."""
         buf = (_u32 * 16)()
@@ -432,7 +438,7 @@ class Solver:
     def phase_names(self) -> list[str]:
         """Names of the entries phase_ms returns."""
         tree = [f"tree_l{lo}" if lo == hi else f"tree_l{lo}-{hi}" for lo, hi in self.tree_launches()]
-        return ["clear", "deposit", "upward", "top"] + tree + ["integrate", "sort"]
+        return ["deposit", "upward", "top"] + tree + ["integrate", "sort"]
 
     def autotune(self, steps: int = 32) -> int:
         v = _u32()

@@ -228,14 +228,14 @@ int rainbow_hip_time_steps(rainbow_hip_t* h, uint32_t steps, float* ms);
 
 /* Runs `steps` steps without the graph, timing every kernel launch with an
  * event pair recorded by the dispatch itself, and writes the mean duration
- * in ms of each phase: out[0] clear, out[1] deposit, out[2] upward
- * (subtrees), out[3] upward (top levels, with the level-1 fields), out[4 +
- * i] tree launch i (M2L + L2L + near of its levels, as listed by
- * rainbow_hip_tree_launches), then integrate, then the body sort (the steps
- * sort when due, as rainbow_hip_step does; the entry is the sort time
- * divided by `steps`).  `n` is the capacity of out (at least 6 + the number
- * of tree launches; 16 always suffices).  Returns the number of entries
- * written (>= 0) or a negative status. */
+ * in ms of each phase: out[0] deposit, out[1] upward (subtrees), out[2]
+ * upward (top levels, with the level-1 fields), out[3 + i] tree launch i
+ * (M2L + L2L + near of its levels, as listed by rainbow_hip_tree_launches),
+ * then integrate, then the body sort (the steps sort when due, as
+ * rainbow_hip_step does; the entry is the sort time divided by `steps`).
+ * `n` is the capacity of out (at least 5 + the number of tree launches; 16
+ * always suffices).  Returns the number of entries written (>= 0) or a
+ * negative status. */
 int rainbow_hip_phase_ms(rainbow_hip_t* h, uint32_t steps, float* out,
                          uint32_t n);
 
@@ -245,10 +245,14 @@ int rainbow_hip_phase_ms(rainbow_hip_t* h, uint32_t steps, float* out,
  * downloads return caller order and no result depends on it. */
 int rainbow_hip_set_sort_interval(rainbow_hip_t* h, uint32_t interval);
 
-/* The tree pass runs levels 2..hi in one launch (2 <= hi <= 6; default 5):
- * a block of that launch recomputes the fields of its item's ancestors down
- * to level 2 instead of waiting for another launch.  Each level deeper than
- * hi has a launch of its own.  Results do not depend on hi. */
+/* The tree pass runs as a few launches, each over a run of levels: a block
+ * of a launch over levels lo..hi recomputes the fields of its item's
+ * ancestors down to level lo instead of waiting for another launch.
+ * rainbow_hip_set_tree_split: bit l of `starts` (2 <= l <= 6) starts a
+ * launch at level l (bit 2 is implied; default: bits 2 and 6, i.e. levels
+ * 2..5 and 6).  rainbow_hip_set_tree_fusion(h, hi): levels 2..hi together,
+ * each deeper level alone.  Results do not depend on the split. */
+int rainbow_hip_set_tree_split(rainbow_hip_t* h, uint32_t starts);
 int rainbow_hip_set_tree_fusion(rainbow_hip_t* h, uint32_t hi);
 
 /* The tree launches of a step, as (lo, hi) level pairs in launch order.
@@ -257,7 +261,8 @@ int rainbow_hip_set_tree_fusion(rainbow_hip_t* h, uint32_t hi);
 int rainbow_hip_tree_launches(rainbow_hip_t* h, uint32_t* lo_hi, uint32_t n);
 
 /* Autotuner: from the same starting state, times `steps` steps at each tree
- * fusion level (2..depth) and keeps the fastest, then at each candidate
+ * split (every partition of levels 2..depth into runs) and keeps the
+ * fastest, then at each candidate
  * sort interval (1, 2, 4, 8, 16, 32, never), keeps the fastest and writes
  * it to *interval.  The bodies are restored afterwards. */
 int rainbow_hip_autotune(rainbow_hip_t* h, uint32_t steps, uint32_t* interval);
@@ -315,9 +320,10 @@ int rainbow_batch_synchronize(rainbow_batch_t* b);
 /* Total bodies over all systems. */
 uint64_t rainbow_batch_particles(rainbow_batch_t* b);
 /* As rainbow_hip_set_sort_interval / rainbow_hip_set_tree_fusion /
- * rainbow_hip_autotune, for the batch. */
+ * rainbow_hip_set_tree_split / rainbow_hip_autotune, for the batch. */
 int rainbow_batch_set_sort_interval(rainbow_batch_t* b, uint32_t interval);
 int rainbow_batch_set_tree_fusion(rainbow_batch_t* b, uint32_t hi);
+int rainbow_batch_set_tree_split(rainbow_batch_t* b, uint32_t starts);
 int rainbow_batch_autotune(rainbow_batch_t* b, uint32_t steps, uint32_t* interval);
 
 /* ------------------------------------------------------------------------ */

@@ -400,6 +400,25 @@ def test_tree_fusion_does_not_change_results(hi, cfg2_reference):
     s.close()
 
 
+@pytest.mark.parametrize("starts", [1 << 5, (1 << 3) | (1 << 5), (1 << 4)])
+def test_tree_split_does_not_change_results(starts, cfg2_reference):
+    say(f"tree split mask {starts:#x}, 1M Plummer")
+    parts, om, of, op = cfg2_reference
+    c = wl.CFG2
+    s = rf.Solver(c["n"], wl.params_for(c))
+    s.set_tree_split(starts)
+    launches = s.tree_launches()
+    bounds = sorted({2} | {l for l in range(3, 7) if (starts >> l) & 1})
+    assert [lo for lo, _ in launches] == bounds and launches[-1][1] == s.depth
+    s.upload(parts)
+    s.solve_fields()
+    assert_bits(s.fields(), of, "fields")
+    s.upload(parts)
+    s.step(2)
+    assert_bits(s.particles(), op, "particles")
+    s.close()
+
+
 @pytest.mark.parametrize("compensated", [False, True])
 def test_tree_fusion_ball_all_levels(compensated):
     say(f"tree fusion 2..6, uniform ball 400k, compensated={compensated}")
