@@ -106,11 +106,12 @@ struct StepArgs {
   uint32_t nsys;
   uint32_t compensated;
   uint32_t sorting;
-  uint32_t parity;  // finest-level buffer of this step (see finest_alt)
+  uint32_t parity;     // finest-level buffer of this step (see finest_alt)
+  uint32_t rank_next;  // the fused deposit writes ranks: the next step sorts
 };
 
-constexpr uint32_t kDepositThreads = 256;
-constexpr uint32_t kDepositPerThread = 4;
+constexpr uint32_t kDepositThreads = 1024;  // one body per thread
+constexpr uint32_t kDepositPerThread = 1;
 constexpr uint32_t kDepositBodies = kDepositThreads * kDepositPerThread;  // per block
 constexpr uint32_t kDepositSlots = 2048;  // LDS hash slots, 2x bodies per block
 constexpr uint32_t kIntegrateThreads = 256;

@@ -125,17 +125,20 @@ __device__ __forceinline__ uint32_t spread3(uint32_t v) {
 // deposit (P2M), src/rainbow_fmm.wgsl:408-422
 // ------------------------------------------------------------------------
 
-__global__ __launch_bounds__(kDepositThreads) void k_deposit(StepArgs a) {
-  __shared__ uint32_t keys[kDepositSlots];  // after the flush: rank base per slot
-  __shared__ int vals[4][kDepositSlots];
-  // The step's work-list counters start at zero (the previous step's tree
-  // pass, their last reader, has finished; upward, their first writer,
-  // runs after this kernel).
-  if (blockIdx.x == 0u)
-    for (uint32_t i = threadIdx.x; i < kLevels * kStripes; i += kDepositThreads) a.list_count[i] = 0u;
-  const BodyBlock blk = a.deposit_blocks[blockIdx.x];
-  const SysDesc d = a.sys[blk.sys];
-  if (!d.live) return;
+// LDS of a deposit block: the open-addressing hash of its cells.
+struct DepositLds {
+  uint32_t keys[kDepositSlots];  // after the flush: rank base per slot
+  int vals[4][kDepositSlots];
+};
+
+// Deposit of one block's bodies first + k * kDepositThreads + thread
+// (positions p[k]) into the finest buffer `parity` of system d; writes the
+// bodies' ranks when `ranks`.  All threads of the block call it.
+__device__ void deposit_block(const StepArgs& a, const SysDesc& d, uint32_t block, uint32_t first,
+                              const float4 (&p)[kDepositPerThread], uint32_t parity, bool ranks,
+                              DepositLds& L) {
+  uint32_t* const keys = L.keys;
+  int (*const vals)[kDepositSlots] = L.vals;
   for (uint32_t s = threadIdx.x; s < kDepositSlots; s += kDepositThreads) {
     keys[s] = kEmptyKey;
     vals[0][s] = 0;
@@ -151,17 +154,17 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(StepArgs a) {
   uint32_t my_slot[kDepositPerThread], my_rank[kDepositPerThread];
 #pragma unroll
   for (uint32_t k = 0; k < kDepositPerThread; ++k) {
-    const uint32_t i = blk.first + k * kDepositThreads + threadIdx.x;
+    const uint32_t i = first + k * kDepositThreads + threadIdx.x;
     const bool valid = i < d.n;
     uint32_t key = kEmptyKey;
     int v[4] = {0, 0, 0, 0};
     if (valid) {
-      const float4 p = d.pos[i];
-      key = lin(cell1(unit_coord(p.x), dim), cell1(unit_coord(p.y), dim), cell1(unit_coord(p.z), dim), dim);
+      const float4 q = p[k];
+      key = lin(cell1(unit_coord(q.x), dim), cell1(unit_coord(q.y), dim), cell1(unit_coord(q.z), dim), dim);
       v[0] = 1;
-      v[1] = to_i32_sat(rintf(p.x * d.scale));
-      v[2] = to_i32_sat(rintf(p.y * d.scale));
-      v[3] = to_i32_sat(rintf(p.z * d.scale));
+      v[1] = to_i32_sat(rintf(q.x * d.scale));
+      v[2] = to_i32_sat(rintf(q.y * d.scale));
+      v[3] = to_i32_sat(rintf(q.z * d.scale));
     }
     // Bodies are kept in Morton order, so equal keys form runs across the
     // wave: a segmented inclusive scan sums each run into its last lane
@@ -207,8 +210,8 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(StepArgs a) {
   // stored.  Any other cell's sums go to `extra` with atomics (bodies that
   // moved since the last sort, and cells shared with other blocks);
   // upward adds `extra` into the finest level.
-  const uint2 own = a.own[blockIdx.x];
-  int4* finest = finest_of(d, a.parity);
+  const uint2 own = a.own[block];
+  int4* finest = finest_of(d, parity);
   for (uint32_t s = threadIdx.x; s < kDepositSlots; s += kDepositThreads) {
     const uint32_t key = keys[s];
     if (key == kEmptyKey) continue;
@@ -229,13 +232,37 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(StepArgs a) {
     atomicAdd(reinterpret_cast<int*>(d.extra + key) + 3, vals[3][s]);
     keys[s] = kRankExtra | static_cast<uint32_t>(old);
   }
-  if (!a.sorting) return;
+  if (!ranks) return;
   __syncthreads();
 #pragma unroll
   for (uint32_t k = 0; k < kDepositPerThread; ++k) {
-    const uint32_t i = blk.first + k * kDepositThreads + threadIdx.x;
+    const uint32_t i = first + k * kDepositThreads + threadIdx.x;
     if (i < d.n) a.rank[d.body_base + i] = keys[my_slot[k]] + my_rank[k];
   }
+}
+
+// Zeroes the work-list counters of the step whose upward follows (their
+// last reader, the previous tree pass, has finished).
+__device__ __forceinline__ void reset_lists(const StepArgs& a) {
+  if (blockIdx.x == 0u)
+    for (uint32_t i = threadIdx.x; i < kLevels * kStripes; i += blockDim.x) a.list_count[i] = 0u;
+}
+
+// Standalone deposit (a solve's first pass; a step normally deposits at the
+// end of the previous step's integrate).
+__global__ __launch_bounds__(kDepositThreads) void k_deposit(StepArgs a) {
+  __shared__ DepositLds L;
+  reset_lists(a);
+  const BodyBlock blk = a.deposit_blocks[blockIdx.x];
+  const SysDesc d = a.sys[blk.sys];
+  if (!d.live) return;
+  float4 p[kDepositPerThread];
+#pragma unroll
+  for (uint32_t k = 0; k < kDepositPerThread; ++k) {
+    const uint32_t i = blk.first + k * kDepositThreads + threadIdx.x;
+    p[k] = i < d.n ? d.pos[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+  deposit_block(a, d, blockIdx.x, blk.first, p, a.parity, a.sorting != 0u, L);
 }
 
 // Morton code of the finest cell of a position.
@@ -1179,18 +1206,13 @@ __device__ __forceinline__ float length3(float x, float y, float z) {
   return sqrtf((x * x + y * y) + z * z);
 }
 
+// One body's integrate (src/rainbow_fmm.wgsl:613-771): the new state
+// replaces p4 (pos, hue) and v4 (vel, excitation).
 template <bool kComp>
-__global__ __launch_bounds__(kIntegrateThreads) void k_integrate(StepArgs a) {
-  const BodyBlock blk = a.integrate_blocks[blockIdx.x];
-  const SysDesc d = a.sys[blk.sys];
-  if (!d.live) return;
-  const uint32_t i = blk.first + threadIdx.x;
-  if (i >= d.n) return;
-  const float4 p4 = d.pos[i];
-  const float4 v4 = d.vel[i];
+__device__ __forceinline__ void integrate_body(const SysDesc& d, uint32_t parity, float4& p4, float4& v4) {
   float pos[3] = {p4.x, p4.y, p4.z};
   float velocity[3] = {v4.x, v4.y, v4.z};
-  const Leaf leaf = leaf_field<kComp, false>(d, a.parity, p4.x, p4.y, p4.z);
+  const Leaf leaf = leaf_field<kComp, false>(d, parity, p4.x, p4.y, p4.z);
   float acceleration[3] = {leaf.ax, leaf.ay, leaf.az};
   const uint32_t count = leaf.count;
 
@@ -1276,8 +1298,36 @@ __global__ __launch_bounds__(kIntegrateThreads) void k_integrate(StepArgs a) {
                              (0.012f + signed_radial_motion * 0.030f + excitation * 0.036f +
                               control_influence * 0.080f);
   const float dynamical_hue = h - floorf(h);
-  d.pos[i] = make_float4(pos[0], pos[1], pos[2], dynamical_hue);
-  d.vel[i] = make_float4(velocity[0], velocity[1], velocity[2], excitation);
+  p4 = make_float4(pos[0], pos[1], pos[2], dynamical_hue);
+  v4 = make_float4(velocity[0], velocity[1], velocity[2], excitation);
+}
+
+// Integrate fused with the next step's deposit: each block integrates the
+// 1024 bodies of its deposit range and deposits their new positions into
+// the other finest-level buffer (zeroed by this step's upward), with the
+// next step's ranks when that step sorts.
+template <bool kComp>
+__global__ __launch_bounds__(kDepositThreads) void k_integrate_deposit(StepArgs a) {
+  __shared__ DepositLds L;
+  reset_lists(a);
+  const BodyBlock blk = a.deposit_blocks[blockIdx.x];
+  const SysDesc d = a.sys[blk.sys];
+  if (!d.live) return;
+  float4 p[kDepositPerThread];
+#pragma unroll
+  for (uint32_t k = 0; k < kDepositPerThread; ++k) {
+    const uint32_t i = blk.first + k * kDepositThreads + threadIdx.x;
+    p[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (i < d.n) {
+      float4 p4 = d.pos[i];
+      float4 v4 = d.vel[i];
+      integrate_body<kComp>(d, a.parity, p4, v4);
+      d.pos[i] = p4;
+      d.vel[i] = v4;
+      p[k] = p4;
+    }
+  }
+  deposit_block(a, d, blockIdx.x, blk.first, p, a.parity ^ 1u, a.rank_next != 0u, L);
 }
 
 // ------------------------------------------------------------------------
@@ -1543,12 +1593,12 @@ hipError_t launch_sort(const StepArgs& a, uint32_t nintegrate, uint32_t ndeposit
   return hipGetLastError();
 }
 
-hipError_t launch_integrate(const StepArgs& a, uint32_t nblocks, hipStream_t stream, const Timing* t) {
-  if (nblocks == 0u) return no_launch(stream, t);
+hipError_t launch_integrate_deposit(const StepArgs& a, uint32_t ndeposit, hipStream_t stream, const Timing* t) {
+  if (ndeposit == 0u) return no_launch(stream, t);
   if (a.compensated)
-    RB_LAUNCH(t, k_integrate<true>, dim3(nblocks), dim3(kIntegrateThreads), stream, a);
+    RB_LAUNCH(t, k_integrate_deposit<true>, dim3(ndeposit), dim3(kDepositThreads), stream, a);
   else
-    RB_LAUNCH(t, k_integrate<false>, dim3(nblocks), dim3(kIntegrateThreads), stream, a);
+    RB_LAUNCH(t, k_integrate_deposit<false>, dim3(ndeposit), dim3(kDepositThreads), stream, a);
   return hipGetLastError();
 }
 

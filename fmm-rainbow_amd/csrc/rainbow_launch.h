@@ -20,7 +20,10 @@ hipError_t launch_top(const StepArgs& a, hipStream_t stream, const Timing* t = n
 // Tree pass of levels lo..hi (2 <= lo <= hi <= 6) in one launch.
 hipError_t launch_tree(const StepArgs& a, uint32_t lo, uint32_t hi, uint32_t nblocks, hipStream_t stream,
                        const Timing* t = nullptr);
-hipError_t launch_integrate(const StepArgs& a, uint32_t nblocks, hipStream_t stream, const Timing* t = nullptr);
+// Integrate plus the next step's deposit (into finest buffer parity ^ 1),
+// one block per deposit range.
+hipError_t launch_integrate_deposit(const StepArgs& a, uint32_t ndeposit, hipStream_t stream,
+                                    const Timing* t = nullptr);
 // The counting sort's scatter, copy-back and owned ranges (after upward and
 // top of a step run with a.sorting set).
 hipError_t launch_sort(const StepArgs& a, uint32_t nintegrate, uint32_t ndeposit, hipStream_t stream);

@@ -92,7 +92,9 @@ class Engine {
   // work tables and drops captured graphs.  Waits for the stream first.
   int configure(const std::vector<SysDesc>& systems);
   int push_descriptors();
-  int enqueue(bool integrate, bool sorting, uint32_t step_parity);
+  int enqueue(bool integrate, bool sorting, uint32_t step_parity, bool deposit_first, bool rank_next);
+  hipError_t zero_finest(uint32_t parity);
+  void plan_next(bool* sorting, bool* rank_next, uint32_t* since_after) const;
   int solve_fields();
   int step(uint32_t steps);
   int step_eager(uint32_t steps);
@@ -127,7 +129,7 @@ class Engine {
   uint32_t parity = 0;
 
  private:
-  int capture(uint32_t steps, uint32_t sort_every, uint32_t start_parity, hipGraphExec_t* out);
+  int capture(uint32_t steps, uint32_t sort_every, bool last_rank, uint32_t start_parity, hipGraphExec_t* out);
   int prepare_graphs(uint32_t steps, uint32_t start_parity);
   uint32_t chunk_steps() const;
   void plan_tree();
@@ -142,11 +144,12 @@ class Engine {
  private:
   uint32_t sort_interval = kDefaultSortInterval;
   uint32_t since_sort = kDefaultSortInterval;
-  // Graphs per starting parity of the finest-level buffers.
-  hipGraphExec_t graph_one[2] = {nullptr, nullptr};
-  hipGraphExec_t graph_sort[2] = {nullptr, nullptr};
+  // Chunk graphs per starting parity of the finest-level buffers.
   hipGraphExec_t graph_chunk[2] = {nullptr, nullptr};
-  hipGraphExec_t graph_chunk_sort[2] = {nullptr, nullptr};
+  // The current finest buffer has not been deposited into for the current
+  // bodies (after an upload, a field solve, a schedule change or a restore):
+  // the next step runs a standalone deposit.
+  bool pending_deposit = true;
   hipStream_t side = nullptr;  // the sort's scatter beside the tree pass
   hipEvent_t ev_fork = nullptr;
   hipEvent_t ev_join = nullptr;

@@ -193,11 +193,10 @@ Engine::~Engine() {
 }
 
 void Engine::drop_graphs() {
-  for (hipGraphExec_t* g : {graph_one, graph_sort, graph_chunk, graph_chunk_sort})
-    for (int k = 0; k < 2; ++k) {
-      if (g[k]) (void)hipGraphExecDestroy(g[k]);
-      g[k] = nullptr;
-    }
+  for (int k = 0; k < 2; ++k) {
+    if (graph_chunk[k]) (void)hipGraphExecDestroy(graph_chunk[k]);
+    graph_chunk[k] = nullptr;
+  }
 }
 
 int Engine::open(uint32_t opts) {
@@ -339,6 +338,7 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
   RB_HIP(hipMemsetAsync(count_buf.p, 0, sizeof(uint32_t) * kLevels * kStripes, stream));
   RB_HIP(hipStreamSynchronize(stream));
   since_sort = sort_interval;
+  pending_deposit = true;
   return RAINBOW_OK;
 }
 
@@ -384,17 +384,25 @@ int Engine::set_tree_fusion(uint32_t hi) {
   return set_tree_split(starts);
 }
 
-// One step (or, without integrate, one field solve).  A sorting step
-// reorders the bodies by the Morton code of their finest cell: its deposit,
-// upward and top also produce the ranks and offsets, and the scatter, copy-
-// back and owned ranges run on the side stream while the tree pass runs
-// (the tree pass does not read the bodies), joined before integrate.
-int Engine::enqueue(bool integrate, bool sorting, uint32_t step_parity) {
+// One step (or, without integrate, one field solve) at finest-level parity
+// `step_parity`.  A step's deposit normally happened at the end of the
+// previous step's integrate (k_integrate_deposit); `deposit_first` runs it
+// here instead, into zeroed buffers.  A sorting step reorders the bodies by
+// the Morton code of their finest cell: its deposit wrote the ranks, its
+// upward and top write the offsets, and the scatter, copy-back and owned
+// ranges run on the side stream while the tree pass runs (the tree pass
+// does not read the bodies), joined before integrate.  `rank_next`: the
+// fused deposit writes ranks for a sorting next step.
+int Engine::enqueue(bool integrate, bool sorting, uint32_t step_parity, bool deposit_first, bool rank_next) {
   if (sys.empty()) return RAINBOW_OK;
   StepArgs a = args;
   a.sorting = sorting ? 1u : 0u;
   a.parity = step_parity;
-  RB_HIP(launch_deposit(a, n_deposit, stream));
+  a.rank_next = rank_next ? 1u : 0u;
+  if (deposit_first) {
+    RB_HIP(zero_finest(step_parity));
+    RB_HIP(launch_deposit(a, n_deposit, stream));
+  }
   RB_HIP(launch_upward(a, n_subtree, stream));
   RB_HIP(launch_top(a, stream));
   if (sorting) {
@@ -405,18 +413,37 @@ int Engine::enqueue(bool integrate, bool sorting, uint32_t step_parity) {
   }
   for (const TreeLaunch& t : tree_launches) RB_HIP(launch_tree(a, t.lo, t.hi, t.blocks, stream));
   if (sorting) RB_HIP(hipStreamWaitEvent(stream, ev_join, 0));
-  if (integrate) RB_HIP(launch_integrate(a, n_integrate, stream));
+  if (integrate) RB_HIP(launch_integrate_deposit(a, n_deposit, stream));
   return RAINBOW_OK;
 }
 
-// Captures `steps` steps; step j sorts when sort_every != 0 and j is a
-// multiple of it.
-int Engine::capture(uint32_t steps, uint32_t sort_every, uint32_t start_parity, hipGraphExec_t* out) {
+// Zeroes every system's finest-level buffer `p` and straggler accumulator
+// (before a standalone deposit; not captured in graphs).
+hipError_t Engine::zero_finest(uint32_t p) {
+  for (const SysDesc& d : sys) {
+    const size_t fb = 16ull * level_cells(d.depth);
+    void* f = p ? static_cast<void*>(d.finest_alt) : static_cast<void*>(d.moments + level_offset(d.depth));
+    hipError_t e = hipMemsetAsync(f, 0, fb, stream);
+    if (e == hipSuccess) e = hipMemsetAsync(d.extra, 0, fb, stream);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// Captures `steps` steps from parity start_parity, with no deposit pending.
+// Steps sort at multiples of sort_every (0: none); a step's fused deposit
+// writes ranks when the following step sorts, and for the last step when
+// last_rank is set.
+int Engine::capture(uint32_t steps, uint32_t sort_every, bool last_rank, uint32_t start_parity,
+                    hipGraphExec_t* out) {
   hipGraph_t graph = nullptr;
   RB_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
   int rc = RAINBOW_OK;
-  for (uint32_t s = 0; s < steps && rc == RAINBOW_OK; ++s)
-    rc = enqueue(true, sort_every != 0u && s % sort_every == 0u, start_parity ^ (s & 1u));
+  for (uint32_t s = 0; s < steps && rc == RAINBOW_OK; ++s) {
+    const bool sorting = sort_every != 0u && s % sort_every == 0u;
+    const bool rank_next = s + 1u < steps ? (sort_every != 0u && (s + 1u) % sort_every == 0u) : last_rank;
+    rc = enqueue(true, sorting, start_parity ^ (s & 1u), false, rank_next);
+  }
   const hipError_t e = hipStreamEndCapture(stream, &graph);
   if (rc != RAINBOW_OK) {
     if (graph) (void)hipGraphDestroy(graph);
@@ -436,23 +463,24 @@ uint32_t Engine::chunk_steps() const {
   return sort_interval * (kGraphSteps / sort_interval);
 }
 
-// Graphs: one step, one sorting step, and chunks of chunk_steps() steps
-// without sorts and with sorts due from their first step, each for the
-// finest-level parity it starts at (captured when first needed).
+// The chunk graph of the current sort interval, for parity p: interval 0,
+// 16 steps without sorts; interval <= kGraphSteps, chunk_steps() steps that
+// start with a sort and sort every interval steps (the next step sorts
+// again); a longer interval, 16 steps without sorts, run only when the
+// next sort lies beyond them.
 int Engine::prepare_graphs(uint32_t steps, uint32_t p) {
-  int rc;
   const uint32_t chunk = chunk_steps();
-  const uint32_t every = sort_interval <= kGraphSteps ? sort_interval : chunk;
-  if (graph_one[p] == nullptr && sort_interval != 1u && (rc = capture(1u, 0u, p, &graph_one[p])) != 0) return rc;
-  if (sort_interval != 0u && graph_sort[p] == nullptr && (rc = capture(1u, 1u, p, &graph_sort[p])) != 0) return rc;
-  if (steps >= chunk && chunk > 1u) {
-    if (sort_interval != 1u && graph_chunk[p] == nullptr && (rc = capture(chunk, 0u, p, &graph_chunk[p])) != 0)
-      return rc;
-    if (sort_interval != 0u && graph_chunk_sort[p] == nullptr &&
-        (rc = capture(chunk, every, p, &graph_chunk_sort[p])) != 0)
-      return rc;
-  }
-  return RAINBOW_OK;
+  if (chunk < 2u || steps < chunk || graph_chunk[p] != nullptr) return RAINBOW_OK;
+  const bool periodic = sort_interval != 0u && sort_interval <= kGraphSteps;
+  return capture(chunk, periodic ? sort_interval : 0u, periodic, p, &graph_chunk[p]);
+}
+
+// The next step's schedule: whether it sorts, and whether its fused
+// deposit must write ranks (the step after it sorts).
+void Engine::plan_next(bool* sorting, bool* rank_next, uint32_t* since_after) const {
+  *sorting = sort_interval != 0u && since_sort >= sort_interval;
+  *since_after = *sorting ? 1u : since_sort + 1u;
+  *rank_next = sort_interval != 0u && *since_after >= sort_interval;
 }
 
 // since_sort counts the steps since the last sorting step (that step
@@ -463,47 +491,44 @@ int Engine::step(uint32_t steps) {
   const uint32_t chunk = chunk_steps();
   uint32_t done = 0;
   while (done < steps) {
-    // Graphs of the current parity (the other parity's are captured when
-    // the parity first reaches it).
-    if ((rc = prepare_graphs(steps - done, parity)) != 0) return rc;
     const uint32_t p = parity;
-    const bool sort_due = sort_interval != 0u && since_sort >= sort_interval;
-    const bool whole = chunk > 1u && steps - done >= chunk;
-    uint32_t ran = 1u;
-    if (sort_due) {
-      if (whole && graph_chunk_sort[p] != nullptr) {
-        RB_HIP(hipGraphLaunch(graph_chunk_sort[p], stream));
-        ran = chunk;
-        since_sort = sort_interval <= kGraphSteps ? sort_interval : chunk;
-      } else {
-        RB_HIP(hipGraphLaunch(graph_sort[p], stream));
-        since_sort = 1u;
-      }
-    } else {
-      const uint64_t free_steps = sort_interval == 0u ? ~0ull : sort_interval - since_sort;
-      if (whole && free_steps >= chunk && graph_chunk[p] != nullptr) {
-        RB_HIP(hipGraphLaunch(graph_chunk[p], stream));
-        ran = chunk;
-      } else {
-        RB_HIP(hipGraphLaunch(graph_one[p], stream));
-      }
-      since_sort += ran;
+    bool use_chunk = !pending_deposit && chunk > 1u && steps - done >= chunk;
+    if (use_chunk) {
+      if (sort_interval != 0u && sort_interval <= kGraphSteps)
+        use_chunk = since_sort >= sort_interval;  // starts with a sort
+      else if (sort_interval != 0u)
+        use_chunk = since_sort + chunk < sort_interval;  // no sort within or right after
     }
-    done += ran;
-    parity ^= ran & 1u;
+    if (use_chunk) {
+      if ((rc = prepare_graphs(steps - done, p)) != 0) return rc;
+      RB_HIP(hipGraphLaunch(graph_chunk[p], stream));
+      done += chunk;
+      parity ^= chunk & 1u;
+      if (sort_interval != 0u && sort_interval <= kGraphSteps)
+        since_sort = sort_interval;
+      else
+        since_sort += chunk;
+    } else {
+      if ((rc = step_eager(1u)) != 0) return rc;
+      done += 1u;
+    }
   }
   return RAINBOW_OK;
 }
 
-// Steps launched directly, without graphs (for a set of systems that
+// Steps launched directly, without graphs (the first step after a change
+// of bodies or schedule, steps between chunks, and a set of systems that
 // changes too often to amortise a capture).
 int Engine::step_eager(uint32_t steps) {
   for (uint32_t k = 0; k < steps; ++k) {
-    const bool sorting = sort_interval != 0u && since_sort >= sort_interval;
-    int rc = enqueue(true, sorting, parity);
+    bool sorting, rank_next;
+    uint32_t after;
+    plan_next(&sorting, &rank_next, &after);
+    int rc = enqueue(true, sorting, parity, pending_deposit, rank_next);
     if (rc) return rc;
     parity ^= 1u;
-    since_sort = sorting ? 1u : since_sort + 1u;
+    since_sort = after;
+    pending_deposit = false;
   }
   return RAINBOW_OK;
 }
@@ -514,6 +539,7 @@ int Engine::set_sort_interval(uint32_t interval) {
   if (interval != sort_interval) drop_graphs();
   sort_interval = interval;
   since_sort = interval;  // sort before the next step
+  pending_deposit = true;  // with ranks for that sort
   return RAINBOW_OK;
 }
 
@@ -539,6 +565,7 @@ int Engine::snapshot(bool restore) {
     }
   }
   RB_HIP(hipStreamSynchronize(stream));
+  if (restore) pending_deposit = true;  // the last fused deposit was of other positions
   return RAINBOW_OK;
 }
 
@@ -588,9 +615,11 @@ int Engine::autotune(uint32_t steps, uint32_t* chosen) {
 }
 
 int Engine::solve_fields() {
-  const int rc = enqueue(false, false, parity);
-  if (rc == RAINBOW_OK) parity ^= 1u;
-  return rc;
+  const int rc = enqueue(false, false, parity, pending_deposit, false);
+  if (rc != RAINBOW_OK) return rc;
+  parity ^= 1u;
+  pending_deposit = true;  // the next step deposits its own bodies
+  return RAINBOW_OK;
 }
 
 int Engine::sync() {
@@ -612,8 +641,10 @@ int Engine::time_steps(uint32_t steps, float* ms) {
 }
 
 int Engine::phase_ms(uint32_t steps, float* out, uint32_t n) {
-  // deposit, upward, top, the tree launches, integrate, then the body sort
-  // (its mean per step: the steps sort when due, as in step()).
+  // deposit (standalone: the first step after a change of bodies), upward,
+  // top, the tree launches, integrate with the next step's deposit, then
+  // the body sort (its mean per step: the steps sort when due, as in
+  // step()).
   const uint32_t nkern = 4u + static_cast<uint32_t>(tree_launches.size());
   const uint32_t nphase = nkern + 1u;
   if (out == nullptr || n < nphase) return fail(RAINBOW_E_INVALID, "phase_ms needs %u entries", nphase);
@@ -627,14 +658,26 @@ int Engine::phase_ms(uint32_t steps, float* out, uint32_t n) {
   std::vector<double> acc(nphase, 0.0);
   hipError_t err = hipSuccess;
   for (uint32_t s = 0; s < steps && err == hipSuccess; ++s) {
-    const bool sorting = sort_interval != 0u && since_sort >= sort_interval;
-    since_sort = sorting ? 1u : since_sort + 1u;
+    bool sorting, rank_next;
+    uint32_t after;
+    plan_next(&sorting, &rank_next, &after);
     StepArgs a = args;
     a.sorting = sorting ? 1u : 0u;
     a.parity = parity;
+    a.rank_next = rank_next ? 1u : 0u;
+    const bool deposit_first = pending_deposit;
+    bool skipped_deposit = false;
     parity ^= 1u;
+    since_sort = after;
+    pending_deposit = false;
     uint32_t k = 0;
-    if (err == hipSuccess) err = launch_deposit(a, n_deposit, stream, &tm[k++]);
+    if (deposit_first) {
+      if (err == hipSuccess) err = zero_finest(a.parity);
+      if (err == hipSuccess) err = launch_deposit(a, n_deposit, stream, &tm[k++]);
+    } else {
+      skipped_deposit = true;
+      ++k;
+    }
     if (err == hipSuccess) err = launch_upward(a, n_subtree, stream, &tm[k++]);
     if (err == hipSuccess) err = launch_top(a, stream, &tm[k++]);
     if (sorting) {
@@ -647,7 +690,7 @@ int Engine::phase_ms(uint32_t steps, float* out, uint32_t n) {
     for (const TreeLaunch& t : tree_launches)
       if (err == hipSuccess) err = launch_tree(a, t.lo, t.hi, t.blocks, stream, &tm[k++]);
     if (sorting && err == hipSuccess) err = hipStreamWaitEvent(stream, sort_ev[1], 0);
-    if (err == hipSuccess) err = launch_integrate(a, n_integrate, stream, &tm[k++]);
+    if (err == hipSuccess) err = launch_integrate_deposit(a, n_deposit, stream, &tm[k++]);
     if (sorting && err == hipSuccess) {
       err = hipEventSynchronize(sort_ev[1]);
       float ms = 0.0f;
@@ -655,7 +698,7 @@ int Engine::phase_ms(uint32_t steps, float* out, uint32_t n) {
       acc[nkern] += ms;
     }
     if (err == hipSuccess) err = hipStreamSynchronize(stream);
-    for (uint32_t i = 0; i < nkern && err == hipSuccess; ++i) {
+    for (uint32_t i = skipped_deposit ? 1u : 0u; i < nkern && err == hipSuccess; ++i) {
       float ms = 0.0f;
       err = hipEventElapsedTime(&ms, tm[i].start, tm[i].stop);
       acc[i] += ms;
@@ -725,6 +768,7 @@ int Engine::upload(uint32_t s, const void* data, size_t bytes) {
   if (s >= sys.size()) return fail(RAINBOW_E_INVALID, "system %u out of range", s);
   RB_HIP(hipStreamSynchronize(stream));
   since_sort = sort_interval;
+  pending_deposit = true;
   return upload_system(sys[s], data, bytes, stream, &io_buf);
 }
 
