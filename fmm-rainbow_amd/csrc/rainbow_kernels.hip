@@ -227,10 +227,15 @@ __device__ void deposit_block(const StepArgs& a, const SysDesc& d, uint32_t bloc
     // places this block's bodies after those already in `extra`.
     const unsigned long long packed = static_cast<unsigned long long>(static_cast<uint32_t>(vals[0][s])) |
                                       (static_cast<unsigned long long>(static_cast<uint32_t>(vals[1][s])) << 32);
-    const unsigned long long old = atomicAdd(reinterpret_cast<unsigned long long*>(d.extra + key), packed);
+    unsigned long long* const ce = reinterpret_cast<unsigned long long*>(d.extra + key);
+    if (ranks) {
+      const unsigned long long old = atomicAdd(ce, packed);
+      keys[s] = kRankExtra | static_cast<uint32_t>(old);
+    } else {
+      (void)atomicAdd(ce, packed);  // no return value needed: the hardware need not wait for it
+    }
     atomicAdd(reinterpret_cast<int*>(d.extra + key) + 2, vals[2][s]);
     atomicAdd(reinterpret_cast<int*>(d.extra + key) + 3, vals[3][s]);
-    keys[s] = kRankExtra | static_cast<uint32_t>(old);
   }
   if (!ranks) return;
   __syncthreads();
@@ -886,9 +891,16 @@ __device__ void ancestor_fields(const SysDesc& d, uint32_t lo, uint32_t level, u
   const uint32_t ci = tid / 10u, ck = tid - 10u * ci;
   Chain<kComp> ch;
   if (ci < na) {
-    const float* row = st + (ci * 10u + ck) * 216u;
-#pragma unroll 8
-    for (uint32_t u = 0; u < 216u; ++u) ch.add(row[u]);
+    // 216 terms = 54 float4 (rows start 16-byte aligned: 216 * 4 = 864).
+    const float4* row = reinterpret_cast<const float4*>(st + (ci * 10u + ck) * 216u);
+#pragma unroll 6
+    for (uint32_t u = 0; u < 54u; ++u) {
+      const float4 v = row[u];
+      ch.add(v.x);
+      ch.add(v.y);
+      ch.add(v.z);
+      ch.add(v.w);
+    }
   }
   // L2L in order, from A_lo up to the parent.
   for (uint32_t i = 0; i < na; ++i) {
