@@ -30,17 +30,22 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 
 
+FLOPS_PER_PAIR = 50  # pair_field: 40 f32 operations (sqrt, div counted once); 10 additions into the field
+
+
 def algorithmic_bytes(n, depth, stats):
-    """Compulsory bytes per launch of each kernel (each unique address once).
+    """Compulsory HBM bytes per launch of each kernel (each unique address once).
 
     deposit    16 N + 16 C_d             read positions, write finest moments
-    upward     read the finest cells once, zero the next step's finest
-               buffer, write levels depth-1..depth-3
+               (standalone; normally inside integrate)
+    upward     read the finest cells, zero the next step's finest buffer,
+               write levels depth-1..depth-3 and the source records
     top        read level depth-3, write the levels above
     tree(l)    16 T_l + 48 T_l + 48 P_l: read the level's target moments,
                write their fields, read their parents' fields (the 6^3
                source windows are served from cache)
-    integrate  64 N + 16 L + 48 L        read+write bodies, leaf moments+fields
+    integrate  64 N + 64 L + 16 C_d      read+write bodies, leaf moments and
+               fields, deposit the next step's finest moments
     with C cells, C_d finest cells, T_l targets of level l, P_l their
     parents, L leaves.
     """
@@ -48,8 +53,9 @@ def algorithmic_bytes(n, depth, stats):
     cd = 8 ** depth
     leaves = stats[7]
     top = (8 ** (depth - 2) - 1) // 7  # cells of levels 0..depth-3
+    mid = cells - cd - top + 8 ** (depth - 3)  # levels depth-1..depth-3
     out = {"deposit": 16 * n + 16 * cd,
-           "upward": 16 * cd + 16 * cd + 16 * (cells - cd - top + 8 ** (depth - 3)),
+           "upward": 16 * cd + 16 * cd + 16 * mid + 16 * (cd + mid),
            "top": 16 * (8 ** (depth - 3)) + 16 * (top - 8 ** (depth - 3))}
     for l in range(2, depth + 1):
         t = stats[l]
@@ -59,10 +65,30 @@ def algorithmic_bytes(n, depth, stats):
     for lo in range(2, depth + 1):
         for hi in range(lo + 1, depth + 1):
             out[f"tree_l{lo}-{hi}"] = sum(out[f"tree_l{l}"] for l in range(lo, hi + 1))
-    out["integrate"] = 64 * n + 16 * leaves + 48 * leaves
-    # sort: read keys+positions+velocities, write the sorted bodies and perm.
-    out["sort"] = 8 * n + 32 * n + 36 * n
+    out["integrate"] = 64 * n + 64 * leaves + 16 * cd
+    # sort: read the bodies and their ranks, write the staged bodies, copy back.
+    out["sort"] = 36 * n + 4 * n + 36 * n + 72 * n
     return out
+
+
+def algorithmic_flops(depth, m2l, near):
+    """FP32 operations of the tree launches: non-empty pairs x FLOPS_PER_PAIR."""
+    out = {f"tree_l{l}": FLOPS_PER_PAIR * (m2l[l] + near[l]) for l in range(2, depth + 1)}
+    for lo in range(2, depth + 1):
+        for hi in range(lo + 1, depth + 1):
+            out[f"tree_l{lo}-{hi}"] = sum(out[f"tree_l{l}"] for l in range(lo, hi + 1))
+    return out
+
+
+def roofline_of(phase, ms, bytes_per, flops_per):
+    """HBM roofline, or for a tree launch the FP32 compute roofline."""
+    if phase in flops_per:
+        achieved = flops_per[phase] / (ms * 1e-3) / 1e12
+        return {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "flops_per_launch": flops_per[phase]}
+    achieved = bytes_per[phase] / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_launch": bytes_per[phase]}
 
 
 TRAFFIC_TABLE = ROOT / "profiles" / "hbm_traffic_plummer-1m.md"
@@ -368,23 +394,17 @@ def main():
     names = solver.phase_names()
     per = dict(zip(names, phases))
     bytes_per = algorithmic_bytes(n, solver.depth, stats)
+    m2l, near = solver.pair_counts()
+    flops_per = algorithmic_flops(solver.depth, m2l, near)
     # The dominant kernel: the phase with the largest mean duration (the
     # sort is several library kernels and is not a candidate).
     dom = max((k for k in per if k != "sort"), key=per.get)
-    achieved = bytes_per[dom] / (per[dom] * 1e-3) / 1e9
     traffic, traffic_src = measured_traffic(dom)
-    roofline = {
-        "bound": "hbm",
-        "kernel": dom,
-        "achieved": round(achieved, 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic,
-        "traffic_source": traffic_src,
-        "mean_ms": round(per[dom], 5),
-        "bytes_per_launch": bytes_per[dom],
-    }
+    roofline = {"kernel": dom, **roofline_of(dom, per[dom], bytes_per, flops_per),
+                "traffic": traffic, "traffic_source": traffic_src, "mean_ms": round(per[dom], 5)}
+    # Phases that run every step (the standalone deposit and the sort do not).
+    phase_roofline = {k: roofline_of(k, v, bytes_per, flops_per)["frac"]
+                      for k, v in per.items() if v > 0 and k in bytes_per and k not in ("deposit", "sort")}
     line = {
         "metric": "particle-steps/s at 1M bodies on 1 GPU; p50/p99 latency for mixed-N serving",
         "value": value,
@@ -407,6 +427,8 @@ def main():
         },
         "roofline": roofline,
         "phases_ms": {k: round(v, 5) for k, v in per.items()},
+        "phases_roofline_frac": phase_roofline,
+        "pairs": {"m2l": m2l[1:solver.depth + 1], "near": near[1:solver.depth + 1]},
         "tree": {"targets_per_level": stats[1:1 + solver.depth], "leaves": stats[7]},
         "wall_s_timed": wall,
     }

@@ -1035,6 +1035,46 @@ int rainbow_hip_tree_stats(rainbow_hip_t* h, uint32_t out[8]) {
   return RAINBOW_OK;
 }
 
+int rainbow_hip_pair_counts(rainbow_hip_t* h, uint64_t out[16]) {
+  RB_CHECK_HANDLE(h);
+  if (out == nullptr) return fail(RAINBOW_E_INVALID, "null output");
+  const rainbow::SysDesc& d = h->batch.engine.sys[0];
+  const uint32_t cells = rainbow::total_cells(d.depth);
+  std::vector<int32_t> m(4u * static_cast<size_t>(cells));
+  const int rc = h->batch.engine.download(0u, RAINBOW_MOMENTS, m.data(), 16u * static_cast<size_t>(cells));
+  if (rc) return rc;
+  for (int k = 0; k < 16; ++k) out[k] = 0u;
+  auto count = [&](uint32_t l, int x, int y, int z) -> int32_t {
+    const int dim = 1 << l;
+    if (x < 0 || y < 0 || z < 0 || x >= dim || y >= dim || z >= dim) return 0;
+    return m[4u * (rainbow::level_offset(l) + static_cast<uint32_t>(x + dim * (y + dim * z)))];
+  };
+  for (uint32_t l = 1; l <= d.depth; ++l) {
+    const int dim = 1 << l;
+    for (int z = 0; z < dim; ++z)
+      for (int y = 0; y < dim; ++y)
+        for (int x = 0; x < dim; ++x) {
+          const int32_t c = count(l, x, y, z);
+          if (c == 0 || count(l - 1u, x / 2, y / 2, z / 2) <= 16) continue;
+          // M2L: children of the parent's neighbours that are not adjacent.
+          for (int pz = z / 2 - 1; pz <= z / 2 + 1; ++pz)
+            for (int py = y / 2 - 1; py <= y / 2 + 1; ++py)
+              for (int px = x / 2 - 1; px <= x / 2 + 1; ++px)
+                for (int k = 0; k < 8; ++k) {
+                  const int sx = 2 * px + (k & 1), sy = 2 * py + ((k >> 1) & 1), sz = 2 * pz + (k >> 2);
+                  if (std::abs(sx - x) <= 1 && std::abs(sy - y) <= 1 && std::abs(sz - z) <= 1) continue;
+                  if (count(l, sx, sy, sz) != 0) out[l] += 1u;
+                }
+          if (c <= 16 || l == d.depth)
+            for (int dz = -1; dz <= 1; ++dz)
+              for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx)
+                  if ((dx | dy | dz) != 0 && count(l, x + dx, y + dy, z + dz) != 0) out[8 + l] += 1u;
+        }
+  }
+  return RAINBOW_OK;
+}
+
 int rainbow_hip_diagnostics(rainbow_hip_t* h, double out[8]) {
   RB_CHECK_HANDLE(h);
   if (out == nullptr) return fail(RAINBOW_E_INVALID, "null output");

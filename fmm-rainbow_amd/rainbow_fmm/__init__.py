@@ -137,6 +137,7 @@ _SIGNATURES = {
     "rainbow_hip_tree_launches": (_i32, [_vp, _P(_u32), _u32]),
     "rainbow_batch_set_tree_fusion": (_i32, [_vp, _u32]),
     "rainbow_hip_tree_stats": (_i32, [_vp, _P(_u32)]),
+    "rainbow_hip_pair_counts": (_i32, [_vp, _P(_u64)]),
     "rainbow_hip_set_sort_interval": (_i32, [_vp, _u32]),
     "rainbow_hip_autotune": (_i32, [_vp, _u32, _P(_u32)]),
     "rainbow_batch_set_sort_interval": (_i32, [_vp, _u32]),
@@ -444,6 +445,12 @@ class Solver:
         v = _u32()
         _check(lib().rainbow_hip_autotune(self._h, steps, ctypes.byref(v)))
         return v.value
+
+    def pair_counts(self) -> tuple[list[int], list[int]]:
+        """(M2L pairs per level, near pairs per level) of the last solve."""
+        buf = (_u64 * 16)()
+        _check(lib().rainbow_hip_pair_counts(self._h, buf))
+        return [buf[i] for i in range(8)], [buf[8 + i] for i in range(8)]
 
     def tree_stats(self) -> list[int]:
         buf = (_u32 * 8)()

@@ -272,6 +272,12 @@ int rainbow_hip_autotune(rainbow_hip_t* h, uint32_t steps, uint32_t* interval);
  * number of near-field leaves.  out[l] for l = 0..6, out[7] = leaves. */
 int rainbow_hip_tree_stats(rainbow_hip_t* h, uint32_t out[8]);
 
+/* Non-empty source pairs of the last solve per level: out[l] M2L pairs,
+ * out[8 + l] near-field pairs (l = 1..6), computed on the host from the
+ * moments.  Each pair costs 40 f32 operations of pair_field plus 10
+ * additions into the target's field. */
+int rainbow_hip_pair_counts(rainbow_hip_t* h, uint64_t out[16]);
+
 /* FP64 diagnostics of the current particle state (precision form):
  * out[0] kinetic energy, out[1] potential energy (the tree's own potential
  * at each body, halved), out[2..4] linear momentum, out[5..7] angular

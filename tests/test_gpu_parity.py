@@ -453,3 +453,49 @@ def test_direct_sum_f64_matches_oracle():
     np.testing.assert_allclose(got[:, :3], want[:, :3], rtol=0, atol=float(1e-12 * scale.max()))
     np.testing.assert_allclose(got[:, 3], want[:, 3], rtol=1e-12)
     s.close()
+
+
+def test_pair_counts_match_oracle_stencils():
+    say("pair counts per level vs the oracle's stencils")
+    n = 65536
+    parts = rf.ic_plummer(n, 31, 0.1, 1.5, 1.0)
+    s = rf.Solver(n, rf.physics(dt=5e-4, gravity=1.0, softening=1e-3))
+    s.upload(parts)
+    s.solve_fields()
+    m2l, near = s.pair_counts()
+    m = s.moments()
+    counts = m[:, 0]
+    want_m2l = [0] * 8
+    want_near = [0] * 8
+    # Brute force from the moments with numpy (same definitions as the pass).
+    from itertools import product
+    off = 0
+    offs = []
+    for level in range(s.depth + 1):
+        offs.append(off)
+        off += 8 ** level
+    def cnt(level, x, y, z):
+        d = 1 << level
+        if min(x, y, z) < 0 or max(x, y, z) >= d:
+            return 0
+        return int(counts[offs[level] + x + d * (y + d * z)])
+    for level in range(1, s.depth + 1):
+        d = 1 << level
+        for z, y, x in product(range(d), repeat=3):
+            c = cnt(level, x, y, z)
+            if c == 0 or cnt(level - 1, x // 2, y // 2, z // 2) <= 16:
+                continue
+            for pz, py, px in product(range(z // 2 - 1, z // 2 + 2), range(y // 2 - 1, y // 2 + 2),
+                                      range(x // 2 - 1, x // 2 + 2)):
+                for k in range(8):
+                    sx, sy, sz = 2 * px + (k & 1), 2 * py + ((k >> 1) & 1), 2 * pz + (k >> 2)
+                    if max(abs(sx - x), abs(sy - y), abs(sz - z)) <= 1:
+                        continue
+                    if cnt(level, sx, sy, sz):
+                        want_m2l[level] += 1
+            if c <= 16 or level == s.depth:
+                for dz, dy, dx in product((-1, 0, 1), repeat=3):
+                    if (dx, dy, dz) != (0, 0, 0) and cnt(level, x + dx, y + dy, z + dz):
+                        want_near[level] += 1
+    assert m2l == want_m2l and near == want_near
+    s.close()
