@@ -816,8 +816,10 @@ __device__ __forceinline__ void stage_window(float4* win, const SysDesc& d, uint
 }
 
 // LDS of the ancestor path per ancestor: its 6^3 window and its 10 x 216
-// pair terms.
-constexpr uint32_t kAncestorBytes = 216u * 16u + 10u * 216u * 4u;
+// pair terms (rows padded to 220 floats so the chains' rows start in
+// different LDS banks).
+constexpr uint32_t kAncRow = 220u;
+constexpr uint32_t kAncestorBytes = 216u * 16u + 10u * kAncRow * 4u;
 // Chunks of 8 x 8 pairs per round of the tree item loop (two per wave).
 constexpr uint32_t kTreeChunksPerRound = 8u;
 // LDS of one item: the window and a round's 10 x 64 chunk terms.
@@ -839,7 +841,7 @@ __device__ void ancestor_fields(const SysDesc& d, uint32_t lo, uint32_t level, u
                                 uint32_t pz, unsigned char* lds, float* pf, uint32_t tid) {
   const uint32_t na = level - lo;
   float4* awin = reinterpret_cast<float4*>(lds);               // [na][216]
-  float* st = reinterpret_cast<float*>(lds + na * 216u * 16u);  // [na][10][216]
+  float* st = reinterpret_cast<float*>(lds + na * 216u * 16u);  // [na][10][kAncRow]
   // One round of loads: every ancestor's window, and the field of A_{lo-1}.
   for (uint32_t w = tid; w < na * 216u; w += kTreeThreads) {
     const uint32_t i = w / 216u, ww = w - i * 216u;
@@ -884,16 +886,16 @@ __device__ void ancestor_fields(const SysDesc& d, uint32_t lo, uint32_t level, u
       if (sr.w != 0.0f) pair_terms(sr, center1(cx, dim), center1(cy, dim), center1(cz, dim), eps2, t);
     }
 #pragma unroll
-    for (int k = 0; k < 10; ++k) st[(i * 10u + k) * 216u + u] = t[k];
+    for (int k = 0; k < 10; ++k) st[(i * 10u + k) * kAncRow + u] = t[k];
   }
   __syncthreads();
   // Chains: thread i * 10 + k sums component k of ancestor i.
   const uint32_t ci = tid / 10u, ck = tid - 10u * ci;
   Chain<kComp> ch;
   if (ci < na) {
-    // 216 terms = 54 float4 (rows start 16-byte aligned: 216 * 4 = 864).
-    const float4* row = reinterpret_cast<const float4*>(st + (ci * 10u + ck) * 216u);
-#pragma unroll 6
+    // 216 terms = 54 float4 (rows start 16-byte aligned: 220 * 4 = 880).
+    const float4* row = reinterpret_cast<const float4*>(st + (ci * 10u + ck) * kAncRow);
+#pragma unroll 3
     for (uint32_t u = 0; u < 54u; ++u) {
       const float4 v = row[u];
       ch.add(v.x);
@@ -1068,8 +1070,10 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t lo, 
             if (sr.w != 0.0f) pair_terms(sr, tcx, tcy, tcz, eps2_near, t);
           }
         }
+        // Terms of chunk `slot` stored per chain (target tt, component k)
+        // as 8 consecutive sources: the chains then read contiguous runs.
 #pragma unroll
-        for (int k = 0; k < 10; ++k) terms[(slot * 10u + k) * 64u + lane] = t[k];
+        for (int k = 0; k < 10; ++k) terms[((slot * 8u + tt) * 10u + k) * 8u + cc] = t[k];
       }
       __syncthreads();
       // The chains add the round's chunks in order; the L2L term goes in
@@ -1078,8 +1082,8 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t lo, 
 #pragma unroll
         for (uint32_t u = 0; u < kChunksPerRound; ++u) {
           if (r0 + u == 27u) ch.reset(ch.value() + inh);
-          const float4 v0 = *reinterpret_cast<const float4*>(&terms[(u * 10u + ck) * 64u + ct * 8u]);
-          const float4 v1 = *reinterpret_cast<const float4*>(&terms[(u * 10u + ck) * 64u + ct * 8u + 4u]);
+          const float4 v0 = *reinterpret_cast<const float4*>(&terms[((u * 8u + ct) * 10u + ck) * 8u]);
+          const float4 v1 = *reinterpret_cast<const float4*>(&terms[((u * 8u + ct) * 10u + ck) * 8u + 4u]);
           ch.add(v0.x);
           ch.add(v0.y);
           ch.add(v0.z);
