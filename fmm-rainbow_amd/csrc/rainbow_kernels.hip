@@ -62,18 +62,29 @@ __device__ __forceinline__ uint32_t lin(uint32_t x, uint32_t y, uint32_t z, uint
   return x + dim * (y + dim * z);
 }
 
-// cell_center, src/rainbow_fmm.wgsl:136-142 (one axis).
+// 1 / dim for dim a power of two (exact), so x * inv_pow2(dim) equals the
+// correctly rounded x / dim for every x: both round the same real number.
+__device__ __forceinline__ float inv_pow2(uint32_t dim) {
+  return __uint_as_float((127u - (31u - static_cast<uint32_t>(__clz(dim)))) << 23);
+}
+
+// cell_center, src/rainbow_fmm.wgsl:136-142 (one axis); 2 / dim is exact.
 __device__ __forceinline__ float center1(uint32_t c, uint32_t dim) {
-  return ((static_cast<float>(c) + 0.5f) * (2.0f / static_cast<float>(dim)) - 1.0f) *
-         kWorldHalfExtent;
+  return ((static_cast<float>(c) + 0.5f) * (2.0f * inv_pow2(dim)) - 1.0f) * kWorldHalfExtent;
 }
 
 // position_cell, src/rainbow_fmm.wgsl:144-152, split in the level-independent
 // part clamp(p / 1.6) * 0.5 + 0.5 and the per-level floor(u * dim).  The
 // product by a power of two is exact, so this equals the reference at every
-// level.
+// level.  p / 1.6f is formed as q = p * 0.625 (= RN(1 / 1.6f)) corrected once
+// by its exact remainder: the clamped result equals the one from the
+// correctly rounded quotient for every float p (all 2^32 inputs are checked
+// on the host by the tests), at a third of a division's instructions.
 __device__ __forceinline__ float unit_coord(float p) {
-  float n = p / kWorldHalfExtent;
+  constexpr float kInv = 0.625f;  // RN(1 / 1.6f)
+  const float q0 = p * kInv;
+  const float rem = __builtin_fmaf(-q0, kWorldHalfExtent, p);
+  float n = __builtin_fmaf(rem, kInv, q0);
   n = fminf(fmaxf(n, -0.99999f), 0.99999f);
   return n * 0.5f + 0.5f;
 }
@@ -872,7 +883,7 @@ __device__ void ancestor_fields(const SysDesc& d, uint32_t lo, uint32_t level, u
     const uint32_t sh = level - 1u - (lo + i);
     const uint32_t dim = 1u << (lo + i);
     const uint32_t cx = px >> sh, cy = py >> sh, cz = pz >> sh;
-    const float cell_epsilon = (0.06f * kWorldHalfExtent) / static_cast<float>(dim);
+    const float cell_epsilon = (0.06f * kWorldHalfExtent) * inv_pow2(dim);
     const float eps2 = cell_epsilon * cell_epsilon;
     const uint32_t q = u >> 3, c = u & 7u;
     const uint32_t wsx = 2u * (q % 3u) + (c & 1u);
@@ -1015,10 +1026,10 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t lo, 
     // Zero terms leave the sums unchanged, so every item walks all 32.
     const bool any_leaf = __syncthreads_or(leaf) != 0;
     // cell_epsilon, src/rainbow_fmm.wgsl:507 (0.06 * WORLD_HALF_EXTENT / dim).
-    const float cell_epsilon = (0.06f * kWorldHalfExtent) / static_cast<float>(dim);
+    const float cell_epsilon = (0.06f * kWorldHalfExtent) * inv_pow2(dim);
     const float eps2_m2l = cell_epsilon * cell_epsilon;
     const float near_epsilon =
-        fmaxf(d.softening, (0.72f * kWorldHalfExtent) / static_cast<float>(dim));
+        fmaxf(d.softening, (0.72f * kWorldHalfExtent) * inv_pow2(dim));
     const float eps2_near = near_epsilon * near_epsilon;
 
     Chain<kComp> ch;
@@ -1172,7 +1183,7 @@ __device__ __forceinline__ Leaf leaf_field(const SysDesc& d, uint32_t parity, fl
     const float4 com = d.source[cell];
     const float ex = com.x - px, ey = com.y - py, ez = com.z - pz;
     const float epsilon =
-        fmaxf(d.softening, (0.62f * kWorldHalfExtent) / static_cast<float>(out.dim));
+        fmaxf(d.softening, (0.62f * kWorldHalfExtent) * inv_pow2(out.dim));
     const float inv_r = 1.0f / sqrtf(((ex * ex + ey * ey) + ez * ez) + epsilon * epsilon);
     const float self_mass = static_cast<float>(count - 1u) / static_cast<float>(d.n);
     self[0] = d.gravity * self_mass * ex * inv_r * inv_r * inv_r;

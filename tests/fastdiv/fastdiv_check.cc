@@ -1,0 +1,50 @@
+// Exhaustive host check of the kernels' division-by-constant forms.
+//
+// The integrate kernels replace a correctly rounded x / d by
+//   q0 = x * r;  rem = fma(-q0, d, x);  q = fma(rem, r, q0)   (r = RN(1 / d))
+// only where the value that is kept afterwards is the same for every float x.
+// q can differ from x / d in the last bit for some x; what is checked here is
+// the quantity the kernel actually uses after the division:
+//   unit  : clamp(q, -0.99999, 0.99999) * 0.5 + 0.5   (position -> unit cube)
+//   clamp01: clamp(q, 0, 1)                            (ramps and weights)
+// Usage: fastdiv_check <d> <unit|clamp01>; prints the mismatch count of the
+// final quantity over all 2^32 float bit patterns and exits 0 iff it is zero.
+// Built with -ffp-contract=off so each line rounds as written.
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+namespace {
+float from_bits(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
+uint32_t to_bits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
+float clampf(float v, float lo, float hi) { return std::fmin(std::fmax(v, lo), hi); }
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc != 3) { std::fprintf(stderr, "usage: %s <d> <unit|clamp01>\n", argv[0]); return 2; }
+  const float d = std::strtof(argv[1], nullptr);
+  const bool unit = std::strcmp(argv[2], "unit") == 0;
+  if (!unit && std::strcmp(argv[2], "clamp01") != 0) return 2;
+  volatile float dv = d;
+  const float r = 1.0f / dv;
+  unsigned long long raw = 0, kept = 0;
+#pragma omp parallel for reduction(+ : raw, kept) schedule(static, 1 << 20)
+  for (long long i = 0; i < (1ll << 32); ++i) {
+    const float x = from_bits(static_cast<uint32_t>(i));
+    if (!std::isfinite(x)) continue;
+    const float exact = x / dv;
+    const float q0 = x * r;
+    const float rem = std::fmaf(-q0, d, x);
+    const float q = std::fmaf(rem, r, q0);
+    if (to_bits(q) == to_bits(exact)) continue;
+    ++raw;
+    const float a = unit ? clampf(exact, -0.99999f, 0.99999f) * 0.5f + 0.5f : clampf(exact, 0.0f, 1.0f);
+    const float b = unit ? clampf(q, -0.99999f, 0.99999f) * 0.5f + 0.5f : clampf(q, 0.0f, 1.0f);
+    if (to_bits(a) != to_bits(b)) ++kept;
+  }
+  std::printf("d=%.9g r=%.9g %s: quotients differing %llu, kept values differing %llu\n", d, r,
+              argv[2], raw, kept);
+  return kept == 0 ? 0 : 1;
+}
