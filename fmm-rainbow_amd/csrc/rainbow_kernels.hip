@@ -43,6 +43,7 @@
 
 #include "rainbow_device.h"
 #include "rainbow_launch.h"
+#include "rainbow_math.h"
 
 namespace rainbow {
 namespace {
@@ -405,7 +406,7 @@ __device__ __forceinline__ void pair_terms(const float4 src, float tx, float ty,
   const float ry = src.y - ty;
   const float rz = src.z - tz;
   const float r2 = ((rx * rx + ry * ry) + rz * rz) + eps2;
-  const float inv_r = 1.0f / sqrtf(r2);
+  const float inv_r = rsqrt_rn(r2);  // = 1.0f / sqrtf(r2)
   const float inv_r3 = inv_r * inv_r * inv_r;
   const float inv_r5 = inv_r3 * inv_r * inv_r;
   const float gm = src.w;
@@ -956,25 +957,42 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t lo, 
   const uint32_t lane = tid & 63u;
   const uint32_t wave = tid >> 6;
 
-  // Items of levels hi, hi-1, ..., lo in that order; per level the stripe
-  // counts and their prefix across lanes 0..31 (every wave computes the
-  // same values).
-  auto stripe_count = [&](uint32_t level) -> uint32_t {
-    const uint32_t lv = level - 1u;
-    return lane < kStripes ? min(a.list_count[lv * kStripes + lane], a.stripe_cap[lv * kStripes + lane]) : 0u;
-  };
-  auto prefix = [&](uint32_t cnt) -> uint32_t {
-    uint32_t incl = cnt;
+  // Items of levels hi, hi-1, ..., lo in that order.  The work table holds,
+  // per level and stripe, the inclusive prefix of the stripe counts and the
+  // stripe's list offset; it is read from memory once per block, so locating
+  // an item costs LDS reads only.
+  __shared__ uint32_t tab_incl[kLevels][kStripes];
+  __shared__ uint32_t tab_off[kLevels][kStripes];
+  if (!kFused) lo = hi;
+  if (tid < (hi - lo + 1u) * kStripes) {
+    const uint32_t level = lo + tid / kStripes, j = tid % kStripes, e = (level - 1u) * kStripes + j;
+    uint32_t incl = min(a.list_count[e], a.stripe_cap[e]);
+    const uint32_t off = a.stripe_off[e];
+    // Prefix over the 32 stripes of a level: one aligned half-wave.
 #pragma unroll
     for (uint32_t o = 1u; o < kStripes; o <<= 1) {
-      const uint32_t v = __shfl_up(incl, o);
-      if (lane >= o) incl += v;
+      const uint32_t v = __shfl_up(incl, o, kStripes);
+      if (j >= o) incl += v;
     }
-    return incl;
-  };
-  if (!kFused) lo = hi;
+    tab_incl[level][j] = incl;
+    tab_off[level][j] = off;
+  }
+  __syncthreads();
   uint32_t total = 0u;
-  for (uint32_t level = hi; level >= lo; --level) total += __shfl(prefix(stripe_count(level)), kStripes - 1u);
+  for (uint32_t level = lo; level <= hi; ++level) total += tab_incl[level][kStripes - 1u];
+  // List position and level of item `it` (< total).
+  auto locate = [&](uint32_t it, uint32_t& level) -> uint32_t {
+    uint32_t rem = it;
+    for (level = hi;; --level) {
+      const uint32_t tl = tab_incl[level][kStripes - 1u];
+      if (rem < tl || level == lo) break;
+      rem -= tl;
+    }
+    const bool below = lane < kStripes && tab_incl[level][lane & (kStripes - 1u)] <= rem;
+    const uint32_t j = static_cast<uint32_t>(__popcll(__ballot(below)));
+    const uint32_t base = j == 0u ? 0u : tab_incl[level][j - 1u];
+    return tab_off[level][j] + (rem - base);
+  };
 
   // Pair lane: target tt (child of the parent), source cc of the chunk.
   const uint32_t tt = lane >> 3, cc = lane & 7u;
@@ -983,25 +1001,23 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t lo, 
   const bool chain = tid < 80u;
   const uint32_t ct = tid / 10u, ck = tid % 10u;
 
+  // Each item's list entry is loaded one item ahead; the system descriptor
+  // is reloaded only when the system changes.
+  uint32_t next_level = lo, next_item = 0u;
+  if (blockIdx.x < total) next_item = a.lists[locate(blockIdx.x, next_level)];
+  uint32_t cur_sys = 0xFFFFFFFFu;
+  SysDesc d;
   for (uint32_t it = blockIdx.x; it < total; it += gridDim.x) {
-    // Level and list position of item `it`.
-    uint32_t level = hi, rem = it, incl = 0u, cnt = 0u;
-    for (;; --level) {
-      cnt = stripe_count(level);
-      incl = prefix(cnt);
-      const uint32_t tl = __shfl(incl, kStripes - 1u);
-      if (rem < tl || level == lo) break;
-      rem -= tl;
-    }
-    const uint32_t lv = level - 1u;
-    const uint32_t excl = incl - cnt;
-    const uint32_t j = static_cast<uint32_t>(__popcll(__ballot(lane < kStripes && incl <= rem)));
-    const uint32_t base = __shfl(excl, j);
-    const uint32_t item = a.lists[a.stripe_off[lv * kStripes + j] + (rem - base)];
+    const uint32_t level = __builtin_amdgcn_readfirstlane(next_level);
+    const uint32_t item = __builtin_amdgcn_readfirstlane(next_item);
+    if (it + gridDim.x < total) next_item = a.lists[locate(it + gridDim.x, next_level)];
     const uint32_t s = item >> 16;
     if (s >= a.nsys) continue;
     const uint32_t pidx = item & 0xFFFFu;
-    const SysDesc d = a.sys[s];
+    if (s != cur_sys) {
+      d = a.sys[s];
+      cur_sys = s;
+    }
     const uint32_t dim = 1u << level;
     const uint32_t pdim = dim >> 1u;
     const uint32_t off = level_off(level);
@@ -1184,7 +1200,7 @@ __device__ __forceinline__ Leaf leaf_field(const SysDesc& d, uint32_t parity, fl
     const float ex = com.x - px, ey = com.y - py, ez = com.z - pz;
     const float epsilon =
         fmaxf(d.softening, (0.62f * kWorldHalfExtent) * inv_pow2(out.dim));
-    const float inv_r = 1.0f / sqrtf(((ex * ex + ey * ey) + ez * ez) + epsilon * epsilon);
+    const float inv_r = rsqrt_rn(((ex * ex + ey * ey) + ez * ez) + epsilon * epsilon);
     const float self_mass = static_cast<float>(count - 1u) / static_cast<float>(d.n);
     self[0] = d.gravity * self_mass * ex * inv_r * inv_r * inv_r;
     self[1] = d.gravity * self_mass * ey * inv_r * inv_r * inv_r;
@@ -1230,7 +1246,7 @@ __device__ __forceinline__ Leaf leaf_field(const SysDesc& d, uint32_t parity, fl
 }
 
 __device__ __forceinline__ float length3(float x, float y, float z) {
-  return sqrtf((x * x + y * y) + z * z);
+  return sqrt_rn((x * x + y * y) + z * z);
 }
 
 // One body's integrate (src/rainbow_fmm.wgsl:613-771): the new state
@@ -1254,7 +1270,7 @@ __device__ __forceinline__ void integrate_body(const SysDesc& d, uint32_t parity
     const float control_radius = fmaxf(d.control_radius, 0.02f);
     control_influence = expf(-2.0f * distance2 / (control_radius * control_radius));
     const float control_strength = d.control_strength;
-    const float isq = 1.0f / sqrtf(distance2 + control_radius * control_radius * 0.018f);
+    const float isq = rsqrt_rn(distance2 + control_radius * control_radius * 0.018f);
     const float dir[3] = {cdx * isq, cdy * isq, cdz * isq};
     const float mode = d.control_axis_mode[3];
     if (mode < 1.5f) {
@@ -1593,11 +1609,35 @@ hipError_t launch_top(const StepArgs& a, hipStream_t stream, const Timing* t) {
   return hipGetLastError();
 }
 
+// Blocks of a tree launch that fit on the device at once (occupancy x CUs),
+// cached per kernel form and level span (which sets the dynamic LDS size).  The tree kernel loops over
+// its items, so a grid of this size keeps every block busy and none is
+// launched only to find no work.
+template <bool kComp, bool kFused>
+static uint32_t tree_resident_blocks(uint32_t span, uint32_t lds) {
+  static uint32_t cache[kLevels] = {};
+  const uint32_t key = span % kLevels;
+  if (cache[key] != 0u) return cache[key];
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tree<kComp, kFused>, kTreeThreads, lds) != hipSuccess ||
+      cus <= 0 || per_cu <= 0)
+    return kTreeMaxBlocks;
+  cache[key] = static_cast<uint32_t>(cus * per_cu);
+  return cache[key];
+}
+
 hipError_t launch_tree(const StepArgs& a, uint32_t lo, uint32_t hi, uint32_t nblocks, hipStream_t stream,
                        const Timing* t) {
   if (nblocks == 0u || lo < 2u || hi < lo || hi >= kLevels) return no_launch(stream, t);
   const bool fused = lo < hi;
   const uint32_t lds = fused ? std::max(kItemBytes, (hi - lo) * kAncestorBytes) : 0u;
+  const uint32_t span = hi - lo;
+  const uint32_t resident =
+      a.compensated ? (fused ? tree_resident_blocks<true, true>(span, lds) : tree_resident_blocks<true, false>(span, lds))
+                    : (fused ? tree_resident_blocks<false, true>(span, lds) : tree_resident_blocks<false, false>(span, lds));
+  nblocks = std::min(nblocks, resident);
   if (a.compensated && fused)
     RB_LAUNCH_LDS(t, (k_tree<true, true>), dim3(nblocks), dim3(kTreeThreads), lds, stream, a, lo, hi);
   else if (a.compensated)

@@ -1,0 +1,78 @@
+// Correctly rounded f32 square root and inverse square root for the kernels,
+// in fewer instructions than the compiler's general expansions.
+//
+// The kernels follow the reference's f32 expressions operation by operation:
+// inverseSqrt(x) is taken as 1 / sqrt(x) with both operations correctly
+// rounded (RN(1 / RN(sqrt(x)))), and length() as RN(sqrt(...)).  HIP's
+// sqrtf and '/' give exactly that, but their expansions carry scaling and
+// special-case handling for the whole float range (about 27 VALU
+// instructions for 1 / sqrtf(x)).  Inside the domain [2^-94, FLT_MAX] the
+// forms below give the same bits:
+//   y = rsq(x)                       hardware estimate of 1/sqrt(x)
+//   s = s0 + (x - s0^2) * y/2        s0 = x*y; one Newton step on sqrt, FMA
+//   q = r + (1 - s r) r              r = rcp(s); one Newton step on 1/s, FMA
+// s equals RN(sqrt(x)) and q equals RN(1 / s) for every float x in the
+// domain: bin/math_check compares them with sqrtf and 1 / sqrtf over all
+// 2^32 inputs on the device.  Outside the domain (zero, tiny, infinite or
+// NaN arguments) a wave that holds such a lane takes the compiler's full
+// expansion for those lanes, so the functions equal the reference forms
+// for every input.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+namespace rainbow {
+
+constexpr float kFastRootMin = 0x1p-94f;
+
+__device__ __forceinline__ bool fast_root_domain(float x) { return x >= kFastRootMin && x <= FLT_MAX; }
+
+// s = RN(sqrt(x)) and y = rsq(x) for x in the domain.
+__device__ __forceinline__ float sqrt_newton(float x, float& y) {
+  y = __builtin_amdgcn_rsqf(x);
+  const float s0 = x * y;
+  const float h = 0.5f * y;
+  return __builtin_fmaf(__builtin_fmaf(-s0, s0, x), h, s0);
+}
+
+// RN(1 / s) for s = RN(sqrt(x)), x in the domain.
+__device__ __forceinline__ float recip_newton(float s) {
+  const float r = __builtin_amdgcn_rcpf(s);
+  return __builtin_fmaf(__builtin_fmaf(-s, r, 1.0f), r, r);
+}
+
+// Equals sqrtf(x) for every float x.
+__device__ __forceinline__ float sqrt_rn(float x) {
+  float y;
+  float s = sqrt_newton(x, y);
+  const bool ok = fast_root_domain(x);
+  if (!__all(ok)) {
+    if (!ok) s = sqrtf(x);
+  }
+  return s;
+}
+
+// Equals 1.0f / sqrtf(x) for every float x.
+__device__ __forceinline__ float rsqrt_rn(float x) {
+  float y;
+  float q = recip_newton(sqrt_newton(x, y));
+  const bool ok = fast_root_domain(x);
+  if (!__all(ok)) {
+    if (!ok) q = 1.0f / sqrtf(x);
+  }
+  return q;
+}
+
+// Unguarded forms, for the checker (exact only inside the domain).
+__device__ __forceinline__ float sqrt_fast_unguarded(float x) {
+  float y;
+  return sqrt_newton(x, y);
+}
+__device__ __forceinline__ float rsqrt_fast_unguarded(float x) {
+  float y;
+  return recip_newton(sqrt_newton(x, y));
+}
+
+}  // namespace rainbow
