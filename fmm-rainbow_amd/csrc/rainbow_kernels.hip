@@ -63,6 +63,14 @@ __device__ __forceinline__ uint32_t lin(uint32_t x, uint32_t y, uint32_t z, uint
   return x + dim * (y + dim * z);
 }
 
+// Inverse of lin for dim = 2^lg (shifts, no integer division).
+__device__ __forceinline__ void unlin(uint32_t i, uint32_t lg, uint32_t& x, uint32_t& y, uint32_t& z) {
+  const uint32_t m = (1u << lg) - 1u;
+  x = i & m;
+  y = (i >> lg) & m;
+  z = i >> (2u * lg);
+}
+
 // 1 / dim for dim a power of two (exact), so x * inv_pow2(dim) equals the
 // correctly rounded x / dim for every x: both round the same real number.
 __device__ __forceinline__ float inv_pow2(uint32_t dim) {
@@ -138,6 +146,19 @@ __device__ __forceinline__ uint32_t spread3(uint32_t v) {
 // ------------------------------------------------------------------------
 
 // LDS of a deposit block: the open-addressing hash of its cells.
+// Inclusive prefix sum over the 64 lanes of a wave (wrapping u32 adds): four
+// DPP row shifts sum within each row of 16 lanes, then the row broadcasts of
+// lanes 15 and 31 carry the rows below into the rows above.
+__device__ __forceinline__ uint32_t wave_prefix_sum(uint32_t x) {
+  x += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x111, 0xF, 0xF, true));
+  x += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x112, 0xF, 0xF, true));
+  x += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x114, 0xF, 0xF, true));
+  x += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x118, 0xF, 0xF, true));
+  x += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x142, 0xA, 0xF, false));
+  x += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x143, 0xC, 0xF, false));
+  return x;
+}
+
 struct DepositLds {
   uint32_t keys[kDepositSlots];  // after the flush: rank base per slot
   int vals[4][kDepositSlots];
@@ -179,20 +200,21 @@ __device__ void deposit_block(const StepArgs& a, const SysDesc& d, uint32_t bloc
       v[3] = to_i32_sat(rintf(q.z * d.scale));
     }
     // Bodies are kept in Morton order, so equal keys form runs across the
-    // wave: a segmented inclusive scan sums each run into its last lane
-    // (integer sums, exact in any order).
+    // wave.  A run's sums are the differences of the wave's inclusive prefix
+    // sums at its last lane and before its first lane (wrapping i32 sums,
+    // exact in any order); its last lane adds them to the table.
     const uint32_t prev_key = __shfl_up(key, 1u);
     const unsigned long long heads = __ballot(lane == 0u || key != prev_key);
     const uint32_t start = 63u - static_cast<uint32_t>(__clzll(heads & ((2ull << lane) - 1ull)));
+    {
+      uint32_t incl[4];
 #pragma unroll
-    for (uint32_t off = 1u; off < 64u; off <<= 1) {
-      int t[4];
+      for (int c = 0; c < 4; ++c) incl[c] = wave_prefix_sum(static_cast<uint32_t>(v[c]));
+      const int src = static_cast<int>(start == 0u ? 0u : start - 1u);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) t[c] = __shfl_up(v[c], off);
-      if (lane >= start + off) {
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-          v[c] = static_cast<int>(static_cast<uint32_t>(v[c]) + static_cast<uint32_t>(t[c]));
+      for (int c = 0; c < 4; ++c) {
+        const uint32_t before = static_cast<uint32_t>(__shfl(static_cast<int>(incl[c]), src));
+        v[c] = static_cast<int>(incl[c] - (start == 0u ? 0u : before));
       }
     }
     // Heads after this lane; the lane before the next head ends the run.
@@ -227,7 +249,8 @@ __device__ void deposit_block(const StepArgs& a, const SysDesc& d, uint32_t bloc
   for (uint32_t s = threadIdx.x; s < kDepositSlots; s += kDepositThreads) {
     const uint32_t key = keys[s];
     if (key == kEmptyKey) continue;
-    const uint32_t z = key / (dim * dim), y = (key / dim) % dim, x = key % dim;
+    uint32_t x, y, z;
+    unlin(key, d.depth, x, y, z);
     const uint32_t morton = spread3(x) | (spread3(y) << 1) | (spread3(z) << 2);
     if (morton >= own.x && morton < own.y) {
       finest[key] = make_int4(vals[0][s], vals[1][s], vals[2][s], vals[3][s]);
@@ -520,11 +543,8 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
   const uint32_t t = threadIdx.x;
   const uint32_t D = d.depth;
   const uint32_t r = D - 3u;
-  const uint32_t dim_r = 1u << r;
-  const uint32_t rz = blk.root / (dim_r * dim_r);
-  const uint32_t rrest = blk.root - rz * dim_r * dim_r;
-  const uint32_t ry = rrest / dim_r;
-  const uint32_t rx = rrest - ry * dim_r;
+  uint32_t rx, ry, rz;
+  unlin(blk.root, r, rx, ry, rz);
   int4* M = d.moments;
   if (t < 4u) ps.n[t] = 0u;
 
@@ -678,10 +698,8 @@ __global__ __launch_bounds__(kUpwardThreads) void k_top(StepArgs a) {
     const uint32_t k = r - j;
     const uint32_t dk = 1u << k;
     if (t < dk * dk * dk) {
-      const uint32_t cz = t / (dk * dk);
-      const uint32_t rest = t - cz * dk * dk;
-      const uint32_t cy = rest / dk;
-      const uint32_t cx = rest - cy * dk;
+      uint32_t cx, cy, cz;
+      unlin(t, k, cx, cy, cz);
       const int4 v = sum8(buf(j - 1u), cx, cy, cz, 2u * dk);
       buf(j)[t] = v;
       M[level_off(k) + t] = v;
@@ -719,10 +737,8 @@ __global__ __launch_bounds__(kUpwardThreads) void k_top(StepArgs a) {
         }
       }
       if (k >= 1u) {
-        const uint32_t cz = i / (dk * dk);
-        const uint32_t rest = i - cz * dk * dk;
-        const uint32_t cy = rest / dk;
-        const uint32_t cx = rest - cy * dk;
+        uint32_t cx, cy, cz;
+        unlin(i, k, cx, cy, cz);
         update_active(d, cell, m.x, buf(j + 1u)[lin(cx >> 1, cy >> 1, cz >> 1, dk >> 1)].x);
       }
     }
@@ -1021,10 +1037,8 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t lo, 
     const uint32_t dim = 1u << level;
     const uint32_t pdim = dim >> 1u;
     const uint32_t off = level_off(level);
-    const uint32_t pz = pidx / (pdim * pdim);
-    const uint32_t prest = pidx - pz * pdim * pdim;
-    const uint32_t py = prest / pdim;
-    const uint32_t px = prest - py * pdim;
+    uint32_t px, py, pz;
+    unlin(pidx, level - 1u, px, py, pz);
 
     // The parent's field: from memory when its level is below lo, else
     // recomputed along the chain of ancestors from level lo - 1.
