@@ -55,6 +55,22 @@ constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
 // deposited by the owning block.
 constexpr uint32_t kRankExtra = 0x80000000u;
 
+// Global-memory view of a buffer pointer: accesses through it are global
+// (not flat) loads, stores and atomics, addressed from a scalar base.  The
+// host pass of this file (which only type-checks the device functions) sees
+// plain pointers.
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+#else
+template <class T>
+using gptr = T*;
+#endif
+template <class T>
+__device__ __forceinline__ gptr<T> G(T* p) {
+  return (gptr<T>)p;
+}
+
 __device__ __forceinline__ uint32_t level_off(uint32_t level) {
   return ((1u << (3u * level)) - 1u) / 7u;
 }
@@ -114,7 +130,7 @@ __device__ __forceinline__ float clampf(float x, float lo, float hi) {
   return fminf(fmaxf(x, lo), hi);
 }
 
-__device__ __forceinline__ void load_field(const float4* fields, uint32_t cell, float4& a, float4& j0,
+__device__ __forceinline__ void load_field(gptr<const float4> fields, uint32_t cell, float4& a, float4& j0,
                                            float4& j1) {
   a = fields[3u * cell + 0u];
   j0 = fields[3u * cell + 1u];
@@ -122,13 +138,13 @@ __device__ __forceinline__ void load_field(const float4* fields, uint32_t cell, 
 }
 
 // The finest level of the step's parity (see SysDesc::finest_alt).
-__device__ __forceinline__ int4* finest_of(const SysDesc& d, uint32_t parity) {
-  return parity ? d.finest_alt : d.moments + level_off(d.depth);
+__device__ __forceinline__ gptr<int4> finest_of(const SysDesc& d, uint32_t parity) {
+  return parity ? G(d.finest_alt) : G(d.moments) + level_off(d.depth);
 }
 
 // Body count of a cell (level, index within the level) at a parity.
 __device__ __forceinline__ int cell_count(const SysDesc& d, uint32_t parity, uint32_t level, uint32_t idx) {
-  return level == d.depth ? finest_of(d, parity)[idx].x : d.moments[level_off(level) + idx].x;
+  return level == d.depth ? finest_of(d, parity)[idx].x : G(d.moments)[level_off(level) + idx].x;
 }
 
 // Interleave the low 10 bits of v with two zero bits between each.
@@ -244,8 +260,8 @@ __device__ void deposit_block(const StepArgs& a, const SysDesc& d, uint32_t bloc
   // stored.  Any other cell's sums go to `extra` with atomics (bodies that
   // moved since the last sort, and cells shared with other blocks);
   // upward adds `extra` into the finest level.
-  const uint2 own = a.own[block];
-  int4* finest = finest_of(d, parity);
+  const uint2 own = G(a.own)[block];
+  const gptr<int4> finest = finest_of(d, parity);
   for (uint32_t s = threadIdx.x; s < kDepositSlots; s += kDepositThreads) {
     const uint32_t key = keys[s];
     if (key == kEmptyKey) continue;
@@ -262,22 +278,22 @@ __device__ void deposit_block(const StepArgs& a, const SysDesc& d, uint32_t bloc
     // places this block's bodies after those already in `extra`.
     const unsigned long long packed = static_cast<unsigned long long>(static_cast<uint32_t>(vals[0][s])) |
                                       (static_cast<unsigned long long>(static_cast<uint32_t>(vals[1][s])) << 32);
-    unsigned long long* const ce = reinterpret_cast<unsigned long long*>(d.extra + key);
+    unsigned long long* const ce = reinterpret_cast<unsigned long long*>(G(d.extra) + key);
     if (ranks) {
       const unsigned long long old = atomicAdd(ce, packed);
       keys[s] = kRankExtra | static_cast<uint32_t>(old);
     } else {
       (void)atomicAdd(ce, packed);  // no return value needed: the hardware need not wait for it
     }
-    atomicAdd(reinterpret_cast<int*>(d.extra + key) + 2, vals[2][s]);
-    atomicAdd(reinterpret_cast<int*>(d.extra + key) + 3, vals[3][s]);
+    atomicAdd(reinterpret_cast<int*>(G(d.extra) + key) + 2, vals[2][s]);
+    atomicAdd(reinterpret_cast<int*>(G(d.extra) + key) + 3, vals[3][s]);
   }
   if (!ranks) return;
   __syncthreads();
 #pragma unroll
   for (uint32_t k = 0; k < kDepositPerThread; ++k) {
     const uint32_t i = first + k * kDepositThreads + threadIdx.x;
-    if (i < d.n) a.rank[d.body_base + i] = keys[my_slot[k]] + my_rank[k];
+    if (i < d.n) G(a.rank)[d.body_base + i] = keys[my_slot[k]] + my_rank[k];
   }
 }
 
@@ -285,7 +301,7 @@ __device__ void deposit_block(const StepArgs& a, const SysDesc& d, uint32_t bloc
 // last reader, the previous tree pass, has finished).
 __device__ __forceinline__ void reset_lists(const StepArgs& a) {
   if (blockIdx.x == 0u)
-    for (uint32_t i = threadIdx.x; i < kLevels * kStripes; i += blockDim.x) a.list_count[i] = 0u;
+    for (uint32_t i = threadIdx.x; i < kLevels * kStripes; i += blockDim.x) G(a.list_count)[i] = 0u;
 }
 
 // Standalone deposit (a solve's first pass; a step normally deposits at the
@@ -300,7 +316,7 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(StepArgs a) {
 #pragma unroll
   for (uint32_t k = 0; k < kDepositPerThread; ++k) {
     const uint32_t i = blk.first + k * kDepositThreads + threadIdx.x;
-    p[k] = i < d.n ? d.pos[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    p[k] = i < d.n ? G(d.pos)[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
   deposit_block(a, d, blockIdx.x, blk.first, p, a.parity, a.sorting != 0u, L);
 }
@@ -351,20 +367,20 @@ __global__ __launch_bounds__(kIntegrateThreads) void k_scatter(StepArgs a) {
   if (!d.live) return;
   const uint32_t i = blk.first + threadIdx.x;
   if (i >= d.n) return;
-  const float4 p = d.pos[i];
+  const float4 p = G(d.pos)[i];
   const uint32_t dim = 1u << d.depth;
   const uint32_t cx = cell1(unit_coord(p.x), dim), cy = cell1(unit_coord(p.y), dim), cz = cell1(unit_coord(p.z), dim);
   const uint32_t m = spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2);
-  const uint32_t r = a.rank[d.body_base + i];
+  const uint32_t r = G(a.rank)[d.body_base + i];
   // Bodies flagged in the rank follow the cell's bodies stored by its
   // owning block.
-  const uint32_t* cs = (r & kRankExtra) ? a.xstart : a.cstart;
-  const uint32_t dst = a.sstart[(d.cell_base >> 9) + (m >> 9)] + cs[d.cell_base + m] + (r & ~kRankExtra);
+  const gptr<uint32_t> cs = (r & kRankExtra) ? G(a.xstart) : G(a.cstart);
+  const uint32_t dst = G(a.sstart)[(d.cell_base >> 9) + (m >> 9)] + cs[d.cell_base + m] + (r & ~kRankExtra);
   if (dst >= d.n) return;  // unreachable when the counts are consistent
   const uint32_t g = d.body_base + dst;
-  a.tpos[g] = p;
-  a.tvel[g] = d.vel[i];
-  a.tperm[g] = d.perm[i];
+  G(a.tpos)[g] = p;
+  G(a.tvel)[g] = G(d.vel)[i];
+  G(a.tperm)[g] = G(d.perm)[i];
 }
 
 __global__ __launch_bounds__(kIntegrateThreads) void k_copy_back(StepArgs a) {
@@ -374,9 +390,9 @@ __global__ __launch_bounds__(kIntegrateThreads) void k_copy_back(StepArgs a) {
   const uint32_t i = blk.first + threadIdx.x;
   if (i >= d.n) return;
   const uint32_t g = d.body_base + i;
-  d.pos[i] = a.tpos[g];
-  d.vel[i] = a.tvel[g];
-  d.perm[i] = a.tperm[g];
+  G(d.pos)[i] = G(a.tpos)[g];
+  G(d.vel)[i] = G(a.tvel)[g];
+  G(d.perm)[i] = G(a.tperm)[g];
 }
 
 // Owned Morton ranges of the deposit blocks from the sorted bodies: block
@@ -389,10 +405,10 @@ __global__ __launch_bounds__(256) void k_own(StepArgs a, uint32_t nblocks) {
   const SysDesc d = a.sys[blk.sys];
   if (!d.live) return;
   const uint32_t dim = 1u << d.depth;
-  const uint32_t lo = blk.first == 0u ? 0u : morton_of(d.pos[blk.first], dim);
+  const uint32_t lo = blk.first == 0u ? 0u : morton_of(G(d.pos)[blk.first], dim);
   const uint32_t next = blk.first + kDepositBodies;
-  const uint32_t hi = next >= d.n ? (1u << 18) : morton_of(d.pos[next], dim);
-  a.own[b] = make_uint2(lo, hi);
+  const uint32_t hi = next >= d.n ? (1u << 18) : morton_of(G(d.pos)[next], dim);
+  G(a.own)[b] = make_uint2(lo, hi);
 }
 
 // ------------------------------------------------------------------------
@@ -489,15 +505,15 @@ __device__ __forceinline__ float4 source_record(const int4 m, const SysDesc& d) 
 __device__ __forceinline__ void update_active(const SysDesc& d, uint32_t cell, int count,
                                               int parent_count) {
   const uint8_t now = (count != 0 && parent_count > kLeafCapacity) ? 1u : 0u;
-  const uint8_t prev = d.active[cell];
+  const uint8_t prev = G(d.active)[cell];
   if (prev != now) {
     if (prev) {
       const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      d.fields[3u * cell + 0u] = z;
-      d.fields[3u * cell + 1u] = z;
-      d.fields[3u * cell + 2u] = z;
+      G(d.fields)[3u * cell + 0u] = z;
+      G(d.fields)[3u * cell + 1u] = z;
+      G(d.fields)[3u * cell + 2u] = z;
     }
-    d.active[cell] = now;
+    G(d.active)[cell] = now;
   }
 }
 
@@ -515,13 +531,13 @@ __device__ __forceinline__ uint32_t stage_item(PushStage& ps, uint32_t k) { retu
 __device__ __forceinline__ void reserve_items(const StepArgs& a, PushStage& ps, const uint32_t levels[4],
                                               uint32_t nlev, uint32_t stripe) {
   for (uint32_t k = 0; k < nlev; ++k)
-    ps.base[k] = ps.n[k] ? atomicAdd(&a.list_count[levels[k] * kStripes + stripe], ps.n[k]) : 0u;
+    ps.base[k] = ps.n[k] ? atomicAdd(&G(a.list_count)[levels[k] * kStripes + stripe], ps.n[k]) : 0u;
 }
 
 __device__ __forceinline__ void write_item(const StepArgs& a, uint32_t level, uint32_t stripe, uint32_t slot,
                                            uint32_t item) {
   const uint32_t si = level * kStripes + stripe;
-  if (slot < a.stripe_cap[si]) a.lists[a.stripe_off[si] + slot] = item;
+  if (slot < G(a.stripe_cap)[si]) G(a.lists)[G(a.stripe_off)[si] + slot] = item;
 }
 
 // upward (M2M), src/rainbow_fmm.wgsl:424-455: one block per 8x8x8 finest
@@ -545,7 +561,7 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
   const uint32_t r = D - 3u;
   uint32_t rx, ry, rz;
   unlin(blk.root, r, rx, ry, rz);
-  int4* M = d.moments;
+  const gptr<int4> M = G(d.moments);
   if (t < 4u) ps.n[t] = 0u;
 
   // Finest level (this step's buffer): 8x8x8 cells, x fastest (rows of 8
@@ -553,21 +569,21 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
   // adds, exact in any order), written back when non-zero; `extra` and the
   // other buffer's cells are zeroed for the next step's deposit.
   __shared__ uint32_t owned[512];
-  int4* const F = finest_of(d, a.parity);
-  int4* const next = finest_of(d, a.parity ^ 1u);
+  const gptr<int4> F = finest_of(d, a.parity);
+  const gptr<int4> next = finest_of(d, a.parity ^ 1u);
   for (uint32_t i = t; i < 512u; i += kUpwardThreads) {
     const uint32_t ix = i & 7u, iy = (i >> 3) & 7u, iz = i >> 6;
     const uint32_t c = lin(8u * rx + ix, 8u * ry + iy, 8u * rz + iz, 1u << D);
     int4 m = F[c];
     owned[i] = static_cast<uint32_t>(m.x);
-    const int4 e = d.extra[c];
+    const int4 e = G(d.extra)[c];
     if (e.x != 0) {
       m.x = static_cast<int>(static_cast<uint32_t>(m.x) + static_cast<uint32_t>(e.x));
       m.y = static_cast<int>(static_cast<uint32_t>(m.y) + static_cast<uint32_t>(e.y));
       m.z = static_cast<int>(static_cast<uint32_t>(m.z) + static_cast<uint32_t>(e.z));
       m.w = static_cast<int>(static_cast<uint32_t>(m.w) + static_cast<uint32_t>(e.w));
       F[c] = m;
-      d.extra[c] = make_int4(0, 0, 0, 0);
+      G(d.extra)[c] = make_int4(0, 0, 0, 0);
     }
     next[c] = make_int4(0, 0, 0, 0);
     lvl0[i] = m;
@@ -585,8 +601,8 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
     const uint32_t root_m = spread3(rx) | (spread3(ry) << 1) | (spread3(rz) << 2);
     for (uint32_t u = t; u < 512u; u += kUpwardThreads) {
       const uint32_t g = d.cell_base + (root_m << 9) + u;
-      a.cstart[g] = sc[u];
-      a.xstart[g] = sc[u] + owned[compact3(u) + 8u * (compact3(u >> 1) + 8u * compact3(u >> 2))];
+      G(a.cstart)[g] = sc[u];
+      G(a.xstart)[g] = sc[u] + owned[compact3(u) + 8u * (compact3(u >> 1) + 8u * compact3(u >> 2))];
     }
   }
   if (t < 64u) {
@@ -614,7 +630,7 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
     const uint32_t ix = i & 7u, iy = (i >> 3) & 7u, iz = i >> 6;
     const uint32_t cell = level_off(D) + lin(8u * rx + ix, 8u * ry + iy, 8u * rz + iz, 1u << D);
     const int4 m = lvl0[i];
-    d.source[cell] = source_record(m, d);
+    G(d.source)[cell] = source_record(m, d);
     update_active(d, cell, m.x, lvl1[lin(ix >> 1, iy >> 1, iz >> 1, 4u)].x);
   }
   uint32_t my_item = 0u, my_slot = 0u, my_level = 0u;
@@ -624,7 +640,7 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
     const uint32_t local = lin(4u * rx + cx, 4u * ry + cy, 4u * rz + cz, 1u << (D - 1u));
     const uint32_t cell = level_off(D - 1u) + local;
     const int4 m = lvl1[t];
-    d.source[cell] = source_record(m, d);
+    G(d.source)[cell] = source_record(m, d);
     update_active(d, cell, m.x, lvl2[lin(cx >> 1, cy >> 1, cz >> 1, 2u)].x);
     if (m.x > kLeafCapacity) {
       pushes = true;
@@ -641,7 +657,7 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
     const uint32_t local = lin(2u * rx + cx, 2u * ry + cy, 2u * rz + cz, 1u << (D - 2u));
     const uint32_t cell = level_off(D - 2u) + local;
     const int4 m = lvl2[u];
-    d.source[cell] = source_record(m, d);
+    G(d.source)[cell] = source_record(m, d);
     update_active(d, cell, m.x, lvl3[0].x);
     if (m.x > kLeafCapacity) {
       pushes2 = true;
@@ -652,7 +668,7 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
   }
   if (t == 72u) {
     const int4 m = lvl3[0];
-    d.source[level_off(r) + blk.root] = source_record(m, d);
+    G(d.source)[level_off(r) + blk.root] = source_record(m, d);
     if (m.x > kLeafCapacity) {
       pushes2 = true;
       my_level2 = 2u;
@@ -684,7 +700,7 @@ __global__ __launch_bounds__(kUpwardThreads) void k_top(StepArgs a) {
   if (!d.live) return;
   const uint32_t t = threadIdx.x;
   const uint32_t r = d.depth - 3u;
-  int4* M = d.moments;
+  const gptr<int4> M = G(d.moments);
   auto buf = [&](uint32_t j) -> int4* {
     return lv + (j == 0u ? 0u : (j == 1u ? 512u : (j == 2u ? 576u : 584u)));
   };
@@ -715,7 +731,7 @@ __global__ __launch_bounds__(kUpwardThreads) void k_top(StepArgs a) {
       sc[u] = u < nr ? static_cast<uint32_t>(buf(0)[lin(compact3(u), compact3(u >> 1), compact3(u >> 2), dr)].x) : 0u;
     __syncthreads();
     scan512(sc, wsum, t);
-    for (uint32_t u = t; u < nr; u += kUpwardThreads) a.sstart[(d.cell_base >> 9) + u] = sc[u];
+    for (uint32_t u = t; u < nr; u += kUpwardThreads) G(a.sstart)[(d.cell_base >> 9) + u] = sc[u];
   }
   // Levels 0..r-1: source records and lists; levels 1..r: flags.
   uint32_t items[4], slots[4], levs[4];
@@ -728,7 +744,7 @@ __global__ __launch_bounds__(kUpwardThreads) void k_top(StepArgs a) {
       const int4 m = buf(j)[i];
       const uint32_t cell = level_off(k) + i;
       if (k < r) {
-        d.source[cell] = source_record(m, d);
+        G(d.source)[cell] = source_record(m, d);
         if (m.x > kLeafCapacity && nmine < 4u) {
           items[nmine] = pack_item(s, i);
           levs[nmine] = k;
@@ -780,7 +796,7 @@ __global__ __launch_bounds__(kUpwardThreads) void k_top(StepArgs a) {
               for (int k = 0; k < 10; ++k) ch[k].add(v[k]);
             }
       }
-      float4* f = d.fields + 3u * (level_off(1u) + t);
+      const gptr<float4> f = G(d.fields) + 3u * (level_off(1u) + t);
       f[0] = make_float4(ch[0].value(), ch[1].value(), ch[2].value(), ch[3].value());
       f[1] = make_float4(ch[4].value(), ch[5].value(), ch[6].value(), ch[7].value());
       f[2] = make_float4(ch[8].value(), ch[9].value(), 0.0f, 0.0f);
@@ -830,7 +846,7 @@ __device__ __forceinline__ void stage_window(float4* win, const SysDesc& d, uint
                                              uint32_t py, uint32_t pz, uint32_t tid) {
   const uint32_t dim = 1u << level;
   const int dim_i = static_cast<int>(dim);
-  const float4* src = d.source + level_off(level);
+  const gptr<float4> src = G(d.source) + level_off(level);
   for (uint32_t w = tid; w < 216u; w += kTreeThreads) {
     const uint32_t wz = w / 36u, wy = (w / 6u) % 6u, wx = w % 6u;
     const int qx = 2 * static_cast<int>(px) - 2 + static_cast<int>(wx);
@@ -884,7 +900,7 @@ __device__ void ancestor_fields(const SysDesc& d, uint32_t lo, uint32_t level, u
     const int qz = 2 * static_cast<int>(qpz) - 2 + static_cast<int>(wz);
     float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (qx >= 0 && qy >= 0 && qz >= 0 && qx < dim_i && qy < dim_i && qz < dim_i)
-      v = d.source[level_off(al) +
+      v = G(d.source)[level_off(al) +
                    lin(static_cast<uint32_t>(qx), static_cast<uint32_t>(qy), static_cast<uint32_t>(qz), dim)];
     awin[w] = v;
   }
@@ -892,7 +908,7 @@ __device__ void ancestor_fields(const SysDesc& d, uint32_t lo, uint32_t level, u
     const uint32_t up = level - lo;  // P >> up = A_{lo-1}
     const uint32_t adim = 1u << (lo - 1u);
     pf[tid] = reinterpret_cast<const float*>(
-        d.fields + 3u * (level_off(lo - 1u) + lin(px >> up, py >> up, pz >> up, adim)))[tid];
+        G(d.fields) + 3u * (level_off(lo - 1u) + lin(px >> up, py >> up, pz >> up, adim)))[tid];
   }
   __syncthreads();
   for (uint32_t w = tid; w < na * 216u; w += kTreeThreads) {
@@ -982,8 +998,8 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t lo, 
   if (!kFused) lo = hi;
   if (tid < (hi - lo + 1u) * kStripes) {
     const uint32_t level = lo + tid / kStripes, j = tid % kStripes, e = (level - 1u) * kStripes + j;
-    uint32_t incl = min(a.list_count[e], a.stripe_cap[e]);
-    const uint32_t off = a.stripe_off[e];
+    uint32_t incl = min(G(a.list_count)[e], G(a.stripe_cap)[e]);
+    const uint32_t off = G(a.stripe_off)[e];
     // Prefix over the 32 stripes of a level: one aligned half-wave.
 #pragma unroll
     for (uint32_t o = 1u; o < kStripes; o <<= 1) {
@@ -1020,13 +1036,13 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t lo, 
   // Each item's list entry is loaded one item ahead; the system descriptor
   // is reloaded only when the system changes.
   uint32_t next_level = lo, next_item = 0u;
-  if (blockIdx.x < total) next_item = a.lists[locate(blockIdx.x, next_level)];
+  if (blockIdx.x < total) next_item = G(a.lists)[locate(blockIdx.x, next_level)];
   uint32_t cur_sys = 0xFFFFFFFFu;
   SysDesc d;
   for (uint32_t it = blockIdx.x; it < total; it += gridDim.x) {
     const uint32_t level = __builtin_amdgcn_readfirstlane(next_level);
     const uint32_t item = __builtin_amdgcn_readfirstlane(next_item);
-    if (it + gridDim.x < total) next_item = a.lists[locate(it + gridDim.x, next_level)];
+    if (it + gridDim.x < total) next_item = G(a.lists)[locate(it + gridDim.x, next_level)];
     const uint32_t s = item >> 16;
     if (s >= a.nsys) continue;
     const uint32_t pidx = item & 0xFFFFu;
@@ -1073,7 +1089,7 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t lo, 
         pj0 = make_float4(pf[4], pf[5], pf[6], pf[7]);
         pj1 = make_float4(pf[8], pf[9], 0.0f, 0.0f);
       } else {
-        load_field(d.fields, level_off(level - 1u) + pidx, pa, pj0, pj1);
+        load_field(G(d.fields), level_off(level - 1u) + pidx, pa, pj0, pj1);
       }
       const float dx = center1(2u * px + (ct & 1u), dim) - center1(px, pdim);
       const float dy = center1(2u * py + ((ct >> 1) & 1u), dim) - center1(py, pdim);
@@ -1143,7 +1159,7 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t lo, 
       const uint32_t x = 2u * px + (ct & 1u), y = 2u * py + ((ct >> 1) & 1u), z = 2u * pz + (ct >> 2);
       const uint32_t g = off + lin(x, y, z, dim);
       if (cell_count(d, a.parity, level, lin(x, y, z, dim)) != 0) {
-        float* f = reinterpret_cast<float*>(d.fields + 3u * g);
+        const gptr<float> f = reinterpret_cast<gptr<float>>(G(d.fields) + 3u * g);
         f[ck] = ch.value();
         if (ck == 9u) {
           f[10] = 0.0f;
@@ -1196,7 +1212,7 @@ __device__ __forceinline__ Leaf leaf_field(const SysDesc& d, uint32_t parity, fl
   out.cell = level_off(chosen) + lin(c[0], c[1], c[2], out.dim);
   const uint32_t cell = out.cell;
   float4 fa, fj0, fj1;
-  load_field(d.fields, cell, fa, fj0, fj1);
+  load_field(G(d.fields), cell, fa, fj0, fj1);
   const float dx = px - center1(c[0], out.dim);
   const float dy = py - center1(c[1], out.dim);
   const float dz = pz - center1(c[2], out.dim);
@@ -1210,7 +1226,7 @@ __device__ __forceinline__ Leaf leaf_field(const SysDesc& d, uint32_t parity, fl
   float self_phi = 0.0f;
   const bool has_self = count > 1u;
   if (has_self) {
-    const float4 com = d.source[cell];
+    const float4 com = G(d.source)[cell];
     const float ex = com.x - px, ey = com.y - py, ez = com.z - pz;
     const float epsilon =
         fmaxf(d.softening, (0.62f * kWorldHalfExtent) * inv_pow2(out.dim));
@@ -1376,11 +1392,11 @@ __global__ __launch_bounds__(kDepositThreads) void k_integrate_deposit(StepArgs 
     const uint32_t i = blk.first + k * kDepositThreads + threadIdx.x;
     p[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (i < d.n) {
-      float4 p4 = d.pos[i];
-      float4 v4 = d.vel[i];
+      float4 p4 = G(d.pos)[i];
+      float4 v4 = G(d.vel)[i];
       integrate_body<kComp>(d, a.parity, p4, v4);
-      d.pos[i] = p4;
-      d.vel[i] = v4;
+      G(d.pos)[i] = p4;
+      G(d.vel)[i] = v4;
       p[k] = p4;
     }
   }
@@ -1398,9 +1414,9 @@ __global__ __launch_bounds__(kIntegrateThreads) void k_accel(StepArgs a, uint32_
   const SysDesc d = a.sys[blk.sys];
   const uint32_t i = blk.first + threadIdx.x;
   if (i >= d.n) return;
-  const float4 p4 = d.pos[i];
+  const float4 p4 = G(d.pos)[i];
   const Leaf leaf = leaf_field<kComp, true>(d, a.parity, p4.x, p4.y, p4.z);
-  out[d.perm[i]] = make_float4(leaf.ax, leaf.ay, leaf.az, leaf.phi);
+  out[G(d.perm)[i]] = make_float4(leaf.ax, leaf.ay, leaf.az, leaf.phi);
 }
 
 // Block partials of {KE, PE, Px, Py, Pz, Lx, Ly, Lz} in FP64 (equal masses
@@ -1414,8 +1430,8 @@ __global__ __launch_bounds__(kIntegrateThreads) void k_diag_partial(StepArgs a, 
   const uint32_t i = blk.first + threadIdx.x;
   double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (blk.sys == sys && i < d.n) {
-    const float4 p4 = d.pos[i];
-    const float4 v4 = d.vel[i];
+    const float4 p4 = G(d.pos)[i];
+    const float4 v4 = G(d.vel)[i];
     const Leaf leaf = leaf_field<kComp, true>(d, a.parity, p4.x, p4.y, p4.z);
     const double m = 1.0 / static_cast<double>(d.n);
     const double x = p4.x, y = p4.y, z = p4.z, vx = v4.x, vy = v4.y, vz = v4.z;
@@ -1459,7 +1475,7 @@ __global__ __launch_bounds__(64) void k_diag_final(const double* partial, uint32
 __global__ __launch_bounds__(256) void k_invert_perm(SysDesc d, uint32_t* inv) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   if (i >= d.n) return;
-  const uint32_t c = d.perm[i];
+  const uint32_t c = G(d.perm)[i];
   if (c < d.n) inv[c] = i;
 }
 
@@ -1471,7 +1487,7 @@ __global__ __launch_bounds__(256) void k_direct_f64(SysDesc d, const uint32_t* i
   const uint32_t self = valid ? min(inv[min(targets[t], d.n - 1u)], d.n - 1u) : 0xFFFFFFFFu;
   double xi = 0.0, yi = 0.0, zi = 0.0;
   if (valid) {
-    const float4 p = d.pos[self];
+    const float4 p = G(d.pos)[self];
     xi = p.x;
     yi = p.y;
     zi = p.z;
@@ -1483,7 +1499,7 @@ __global__ __launch_bounds__(256) void k_direct_f64(SysDesc d, const uint32_t* i
   for (uint32_t jt = j0; jt < j1; jt += 256u) {
     const uint32_t j = jt + threadIdx.x;
     if (j < j1) {
-      const float4 q = d.pos[j];
+      const float4 q = G(d.pos)[j];
       sx[threadIdx.x] = q.x;
       sy[threadIdx.x] = q.y;
       sz[threadIdx.x] = q.z;
@@ -1532,17 +1548,17 @@ __global__ __launch_bounds__(256) void k_direct_final(SysDesc d, uint32_t nt, ui
 __global__ __launch_bounds__(256) void k_split_particles(SysDesc d, const float4* in) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   if (i >= d.n) return;
-  d.pos[i] = in[2u * i];
-  d.vel[i] = in[2u * i + 1u];
-  d.perm[i] = i;
+  G(d.pos)[i] = in[2u * i];
+  G(d.vel)[i] = in[2u * i + 1u];
+  G(d.perm)[i] = i;
 }
 
 __global__ __launch_bounds__(256) void k_join_particles(SysDesc d, float4* out) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   if (i >= d.n) return;
-  const uint32_t j = d.perm[i] < d.n ? d.perm[i] : i;
-  out[2u * j] = d.pos[i];
-  out[2u * j + 1u] = d.vel[i];
+  const uint32_t j = G(d.perm)[i] < d.n ? G(d.perm)[i] : i;
+  out[2u * j] = G(d.pos)[i];
+  out[2u * j + 1u] = G(d.vel)[i];
 }
 
 }  // namespace
