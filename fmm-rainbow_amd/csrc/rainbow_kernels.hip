@@ -502,10 +502,11 @@ __device__ __forceinline__ float4 source_record(const int4 m, const SysDesc& d) 
 // holds more than 16 bodies (src/rainbow_fmm.wgsl:472-476, 593-595).  Every
 // field outside the current target set is zero; a cell that leaves the set
 // has its field cleared here, before the tree passes run.
-__device__ __forceinline__ void update_active(const SysDesc& d, uint32_t cell, int count,
-                                              int parent_count) {
+// `prev` is the cell's flag before this step (loaded by the caller ahead of
+// time, off the critical path).
+__device__ __forceinline__ void update_active(const SysDesc& d, uint32_t cell, int count, int parent_count,
+                                              uint8_t prev) {
   const uint8_t now = (count != 0 && parent_count > kLeafCapacity) ? 1u : 0u;
-  const uint8_t prev = G(d.active)[cell];
   if (prev != now) {
     if (prev) {
       const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -523,7 +524,20 @@ __device__ __forceinline__ void update_active(const SysDesc& d, uint32_t cell, i
 struct PushStage {
   uint32_t n[4];
   uint32_t base[4];
+  uint32_t cap[4];  // the stripe's slots and first slot per staged level,
+  uint32_t off[4];  // loaded when the block starts
 };
+
+// Threads 0..nlev-1: load the stripe's capacity and offset of each staged
+// level (constant between configurations), off the critical path.
+__device__ __forceinline__ void prefetch_stripes(const StepArgs& a, PushStage& ps, const uint32_t levels[4],
+                                                 uint32_t nlev, uint32_t stripe, uint32_t t) {
+  if (t < nlev) {
+    const uint32_t si = levels[t] * kStripes + stripe;
+    ps.cap[t] = G(a.stripe_cap)[si];
+    ps.off[t] = G(a.stripe_off)[si];
+  }
+}
 
 __device__ __forceinline__ uint32_t stage_item(PushStage& ps, uint32_t k) { return atomicAdd(&ps.n[k], 1u); }
 
@@ -534,10 +548,11 @@ __device__ __forceinline__ void reserve_items(const StepArgs& a, PushStage& ps, 
     ps.base[k] = ps.n[k] ? atomicAdd(&G(a.list_count)[levels[k] * kStripes + stripe], ps.n[k]) : 0u;
 }
 
-__device__ __forceinline__ void write_item(const StepArgs& a, uint32_t level, uint32_t stripe, uint32_t slot,
+// Stores an item of staged level k at its reserved slot (items beyond the
+// stripe's capacity are dropped; the planner sizes stripes so none is).
+__device__ __forceinline__ void write_item(const StepArgs& a, const PushStage& ps, uint32_t k, uint32_t slot,
                                            uint32_t item) {
-  const uint32_t si = level * kStripes + stripe;
-  if (slot < G(a.stripe_cap)[si]) G(a.lists)[G(a.stripe_off)[si] + slot] = item;
+  if (slot < ps.cap[k]) G(a.lists)[ps.off[k] + slot] = item;
 }
 
 // upward (M2M), src/rainbow_fmm.wgsl:424-455: one block per 8x8x8 finest
@@ -563,12 +578,28 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
   unlin(blk.root, r, rx, ry, rz);
   const gptr<int4> M = G(d.moments);
   if (t < 4u) ps.n[t] = 0u;
+  const uint32_t stripe = blockIdx.x % kStripes;
+  const uint32_t lev_of[4] = {D - 1u, D - 2u, r, 0u};
+  prefetch_stripes(a, ps, lev_of, 3u, stripe, t);
 
   // Finest level (this step's buffer): 8x8x8 cells, x fastest (rows of 8
   // contiguous moments), plus the sums deposited into `extra` (wrapping i32
   // adds, exact in any order), written back when non-zero; `extra` and the
   // other buffer's cells are zeroed for the next step's deposit.
   __shared__ uint32_t owned[512];
+  // Target flags of this block's cells before this step, loaded with the
+  // finest cells (the flag updates below then wait on nothing).
+  uint8_t prev_f[2], prev_1 = 0u, prev_2 = 0u;
+#pragma unroll
+  for (uint32_t q = 0; q < 2u; ++q) {
+    const uint32_t i = t + q * kUpwardThreads;
+    prev_f[q] = G(d.active)[level_off(D) + lin(8u * rx + (i & 7u), 8u * ry + ((i >> 3) & 7u), 8u * rz + (i >> 6), 1u << D)];
+  }
+  if (t < 64u)
+    prev_1 = G(d.active)[level_off(D - 1u) + lin(4u * rx + (t & 3u), 4u * ry + ((t >> 2) & 3u), 4u * rz + (t >> 4), 1u << (D - 1u))];
+  else if (t < 72u)
+    prev_2 = G(d.active)[level_off(D - 2u) + lin(2u * rx + ((t - 64u) & 1u), 2u * ry + (((t - 64u) >> 1) & 1u),
+                                                   2u * rz + ((t - 64u) >> 2), 1u << (D - 2u))];
   const gptr<int4> F = finest_of(d, a.parity);
   const gptr<int4> next = finest_of(d, a.parity ^ 1u);
   for (uint32_t i = t; i < 512u; i += kUpwardThreads) {
@@ -626,12 +657,14 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
   }
   __syncthreads();
 
-  for (uint32_t i = t; i < 512u; i += kUpwardThreads) {
+#pragma unroll
+  for (uint32_t q = 0; q < 2u; ++q) {
+    const uint32_t i = t + q * kUpwardThreads;
     const uint32_t ix = i & 7u, iy = (i >> 3) & 7u, iz = i >> 6;
     const uint32_t cell = level_off(D) + lin(8u * rx + ix, 8u * ry + iy, 8u * rz + iz, 1u << D);
     const int4 m = lvl0[i];
     G(d.source)[cell] = source_record(m, d);
-    update_active(d, cell, m.x, lvl1[lin(ix >> 1, iy >> 1, iz >> 1, 4u)].x);
+    update_active(d, cell, m.x, lvl1[lin(ix >> 1, iy >> 1, iz >> 1, 4u)].x, prev_f[q]);
   }
   uint32_t my_item = 0u, my_slot = 0u, my_level = 0u;
   bool pushes = false;
@@ -641,7 +674,7 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
     const uint32_t cell = level_off(D - 1u) + local;
     const int4 m = lvl1[t];
     G(d.source)[cell] = source_record(m, d);
-    update_active(d, cell, m.x, lvl2[lin(cx >> 1, cy >> 1, cz >> 1, 2u)].x);
+    update_active(d, cell, m.x, lvl2[lin(cx >> 1, cy >> 1, cz >> 1, 2u)].x, prev_1);
     if (m.x > kLeafCapacity) {
       pushes = true;
       my_level = 0u;
@@ -658,7 +691,7 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
     const uint32_t cell = level_off(D - 2u) + local;
     const int4 m = lvl2[u];
     G(d.source)[cell] = source_record(m, d);
-    update_active(d, cell, m.x, lvl3[0].x);
+    update_active(d, cell, m.x, lvl3[0].x, prev_2);
     if (m.x > kLeafCapacity) {
       pushes2 = true;
       my_level2 = 1u;
@@ -677,15 +710,10 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
     }
   }
   __syncthreads();
-  const uint32_t stripe = blockIdx.x % kStripes;
-  if (t == 0u) {
-    const uint32_t levels[4] = {D - 1u, D - 2u, r, 0u};
-    reserve_items(a, ps, levels, 3u, stripe);
-  }
+  if (t == 0u) reserve_items(a, ps, lev_of, 3u, stripe);
   __syncthreads();
-  const uint32_t lev_of[3] = {D - 1u, D - 2u, r};
-  if (pushes) write_item(a, lev_of[my_level], stripe, ps.base[my_level] + my_slot, my_item);
-  if (pushes2) write_item(a, lev_of[my_level2], stripe, ps.base[my_level2] + my_slot2, my_item2);
+  if (pushes) write_item(a, ps, my_level, ps.base[my_level] + my_slot, my_item);
+  if (pushes2) write_item(a, ps, my_level2, ps.base[my_level2] + my_slot2, my_item2);
 }
 
 // The levels above the subtrees, one block per system: levels r-1..0 from
@@ -705,10 +733,16 @@ __global__ __launch_bounds__(kUpwardThreads) void k_top(StepArgs a) {
     return lv + (j == 0u ? 0u : (j == 1u ? 512u : (j == 2u ? 576u : 584u)));
   };
   if (t < 4u) ps.n[t] = 0u;
+  const uint32_t stripe = s % kStripes;
+  const uint32_t levels[4] = {0u, 1u, 2u, 3u};
+  prefetch_stripes(a, ps, levels, r, stripe, t);
   // Level r (8^r <= 512 cells) into buf(0), then levels r-1, r-2, ... into
-  // buf(1), buf(2), buf(3).
+  // buf(1), buf(2), buf(3).  The target flags of levels 0..r before this
+  // step are loaded alongside.
+  __shared__ uint8_t prev[1u + 8u + 64u + 512u];
   const uint32_t nr = 1u << (3u * r);
   for (uint32_t i = t; i < nr; i += kUpwardThreads) buf(0)[i] = M[level_off(r) + i];
+  for (uint32_t c = t; c < level_off(r + 1u); c += kUpwardThreads) prev[c] = G(d.active)[c];
   __syncthreads();
   for (uint32_t j = 1u; j <= r; ++j) {
     const uint32_t k = r - j;
@@ -733,9 +767,37 @@ __global__ __launch_bounds__(kUpwardThreads) void k_top(StepArgs a) {
     scan512(sc, wsum, t);
     for (uint32_t u = t; u < nr; u += kUpwardThreads) G(a.sstart)[(d.cell_base >> 9) + u] = sc[u];
   }
-  // Levels 0..r-1: source records and lists; levels 1..r: flags.
+  // Internal cells of levels 0..r-1 are staged first, so that thread 0's
+  // list reservation is in flight while the records and flags are stored.
   uint32_t items[4], slots[4], levs[4];
   uint32_t nmine = 0u;
+  for (uint32_t j = 1u; j <= r; ++j) {
+    const uint32_t k = r - j;
+    const uint32_t dk = 1u << k;
+    const uint32_t ncell = dk * dk * dk;
+    for (uint32_t i = t; i < ncell; i += kUpwardThreads) {
+      if (buf(j)[i].x > kLeafCapacity && nmine < 4u) {
+        items[nmine] = pack_item(s, i);
+        levs[nmine] = k;
+        slots[nmine] = stage_item(ps, k);
+        ++nmine;
+      }
+    }
+  }
+  __syncthreads();
+  if (t == 0u) {
+    if (a.nsys == 1u) {
+      // One system: levels below r are listed by this block alone, so the
+      // counts are stored and the slots start at 0 (no atomics to wait on).
+      for (uint32_t k = 0; k < r; ++k) {
+        G(a.list_count)[k * kStripes + stripe] = ps.n[k];
+        ps.base[k] = 0u;
+      }
+    } else {
+      reserve_items(a, ps, levels, r, stripe);
+    }
+  }
+  // Levels 0..r-1: source records; levels 1..r: flags.
   for (uint32_t j = 0u; j <= r; ++j) {
     const uint32_t k = r - j;
     const uint32_t dk = 1u << k;
@@ -743,30 +805,16 @@ __global__ __launch_bounds__(kUpwardThreads) void k_top(StepArgs a) {
     for (uint32_t i = t; i < ncell; i += kUpwardThreads) {
       const int4 m = buf(j)[i];
       const uint32_t cell = level_off(k) + i;
-      if (k < r) {
-        G(d.source)[cell] = source_record(m, d);
-        if (m.x > kLeafCapacity && nmine < 4u) {
-          items[nmine] = pack_item(s, i);
-          levs[nmine] = k;
-          slots[nmine] = stage_item(ps, k);
-          ++nmine;
-        }
-      }
+      if (k < r) G(d.source)[cell] = source_record(m, d);
       if (k >= 1u) {
         uint32_t cx, cy, cz;
         unlin(i, k, cx, cy, cz);
-        update_active(d, cell, m.x, buf(j + 1u)[lin(cx >> 1, cy >> 1, cz >> 1, dk >> 1)].x);
+        update_active(d, cell, m.x, buf(j + 1u)[lin(cx >> 1, cy >> 1, cz >> 1, dk >> 1)].x, prev[cell]);
       }
     }
   }
   __syncthreads();
-  const uint32_t stripe = s % kStripes;
-  if (t == 0u) {
-    const uint32_t levels[4] = {0u, 1u, 2u, 3u};
-    reserve_items(a, ps, levels, r, stripe);
-  }
-  __syncthreads();
-  for (uint32_t q = 0; q < nmine; ++q) write_item(a, levs[q], stripe, ps.base[levs[q]] + slots[q], items[q]);
+  for (uint32_t q = 0; q < nmine; ++q) write_item(a, ps, levs[q], ps.base[levs[q]] + slots[q], items[q]);
 
   // Level 1 (tree pass of level 1, src/rainbow_fmm.wgsl:457-611): the root's
   // children have no well-separated source (every level-1 cell is a
