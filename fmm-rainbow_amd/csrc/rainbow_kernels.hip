@@ -439,13 +439,17 @@ __device__ __forceinline__ int4 sum8(const int4* src, uint32_t cx, uint32_t cy, 
 // The source record {com, G m} holds com = f32(sum) * (1 / (scale
 // f32(count))) and gm = G (f32(count) / f32(N)), formed once per cell with
 // the same operations the reference repeats per pair.
+// kFast: the caller guarantees r2 in [2^-94, FLT_MAX] (finite coordinates
+// of magnitude <= 2^60 and eps2 in [2^-94, 2^100]), where the unguarded
+// Newton form equals 1.0f / sqrtf(r2).
+template <bool kFast = false>
 __device__ __forceinline__ void pair_terms(const float4 src, float tx, float ty, float tz, float eps2,
                                            float out[10]) {
   const float rx = src.x - tx;
   const float ry = src.y - ty;
   const float rz = src.z - tz;
   const float r2 = ((rx * rx + ry * ry) + rz * rz) + eps2;
-  const float inv_r = rsqrt_rn(r2);  // = 1.0f / sqrtf(r2)
+  const float inv_r = kFast ? rsqrt_fast_unguarded(r2) : rsqrt_rn(r2);  // = 1.0f / sqrtf(r2)
   const float inv_r3 = inv_r * inv_r * inv_r;
   const float inv_r5 = inv_r3 * inv_r * inv_r;
   const float gm = src.w;
@@ -890,8 +894,11 @@ __device__ __forceinline__ float l2l_term(uint32_t k, float4 pa, float4 pj0, flo
 
 // Stages the source records of cells 2P-2 .. 2P+3 per axis of level `level`
 // (P = the parent's coordinates); cells outside the level read as empty.
-__device__ __forceinline__ void stage_window(float4* win, const SysDesc& d, uint32_t level, uint32_t px,
+// Returns false when a source this thread staged has a coordinate that is
+// not finite or exceeds 2^60 in magnitude.
+__device__ __forceinline__ bool stage_window(float4* win, const SysDesc& d, uint32_t level, uint32_t px,
                                              uint32_t py, uint32_t pz, uint32_t tid) {
+  bool bounded = true;
   const uint32_t dim = 1u << level;
   const int dim_i = static_cast<int>(dim);
   const gptr<float4> src = G(d.source) + level_off(level);
@@ -904,7 +911,9 @@ __device__ __forceinline__ void stage_window(float4* win, const SysDesc& d, uint
     if (qx >= 0 && qy >= 0 && qz >= 0 && qx < dim_i && qy < dim_i && qz < dim_i)
       v = src[lin(static_cast<uint32_t>(qx), static_cast<uint32_t>(qy), static_cast<uint32_t>(qz), dim)];
     win[w] = v;
+    bounded = bounded && fabsf(v.x) <= 0x1p60f && fabsf(v.y) <= 0x1p60f && fabsf(v.z) <= 0x1p60f;
   }
+  return bounded;
 }
 
 // LDS of the ancestor path per ancestor: its 6^3 window and its 10 x 216
@@ -1031,6 +1040,7 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t lo, 
   __shared__ float terms_s[kFused ? 1 : kChunksPerRound * 10 * 64];
   extern __shared__ float4 dyn[];
   __shared__ float pf[12];
+  __shared__ uint32_t votes[kTreeThreads / 64u];
   float4* win = kFused ? dyn : win_s;
   float* terms = kFused ? reinterpret_cast<float*>(dyn + 216) : terms_s;
   const uint32_t tid = threadIdx.x;
@@ -1109,7 +1119,7 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t lo, 
     const bool recompute = kFused && level - 1u >= lo;
     if (kFused && recompute) ancestor_fields<kComp>(d, lo, level, px, py, pz, reinterpret_cast<unsigned char*>(dyn), pf, tid);
 
-    stage_window(win, d, level, px, py, pz, tid);
+    const bool bounded = stage_window(win, d, level, px, py, pz, tid);
     const uint32_t tx = 2u * px + (tt & 1u), ty = 2u * py + ((tt >> 1) & 1u), tz = 2u * pz + (tt >> 2);
     const int tcount = cell_count(d, a.parity, level, lin(tx, ty, tz, dim));
     const float tcx = center1(tx, dim), tcy = center1(ty, dim), tcz = center1(tz, dim);
@@ -1117,14 +1127,29 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t lo, 
     // 32 chunks of 8 sources: the 27 source-parent blocks in pz, py, px
     // order (M2L; parents outside the level were staged as empty), then 4
     // blocks of 8 near offsets (leaf targets only), then one empty chunk.
-    // Zero terms leave the sums unchanged, so every item walks all 32.
-    const bool any_leaf = __syncthreads_or(leaf) != 0;
+    // Zero terms leave the sums unchanged (a running sum that starts at +0
+    // never becomes -0, so adding +0 or -0 is the identity), so every item
+    // walks all 32.  The waves' votes (any leaf target, any unbounded
+    // source) are combined after the barrier.
+    const uint32_t vote = (__any(leaf) ? 1u : 0u) | (__any(!bounded) ? 2u : 0u);
+    if (lane == 0u) votes[wave] = vote;
+    __syncthreads();
+    uint32_t all_votes = 0u;
+#pragma unroll
+    for (uint32_t w = 0; w < kWaves; ++w) all_votes |= votes[w];
+    const bool any_leaf = (all_votes & 1u) != 0u;
     // cell_epsilon, src/rainbow_fmm.wgsl:507 (0.06 * WORLD_HALF_EXTENT / dim).
     const float cell_epsilon = (0.06f * kWorldHalfExtent) * inv_pow2(dim);
     const float eps2_m2l = cell_epsilon * cell_epsilon;
     const float near_epsilon =
         fmaxf(d.softening, (0.72f * kWorldHalfExtent) * inv_pow2(dim));
     const float eps2_near = near_epsilon * near_epsilon;
+    // With every source coordinate bounded by 2^60 and eps2 in [2^-94,
+    // 2^100], each pair's r2 lies in [2^-94, FLT_MAX]: the pair fields use
+    // the unguarded inverse square root, and lanes without a pair evaluate
+    // a zero-mass source (terms of +-0) instead of branching.
+    const bool fast = (all_votes & 2u) == 0u && eps2_m2l >= 0x1p-94f && eps2_near >= 0x1p-94f &&
+                      eps2_m2l <= 0x1p100f && eps2_near <= 0x1p100f;
 
     Chain<kComp> ch;
     float inh = 0.0f;
@@ -1152,27 +1177,36 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t lo, 
       for (uint32_t j = 0; j < kChunksPerRound / kWaves; ++j) {
         const uint32_t slot = wave * (kChunksPerRound / kWaves) + j;
         const uint32_t q = r0 + slot;
-        float t[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        // This lane's source in the window and whether its pair counts.
+        uint32_t widx = 0u;
+        bool active = false;
+        float eps2 = eps2_m2l;
         if (q < 27u) {
           const uint32_t wsx = 2u * (q % 3u) + (cc & 1u);
           const uint32_t wsy = 2u * ((q / 3u) % 3u) + ((cc >> 1) & 1u);
           const uint32_t wsz = 2u * (q / 9u) + (cc >> 2);
           const bool near = (wsx + 1u - wtx) <= 2u && (wsy + 1u - wty) <= 2u && (wsz + 1u - wtz) <= 2u;
-          if (tcount != 0 && !near) {
-            const float4 sr = win[wsx + 6u * (wsy + 6u * wsz)];
-            if (sr.w != 0.0f) pair_terms(sr, tcx, tcy, tcz, eps2_m2l, t);
-          }
-        } else if (any_leaf) {
+          widx = wsx + 6u * (wsy + 6u * wsz);
+          active = tcount != 0 && !near;
+        } else {
           // Near field of leaf targets (near_all, src/rainbow_fmm.wgsl:
           // 533-571): offsets dz, dy, dx; the centre is skipped; neighbours
           // outside the level were staged as empty.
           const uint32_t idx = (q - 27u) * 8u + cc;
-          if (leaf && idx < 27u && idx != 13u) {
-            const uint32_t wx = wtx + idx % 3u - 1u;
-            const uint32_t wy = wty + (idx / 3u) % 3u - 1u;
-            const uint32_t wz = wtz + idx / 9u - 1u;
-            const float4 sr = win[wx + 6u * (wy + 6u * wz)];
-            if (sr.w != 0.0f) pair_terms(sr, tcx, tcy, tcz, eps2_near, t);
+          if (idx < 27u) {
+            widx = (wtx + idx % 3u - 1u) + 6u * ((wty + (idx / 3u) % 3u - 1u) + 6u * (wtz + idx / 9u - 1u));
+            active = leaf && idx != 13u;
+          }
+          eps2 = eps2_near;
+        }
+        float t[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (q < 27u || any_leaf) {
+          float4 sr = win[widx];
+          if (fast) {
+            if (!active) sr.w = 0.0f;
+            if (__any(sr.w != 0.0f)) pair_terms<true>(sr, tcx, tcy, tcz, eps2, t);
+          } else if (active && sr.w != 0.0f) {
+            pair_terms<false>(sr, tcx, tcy, tcz, eps2, t);
           }
         }
         // Terms of chunk `slot` stored per chain (target tt, component k)
