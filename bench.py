@@ -4,7 +4,7 @@ Workload (default): 1,048,576 equal-mass bodies in a seeded Plummer sphere
 (scale radius 0.1, cut at 1.5, virial velocities, seed 2), G=1, softening
 1e-3, dt 5e-4, planner depth 6; K timed steps after W warm-up steps.  A
 step is deposit, upward, top, tree (levels 2..6 in a few launches),
-integrate, with the body sort every few steps beside the tree pass,
+integrate, with the body sort every few steps before the tree pass,
 replayed from captured HIP graphs with the bodies resident in HBM.
 
 Prints ONE JSON line.  `roofline` describes the kernel with the largest

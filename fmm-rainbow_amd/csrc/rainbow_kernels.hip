@@ -21,7 +21,7 @@
 //   integrate  adaptive descent, local expansion, own-leaf monopole,
 //              symplectic Euler kick-drift and the soft absorbing boundary
 // A sorting step also reorders the bodies by Morton code (scatter, copy-back
-// and owned ranges) on a second stream beside the tree pass.
+// and owned ranges) between top and the tree pass.
 //
 // The reference runs the same physics as 2d+5 dispatches
 // (native/main.rs:411-421): clear_cells, deposit, upward per level, m2l_all,

@@ -150,9 +150,6 @@ class Engine {
   // bodies (after an upload, a field solve, a schedule change or a restore):
   // the next step runs a standalone deposit.
   bool pending_deposit = true;
-  hipStream_t side = nullptr;  // the sort's scatter beside the tree pass
-  hipEvent_t ev_fork = nullptr;
-  hipEvent_t ev_join = nullptr;
   hipEvent_t ev_a = nullptr;
   hipEvent_t ev_b = nullptr;
 };

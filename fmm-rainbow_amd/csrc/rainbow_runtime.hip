@@ -180,15 +180,13 @@ void DevBuf::release() {
 
 Engine::~Engine() {
   if (stream) (void)hipStreamSynchronize(stream);
-  if (side) (void)hipStreamSynchronize(side);
   drop_graphs();
   for (DevBuf* b : {&sys_buf, &deposit_buf, &integrate_buf, &subtree_buf, &list_buf, &count_buf,
                     &stripe_buf, &diag_buf, &body_buf, &snap_buf, &own_buf, &rank_buf, &cstart_buf, &sstart_buf,
                     &io_buf})
     b->release();
-  for (hipEvent_t e : {ev_a, ev_b, ev_fork, ev_join})
+  for (hipEvent_t e : {ev_a, ev_b})
     if (e) (void)hipEventDestroy(e);
-  if (side) (void)hipStreamDestroy(side);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -202,18 +200,14 @@ void Engine::drop_graphs() {
 int Engine::open(uint32_t opts) {
   options = opts;
   RB_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-  RB_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
   RB_HIP(hipEventCreate(&ev_a));
   RB_HIP(hipEventCreate(&ev_b));
-  RB_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
-  RB_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
   return RAINBOW_OK;
 }
 
 int Engine::configure(const std::vector<SysDesc>& systems) {
   if (systems.size() > 65535u) return fail(RAINBOW_E_INVALID, "at most 65535 systems");
   RB_HIP(hipStreamSynchronize(stream));
-  RB_HIP(hipStreamSynchronize(side));
   drop_graphs();
   // Owned deposit ranges of the systems that stay (same body buffer and
   // count) carry over; the others start owning nothing until their sort.
@@ -390,9 +384,9 @@ int Engine::set_tree_fusion(uint32_t hi) {
 // here instead, into zeroed buffers.  A sorting step reorders the bodies by
 // the Morton code of their finest cell: its deposit wrote the ranks, its
 // upward and top write the offsets, and the scatter, copy-back and owned
-// ranges run on the side stream while the tree pass runs (the tree pass
-// does not read the bodies), joined before integrate.  `rank_next`: the
-// fused deposit writes ranks for a sorting next step.
+// ranges run after top, in the same stream (a second stream's fork and
+// join cost more here than the overlap with the tree pass gains).
+// `rank_next`: the fused deposit writes ranks for a sorting next step.
 int Engine::enqueue(bool integrate, bool sorting, uint32_t step_parity, bool deposit_first, bool rank_next) {
   if (sys.empty()) return RAINBOW_OK;
   StepArgs a = args;
@@ -405,14 +399,8 @@ int Engine::enqueue(bool integrate, bool sorting, uint32_t step_parity, bool dep
   }
   RB_HIP(launch_upward(a, n_subtree, stream));
   RB_HIP(launch_top(a, stream));
-  if (sorting) {
-    RB_HIP(hipEventRecord(ev_fork, stream));
-    RB_HIP(hipStreamWaitEvent(side, ev_fork, 0));
-    RB_HIP(launch_sort(a, n_integrate, n_deposit, side));
-    RB_HIP(hipEventRecord(ev_join, side));
-  }
+  if (sorting) RB_HIP(launch_sort(a, n_integrate, n_deposit, stream));
   for (const TreeLaunch& t : tree_launches) RB_HIP(launch_tree(a, t.lo, t.hi, t.blocks, stream));
-  if (sorting) RB_HIP(hipStreamWaitEvent(stream, ev_join, 0));
   if (integrate) RB_HIP(launch_integrate_deposit(a, n_deposit, stream));
   return RAINBOW_OK;
 }
@@ -681,15 +669,12 @@ int Engine::phase_ms(uint32_t steps, float* out, uint32_t n) {
     if (err == hipSuccess) err = launch_upward(a, n_subtree, stream, &tm[k++]);
     if (err == hipSuccess) err = launch_top(a, stream, &tm[k++]);
     if (sorting) {
-      // The sort kernels run on the side stream beside the tree pass.
       if (err == hipSuccess) err = hipEventRecord(sort_ev[0], stream);
-      if (err == hipSuccess) err = hipStreamWaitEvent(side, sort_ev[0], 0);
-      if (err == hipSuccess) err = launch_sort(a, n_integrate, n_deposit, side);
-      if (err == hipSuccess) err = hipEventRecord(sort_ev[1], side);
+      if (err == hipSuccess) err = launch_sort(a, n_integrate, n_deposit, stream);
+      if (err == hipSuccess) err = hipEventRecord(sort_ev[1], stream);
     }
     for (const TreeLaunch& t : tree_launches)
       if (err == hipSuccess) err = launch_tree(a, t.lo, t.hi, t.blocks, stream, &tm[k++]);
-    if (sorting && err == hipSuccess) err = hipStreamWaitEvent(stream, sort_ev[1], 0);
     if (err == hipSuccess) err = launch_integrate_deposit(a, n_deposit, stream, &tm[k++]);
     if (sorting && err == hipSuccess) {
       err = hipEventSynchronize(sort_ev[1]);

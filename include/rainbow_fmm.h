@@ -241,7 +241,7 @@ int rainbow_hip_phase_ms(rainbow_hip_t* h, uint32_t steps, float* out,
 
 /* Bodies are stored in Morton order of their finest cell: every
  * `interval`-th step (0 = never; default 8) counting-sorts them by the cell
- * its deposit found, beside its tree pass.  The order is internal:
+ * its deposit found, before its tree pass.  The order is internal:
  * downloads return caller order and no result depends on it. */
 int rainbow_hip_set_sort_interval(rainbow_hip_t* h, uint32_t interval);
 
