@@ -20,8 +20,9 @@
 //              child of an internal cell), four waves per internal parent
 //   integrate  adaptive descent, local expansion, own-leaf monopole,
 //              symplectic Euler kick-drift and the soft absorbing boundary
-// A sorting step also reorders the bodies by Morton code (scatter, copy-back
-// and owned ranges) between top and the tree pass.
+// A sorting step also reorders the bodies by Morton code: a scatter into
+// staging arrays between top and the tree pass, from which that step's
+// integrate reads them.
 //
 // The reference runs the same physics as 2d+5 dispatches
 // (native/main.rs:411-421): clear_cells, deposit, upward per level, m2l_all,
@@ -185,7 +186,7 @@ struct DepositLds {
 // bodies' ranks when `ranks`.  All threads of the block call it.
 __device__ void deposit_block(const StepArgs& a, const SysDesc& d, uint32_t block, uint32_t first,
                               const float4 (&p)[kDepositPerThread], uint32_t parity, bool ranks,
-                              DepositLds& L) {
+                              DepositLds& L, const uint2* own_lds = nullptr) {
   uint32_t* const keys = L.keys;
   int (*const vals)[kDepositSlots] = L.vals;
   for (uint32_t s = threadIdx.x; s < kDepositSlots; s += kDepositThreads) {
@@ -260,7 +261,7 @@ __device__ void deposit_block(const StepArgs& a, const SysDesc& d, uint32_t bloc
   // stored.  Any other cell's sums go to `extra` with atomics (bodies that
   // moved since the last sort, and cells shared with other blocks);
   // upward adds `extra` into the finest level.
-  const uint2 own = G(a.own)[block];
+  const uint2 own = own_lds != nullptr ? *own_lds : G(a.own)[block];
   const gptr<int4> finest = finest_of(d, parity);
   for (uint32_t s = threadIdx.x; s < kDepositSlots; s += kDepositThreads) {
     const uint32_t key = keys[s];
@@ -381,34 +382,6 @@ __global__ __launch_bounds__(kIntegrateThreads) void k_scatter(StepArgs a) {
   G(a.tpos)[g] = p;
   G(a.tvel)[g] = G(d.vel)[i];
   G(a.tperm)[g] = G(d.perm)[i];
-}
-
-__global__ __launch_bounds__(kIntegrateThreads) void k_copy_back(StepArgs a) {
-  const BodyBlock blk = a.integrate_blocks[blockIdx.x];
-  const SysDesc d = a.sys[blk.sys];
-  if (!d.live) return;
-  const uint32_t i = blk.first + threadIdx.x;
-  if (i >= d.n) return;
-  const uint32_t g = d.body_base + i;
-  G(d.pos)[i] = G(a.tpos)[g];
-  G(d.vel)[i] = G(a.tvel)[g];
-  G(d.perm)[i] = G(a.tperm)[g];
-}
-
-// Owned Morton ranges of the deposit blocks from the sorted bodies: block
-// b owns [cell of its first body, cell of the next block's first body); a
-// system's first block starts at 0 and its last ends at 2^18.
-__global__ __launch_bounds__(256) void k_own(StepArgs a, uint32_t nblocks) {
-  const uint32_t b = blockIdx.x * 256u + threadIdx.x;
-  if (b >= nblocks) return;
-  const BodyBlock blk = a.deposit_blocks[b];
-  const SysDesc d = a.sys[blk.sys];
-  if (!d.live) return;
-  const uint32_t dim = 1u << d.depth;
-  const uint32_t lo = blk.first == 0u ? 0u : morton_of(G(d.pos)[blk.first], dim);
-  const uint32_t next = blk.first + kDepositBodies;
-  const uint32_t hi = next >= d.n ? (1u << 18) : morton_of(G(d.pos)[next], dim);
-  G(a.own)[b] = make_uint2(lo, hi);
 }
 
 // ------------------------------------------------------------------------
@@ -1470,28 +1443,48 @@ __device__ __forceinline__ void integrate_body(const SysDesc& d, uint32_t parity
 // 1024 bodies of its deposit range and deposits their new positions into
 // the other finest-level buffer (zeroed by this step's upward), with the
 // next step's ranks when that step sorts.
+//
+// On a sorting step the bodies are read from the staging arrays the scatter
+// filled in Morton order (and written back to the body arrays, with their
+// caller indices), and each block takes the Morton range it owns from then
+// on: [cell of its first body, cell of the next block's first body); a
+// system's first block starts at 0 and its last ends at 2^18.
 template <bool kComp>
 __global__ __launch_bounds__(kDepositThreads) void k_integrate_deposit(StepArgs a) {
   __shared__ DepositLds L;
+  __shared__ uint2 own_new;
   reset_lists(a);
   const BodyBlock blk = a.deposit_blocks[blockIdx.x];
   const SysDesc d = a.sys[blk.sys];
   if (!d.live) return;
+  const bool staged = a.sorting != 0u;
+  const uint32_t dim = 1u << d.depth;
   float4 p[kDepositPerThread];
 #pragma unroll
   for (uint32_t k = 0; k < kDepositPerThread; ++k) {
     const uint32_t i = blk.first + k * kDepositThreads + threadIdx.x;
     p[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (i < d.n) {
-      float4 p4 = G(d.pos)[i];
-      float4 v4 = G(d.vel)[i];
+      const uint32_t g = d.body_base + i;
+      float4 p4 = staged ? G(a.tpos)[g] : G(d.pos)[i];
+      float4 v4 = staged ? G(a.tvel)[g] : G(d.vel)[i];
+      if (staged) {
+        G(d.perm)[i] = G(a.tperm)[g];
+        if (k == 0u && threadIdx.x == 0u) {
+          const uint32_t next = blk.first + kDepositBodies;
+          const uint32_t lo = blk.first == 0u ? 0u : morton_of(p4, dim);
+          const uint32_t hi = next >= d.n ? (1u << 18) : morton_of(G(a.tpos)[d.body_base + next], dim);
+          own_new = make_uint2(lo, hi);
+          G(a.own)[blockIdx.x] = own_new;
+        }
+      }
       integrate_body<kComp>(d, a.parity, p4, v4);
       G(d.pos)[i] = p4;
       G(d.vel)[i] = v4;
       p[k] = p4;
     }
   }
-  deposit_block(a, d, blockIdx.x, blk.first, p, a.parity ^ 1u, a.rank_next != 0u, L);
+  deposit_block(a, d, blockIdx.x, blk.first, p, a.parity ^ 1u, a.rank_next != 0u, L, staged ? &own_new : nullptr);
 }
 
 // ------------------------------------------------------------------------
@@ -1770,14 +1763,9 @@ hipError_t launch_tree(const StepArgs& a, uint32_t lo, uint32_t hi, uint32_t nbl
   return hipGetLastError();
 }
 
-hipError_t launch_sort(const StepArgs& a, uint32_t nintegrate, uint32_t ndeposit, hipStream_t stream) {
+hipError_t launch_sort(const StepArgs& a, uint32_t nintegrate, hipStream_t stream) {
   if (nintegrate == 0u) return hipSuccess;
   hipLaunchKernelGGL(k_scatter, dim3(nintegrate), dim3(kIntegrateThreads), 0, stream, a);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_copy_back, dim3(nintegrate), dim3(kIntegrateThreads), 0, stream, a);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_own, dim3((ndeposit + 255u) / 256u), dim3(256), 0, stream, a, ndeposit);
   return hipGetLastError();
 }
 

@@ -24,9 +24,9 @@ hipError_t launch_tree(const StepArgs& a, uint32_t lo, uint32_t hi, uint32_t nbl
 // one block per deposit range.
 hipError_t launch_integrate_deposit(const StepArgs& a, uint32_t ndeposit, hipStream_t stream,
                                     const Timing* t = nullptr);
-// The counting sort's scatter, copy-back and owned ranges (after upward and
-// top of a step run with a.sorting set).
-hipError_t launch_sort(const StepArgs& a, uint32_t nintegrate, uint32_t ndeposit, hipStream_t stream);
+// The counting sort's scatter into the staging arrays (after upward and top
+// of a step run with a.sorting set; that step's integrate reads them).
+hipError_t launch_sort(const StepArgs& a, uint32_t nintegrate, hipStream_t stream);
 // FP64 direct sum at `nt` target bodies (caller indices, device array):
 // out[4 t ..] = {ax, ay, az, phi}.  inv: n words of scratch; partial:
 // 4 nt nchunks doubles of scratch.

@@ -383,12 +383,15 @@ int Engine::set_tree_fusion(uint32_t hi) {
 // previous step's integrate (k_integrate_deposit); `deposit_first` runs it
 // here instead, into zeroed buffers.  A sorting step reorders the bodies by
 // the Morton code of their finest cell: its deposit wrote the ranks, its
-// upward and top write the offsets, and the scatter, copy-back and owned
-// ranges run after top, in the same stream (a second stream's fork and
-// join cost more here than the overlap with the tree pass gains).
+// upward and top write the offsets, and the scatter runs after top, in the
+// same stream (a second stream's fork and join cost more here than the
+// overlap with the tree pass gains); the step's integrate reads the sorted
+// bodies from the staging arrays and sets the blocks' owned ranges.
 // `rank_next`: the fused deposit writes ranks for a sorting next step.
 int Engine::enqueue(bool integrate, bool sorting, uint32_t step_parity, bool deposit_first, bool rank_next) {
   if (sys.empty()) return RAINBOW_OK;
+  // The sorted bodies reach the body arrays only through integrate.
+  sorting = sorting && integrate;
   StepArgs a = args;
   a.sorting = sorting ? 1u : 0u;
   a.parity = step_parity;
@@ -399,7 +402,7 @@ int Engine::enqueue(bool integrate, bool sorting, uint32_t step_parity, bool dep
   }
   RB_HIP(launch_upward(a, n_subtree, stream));
   RB_HIP(launch_top(a, stream));
-  if (sorting) RB_HIP(launch_sort(a, n_integrate, n_deposit, stream));
+  if (sorting) RB_HIP(launch_sort(a, n_integrate, stream));
   for (const TreeLaunch& t : tree_launches) RB_HIP(launch_tree(a, t.lo, t.hi, t.blocks, stream));
   if (integrate) RB_HIP(launch_integrate_deposit(a, n_deposit, stream));
   return RAINBOW_OK;
@@ -670,7 +673,7 @@ int Engine::phase_ms(uint32_t steps, float* out, uint32_t n) {
     if (err == hipSuccess) err = launch_top(a, stream, &tm[k++]);
     if (sorting) {
       if (err == hipSuccess) err = hipEventRecord(sort_ev[0], stream);
-      if (err == hipSuccess) err = launch_sort(a, n_integrate, n_deposit, stream);
+      if (err == hipSuccess) err = launch_sort(a, n_integrate, stream);
       if (err == hipSuccess) err = hipEventRecord(sort_ev[1], stream);
     }
     for (const TreeLaunch& t : tree_launches)
