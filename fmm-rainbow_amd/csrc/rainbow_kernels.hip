@@ -697,7 +697,7 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
 // the 8^r subtree roots, their source records and lists, the target flags
 // of levels 1..r, and the fields of level 1.
 template <bool kComp>
-__global__ __launch_bounds__(kUpwardThreads) void k_top(StepArgs a) {
+__global__ __launch_bounds__(kTopThreads) void k_top(StepArgs a) {
   __shared__ int4 lv[512 + 64 + 8 + 1];
   __shared__ PushStage ps;
   const uint32_t s = blockIdx.x;
@@ -718,8 +718,8 @@ __global__ __launch_bounds__(kUpwardThreads) void k_top(StepArgs a) {
   // step are loaded alongside.
   __shared__ uint8_t prev[1u + 8u + 64u + 512u];
   const uint32_t nr = 1u << (3u * r);
-  for (uint32_t i = t; i < nr; i += kUpwardThreads) buf(0)[i] = M[level_off(r) + i];
-  for (uint32_t c = t; c < level_off(r + 1u); c += kUpwardThreads) prev[c] = G(d.active)[c];
+  for (uint32_t i = t; i < nr; i += kTopThreads) buf(0)[i] = M[level_off(r) + i];
+  for (uint32_t c = t; c < level_off(r + 1u); c += kTopThreads) prev[c] = G(d.active)[c];
   __syncthreads();
   for (uint32_t j = 1u; j <= r; ++j) {
     const uint32_t k = r - j;
@@ -735,14 +735,22 @@ __global__ __launch_bounds__(kUpwardThreads) void k_top(StepArgs a) {
   }
   if (a.sorting) {
     // Offsets of the subtrees (level r cells, Morton order) in the system.
-    __shared__ uint32_t sc[512];
-    __shared__ uint32_t wsum[4];
     const uint32_t dr = 1u << r;
-    for (uint32_t u = t; u < 512u; u += kUpwardThreads)
-      sc[u] = u < nr ? static_cast<uint32_t>(buf(0)[lin(compact3(u), compact3(u >> 1), compact3(u >> 2), dr)].x) : 0u;
+    __shared__ uint32_t wsum8[8];
+    // One value per thread of the first 512: wave prefix sums, then the
+    // waves' totals.
+    uint32_t own = 0u, incl = 0u;
+    if (t < 512u) {
+      own = t < nr ? static_cast<uint32_t>(buf(0)[lin(compact3(t), compact3(t >> 1), compact3(t >> 2), dr)].x) : 0u;
+      incl = wave_prefix_sum(own);
+      if ((t & 63u) == 63u) wsum8[t >> 6] = incl;
+    }
     __syncthreads();
-    scan512(sc, wsum, t);
-    for (uint32_t u = t; u < nr; u += kUpwardThreads) G(a.sstart)[(d.cell_base >> 9) + u] = sc[u];
+    if (t < nr) {
+      uint32_t base = 0u;
+      for (uint32_t w = 0; w < (t >> 6); ++w) base += wsum8[w];
+      G(a.sstart)[(d.cell_base >> 9) + t] = base + incl - own;
+    }
   }
   // Internal cells of levels 0..r-1 are staged first, so that thread 0's
   // list reservation is in flight while the records and flags are stored.
@@ -752,7 +760,7 @@ __global__ __launch_bounds__(kUpwardThreads) void k_top(StepArgs a) {
     const uint32_t k = r - j;
     const uint32_t dk = 1u << k;
     const uint32_t ncell = dk * dk * dk;
-    for (uint32_t i = t; i < ncell; i += kUpwardThreads) {
+    for (uint32_t i = t; i < ncell; i += kTopThreads) {
       if (buf(j)[i].x > kLeafCapacity && nmine < 4u) {
         items[nmine] = pack_item(s, i);
         levs[nmine] = k;
@@ -774,20 +782,17 @@ __global__ __launch_bounds__(kUpwardThreads) void k_top(StepArgs a) {
       reserve_items(a, ps, levels, r, stripe);
     }
   }
-  // Levels 0..r-1: source records; levels 1..r: flags.
-  for (uint32_t j = 0u; j <= r; ++j) {
-    const uint32_t k = r - j;
-    const uint32_t dk = 1u << k;
-    const uint32_t ncell = dk * dk * dk;
-    for (uint32_t i = t; i < ncell; i += kUpwardThreads) {
-      const int4 m = buf(j)[i];
-      const uint32_t cell = level_off(k) + i;
-      if (k < r) G(d.source)[cell] = source_record(m, d);
-      if (k >= 1u) {
-        uint32_t cx, cy, cz;
-        unlin(i, k, cx, cy, cz);
-        update_active(d, cell, m.x, buf(j + 1u)[lin(cx >> 1, cy >> 1, cz >> 1, dk >> 1)].x, prev[cell]);
-      }
+  // Levels 0..r-1: source records; levels 1..r: flags; one cell per thread.
+  for (uint32_t cell = t; cell < level_off(r + 1u); cell += kTopThreads) {
+    uint32_t k = 0u;
+    while (cell >= level_off(k + 1u)) ++k;
+    const uint32_t j = r - k, dk = 1u << k, i = cell - level_off(k);
+    const int4 m = buf(j)[i];
+    if (k < r) G(d.source)[cell] = source_record(m, d);
+    if (k >= 1u) {
+      uint32_t cx, cy, cz;
+      unlin(i, k, cx, cy, cz);
+      update_active(d, cell, m.x, buf(j + 1u)[lin(cx >> 1, cy >> 1, cz >> 1, dk >> 1)].x, prev[cell]);
     }
   }
   __syncthreads();
@@ -1717,9 +1722,9 @@ hipError_t launch_upward(const StepArgs& a, uint32_t nblocks, hipStream_t stream
 hipError_t launch_top(const StepArgs& a, hipStream_t stream, const Timing* t) {
   if (a.nsys == 0u) return no_launch(stream, t);
   if (a.compensated)
-    RB_LAUNCH(t, k_top<true>, dim3(a.nsys), dim3(kUpwardThreads), stream, a);
+    RB_LAUNCH(t, k_top<true>, dim3(a.nsys), dim3(kTopThreads), stream, a);
   else
-    RB_LAUNCH(t, k_top<false>, dim3(a.nsys), dim3(kUpwardThreads), stream, a);
+    RB_LAUNCH(t, k_top<false>, dim3(a.nsys), dim3(kTopThreads), stream, a);
   return hipGetLastError();
 }
 
