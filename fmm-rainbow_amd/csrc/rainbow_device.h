@@ -115,7 +115,7 @@ constexpr uint32_t kDepositPerThread = 1;
 constexpr uint32_t kDepositBodies = kDepositThreads * kDepositPerThread;  // per block
 constexpr uint32_t kDepositSlots = 2048;  // LDS hash slots, 2x bodies per block
 constexpr uint32_t kIntegrateThreads = 256;
-constexpr uint32_t kUpwardThreads = 256;
+constexpr uint32_t kUpwardThreads = 512;  // one thread per finest cell of a subtree
 constexpr uint32_t kTopThreads = 1024;  // one block per system for the levels above the subtrees
 constexpr uint32_t kTreeThreads = 256;  // four waves per internal parent
 constexpr uint32_t kTreeMaxBlocks = 16384;

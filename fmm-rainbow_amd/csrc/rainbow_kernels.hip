@@ -338,27 +338,6 @@ __device__ __forceinline__ uint32_t compact3(uint32_t v) {
   return v;
 }
 
-// Exclusive prefix sum of v[0..511] in LDS, in place, two values per
-// thread of a 256-thread block (wsum: 4 words of LDS).
-__device__ void scan512(uint32_t* v, uint32_t* wsum, uint32_t t) {
-  const uint32_t a0 = v[2u * t], a1 = v[2u * t + 1u];
-  const uint32_t lane = t & 63u, w = t >> 6;
-  uint32_t x = a0 + a1;
-#pragma unroll
-  for (uint32_t o = 1u; o < 64u; o <<= 1) {
-    const uint32_t y = __shfl_up(x, o);
-    if (lane >= o) x += y;
-  }
-  if (lane == 63u) wsum[w] = x;
-  __syncthreads();
-  uint32_t base = 0u;
-  for (uint32_t j = 0; j < w; ++j) base += wsum[j];
-  const uint32_t excl = base + x - (a0 + a1);
-  v[2u * t] = excl;
-  v[2u * t + 1u] = excl + a0;
-  __syncthreads();
-}
-
 // Scatter of the counting sort (bodies in Morton order of their finest
 // cell): body i goes to slot sstart[subtree] + cstart[cell] + its index in
 // the cell, in the staging buffers.
@@ -537,10 +516,12 @@ __device__ __forceinline__ void write_item(const StepArgs& a, const PushStage& p
 // LDS, writes them with their source records, updates the target flags of
 // the levels whose parents it holds, and lists the internal cells.
 __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
+  static_assert(kUpwardThreads == 512u, "one thread per finest cell of the subtree");
   // Levels of the subtree in one LDS array: 8^3 finest cells at 0, then
   // 64, 8 and 1 cells of the three levels above.
   __shared__ int4 lv[512 + 64 + 8 + 1];
   __shared__ PushStage ps;
+  __shared__ uint32_t wsum8[8];
   int4* const lvl0 = lv;
   int4* const lvl1 = lv + 512;
   int4* const lvl2 = lv + 576;
@@ -559,59 +540,62 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
   const uint32_t lev_of[4] = {D - 1u, D - 2u, r, 0u};
   prefetch_stripes(a, ps, lev_of, 3u, stripe, t);
 
-  // Finest level (this step's buffer): 8x8x8 cells, x fastest (rows of 8
-  // contiguous moments), plus the sums deposited into `extra` (wrapping i32
-  // adds, exact in any order), written back when non-zero; `extra` and the
-  // other buffer's cells are zeroed for the next step's deposit.
-  __shared__ uint32_t owned[512];
-  // Target flags of this block's cells before this step, loaded with the
-  // finest cells (the flag updates below then wait on nothing).
-  uint8_t prev_f[2], prev_1 = 0u, prev_2 = 0u;
-#pragma unroll
-  for (uint32_t q = 0; q < 2u; ++q) {
-    const uint32_t i = t + q * kUpwardThreads;
-    prev_f[q] = G(d.active)[level_off(D) + lin(8u * rx + (i & 7u), 8u * ry + ((i >> 3) & 7u), 8u * rz + (i >> 6), 1u << D)];
-  }
+  // This thread's cells: finest cell t (x fastest, rows of 8 contiguous
+  // moments), level D-1 cell t (t < 64), level D-2 cell t - 64 (64 <= t <
+  // 72), the root (t == 72).  Their target flags before this step are
+  // loaded with the finest cells (the flag updates below then wait on
+  // nothing).
+  const uint32_t ix = t & 7u, iy = (t >> 3) & 7u, iz = t >> 6;
+  const uint32_t fcell = lin(8u * rx + ix, 8u * ry + iy, 8u * rz + iz, 1u << D);
+  const uint8_t prev_f = G(d.active)[level_off(D) + fcell];
+  uint8_t prev_1 = 0u, prev_2 = 0u;
   if (t < 64u)
     prev_1 = G(d.active)[level_off(D - 1u) + lin(4u * rx + (t & 3u), 4u * ry + ((t >> 2) & 3u), 4u * rz + (t >> 4), 1u << (D - 1u))];
   else if (t < 72u)
     prev_2 = G(d.active)[level_off(D - 2u) + lin(2u * rx + ((t - 64u) & 1u), 2u * ry + (((t - 64u) >> 1) & 1u),
                                                    2u * rz + ((t - 64u) >> 2), 1u << (D - 2u))];
+  // Finest level (this step's buffer) plus the sums deposited into `extra`
+  // (wrapping i32 adds, exact in any order), written back when non-zero;
+  // `extra` and the other buffer's cells are zeroed for the next step's
+  // deposit.
   const gptr<int4> F = finest_of(d, a.parity);
   const gptr<int4> next = finest_of(d, a.parity ^ 1u);
-  for (uint32_t i = t; i < 512u; i += kUpwardThreads) {
-    const uint32_t ix = i & 7u, iy = (i >> 3) & 7u, iz = i >> 6;
-    const uint32_t c = lin(8u * rx + ix, 8u * ry + iy, 8u * rz + iz, 1u << D);
-    int4 m = F[c];
-    owned[i] = static_cast<uint32_t>(m.x);
-    const int4 e = G(d.extra)[c];
+  int4 m0 = F[fcell];
+  const uint32_t owned = static_cast<uint32_t>(m0.x);
+  {
+    const int4 e = G(d.extra)[fcell];
     if (e.x != 0) {
-      m.x = static_cast<int>(static_cast<uint32_t>(m.x) + static_cast<uint32_t>(e.x));
-      m.y = static_cast<int>(static_cast<uint32_t>(m.y) + static_cast<uint32_t>(e.y));
-      m.z = static_cast<int>(static_cast<uint32_t>(m.z) + static_cast<uint32_t>(e.z));
-      m.w = static_cast<int>(static_cast<uint32_t>(m.w) + static_cast<uint32_t>(e.w));
-      F[c] = m;
-      G(d.extra)[c] = make_int4(0, 0, 0, 0);
+      m0.x = static_cast<int>(static_cast<uint32_t>(m0.x) + static_cast<uint32_t>(e.x));
+      m0.y = static_cast<int>(static_cast<uint32_t>(m0.y) + static_cast<uint32_t>(e.y));
+      m0.z = static_cast<int>(static_cast<uint32_t>(m0.z) + static_cast<uint32_t>(e.z));
+      m0.w = static_cast<int>(static_cast<uint32_t>(m0.w) + static_cast<uint32_t>(e.w));
+      F[fcell] = m0;
+      G(d.extra)[fcell] = make_int4(0, 0, 0, 0);
     }
-    next[c] = make_int4(0, 0, 0, 0);
-    lvl0[i] = m;
   }
+  next[fcell] = make_int4(0, 0, 0, 0);
+  lvl0[t] = m0;
   __syncthreads();
   if (a.sorting) {
     // Offsets of the subtree's cells in Morton order (the subtree's 512
-    // finest cells are the contiguous Morton range root << 9 ..).
-    __shared__ uint32_t sc[512];
-    __shared__ uint32_t wsum[4];
-    for (uint32_t u = t; u < 512u; u += kUpwardThreads)
-      sc[u] = static_cast<uint32_t>(lvl0[compact3(u) + 8u * (compact3(u >> 1) + 8u * compact3(u >> 2))].x);
+    // finest cells are the contiguous Morton range root << 9 ..): thread u
+    // takes Morton cell u; wave prefix sums, then the waves' totals.
+    const uint32_t mi = compact3(t) + 8u * (compact3(t >> 1) + 8u * compact3(t >> 2));
+    const uint32_t cnt = static_cast<uint32_t>(lvl0[mi].x);
+    const uint32_t incl = wave_prefix_sum(cnt);
+    if ((t & 63u) == 63u) wsum8[t >> 6] = incl;
     __syncthreads();
-    scan512(sc, wsum, t);
+    uint32_t base = 0u;
+    for (uint32_t w = 0; w < (t >> 6); ++w) base += wsum8[w];
+    const uint32_t start = base + incl - cnt;
     const uint32_t root_m = spread3(rx) | (spread3(ry) << 1) | (spread3(rz) << 2);
-    for (uint32_t u = t; u < 512u; u += kUpwardThreads) {
-      const uint32_t g = d.cell_base + (root_m << 9) + u;
-      G(a.cstart)[g] = sc[u];
-      G(a.xstart)[g] = sc[u] + owned[compact3(u) + 8u * (compact3(u >> 1) + 8u * compact3(u >> 2))];
-    }
+    const uint32_t g = d.cell_base + (root_m << 9) + t;
+    // The owned count of Morton cell t lives with the thread of its lin index.
+    __shared__ uint32_t owned_s[512];
+    owned_s[t] = owned;
+    __syncthreads();
+    G(a.cstart)[g] = start;
+    G(a.xstart)[g] = start + owned_s[mi];
   }
   if (t < 64u) {
     const uint32_t cx = t & 3u, cy = (t >> 2) & 3u, cz = t >> 4;
@@ -634,63 +618,60 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
   }
   __syncthreads();
 
-#pragma unroll
-  for (uint32_t q = 0; q < 2u; ++q) {
-    const uint32_t i = t + q * kUpwardThreads;
-    const uint32_t ix = i & 7u, iy = (i >> 3) & 7u, iz = i >> 6;
-    const uint32_t cell = level_off(D) + lin(8u * rx + ix, 8u * ry + iy, 8u * rz + iz, 1u << D);
-    const int4 m = lvl0[i];
-    G(d.source)[cell] = source_record(m, d);
-    update_active(d, cell, m.x, lvl1[lin(ix >> 1, iy >> 1, iz >> 1, 4u)].x, prev_f[q]);
-  }
-  uint32_t my_item = 0u, my_slot = 0u, my_level = 0u;
-  bool pushes = false;
+  // Internal cells (levels D-1, D-2 and the root) are staged first, so that
+  // thread 0's list reservation is in flight while the records and flags
+  // are stored.
+  uint32_t my_item = 0u, my_slot = 0u, my_level = 0u, local = 0u, lcell = 0u;
+  int4 lm = make_int4(0, 0, 0, 0);
+  int lparent = 0;
+  uint8_t lprev = 0u;
+  bool mine = false;
   if (t < 64u) {
     const uint32_t cx = t & 3u, cy = (t >> 2) & 3u, cz = t >> 4;
-    const uint32_t local = lin(4u * rx + cx, 4u * ry + cy, 4u * rz + cz, 1u << (D - 1u));
-    const uint32_t cell = level_off(D - 1u) + local;
-    const int4 m = lvl1[t];
-    G(d.source)[cell] = source_record(m, d);
-    update_active(d, cell, m.x, lvl2[lin(cx >> 1, cy >> 1, cz >> 1, 2u)].x, prev_1);
-    if (m.x > kLeafCapacity) {
-      pushes = true;
-      my_level = 0u;
-      my_item = pack_item(blk.sys, local);
-      my_slot = stage_item(ps, 0u);
-    }
-  }
-  uint32_t my_item2 = 0u, my_slot2 = 0u, my_level2 = 0u;
-  bool pushes2 = false;
-  if (t >= 64u && t < 72u) {
+    local = lin(4u * rx + cx, 4u * ry + cy, 4u * rz + cz, 1u << (D - 1u));
+    lcell = level_off(D - 1u) + local;
+    lm = lvl1[t];
+    lparent = lvl2[lin(cx >> 1, cy >> 1, cz >> 1, 2u)].x;
+    lprev = prev_1;
+    my_level = 0u;
+    mine = true;
+  } else if (t < 72u) {
     const uint32_t u = t - 64u;
     const uint32_t cx = u & 1u, cy = (u >> 1) & 1u, cz = u >> 2;
-    const uint32_t local = lin(2u * rx + cx, 2u * ry + cy, 2u * rz + cz, 1u << (D - 2u));
-    const uint32_t cell = level_off(D - 2u) + local;
-    const int4 m = lvl2[u];
-    G(d.source)[cell] = source_record(m, d);
-    update_active(d, cell, m.x, lvl3[0].x, prev_2);
-    if (m.x > kLeafCapacity) {
-      pushes2 = true;
-      my_level2 = 1u;
-      my_item2 = pack_item(blk.sys, local);
-      my_slot2 = stage_item(ps, 1u);
-    }
+    local = lin(2u * rx + cx, 2u * ry + cy, 2u * rz + cz, 1u << (D - 2u));
+    lcell = level_off(D - 2u) + local;
+    lm = lvl2[u];
+    lparent = lvl3[0].x;
+    lprev = prev_2;
+    my_level = 1u;
+    mine = true;
+  } else if (t == 72u) {
+    local = blk.root;
+    lcell = level_off(r) + blk.root;
+    lm = lvl3[0];
+    my_level = 2u;
+    mine = true;
   }
-  if (t == 72u) {
-    const int4 m = lvl3[0];
-    G(d.source)[level_off(r) + blk.root] = source_record(m, d);
-    if (m.x > kLeafCapacity) {
-      pushes2 = true;
-      my_level2 = 2u;
-      my_item2 = pack_item(blk.sys, blk.root);
-      my_slot2 = stage_item(ps, 2u);
-    }
+  const bool pushes = mine && lm.x > kLeafCapacity;
+  if (pushes) {
+    my_item = pack_item(blk.sys, local);
+    my_slot = stage_item(ps, my_level);
   }
   __syncthreads();
   if (t == 0u) reserve_items(a, ps, lev_of, 3u, stripe);
+
+  {
+    const int4 m = lvl0[t];
+    G(d.source)[level_off(D) + fcell] = source_record(m, d);
+    update_active(d, level_off(D) + fcell, m.x, lvl1[lin(ix >> 1, iy >> 1, iz >> 1, 4u)].x, prev_f);
+  }
+  if (mine) {
+    G(d.source)[lcell] = source_record(lm, d);
+    // The root's flag (level r) is set by the pass above the subtrees.
+    if (my_level < 2u) update_active(d, lcell, lm.x, lparent, lprev);
+  }
   __syncthreads();
   if (pushes) write_item(a, ps, my_level, ps.base[my_level] + my_slot, my_item);
-  if (pushes2) write_item(a, ps, my_level2, ps.base[my_level2] + my_slot2, my_item2);
 }
 
 // The levels above the subtrees, one block per system: levels r-1..0 from
