@@ -1236,27 +1236,31 @@ struct Leaf {
 // (:640-642) and own-leaf monopole (:644-657).  phi is the matching
 // potential phi_c - a_c.d - d.(J d)/2 - G (count-1)/N / r_soft, which the
 // reference does not form (energy diagnostic only).
-template <bool kComp, bool kPhi>
+// kDepth: the system's depth when known at compile time (0: d.depth).
+template <bool kComp, bool kPhi, uint32_t kDepth = 0u>
 __device__ __forceinline__ Leaf leaf_field(const SysDesc& d, uint32_t parity, float px, float py, float pz) {
+  const uint32_t depth = kDepth != 0u ? kDepth : d.depth;
   const float ux = unit_coord(px), uy = unit_coord(py), uz = unit_coord(pz);
   Leaf out;
   // The cell of every level depends only on the position, so all counts are
   // fetched at once; the leaf is then the first level with count <= 16, or
   // the finest level.
-  const uint32_t fx = cell1(ux, 1u << d.depth), fy = cell1(uy, 1u << d.depth), fz = cell1(uz, 1u << d.depth);
+  const uint32_t fx = cell1(ux, 1u << depth), fy = cell1(uy, 1u << depth), fz = cell1(uz, 1u << depth);
+  const gptr<int4> finest = finest_of(d, parity);
   int lc[kLevels];
 #pragma unroll
   for (uint32_t level = 1u; level < kLevels; ++level) {
-    if (level <= d.depth) {
-      const uint32_t sh = d.depth - level;
-      lc[level] = cell_count(d, parity, level, lin(fx >> sh, fy >> sh, fz >> sh, 1u << level));
+    if (level <= depth) {
+      const uint32_t sh = depth - level;
+      const uint32_t idx = lin(fx >> sh, fy >> sh, fz >> sh, 1u << level);
+      lc[level] = level == depth ? finest[idx].x : G(d.moments)[level_off(level) + idx].x;
     }
   }
-  uint32_t chosen = d.depth;
+  uint32_t chosen = depth;
 #pragma unroll
   for (uint32_t level = kLevels - 1u; level >= 1u; --level)
-    if (level < d.depth && lc[level] <= kLeafCapacity) chosen = level;
-  const uint32_t sh = d.depth - chosen;
+    if (level < depth && lc[level] <= kLeafCapacity) chosen = level;
+  const uint32_t sh = depth - chosen;
   const uint32_t c[3] = {fx >> sh, fy >> sh, fz >> sh};
   out.dim = 1u << chosen;
   out.cell = level_off(chosen) + lin(c[0], c[1], c[2], out.dim);
@@ -1331,11 +1335,11 @@ __device__ __forceinline__ float length3(float x, float y, float z) {
 
 // One body's integrate (src/rainbow_fmm.wgsl:613-771): the new state
 // replaces p4 (pos, hue) and v4 (vel, excitation).
-template <bool kComp>
+template <bool kComp, uint32_t kDepth = 0u>
 __device__ __forceinline__ void integrate_body(const SysDesc& d, uint32_t parity, float4& p4, float4& v4) {
   float pos[3] = {p4.x, p4.y, p4.z};
   float velocity[3] = {v4.x, v4.y, v4.z};
-  const Leaf leaf = leaf_field<kComp, false>(d, parity, p4.x, p4.y, p4.z);
+  const Leaf leaf = leaf_field<kComp, false, kDepth>(d, parity, p4.x, p4.y, p4.z);
   float acceleration[3] = {leaf.ax, leaf.ay, leaf.az};
   const uint32_t count = leaf.count;
 
@@ -1464,7 +1468,14 @@ __global__ __launch_bounds__(kDepositThreads) void k_integrate_deposit(StepArgs 
           G(a.own)[blockIdx.x] = own_new;
         }
       }
-      integrate_body<kComp>(d, a.parity, p4, v4);
+      // Depth as a constant: the level loops of the leaf search unroll to
+      // fixed shifts and offsets.
+      switch (d.depth) {
+        case 4u: integrate_body<kComp, 4u>(d, a.parity, p4, v4); break;
+        case 5u: integrate_body<kComp, 5u>(d, a.parity, p4, v4); break;
+        case 6u: integrate_body<kComp, 6u>(d, a.parity, p4, v4); break;
+        default: integrate_body<kComp>(d, a.parity, p4, v4); break;
+      }
       G(d.pos)[i] = p4;
       G(d.vel)[i] = v4;
       p[k] = p4;
