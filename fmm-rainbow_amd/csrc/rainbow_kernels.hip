@@ -106,11 +106,18 @@ __device__ __forceinline__ float center1(uint32_t c, uint32_t dim) {
 // by its exact remainder: the clamped result equals the one from the
 // correctly rounded quotient for every float p (all 2^32 inputs are checked
 // on the host by the tests), at a third of a division's instructions.
+// x / d for a constant d, r = RN(1 / d): the product x * r corrected once by
+// its exact remainder (three instructions instead of a division).  Used only
+// where the value kept afterwards is shown equal to the correctly rounded
+// quotient's for every input.
+__device__ __forceinline__ float div_corrected(float x, float d, float r) {
+  const float q0 = x * r;
+  return __builtin_fmaf(__builtin_fmaf(-q0, d, x), r, q0);
+}
+
 __device__ __forceinline__ float unit_coord(float p) {
   constexpr float kInv = 0.625f;  // RN(1 / 1.6f)
-  const float q0 = p * kInv;
-  const float rem = __builtin_fmaf(-q0, kWorldHalfExtent, p);
-  float n = __builtin_fmaf(rem, kInv, q0);
+  float n = div_corrected(p, kWorldHalfExtent, kInv);
   n = fminf(fmaxf(n, -0.99999f), 0.99999f);
   return n * 0.5f + 0.5f;
 }
@@ -1386,8 +1393,13 @@ __device__ __forceinline__ void integrate_body(const SysDesc& d, uint32_t parity
       -1.0f, 1.0f);
   const float radial_fraction = fabsf(signed_radial_motion);
   const float radial_shock = radial_fraction * radial_fraction * radial_fraction;
-  const float density_pulse =
-      clampf((static_cast<float>(count) - static_cast<float>(kLeafCapacity)) / 72.0f, 0.0f, 1.0f);
+  // clamp((count - 16) / 72, 0, 1); the dividend clamped to [-1, 73] leaves
+  // the ramp unchanged and keeps the corrected product exact (every uint32
+  // count is checked on the host by the tests).
+  const float density_pulse = clampf(
+      div_corrected(clampf(static_cast<float>(count) - static_cast<float>(kLeafCapacity), -1.0f, 73.0f), 72.0f,
+                    1.0f / 72.0f),
+      0.0f, 1.0f);
   const float field_strength = clampf(
       length3(acceleration[0], acceleration[1], acceleration[2]) * (radius + 0.12f) * 0.58f, 0.0f, 1.0f);
   const float target_excitation =
@@ -1419,7 +1431,11 @@ __device__ __forceinline__ void integrate_body(const SysDesc& d, uint32_t parity
     }
   }
 
-  const float st = clampf((excitation - 0.30f) / (0.70f - 0.30f), 0.0f, 1.0f);
+  // clamp((e - 0.30) / (0.70 - 0.30), 0, 1); the dividend clamped to [-1, 1]
+  // leaves the ramp unchanged and keeps the corrected product exact (every
+  // float e is checked on the host by the tests).
+  constexpr float kBand = 0.70f - 0.30f;
+  const float st = clampf(div_corrected(clampf(excitation - 0.30f, -1.0f, 1.0f), kBand, 1.0f / kBand), 0.0f, 1.0f);
   const float hue_drift = 1.0f - st * st * (3.0f - 2.0f * st);
   const float h = p4.w + dt * hue_drift *
                              (0.012f + signed_radial_motion * 0.030f + excitation * 0.036f +

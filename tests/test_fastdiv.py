@@ -1,8 +1,10 @@
 """The integrate kernels form p / 1.6f (the position -> unit-cube map of
-position_cell, src/rainbow_fmm.wgsl:144-152) with a remainder-corrected
-multiplication instead of a division.  This compiles the host checker and runs
-it over all 2^32 float inputs: the clamped unit coordinate must equal the one
-from the correctly rounded quotient for every input.  The same checker shows
+position_cell, src/rainbow_fmm.wgsl:144-152) and the two clamped ramps
+(count - 16) / 72 (:703-705) and smoothstep(0.30, 0.70, excitation)'s
+(excitation - 0.30) / (0.70 - 0.30) (:758) with a
+remainder-corrected multiplication instead of a division.  This compiles the
+host checker and runs it over all 2^32 inputs of each: the value the kernel
+keeps must equal the one from the correctly rounded quotient for every input.  The same checker shows
 that the form is NOT exact for other constants (a plausible misuse), so the
 test also pins that it detects a difference."""
 
@@ -31,6 +33,15 @@ def test_unit_coord_division_exact_for_all_floats(checker):
     r = subprocess.run([str(checker), "1.6", "unit"], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "kept values differing 0" in r.stdout
+
+
+def test_integrate_ramps_exact_for_all_inputs(checker):
+    # The excitation ramp over every float excitation, the density ramp over
+    # every uint32 body count (divisors as the kernel forms them).
+    for d, mode in (("0.39999998", "excitation"), ("72", "density")):
+        r = subprocess.run([str(checker), d, mode], capture_output=True, text=True, timeout=600)
+        assert r.returncode == 0, r.stdout + r.stderr
+        assert "kept values differing 0" in r.stdout
 
 
 def test_checker_detects_inexact_constant(checker):
