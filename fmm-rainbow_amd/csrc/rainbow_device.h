@@ -114,6 +114,8 @@ constexpr uint32_t kDepositThreads = 1024;  // one body per thread
 constexpr uint32_t kDepositPerThread = 1;
 constexpr uint32_t kDepositBodies = kDepositThreads * kDepositPerThread;  // per block
 constexpr uint32_t kDepositSlots = 2048;  // LDS hash slots, 2x bodies per block
+constexpr uint32_t kDepositSlotBits = 31u - static_cast<uint32_t>(__builtin_clz(kDepositSlots));
+static_assert((1u << kDepositSlotBits) == kDepositSlots, "hash table size is a power of two");
 constexpr uint32_t kIntegrateThreads = 256;
 constexpr uint32_t kUpwardThreads = 512;  // one thread per finest cell of a subtree
 constexpr uint32_t kTopThreads = 1024;  // one block per system for the levels above the subtrees

@@ -248,7 +248,7 @@ __device__ void deposit_block(const StepArgs& a, const SysDesc& d, uint32_t bloc
     if (valid && tail) {
       // Open addressing; at most kDepositBodies distinct keys in a table of
       // 2 kDepositBodies slots, so an empty slot always exists.
-      h = (key * 2654435761u) >> (32 - 11);
+      h = (key * 2654435761u) >> (32u - kDepositSlotBits);
       for (;;) {
         const uint32_t prev = atomicCAS(&keys[h], kEmptyKey, key);
         if (prev == kEmptyKey || prev == key) break;
