@@ -1062,6 +1062,28 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t lo, 
   // Pair lane: target tt (child of the parent), source cc of the chunk.
   const uint32_t tt = lane >> 3, cc = lane & 7u;
   const uint32_t wtx = 2u + (tt & 1u), wty = 2u + ((tt >> 1) & 1u), wtz = 2u + (tt >> 2);
+  // Fixed per lane: the window offset of source child cc within a source
+  // parent's 2x2x2 block, the window index of target tt, and bit q set for
+  // the source parents q (pz, py, px order) whose child cc is adjacent to
+  // target tt (per axis: the middle parent always; the low one when the
+  // child is its high cell and the target the low cell of the middle, the
+  // high one conversely).
+  const uint32_t lane_src = (cc & 1u) + 6u * ((cc >> 1) & 1u) + 36u * (cc >> 2);
+  const uint32_t lane_tgt = wtx + 6u * (wty + 6u * wtz);
+  uint32_t near_q = 0u;
+  {
+    auto adjacent = [](uint32_t c, uint32_t w) -> uint32_t {
+      return 2u | (c == 1u && w == 2u ? 1u : 0u) | (c == 0u && w == 3u ? 4u : 0u);
+    };
+    const uint32_t mx = adjacent(cc & 1u, wtx), my = adjacent((cc >> 1) & 1u, wty), mz = adjacent(cc >> 2, wtz);
+    // Outer product of the axis masks: the x mask repeated at bit 3 qy for
+    // each qy in my, that row repeated at bit 9 qz for each qz in mz (the
+    // copies do not overlap, so the products carry nothing).
+    const uint32_t ys = (my & 1u) | ((my & 2u) << 2) | ((my & 4u) << 4);
+    const uint32_t zs = (mz & 1u) | ((mz & 2u) << 8) | ((mz & 4u) << 16);
+    near_q = (mx * ys) * zs;
+  }
+  const uint32_t wave_s = __builtin_amdgcn_readfirstlane(wave);
   // Chain threads 0..79: target tid / 10, component tid % 10.
   const bool chain = tid < 80u;
   const uint32_t ct = tid / 10u, ck = tid % 10u;
@@ -1146,30 +1168,28 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t lo, 
     }
     __syncthreads();
 
+#pragma unroll 1
     for (uint32_t r0 = 0; r0 < 32u; r0 += kChunksPerRound) {
       // Each wave evaluates two chunks' 8 x 8 pair fields.
 #pragma unroll
       for (uint32_t j = 0; j < kChunksPerRound / kWaves; ++j) {
-        const uint32_t slot = wave * (kChunksPerRound / kWaves) + j;
-        const uint32_t q = r0 + slot;
+        const uint32_t slot = wave_s * (kChunksPerRound / kWaves) + j;
+        const uint32_t q = r0 + slot;  // uniform
         // This lane's source in the window and whether its pair counts.
         uint32_t widx = 0u;
         bool active = false;
         float eps2 = eps2_m2l;
         if (q < 27u) {
-          const uint32_t wsx = 2u * (q % 3u) + (cc & 1u);
-          const uint32_t wsy = 2u * ((q / 3u) % 3u) + ((cc >> 1) & 1u);
-          const uint32_t wsz = 2u * (q / 9u) + (cc >> 2);
-          const bool near = (wsx + 1u - wtx) <= 2u && (wsy + 1u - wty) <= 2u && (wsz + 1u - wtz) <= 2u;
-          widx = wsx + 6u * (wsy + 6u * wsz);
-          active = tcount != 0 && !near;
+          // Source parent q's block starts at (2 qx, 2 qy, 2 qz).
+          widx = (2u * (q % 3u) + 12u * ((q / 3u) % 3u) + 72u * (q / 9u)) + lane_src;
+          active = tcount != 0 && ((near_q >> q) & 1u) == 0u;
         } else {
           // Near field of leaf targets (near_all, src/rainbow_fmm.wgsl:
           // 533-571): offsets dz, dy, dx; the centre is skipped; neighbours
           // outside the level were staged as empty.
           const uint32_t idx = (q - 27u) * 8u + cc;
           if (idx < 27u) {
-            widx = (wtx + idx % 3u - 1u) + 6u * ((wty + (idx / 3u) % 3u - 1u) + 6u * (wtz + idx / 9u - 1u));
+            widx = lane_tgt + (idx % 3u + 6u * ((idx / 3u) % 3u) + 36u * (idx / 9u)) - 43u;
             active = leaf && idx != 13u;
           }
           eps2 = eps2_near;
