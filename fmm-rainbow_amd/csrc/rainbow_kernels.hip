@@ -1004,8 +1004,10 @@ __device__ void ancestor_fields(const SysDesc& d, uint32_t lo, uint32_t level, u
 
 // kFused: the launch covers several levels (lo < hi); the single-level
 // variant compiles without the ancestor path and keeps fewer registers.
+// Occupancy: 6 waves per SIMD for single-level launches (their LDS allows 6
+// blocks per CU), 5 for fused ones.
 template <bool kComp, bool kFused>
-__global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t lo, uint32_t hi) {
+__global__ __launch_bounds__(kTreeThreads, kFused ? 5 : 6) void k_tree(StepArgs a, uint32_t lo, uint32_t hi) {
   constexpr uint32_t kWaves = kTreeThreads / 64u;
   constexpr uint32_t kChunksPerRound = kTreeChunksPerRound;
   // Single level: static LDS.  Fused: one dynamic region of
@@ -1194,14 +1196,21 @@ __global__ __launch_bounds__(kTreeThreads) void k_tree(StepArgs a, uint32_t lo, 
           }
           eps2 = eps2_near;
         }
-        float t[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-        if (q < 27u || any_leaf) {
+        // Chunk 13 (the parent's own block: every pair adjacent), chunk 31
+        // and, without a leaf target, the near chunks hold no pair.
+        const bool eval = q < 27u ? q != 13u : (any_leaf && q < 31u);
+        float t[10];
+        if (eval && fast) {
+          // Every lane evaluates; lanes without a pair see a zero mass.
           float4 sr = win[widx];
-          if (fast) {
-            if (!active) sr.w = 0.0f;
-            if (__any(sr.w != 0.0f)) pair_terms<true>(sr, tcx, tcy, tcz, eps2, t);
-          } else if (active && sr.w != 0.0f) {
-            pair_terms<false>(sr, tcx, tcy, tcz, eps2, t);
+          if (!active) sr.w = 0.0f;
+          pair_terms<true>(sr, tcx, tcy, tcz, eps2, t);
+        } else {
+#pragma unroll
+          for (int k = 0; k < 10; ++k) t[k] = 0.0f;
+          if (eval) {
+            const float4 sr = win[widx];
+            if (active && sr.w != 0.0f) pair_terms<false>(sr, tcx, tcy, tcz, eps2, t);
           }
         }
         // Terms of chunk `slot` stored per chain (target tt, component k)
