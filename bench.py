@@ -168,10 +168,11 @@ def batch_form(args):
         s.step(args.warmup)
         s.sync()
         solvers.append(s)
+    # One after another: each solver's steps complete before the next
+    # solver starts (separate streams would otherwise overlap them).
     t0 = time.perf_counter()
     for s in solvers:
         s.step(args.steps)
-    for s in solvers:
         s.sync()
     seq_s = time.perf_counter() - t0
     for s in solvers:

@@ -1784,6 +1784,10 @@ static uint32_t tree_resident_blocks(uint32_t span, uint32_t lds) {
   return cache[key];
 }
 
+uint32_t tree_single_level_capacity(bool compensated) {
+  return compensated ? tree_resident_blocks<true, false>(0u, 0u) : tree_resident_blocks<false, false>(0u, 0u);
+}
+
 hipError_t launch_tree(const StepArgs& a, uint32_t lo, uint32_t hi, uint32_t nblocks, hipStream_t stream,
                        const Timing* t) {
   if (nblocks == 0u || lo < 2u || hi < lo || hi >= kLevels) return no_launch(stream, t);

@@ -18,6 +18,8 @@ hipError_t launch_deposit(const StepArgs& a, uint32_t nblocks, hipStream_t strea
 hipError_t launch_upward(const StepArgs& a, uint32_t nblocks, hipStream_t stream, const Timing* t = nullptr);
 hipError_t launch_top(const StepArgs& a, hipStream_t stream, const Timing* t = nullptr);
 // Tree pass of levels lo..hi (2 <= lo <= hi <= 6) in one launch.
+// Blocks of a single-level tree launch resident on the device at once.
+uint32_t tree_single_level_capacity(bool compensated);
 hipError_t launch_tree(const StepArgs& a, uint32_t lo, uint32_t hi, uint32_t nblocks, hipStream_t stream,
                        const Timing* t = nullptr);
 // Integrate plus the next step's deposit (into finest buffer parity ^ 1),

@@ -92,7 +92,8 @@ class Engine {
   // work tables and drops captured graphs.  Waits for the stream first.
   int configure(const std::vector<SysDesc>& systems);
   int push_descriptors();
-  int enqueue(bool integrate, bool sorting, uint32_t step_parity, bool deposit_first, bool rank_next);
+  int enqueue(bool integrate, bool sorting, uint32_t step_parity, bool deposit_first, bool rank_next,
+              bool record_counts = false);
   hipError_t zero_finest(uint32_t parity);
   void plan_next(bool* sorting, bool* rank_next, uint32_t* since_after) const;
   int solve_fields();
@@ -108,7 +109,7 @@ class Engine {
   int direct_sum_f64(uint32_t sys, const uint32_t* targets, uint32_t nt, double* out);
   int set_sort_interval(uint32_t interval);
   int set_tree_fusion(uint32_t hi);
-  int set_tree_split(uint32_t starts);
+  int set_tree_split(uint32_t starts);  // fixes the split (no measured choice)
   int autotune(uint32_t steps, uint32_t* chosen);
   int snapshot(bool restore);
 
@@ -124,6 +125,10 @@ class Engine {
   uint32_t tree_blocks[kLevels + 1] = {};
   std::vector<TreeLaunch> tree_launches;
   uint32_t tree_split = kDefaultTreeSplit;
+  // Until a split is set explicitly, the first eager step after a
+  // configuration records its per-level internal-cell counts and the split
+  // is chosen from them (adapt_split).
+  bool split_auto = true;
   // Finest-level buffer the next solve deposits into; the last solve used
   // parity ^ 1.
   uint32_t parity = 0;
@@ -133,6 +138,8 @@ class Engine {
   int prepare_graphs(uint32_t steps, uint32_t start_parity);
   uint32_t chunk_steps() const;
   void plan_tree();
+  int apply_tree_split(uint32_t starts);
+  int adapt_split(bool wait);
   void drop_graphs();
 
   DevBuf sys_buf, deposit_buf, integrate_buf, subtree_buf, list_buf, count_buf,
@@ -152,6 +159,11 @@ class Engine {
   bool pending_deposit = true;
   hipEvent_t ev_a = nullptr;
   hipEvent_t ev_b = nullptr;
+  bool split_wanted = false;   // a measured split choice is due
+  bool counts_pending = false;  // counts_host is being filled (ev_counts)
+  uint32_t* counts_host = nullptr;  // pinned copy of the list counts
+  hipEvent_t ev_counts = nullptr;
+  std::vector<uint32_t> caps_host;  // list capacity per level and stripe
 };
 
 // A fixed set of systems (the solver is a batch of one).

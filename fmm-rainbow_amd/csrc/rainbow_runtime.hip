@@ -185,8 +185,9 @@ Engine::~Engine() {
                     &stripe_buf, &diag_buf, &body_buf, &snap_buf, &own_buf, &rank_buf, &cstart_buf, &sstart_buf,
                     &io_buf})
     b->release();
-  for (hipEvent_t e : {ev_a, ev_b})
+  for (hipEvent_t e : {ev_a, ev_b, ev_counts})
     if (e) (void)hipEventDestroy(e);
+  if (counts_host) (void)hipHostFree(counts_host);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -202,6 +203,9 @@ int Engine::open(uint32_t opts) {
   RB_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   RB_HIP(hipEventCreate(&ev_a));
   RB_HIP(hipEventCreate(&ev_b));
+  RB_HIP(hipEventCreateWithFlags(&ev_counts, hipEventDisableTiming));
+  RB_HIP(hipHostMalloc(reinterpret_cast<void**>(&counts_host), sizeof(uint32_t) * kLevels * kStripes,
+                       hipHostMallocDefault));
   return RAINBOW_OK;
 }
 
@@ -267,6 +271,9 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
     stripes[kLevels * kStripes + i] = caps[i];
     list_total += caps[i];
   }
+  caps_host = caps;
+  counts_pending = false;
+  split_wanted = split_auto;
   int rc;
   if ((rc = sys_buf.ensure(sizeof(SysDesc) * nsys)) ||
       (rc = deposit_buf.ensure(sizeof(BodyBlock) * n_deposit)) ||
@@ -362,12 +369,49 @@ void Engine::plan_tree() {
 
 int Engine::set_tree_split(uint32_t starts) {
   if (starts & ~0x7Cu) return fail(RAINBOW_E_INVALID, "tree split bits must be within levels 2..6");
+  split_auto = false;
+  split_wanted = false;
+  return apply_tree_split(starts);
+}
+
+int Engine::apply_tree_split(uint32_t starts) {
   starts |= 1u << 2;
   RB_HIP(hipStreamSynchronize(stream));
   if (starts != tree_split) drop_graphs();
   tree_split = starts;
   plan_tree();
   return RAINBOW_OK;
+}
+
+// The measured split: a level starts a launch of its own when its internal
+// parents fill two thirds or more of the resident blocks of a single-level
+// launch (then a fused launch would recompute ancestors for many parents);
+// the levels with fewer parents join the launch below them, where one
+// launch's latency serves several levels.  (Measured on the 1M-body Plummer
+// step: 1343 level-6 parents for 1536 resident blocks run best alone, 603
+// level-5 parents best fused.)  Without `wait`, returns when the counts of
+// the recording step are not in yet.
+int Engine::adapt_split(bool wait) {
+  if (!counts_pending) return RAINBOW_OK;
+  if (!wait) {
+    const hipError_t q = hipEventQuery(ev_counts);
+    if (q == hipErrorNotReady) return RAINBOW_OK;
+    if (q != hipSuccess) return fail(RAINBOW_E_HIP, "split counts: %s", hipGetErrorString(q));
+  }
+  RB_HIP(hipEventSynchronize(ev_counts));
+  counts_pending = false;
+  split_wanted = false;
+  const uint32_t cap = 2u * tree_single_level_capacity((options & RAINBOW_OPT_COMPENSATED) != 0u) / 3u;
+  uint32_t starts = 1u << 2;
+  for (uint32_t level = 3u; level <= dmax && level < kLevels; ++level) {
+    uint64_t items = 0;  // internal parents (level - 1) of this level's targets
+    for (uint32_t j = 0; j < kStripes; ++j) {
+      const uint32_t e = (level - 1u) * kStripes + j;
+      items += std::min(counts_host[e], caps_host.empty() ? 0u : caps_host[e]);
+    }
+    if (items >= cap) starts |= 1u << level;
+  }
+  return apply_tree_split(starts);
 }
 
 // Levels 2..hi in one launch, every deeper level alone.
@@ -388,7 +432,8 @@ int Engine::set_tree_fusion(uint32_t hi) {
 // overlap with the tree pass gains); the step's integrate reads the sorted
 // bodies from the staging arrays and sets the blocks' owned ranges.
 // `rank_next`: the fused deposit writes ranks for a sorting next step.
-int Engine::enqueue(bool integrate, bool sorting, uint32_t step_parity, bool deposit_first, bool rank_next) {
+int Engine::enqueue(bool integrate, bool sorting, uint32_t step_parity, bool deposit_first, bool rank_next,
+                    bool record_counts) {
   if (sys.empty()) return RAINBOW_OK;
   // The sorted bodies reach the body arrays only through integrate.
   sorting = sorting && integrate;
@@ -404,6 +449,13 @@ int Engine::enqueue(bool integrate, bool sorting, uint32_t step_parity, bool dep
   RB_HIP(launch_top(a, stream));
   if (sorting) RB_HIP(launch_sort(a, n_integrate, stream));
   for (const TreeLaunch& t : tree_launches) RB_HIP(launch_tree(a, t.lo, t.hi, t.blocks, stream));
+  if (record_counts) {
+    // The step's internal-cell counts, for the measured tree split (the
+    // integrate kernel clears them for the next step).
+    RB_HIP(hipMemcpyAsync(counts_host, count_buf.p, sizeof(uint32_t) * kLevels * kStripes, hipMemcpyDeviceToHost,
+                          stream));
+    RB_HIP(hipEventRecord(ev_counts, stream));
+  }
   if (integrate) RB_HIP(launch_integrate_deposit(a, n_deposit, stream));
   return RAINBOW_OK;
 }
@@ -491,6 +543,7 @@ int Engine::step(uint32_t steps) {
         use_chunk = since_sort + chunk < sort_interval;  // no sort within or right after
     }
     if (use_chunk) {
+      if ((rc = adapt_split(true)) != 0) return rc;
       if ((rc = prepare_graphs(steps - done, p)) != 0) return rc;
       RB_HIP(hipGraphLaunch(graph_chunk[p], stream));
       done += chunk;
@@ -511,12 +564,16 @@ int Engine::step(uint32_t steps) {
 // of bodies or schedule, steps between chunks, and a set of systems that
 // changes too often to amortise a capture).
 int Engine::step_eager(uint32_t steps) {
+  int rc = adapt_split(false);
+  if (rc) return rc;
   for (uint32_t k = 0; k < steps; ++k) {
     bool sorting, rank_next;
     uint32_t after;
     plan_next(&sorting, &rank_next, &after);
-    int rc = enqueue(true, sorting, parity, pending_deposit, rank_next);
+    const bool record = split_wanted && !counts_pending;
+    rc = enqueue(true, sorting, parity, pending_deposit, rank_next, record);
     if (rc) return rc;
+    if (record) counts_pending = true;
     parity ^= 1u;
     since_sort = after;
     pending_deposit = false;
@@ -620,7 +677,9 @@ int Engine::sync() {
 
 int Engine::time_steps(uint32_t steps, float* ms) {
   int rc;
-  // Graphs of both parities are captured before the timed region.
+  // The split is settled and graphs of both parities are captured before
+  // the timed region.
+  if ((rc = adapt_split(true)) != 0) return rc;
   if ((rc = prepare_graphs(steps, 0u)) != 0 || (rc = prepare_graphs(steps, 1u)) != 0) return rc;
   RB_HIP(hipStreamSynchronize(stream));
   RB_HIP(hipEventRecord(ev_a, stream));

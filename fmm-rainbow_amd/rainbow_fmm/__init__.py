@@ -526,6 +526,9 @@ class Batch:
     def set_tree_fusion(self, hi: int):
         _check(lib().rainbow_batch_set_tree_fusion(self._h, hi))
 
+    def set_tree_split(self, starts: int):
+        _check(lib().rainbow_batch_set_tree_split(self._h, starts))
+
     def autotune(self, steps: int = 32) -> int:
         v = _u32()
         _check(lib().rainbow_batch_autotune(self._h, steps, ctypes.byref(v)))

@@ -249,9 +249,14 @@ int rainbow_hip_set_sort_interval(rainbow_hip_t* h, uint32_t interval);
  * of a launch over levels lo..hi recomputes the fields of its item's
  * ancestors down to level lo instead of waiting for another launch.
  * rainbow_hip_set_tree_split: bit l of `starts` (2 <= l <= 6) starts a
- * launch at level l (bit 2 is implied; default: bits 2 and 6, i.e. levels
- * 2..5 and 6).  rainbow_hip_set_tree_fusion(h, hi): levels 2..hi together,
- * each deeper level alone.  Results do not depend on the split. */
+ * launch at level l (bit 2 is implied).  rainbow_hip_set_tree_fusion(h,
+ * hi): levels 2..hi together, each deeper level alone.  Until one of them
+ * (or autotune) is called, the split is measured: the first step after a
+ * configuration records how many internal parents each level has, and a
+ * level starts its own launch when they fill two thirds of the device's
+ * resident blocks (a 1M-body Plummer system: levels 2..5 and 6; 256
+ * systems of up to 262144 bodies: one launch per level).  Results do not depend on the
+ * split. */
 int rainbow_hip_set_tree_split(rainbow_hip_t* h, uint32_t starts);
 int rainbow_hip_set_tree_fusion(rainbow_hip_t* h, uint32_t hi);
 

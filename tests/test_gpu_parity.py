@@ -419,6 +419,28 @@ def test_tree_split_does_not_change_results(starts, cfg2_reference):
     s.close()
 
 
+def test_measured_split(cfg2_reference):
+    # No split set: the first step records the per-level parent counts and
+    # the split follows from them (1343 level-6 parents here: levels 2..5
+    # fused, level 6 alone); results are the same as with any fixed split.
+    say("measured tree split, 1M Plummer")
+    parts, om, of, op = cfg2_reference
+    c = wl.CFG2
+    s = rf.Solver(c["n"], wl.params_for(c))
+    s.upload(parts)
+    s.step(2)
+    assert_bits(s.particles(), op, "particles")
+    s.step(16)
+    assert s.tree_launches() == [(2, 5), (6, 6)]
+    t = rf.Solver(c["n"], wl.params_for(c))
+    t.set_tree_fusion(6)
+    t.upload(parts)
+    t.step(18)
+    assert s.digest(0) == t.digest(0)
+    s.close()
+    t.close()
+
+
 @pytest.mark.parametrize("compensated", [False, True])
 def test_tree_fusion_ball_all_levels(compensated):
     say(f"tree fusion 2..6, uniform ball 400k, compensated={compensated}")
