@@ -43,13 +43,18 @@ __device__ __forceinline__ float recip_newton(float s) {
   return __builtin_fmaf(__builtin_fmaf(-s, r, 1.0f), r, r);
 }
 
+// The full expansions for the lanes outside the domain, kept out of line so
+// that the compiler cannot evaluate them speculatively for every lane.
+__device__ __attribute__((noinline)) float sqrt_slow(float x) { return sqrtf(x); }
+__device__ __attribute__((noinline)) float rsqrt_slow(float x) { return 1.0f / sqrtf(x); }
+
 // Equals sqrtf(x) for every float x.
 __device__ __forceinline__ float sqrt_rn(float x) {
   float y;
   float s = sqrt_newton(x, y);
   const bool ok = fast_root_domain(x);
   if (!__all(ok)) {
-    if (!ok) s = sqrtf(x);
+    if (!ok) s = sqrt_slow(x);
   }
   return s;
 }
@@ -60,7 +65,7 @@ __device__ __forceinline__ float rsqrt_rn(float x) {
   float q = recip_newton(sqrt_newton(x, y));
   const bool ok = fast_root_domain(x);
   if (!__all(ok)) {
-    if (!ok) q = 1.0f / sqrtf(x);
+    if (!ok) q = rsqrt_slow(x);
   }
   return q;
 }
