@@ -65,18 +65,44 @@ def algorithmic_bytes(n, depth, stats):
     for lo in range(2, depth + 1):
         for hi in range(lo + 1, depth + 1):
             out[f"tree_l{lo}-{hi}"] = sum(out[f"tree_l{l}"] for l in range(lo, hi + 1))
+    # Two-launch form: the M2L launch reads the target moments and writes
+    # the sums (48 B) of levels lo..hi < depth; the finish launch reads
+    # those sums back, the target moments and the parents' ancestor sums,
+    # and writes every level's fields.
+    for lo, hi in tree_spans(depth):
+        m2l_bytes = sum(16 * stats[l] + 48 * stats[l] for l in range(lo, hi + 1) if l < depth)
+        out[f"tree_m2l_{span_name(lo, hi)}"] = m2l_bytes
+        out[f"tree_finish_{span_name(lo, hi)}"] = sum(
+            out[f"tree_l{l}"] + (48 * stats[l] if l < depth else 0) for l in range(lo, hi + 1))
     out["integrate"] = 64 * n + 64 * leaves + 16 * cd
     # sort: read the bodies and their ranks, write the staged bodies, copy back.
     out["sort"] = 36 * n + 4 * n + 36 * n + 72 * n
     return out
 
 
+def tree_spans(depth):
+    """Every level span lo..hi of a tree launch."""
+    return [(lo, hi) for lo in range(2, depth + 1) for hi in range(lo, depth + 1)]
+
+
+def span_name(lo, hi):
+    return f"l{lo}" if lo == hi else f"l{lo}-{hi}"
+
+
 def algorithmic_flops(depth, m2l, near):
-    """FP32 operations of the tree launches: non-empty pairs x FLOPS_PER_PAIR."""
+    """FP32 operations of the tree launches: non-empty pairs x FLOPS_PER_PAIR.
+
+    Two-launch form: the M2L launch evaluates the M2L pairs above the finest
+    level, the finish launch the finest level's M2L pairs and every level's
+    near pairs."""
     out = {f"tree_l{l}": FLOPS_PER_PAIR * (m2l[l] + near[l]) for l in range(2, depth + 1)}
     for lo in range(2, depth + 1):
         for hi in range(lo + 1, depth + 1):
             out[f"tree_l{lo}-{hi}"] = sum(out[f"tree_l{l}"] for l in range(lo, hi + 1))
+    for lo, hi in tree_spans(depth):
+        out[f"tree_m2l_{span_name(lo, hi)}"] = FLOPS_PER_PAIR * sum(m2l[l] for l in range(lo, hi + 1) if l < depth)
+        out[f"tree_finish_{span_name(lo, hi)}"] = FLOPS_PER_PAIR * sum(
+            near[l] + (m2l[l] if l == depth else 0) for l in range(lo, hi + 1))
     return out
 
 

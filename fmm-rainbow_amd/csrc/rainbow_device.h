@@ -16,6 +16,9 @@
 //                       between the moments array's finest level (parity 0)
 //                       and this one (parity 1); a step's upward zeroes the
 //                       buffer the next step deposits into
+//   m2l     float4[3 level_offset(D)]  M2L sums of the cells above the
+//                       finest level, written by the tree pass's first
+//                       launch and read by its second
 // with the reference's cell numbering level_offset(l) + x + 2^l (y + 2^l z).
 // A step works on a set of systems (a plain solve is a set of one); each
 // system's descriptor carries its own buffer pointers, so systems can join
@@ -38,6 +41,7 @@ struct alignas(16) SysDesc {
   uint8_t* active;
   int4* extra;
   int4* finest_alt;
+  float4* m2l;         // M2L sums of the levels below the finest, fields layout
   uint32_t n;          // bodies (particle_count)
   uint32_t depth;      // octree depth
   uint32_t body_base;  // first index of the system in the batch's body space

@@ -1002,12 +1002,80 @@ __device__ void ancestor_fields(const SysDesc& d, uint32_t lo, uint32_t level, u
   }
 }
 
-// kFused: the launch covers several levels (lo < hi); the single-level
-// variant compiles without the ancestor path and keeps fewer registers.
-// Occupancy: 6 waves per SIMD for single-level launches (their LDS allows 6
-// blocks per CU), 5 for fused ones.
-template <bool kComp, bool kFused>
-__global__ __launch_bounds__(kTreeThreads, kFused ? 5 : 6) void k_tree(StepArgs a, uint32_t lo, uint32_t hi) {
+// Forms of the tree kernel:
+//   kTreeSingle  one level; the parent's field from memory.
+//   kTreeFused   levels lo..hi; the parent's field recomputed along the
+//                ancestor path (ancestor_fields) when its level is >= lo.
+//   kTreeM2l     levels lo..hi, items below their system's finest level
+//                only: each target's M2L sum, into the system's m2l buffer.
+//   kTreeFinish  levels lo..hi after a kTreeM2l launch over the levels below
+//                the finest: the parent's field from the level-1 field and
+//                the ancestors' M2L sums (L2L additions in order, one wave),
+//                then per target its M2L sum (evaluated here at the finest
+//                level, read from the m2l buffer above it), + the L2L term,
+//                + the near field of leaves.  The M2L sums do not depend on
+//                the parents' fields, so the first launch has no order
+//                between its items and nothing is recomputed.
+constexpr uint32_t kTreeSingle = 0u, kTreeFused = 1u, kTreeM2l = 2u, kTreeFinish = 3u;
+
+// The L2L additions of the ancestor path from the M2L sums (kTreeFinish):
+// F(A_1) from memory, F(A_j) = M2L(A_j) + L2L term of F(A_{j-1}) for j =
+// 2..level-1, the same operations as ancestor_fields.  Run by one whole
+// wave, lane k < 10 on component k: path_loads issues the loads (before the
+// item's other loads, so that their latencies overlap), path_field adds.
+struct PathSums {
+  float f;                // component of F(A_1)
+  float m[kLevels - 3u];  // components of the M2L sums of A_2 .. A_5
+};
+
+__device__ __forceinline__ PathSums path_loads(const SysDesc& d, uint32_t level, uint32_t px, uint32_t py, uint32_t pz,
+                                               uint32_t lane) {
+  PathSums p;
+  const uint32_t k = lane < 10u ? lane : 9u;
+  const uint32_t up = level - 2u;  // P >> up = A_1
+  p.f = reinterpret_cast<gptr<float>>(G(d.fields) + 3u * (level_off(1u) + lin(px >> up, py >> up, pz >> up, 2u)))[k];
+#pragma unroll
+  for (uint32_t j = 2u; j < kLevels - 1u; ++j) {
+    p.m[j - 2u] = 0.0f;
+    if (j < level) {
+      const uint32_t sh = level - 1u - j;
+      p.m[j - 2u] =
+          reinterpret_cast<gptr<float>>(G(d.m2l) + 3u * (level_off(j) + lin(px >> sh, py >> sh, pz >> sh, 1u << j)))[k];
+    }
+  }
+  return p;
+}
+
+// Component `lane` (< 10) of the parent's field.
+__device__ __forceinline__ float path_field(const PathSums& p, uint32_t level, uint32_t px, uint32_t py, uint32_t pz,
+                                            uint32_t lane) {
+  const uint32_t k = lane < 10u ? lane : 9u;
+  float f = p.f;
+  auto at = [&](int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(f), l)); };
+#pragma unroll
+  for (uint32_t j = 2u; j < kLevels - 1u; ++j) {
+    if (j < level) {
+      const float4 pa = make_float4(at(0), at(1), at(2), at(3));
+      const float4 pj0 = make_float4(at(4), at(5), at(6), at(7));
+      const float4 pj1 = make_float4(at(8), at(9), 0.0f, 0.0f);
+      const uint32_t sh = level - 1u - j;
+      const uint32_t dim = 1u << j;
+      const uint32_t cx = px >> sh, cy = py >> sh, cz = pz >> sh;
+      const float dx = center1(cx, dim) - center1(cx >> 1, dim >> 1);
+      const float dy = center1(cy, dim) - center1(cy >> 1, dim >> 1);
+      const float dz = center1(cz, dim) - center1(cz >> 1, dim >> 1);
+      f = p.m[j - 2u] + l2l_term(k, pa, pj0, pj1, dx, dy, dz);
+    }
+  }
+  return f;
+}
+
+// Occupancy: 6 waves per SIMD for the unfused forms (their LDS allows 6
+// blocks per CU), 5 for fused launches, which carry the ancestor path.
+template <bool kComp, uint32_t kKind>
+__global__ __launch_bounds__(kTreeThreads, kKind == kTreeFused ? 5 : 6) void k_tree(StepArgs a, uint32_t lo,
+                                                                                    uint32_t hi) {
+  constexpr bool kFused = kKind == kTreeFused;
   constexpr uint32_t kWaves = kTreeThreads / 64u;
   constexpr uint32_t kChunksPerRound = kTreeChunksPerRound;
   // Single level: static LDS.  Fused: one dynamic region of
@@ -1030,7 +1098,7 @@ __global__ __launch_bounds__(kTreeThreads, kFused ? 5 : 6) void k_tree(StepArgs 
   // an item costs LDS reads only.
   __shared__ uint32_t tab_incl[kLevels][kStripes];
   __shared__ uint32_t tab_off[kLevels][kStripes];
-  if (!kFused) lo = hi;
+  if (kKind == kTreeSingle) lo = hi;
   if (tid < (hi - lo + 1u) * kStripes) {
     const uint32_t level = lo + tid / kStripes, j = tid % kStripes, e = (level - 1u) * kStripes + j;
     uint32_t incl = min(G(a.list_count)[e], G(a.stripe_cap)[e]);
@@ -1107,6 +1175,10 @@ __global__ __launch_bounds__(kTreeThreads, kFused ? 5 : 6) void k_tree(StepArgs 
       d = a.sys[s];
       cur_sys = s;
     }
+    // The M2L launch leaves each system's finest level to the finish launch;
+    // there a target's M2L sum is evaluated at the finest level only.
+    if (kKind == kTreeM2l && level >= d.depth) continue;
+    const bool full = kKind != kTreeFinish || level == d.depth;
     const uint32_t dim = 1u << level;
     const uint32_t pdim = dim >> 1u;
     const uint32_t off = level_off(level);
@@ -1114,13 +1186,22 @@ __global__ __launch_bounds__(kTreeThreads, kFused ? 5 : 6) void k_tree(StepArgs 
     unlin(pidx, level - 1u, px, py, pz);
 
     // The parent's field: from memory when its level is below lo, else
-    // recomputed along the chain of ancestors from level lo - 1.
+    // recomputed along the chain of ancestors from level lo - 1 (fused), or
+    // from the ancestors' M2L sums (finish; pf is read after the barrier).
     const bool recompute = kFused && level - 1u >= lo;
     if (kFused && recompute) ancestor_fields<kComp>(d, lo, level, px, py, pz, reinterpret_cast<unsigned char*>(dyn), pf, tid);
-
-    const bool bounded = stage_window(win, d, level, px, py, pz, tid);
+    PathSums path{};
+    if (kKind == kTreeFinish && wave_s == 0u) path = path_loads(d, level, px, py, pz, lane);
+    // Above the finest level a finish chain starts from its target's M2L
+    // sum (an empty target's slot is stale and its field is not stored).
+    float m2l_sum = 0.0f;
+    if (kKind == kTreeFinish && !full && chain) {
+      const uint32_t x = 2u * px + (ct & 1u), y = 2u * py + ((ct >> 1) & 1u), z = 2u * pz + (ct >> 2);
+      m2l_sum = reinterpret_cast<gptr<float>>(G(d.m2l) + 3u * (off + lin(x, y, z, dim)))[ck];
+    }
     const uint32_t tx = 2u * px + (tt & 1u), ty = 2u * py + ((tt >> 1) & 1u), tz = 2u * pz + (tt >> 2);
     const int tcount = cell_count(d, a.parity, level, lin(tx, ty, tz, dim));
+    const bool bounded = stage_window(win, d, level, px, py, pz, tid);
     const float tcx = center1(tx, dim), tcy = center1(ty, dim), tcz = center1(tz, dim);
     const bool leaf = tcount != 0 && (tcount <= kLeafCapacity || level == d.depth);
     // 32 chunks of 8 sources: the 27 source-parent blocks in pz, py, px
@@ -1130,6 +1211,10 @@ __global__ __launch_bounds__(kTreeThreads, kFused ? 5 : 6) void k_tree(StepArgs 
     // never becomes -0, so adding +0 or -0 is the identity), so every item
     // walks all 32.  The waves' votes (any leaf target, any unbounded
     // source) are combined after the barrier.
+    if (kKind == kTreeFinish && wave_s == 0u) {
+      const float f = path_field(path, level, px, py, pz, lane);
+      if (lane < 10u) pf[lane] = f;
+    }
     const uint32_t vote = (__any(leaf) ? 1u : 0u) | (__any(!bounded) ? 2u : 0u);
     if (lane == 0u) votes[wave] = vote;
     __syncthreads();
@@ -1152,11 +1237,11 @@ __global__ __launch_bounds__(kTreeThreads, kFused ? 5 : 6) void k_tree(StepArgs 
 
     Chain<kComp> ch;
     float inh = 0.0f;
-    if (chain) {
+    if (kKind != kTreeM2l && chain) {
       // L2L: this chain's component of the parent's field shifted to its
       // target's centre.
       float4 pa, pj0, pj1;
-      if (kFused && recompute) {
+      if ((kFused && recompute) || kKind == kTreeFinish) {
         pa = make_float4(pf[0], pf[1], pf[2], pf[3]);
         pj0 = make_float4(pf[4], pf[5], pf[6], pf[7]);
         pj1 = make_float4(pf[8], pf[9], 0.0f, 0.0f);
@@ -1168,10 +1253,13 @@ __global__ __launch_bounds__(kTreeThreads, kFused ? 5 : 6) void k_tree(StepArgs 
       const float dz = center1(2u * pz + (ct >> 2), dim) - center1(pz, pdim);
       inh = l2l_term(ck, pa, pj0, pj1, dx, dy, dz);
     }
+    if (kKind == kTreeFinish && !full && chain) ch.reset(m2l_sum);
     __syncthreads();
 
+    // Items without M2L chunks start at the round of chunk 24 (chunks 24-26
+    // then hold zero terms).
 #pragma unroll 1
-    for (uint32_t r0 = 0; r0 < 32u; r0 += kChunksPerRound) {
+    for (uint32_t r0 = full ? 0u : 24u; r0 < 32u; r0 += kChunksPerRound) {
       // Each wave evaluates two chunks' 8 x 8 pair fields.
 #pragma unroll
       for (uint32_t j = 0; j < kChunksPerRound / kWaves; ++j) {
@@ -1198,7 +1286,7 @@ __global__ __launch_bounds__(kTreeThreads, kFused ? 5 : 6) void k_tree(StepArgs 
         }
         // Chunk 13 (the parent's own block: every pair adjacent), chunk 31
         // and, without a leaf target, the near chunks hold no pair.
-        const bool eval = q < 27u ? q != 13u : (any_leaf && q < 31u);
+        const bool eval = q < 27u ? (full && q != 13u) : (kKind != kTreeM2l && any_leaf && q < 31u);
         float t[10];
         if (eval && fast) {
           // Every lane evaluates; lanes without a pair see a zero mass.
@@ -1224,7 +1312,7 @@ __global__ __launch_bounds__(kTreeThreads, kFused ? 5 : 6) void k_tree(StepArgs 
       if (chain) {
 #pragma unroll
         for (uint32_t u = 0; u < kChunksPerRound; ++u) {
-          if (r0 + u == 27u) ch.reset(ch.value() + inh);
+          if (kKind != kTreeM2l && r0 + u == 27u) ch.reset(ch.value() + inh);
           const float4 v0 = *reinterpret_cast<const float4*>(&terms[((u * 8u + ct) * 10u + ck) * 8u]);
           const float4 v1 = *reinterpret_cast<const float4*>(&terms[((u * 8u + ct) * 10u + ck) * 8u + 4u]);
           ch.add(v0.x);
@@ -1245,9 +1333,9 @@ __global__ __launch_bounds__(kTreeThreads, kFused ? 5 : 6) void k_tree(StepArgs 
       const uint32_t x = 2u * px + (ct & 1u), y = 2u * py + ((ct >> 1) & 1u), z = 2u * pz + (ct >> 2);
       const uint32_t g = off + lin(x, y, z, dim);
       if (cell_count(d, a.parity, level, lin(x, y, z, dim)) != 0) {
-        const gptr<float> f = reinterpret_cast<gptr<float>>(G(d.fields) + 3u * g);
+        const gptr<float> f = reinterpret_cast<gptr<float>>(G(kKind == kTreeM2l ? d.m2l : d.fields) + 3u * g);
         f[ck] = ch.value();
-        if (ck == 9u) {
+        if (kKind != kTreeM2l && ck == 9u) {
           f[10] = 0.0f;
           f[11] = 0.0f;
         }
@@ -1766,10 +1854,10 @@ hipError_t launch_top(const StepArgs& a, hipStream_t stream, const Timing* t) {
 }
 
 // Blocks of a tree launch that fit on the device at once (occupancy x CUs),
-// cached per kernel form and level span (which sets the dynamic LDS size).  The tree kernel loops over
-// its items, so a grid of this size keeps every block busy and none is
-// launched only to find no work.
-template <bool kComp, bool kFused>
+// cached per kernel form and level span (which sets the dynamic LDS size).
+// The tree kernel loops over its items, so a grid of this size keeps every
+// block busy and none is launched only to find no work.
+template <bool kComp, uint32_t kKind>
 static uint32_t tree_resident_blocks(uint32_t span, uint32_t lds) {
   static uint32_t cache[kLevels] = {};
   const uint32_t key = span % kLevels;
@@ -1777,7 +1865,7 @@ static uint32_t tree_resident_blocks(uint32_t span, uint32_t lds) {
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tree<kComp, kFused>, kTreeThreads, lds) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tree<kComp, kKind>, kTreeThreads, lds) != hipSuccess ||
       cus <= 0 || per_cu <= 0)
     return kTreeMaxBlocks;
   cache[key] = static_cast<uint32_t>(cus * per_cu);
@@ -1785,28 +1873,40 @@ static uint32_t tree_resident_blocks(uint32_t span, uint32_t lds) {
 }
 
 uint32_t tree_single_level_capacity(bool compensated) {
-  return compensated ? tree_resident_blocks<true, false>(0u, 0u) : tree_resident_blocks<false, false>(0u, 0u);
+  return compensated ? tree_resident_blocks<true, kTreeSingle>(0u, 0u)
+                     : tree_resident_blocks<false, kTreeSingle>(0u, 0u);
 }
 
-hipError_t launch_tree(const StepArgs& a, uint32_t lo, uint32_t hi, uint32_t nblocks, hipStream_t stream,
-                       const Timing* t) {
-  if (nblocks == 0u || lo < 2u || hi < lo || hi >= kLevels) return no_launch(stream, t);
-  const bool fused = lo < hi;
-  const uint32_t lds = fused ? std::max(kItemBytes, (hi - lo) * kAncestorBytes) : 0u;
-  const uint32_t span = hi - lo;
-  const uint32_t resident =
-      a.compensated ? (fused ? tree_resident_blocks<true, true>(span, lds) : tree_resident_blocks<true, false>(span, lds))
-                    : (fused ? tree_resident_blocks<false, true>(span, lds) : tree_resident_blocks<false, false>(span, lds));
-  nblocks = std::min(nblocks, resident);
-  if (a.compensated && fused)
-    RB_LAUNCH_LDS(t, (k_tree<true, true>), dim3(nblocks), dim3(kTreeThreads), lds, stream, a, lo, hi);
-  else if (a.compensated)
-    RB_LAUNCH(t, (k_tree<true, false>), dim3(nblocks), dim3(kTreeThreads), stream, a, lo, hi);
-  else if (fused)
-    RB_LAUNCH_LDS(t, (k_tree<false, true>), dim3(nblocks), dim3(kTreeThreads), lds, stream, a, lo, hi);
+template <bool kComp, uint32_t kKind>
+static hipError_t launch_tree_form(const StepArgs& a, uint32_t lo, uint32_t hi, uint32_t nblocks, uint32_t lds,
+                                   hipStream_t stream, const Timing* t) {
+  nblocks = std::min(nblocks, tree_resident_blocks<kComp, kKind>(hi - lo, lds));
+  if (lds != 0u)
+    RB_LAUNCH_LDS(t, (k_tree<kComp, kKind>), dim3(nblocks), dim3(kTreeThreads), lds, stream, a, lo, hi);
   else
-    RB_LAUNCH(t, (k_tree<false, false>), dim3(nblocks), dim3(kTreeThreads), stream, a, lo, hi);
+    RB_LAUNCH(t, (k_tree<kComp, kKind>), dim3(nblocks), dim3(kTreeThreads), stream, a, lo, hi);
   return hipGetLastError();
+}
+
+template <bool kComp>
+static hipError_t launch_tree_comp(const StepArgs& a, uint32_t kind, uint32_t lo, uint32_t hi, uint32_t nblocks,
+                                   hipStream_t stream, const Timing* t) {
+  switch (kind) {
+    case kTreePassM2l: return launch_tree_form<kComp, kTreeM2l>(a, lo, hi, nblocks, 0u, stream, t);
+    case kTreePassFinish: return launch_tree_form<kComp, kTreeFinish>(a, lo, hi, nblocks, 0u, stream, t);
+    default:
+      if (lo < hi)
+        return launch_tree_form<kComp, kTreeFused>(a, lo, hi, nblocks, std::max(kItemBytes, (hi - lo) * kAncestorBytes),
+                                                   stream, t);
+      return launch_tree_form<kComp, kTreeSingle>(a, lo, hi, nblocks, 0u, stream, t);
+  }
+}
+
+hipError_t launch_tree(const StepArgs& a, uint32_t kind, uint32_t lo, uint32_t hi, uint32_t nblocks,
+                       hipStream_t stream, const Timing* t) {
+  if (nblocks == 0u || lo < 2u || hi < lo || hi >= kLevels) return no_launch(stream, t);
+  return a.compensated ? launch_tree_comp<true>(a, kind, lo, hi, nblocks, stream, t)
+                       : launch_tree_comp<false>(a, kind, lo, hi, nblocks, stream, t);
 }
 
 hipError_t launch_sort(const StepArgs& a, uint32_t nintegrate, hipStream_t stream) {

@@ -17,11 +17,19 @@ struct Timing {
 hipError_t launch_deposit(const StepArgs& a, uint32_t nblocks, hipStream_t stream, const Timing* t = nullptr);
 hipError_t launch_upward(const StepArgs& a, uint32_t nblocks, hipStream_t stream, const Timing* t = nullptr);
 hipError_t launch_top(const StepArgs& a, hipStream_t stream, const Timing* t = nullptr);
-// Tree pass of levels lo..hi (2 <= lo <= hi <= 6) in one launch.
+// Tree pass launches over levels lo..hi (2 <= lo <= hi <= 6):
+//   kTreePassRun     M2L, L2L and near of the levels, parents' fields from
+//                    memory below lo and recomputed from lo on;
+//   kTreePassM2l     the M2L sums of the targets below each system's finest
+//                    level (into its m2l buffer);
+//   kTreePassFinish  after a kTreePassM2l launch over lo..hi-1: every
+//                    target's field from those sums (the finest level
+//                    evaluates its own), L2L and near.
+constexpr uint32_t kTreePassRun = 0u, kTreePassM2l = 1u, kTreePassFinish = 2u;
 // Blocks of a single-level tree launch resident on the device at once.
 uint32_t tree_single_level_capacity(bool compensated);
-hipError_t launch_tree(const StepArgs& a, uint32_t lo, uint32_t hi, uint32_t nblocks, hipStream_t stream,
-                       const Timing* t = nullptr);
+hipError_t launch_tree(const StepArgs& a, uint32_t kind, uint32_t lo, uint32_t hi, uint32_t nblocks,
+                       hipStream_t stream, const Timing* t = nullptr);
 // Integrate plus the next step's deposit (into finest buffer parity ^ 1),
 // one block per deposit range.
 hipError_t launch_integrate_deposit(const StepArgs& a, uint32_t ndeposit, hipStream_t stream,

@@ -96,9 +96,10 @@ void apply_physics(SysDesc& s, const Physics& ph) {
 size_t system_storage_bytes(uint32_t n) {
   const size_t cells = total_cells(depth_for_particles(n));
   const size_t finest = level_cells(depth_for_particles(n));
+  const size_t above = level_offset(depth_for_particles(n));
   return 2u * align_up(16u * static_cast<size_t>(n)) + align_up(4u * static_cast<size_t>(n)) +
          align_up(16u * cells) + align_up(48u * cells) + align_up(16u * cells) + align_up(cells) +
-         2u * align_up(16u * finest);
+         2u * align_up(16u * finest) + align_up(48u * above);
 }
 
 int alloc_system(uint32_t n, const Physics& ph, float scale, hipStream_t stream, SystemStorage* out) {
@@ -136,6 +137,8 @@ int alloc_system(uint32_t n, const Physics& ph, float scale, hipStream_t stream,
   d.extra = reinterpret_cast<int4*>(p);
   p += align_up(16u * level_cells(plan.depth));
   d.finest_alt = reinterpret_cast<int4*>(p);
+  p += align_up(16u * level_cells(plan.depth));
+  d.m2l = reinterpret_cast<float4*>(p);
   d.n = n;
   d.depth = plan.depth;
   d.scale = scale != 0.0f ? scale : plan.scale;
@@ -353,22 +356,31 @@ int Engine::push_descriptors() {
 
 // Tree launches from the split mask: a launch starts at level 2 and at
 // every level whose bit is set, and runs up to the next start (or dmax).
+// Bit 0 instead selects the two-launch form: the M2L sums of levels
+// 2..dmax-1, then every level's fields from them.
 void Engine::plan_tree() {
   tree_launches.clear();
+  auto blocks_of = [&](uint32_t lo, uint32_t hi) {
+    uint64_t blocks = 0;
+    for (uint32_t k = lo; k <= hi; ++k) blocks += tree_blocks[k];
+    return static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(blocks, 1u), kTreeMaxBlocks));
+  };
+  if (tree_split & 1u) {
+    if (dmax >= 3u) tree_launches.push_back(TreeLaunch{kTreePassM2l, 2u, dmax - 1u, blocks_of(2u, dmax - 1u)});
+    if (dmax >= 2u) tree_launches.push_back(TreeLaunch{kTreePassFinish, 2u, dmax, blocks_of(2u, dmax)});
+    return;
+  }
   uint32_t l = 2u;
   while (l <= dmax) {
     uint32_t hi = l;
     while (hi + 1u <= dmax && !((tree_split >> (hi + 1u)) & 1u)) ++hi;
-    uint64_t blocks = 0;
-    for (uint32_t k = l; k <= hi; ++k) blocks += tree_blocks[k];
-    tree_launches.push_back(
-        TreeLaunch{l, hi, static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(blocks, 1u), kTreeMaxBlocks))});
+    tree_launches.push_back(TreeLaunch{kTreePassRun, l, hi, blocks_of(l, hi)});
     l = hi + 1u;
   }
 }
 
 int Engine::set_tree_split(uint32_t starts) {
-  if (starts & ~0x7Cu) return fail(RAINBOW_E_INVALID, "tree split bits must be within levels 2..6");
+  if (starts & ~0x7Du) return fail(RAINBOW_E_INVALID, "tree split bits must be 0 or within levels 2..6");
   split_auto = false;
   split_wanted = false;
   return apply_tree_split(starts);
@@ -387,7 +399,8 @@ int Engine::apply_tree_split(uint32_t starts) {
 // parents fill two thirds or more of the resident blocks of a single-level
 // launch (then a fused launch would recompute ancestors for many parents);
 // the levels with fewer parents join the launch below them, where one
-// launch's latency serves several levels.  (Measured on the 1M-body Plummer
+// launch's latency serves several levels.  When that fuses every level but
+// the finest, the two-launch form replaces the runs.  (Measured on the 1M-body Plummer
 // step: 1343 level-6 parents for 1536 resident blocks run best alone, 603
 // level-5 parents best fused.)  Without `wait`, returns when the counts of
 // the recording step are not in yet.
@@ -411,6 +424,10 @@ int Engine::adapt_split(bool wait) {
     }
     if (items >= cap) starts |= 1u << level;
   }
+  // Levels 2..dmax-1 fused below a finest level alone: the two-launch form
+  // instead (the M2L sums of the upper levels without the ancestor path,
+  // then all fields; 1M-body Plummer: 25.6 us against 28.2).
+  if (dmax >= 4u && starts == ((1u << 2) | (1u << dmax))) starts = 1u;
   return apply_tree_split(starts);
 }
 
@@ -448,7 +465,7 @@ int Engine::enqueue(bool integrate, bool sorting, uint32_t step_parity, bool dep
   RB_HIP(launch_upward(a, n_subtree, stream));
   RB_HIP(launch_top(a, stream));
   if (sorting) RB_HIP(launch_sort(a, n_integrate, stream));
-  for (const TreeLaunch& t : tree_launches) RB_HIP(launch_tree(a, t.lo, t.hi, t.blocks, stream));
+  for (const TreeLaunch& t : tree_launches) RB_HIP(launch_tree(a, t.kind, t.lo, t.hi, t.blocks, stream));
   if (record_counts) {
     // The step's internal-cell counts, for the measured tree split (the
     // integrate kernel clears them for the next step).
@@ -624,13 +641,14 @@ int Engine::autotune(uint32_t steps, uint32_t* chosen) {
   if (steps == 0u) return fail(RAINBOW_E_INVALID, "steps must be >= 1");
   int rc = snapshot(false);
   if (rc) return rc;
-  // Tree launch split: every partition of levels 2..dmax into runs.
+  // Tree launch split: every partition of levels 2..dmax into runs, and the
+  // two-launch form (mask 1, tried last).
   {
     float best = 0.0f;
     uint32_t best_split = tree_split;
     const uint32_t free_bits = dmax > 2u ? dmax - 2u : 0u;  // levels 3..dmax
-    for (uint32_t m = 0; m < (1u << free_bits); ++m) {
-      if ((rc = set_tree_split((1u << 2) | (m << 3))) != 0) return rc;
+    for (uint32_t m = 0; m <= (1u << free_bits); ++m) {
+      if ((rc = set_tree_split(m < (1u << free_bits) ? (1u << 2) | (m << 3) : 1u)) != 0) return rc;
       float ms = 0.0f;
       if ((rc = time_steps(std::min<uint32_t>(steps, 4u), &ms)) != 0) return rc;  // warm graphs
       if ((rc = snapshot(true)) != 0) return rc;
@@ -736,7 +754,7 @@ int Engine::phase_ms(uint32_t steps, float* out, uint32_t n) {
       if (err == hipSuccess) err = hipEventRecord(sort_ev[1], stream);
     }
     for (const TreeLaunch& t : tree_launches)
-      if (err == hipSuccess) err = launch_tree(a, t.lo, t.hi, t.blocks, stream, &tm[k++]);
+      if (err == hipSuccess) err = launch_tree(a, t.kind, t.lo, t.hi, t.blocks, stream, &tm[k++]);
     if (err == hipSuccess) err = launch_integrate_deposit(a, n_deposit, stream, &tm[k++]);
     if (sorting && err == hipSuccess) {
       err = hipEventSynchronize(sort_ev[1]);
@@ -1047,7 +1065,7 @@ int rainbow_hip_tree_launches(rainbow_hip_t* h, uint32_t* lo_hi, uint32_t n) {
   const auto& tl = h->batch.engine.tree_launches;
   if (lo_hi == nullptr || n < 2u * tl.size()) return fail(RAINBOW_E_INVALID, "tree_launches needs %zu entries", 2u * tl.size());
   for (size_t i = 0; i < tl.size(); ++i) {
-    lo_hi[2u * i] = tl[i].lo;
+    lo_hi[2u * i] = tl[i].lo | (tl[i].kind << 8);
     lo_hi[2u * i + 1u] = tl[i].hi;
   }
   return static_cast<int>(tl.size());

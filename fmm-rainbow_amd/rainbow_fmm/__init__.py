@@ -427,18 +427,29 @@ class Solver:
         _check(lib().rainbow_hip_set_tree_fusion(self._h, hi))
 
     def set_tree_split(self, starts: int):
-        """Bit l of `starts` begins a tree launch at level l (bit 2 implied)."""
+        """Bit l of `starts` begins a tree launch at level l (bit 2 implied);
+        starts=1 selects the two-launch form (M2L sums, then fields)."""
         _check(lib().rainbow_hip_set_tree_split(self._h, starts))
 
     def tree_launches(self) -> list[tuple[int, int]]:
         """(lo, hi) levels of each tree launch of a step, in order."""
+        return [(lo, hi) for _, lo, hi in self.tree_launch_forms()]
+
+    def tree_launch_forms(self) -> list[tuple[str, int, int]]:
+        """(form, lo, hi) of each tree launch: form "run" (M2L, L2L and near
+        of levels lo..hi), "m2l" (the M2L sums below the finest level) or
+        "finish" (every level's fields from those sums)."""
         buf = (_u32 * 16)()
         n = _check(lib().rainbow_hip_tree_launches(self._h, buf, 16))
-        return [(buf[2 * i], buf[2 * i + 1]) for i in range(n)]
+        forms = {0: "run", 1: "m2l", 2: "finish"}
+        return [(forms[buf[2 * i] >> 8], buf[2 * i] & 0xFF, buf[2 * i + 1]) for i in range(n)]
 
     def phase_names(self) -> list[str]:
         """Names of the entries phase_ms returns."""
-        tree = [f"tree_l{lo}" if lo == hi else f"tree_l{lo}-{hi}" for lo, hi in self.tree_launches()]
+        tree = []
+        for form, lo, hi in self.tree_launch_forms():
+            levels = f"l{lo}" if lo == hi else f"l{lo}-{hi}"
+            tree.append(f"tree_{levels}" if form == "run" else f"tree_{form}_{levels}")
         return ["deposit", "upward", "top"] + tree + ["integrate", "sort"]
 
     def autotune(self, steps: int = 32) -> int:

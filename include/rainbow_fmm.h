@@ -230,7 +230,7 @@ int rainbow_hip_time_steps(rainbow_hip_t* h, uint32_t steps, float* ms);
  * event pair recorded by the dispatch itself, and writes the mean duration
  * in ms of each phase: out[0] deposit, out[1] upward (subtrees), out[2]
  * upward (top levels, with the level-1 fields), out[3 + i] tree launch i
- * (M2L + L2L + near of its levels, as listed by rainbow_hip_tree_launches),
+ * (as listed by rainbow_hip_tree_launches),
  * then integrate, then the body sort (the steps sort when due, as
  * rainbow_hip_step does; the entry is the sort time divided by `steps`).
  * `n` is the capacity of out (at least 5 + the number of tree launches; 16
@@ -245,29 +245,38 @@ int rainbow_hip_phase_ms(rainbow_hip_t* h, uint32_t steps, float* out,
  * downloads return caller order and no result depends on it. */
 int rainbow_hip_set_sort_interval(rainbow_hip_t* h, uint32_t interval);
 
-/* The tree pass runs as a few launches, each over a run of levels: a block
- * of a launch over levels lo..hi recomputes the fields of its item's
- * ancestors down to level lo instead of waiting for another launch.
- * rainbow_hip_set_tree_split: bit l of `starts` (2 <= l <= 6) starts a
- * launch at level l (bit 2 is implied).  rainbow_hip_set_tree_fusion(h,
- * hi): levels 2..hi together, each deeper level alone.  Until one of them
- * (or autotune) is called, the split is measured: the first step after a
- * configuration records how many internal parents each level has, and a
- * level starts its own launch when they fill two thirds of the device's
- * resident blocks (a 1M-body Plummer system: levels 2..5 and 6; 256
- * systems of up to 262144 bodies: one launch per level).  Results do not depend on the
- * split. */
+/* The tree pass runs as a few launches, in one of two forms.
+ * Runs of levels: each launch covers levels lo..hi, and a block of it
+ * recomputes the fields of its item's ancestors down to level lo instead of
+ * waiting for another launch.  rainbow_hip_set_tree_split: bit l of
+ * `starts` (2 <= l <= 6) starts a launch at level l (bit 2 is implied).
+ * rainbow_hip_set_tree_fusion(h, hi): levels 2..hi together, each deeper
+ * level alone.
+ * Two launches (`starts` = 1, bit 0): the M2L sums of every level above the
+ * finest (they depend on no parent's field), then every level's fields:
+ * each parent's field from the level-1 field and its ancestors' sums (the
+ * L2L additions in order), the finest level's M2L, the L2L terms and the
+ * near fields.
+ * Until one of the calls above (or autotune) is made, the split is
+ * measured: the first step after a configuration records how many internal
+ * parents each level has, and a level starts its own launch when they fill
+ * two thirds of the device's resident blocks; when that leaves every level
+ * but the finest in one run, the two-launch form is used instead (a
+ * 1M-body Plummer system: two launches; 256 systems of up to 262144 bodies:
+ * one launch per level).  Results do not depend on the split. */
 int rainbow_hip_set_tree_split(rainbow_hip_t* h, uint32_t starts);
 int rainbow_hip_set_tree_fusion(rainbow_hip_t* h, uint32_t hi);
 
-/* The tree launches of a step, as (lo, hi) level pairs in launch order.
- * `n` is the capacity of lo_hi (2 entries per launch).  Returns the number
- * of launches or a negative status. */
+/* The tree launches of a step, as (lo | form << 8, hi) pairs in launch
+ * order; form 0: a run of levels lo..hi, 1: the M2L sums of levels lo..hi
+ * (two-launch form), 2: the fields of levels lo..hi from them.  `n` is the
+ * capacity of lo_hi (2 entries per launch).  Returns the number of
+ * launches or a negative status. */
 int rainbow_hip_tree_launches(rainbow_hip_t* h, uint32_t* lo_hi, uint32_t n);
 
 /* Autotuner: from the same starting state, times `steps` steps at each tree
- * split (every partition of levels 2..depth into runs) and keeps the
- * fastest, then at each candidate
+ * split (every partition of levels 2..depth into runs, and the two-launch
+ * form) and keeps the fastest, then at each candidate
  * sort interval (1, 2, 4, 8, 16, 32, never), keeps the fastest and writes
  * it to *interval.  The bodies are restored afterwards. */
 int rainbow_hip_autotune(rainbow_hip_t* h, uint32_t steps, uint32_t* interval);

@@ -422,7 +422,8 @@ def test_tree_split_does_not_change_results(starts, cfg2_reference):
 def test_measured_split(cfg2_reference):
     # No split set: the first step records the per-level parent counts and
     # the split follows from them (1343 level-6 parents here: levels 2..5
-    # fused, level 6 alone); results are the same as with any fixed split.
+    # would be fused and level 6 alone, so the two-launch form runs
+    # instead); results are the same as with any fixed split.
     say("measured tree split, 1M Plummer")
     parts, om, of, op = cfg2_reference
     c = wl.CFG2
@@ -431,7 +432,7 @@ def test_measured_split(cfg2_reference):
     s.step(2)
     assert_bits(s.particles(), op, "particles")
     s.step(16)
-    assert s.tree_launches() == [(2, 5), (6, 6)]
+    assert s.tree_launch_forms() == [("m2l", 2, 5), ("finish", 2, 6)]
     t = rf.Solver(c["n"], wl.params_for(c))
     t.set_tree_fusion(6)
     t.upload(parts)
@@ -439,6 +440,55 @@ def test_measured_split(cfg2_reference):
     assert s.digest(0) == t.digest(0)
     s.close()
     t.close()
+
+
+def test_two_launch_tree_matches_oracle(cfg2_reference):
+    # Bit 0 of the split mask: the M2L sums of levels 2..5 in one launch,
+    # then every level's fields (L2L from those sums, the finest level's own
+    # M2L, near) in a second.
+    say("two-launch tree, 1M Plummer")
+    parts, om, of, op = cfg2_reference
+    c = wl.CFG2
+    s = rf.Solver(c["n"], wl.params_for(c))
+    s.set_tree_split(1)
+    assert s.tree_launch_forms() == [("m2l", 2, 5), ("finish", 2, 6)]
+    s.upload(parts)
+    s.solve_fields()
+    assert_bits(s.moments(), om, "moments")
+    assert_bits(s.fields(), of, "fields")
+    s.upload(parts)
+    s.step(2)
+    assert_bits(s.particles(), op, "particles")
+    s.close()
+
+
+@pytest.mark.parametrize("compensated", [False, True])
+def test_two_launch_tree_ball_and_uneven_batch(compensated):
+    say(f"two-launch tree, uniform ball 400k and an uneven batch, compensated={compensated}")
+    n = 400000
+    parts = rf.ic_uniform_ball(n, 17, 1.0)
+    dt, g, soft = 1e-3, 1.0, 1e-3
+    s = rf.Solver(n, rf.physics(dt=dt, gravity=g, softening=soft), compensated=compensated)
+    s.set_tree_split(1)
+    s.upload(parts)
+    s.solve_fields()
+    om, of = orc.solve_fields(slots_for(n, dt, g, soft), parts, compensated=compensated)
+    assert_bits(s.fields(), of, "fields")
+    s.close()
+    # Systems of depths 4, 4 and 5 in one batch: the finest level differs
+    # per system.
+    counts = [4096, 1, 70000]
+    ins = [rf.ic_plummer(m, 60 + i, 0.1, 1.5, 1.0) for i, m in enumerate(counts)]
+    b = rf.Batch(counts, rf.physics(dt=dt, gravity=g, softening=soft), compensated=compensated)
+    b.set_tree_split(1)
+    for i, p in enumerate(ins):
+        b.upload(i, p)
+    b.step(3)
+    for i, m in enumerate(counts):
+        op, _, of = orc.step(slots_for(m, dt, g, soft), ins[i], 3, compensated=compensated)
+        assert_bits(b.download(i, rf.PARTICLES), op, f"batch system {i}")
+        assert_bits(b.download(i, rf.FIELDS), of, f"batch system {i} fields")
+    b.close()
 
 
 @pytest.mark.parametrize("compensated", [False, True])
