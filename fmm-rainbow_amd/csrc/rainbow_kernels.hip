@@ -1767,6 +1767,133 @@ __global__ __launch_bounds__(256) void k_split_particles(SysDesc d, const float4
   G(d.perm)[i] = i;
 }
 
+// ------------------------------------------------------------------------
+// galaxy seeding: initialize, src/rainbow_fmm.wgsl:205-374
+// ------------------------------------------------------------------------
+
+// hash32 and random01, src/rainbow_fmm.wgsl:205-214.
+__device__ __forceinline__ uint32_t galaxy_hash(uint32_t x) {
+  x = ((x >> 16) ^ x) * 0x45d9f3bu;
+  x = ((x >> 16) ^ x) * 0x45d9f3bu;
+  return (x >> 16) ^ x;
+}
+
+__device__ __forceinline__ float galaxy_random(uint32_t seed) {
+  return static_cast<float>(galaxy_hash(seed)) * (1.0f / 4294967296.0f);
+}
+
+// seeded_enclosed_mass, src/rainbow_fmm.wgsl:216-230: mass fraction inside
+// `radius` of the 5% bulge, 70% halo, 18% disc and 7% polar populations.
+__device__ __forceinline__ float galaxy_enclosed_mass(float radius) {
+  const float bulge = 0.05f * powf(clampf(radius / 0.11f, 0.0f, 1.0f), 3.0f);
+  const float halo = 0.70f * powf(clampf(radius / 0.70f, 0.0f, 1.0f), 3.0f);
+  const float disc = 0.18f * powf(clampf((radius - 0.055f) / 0.60f, 0.0f, 1.0f), 1.0f / 0.72f);
+  const float polar = 0.07f * powf(clampf((radius - 0.22f) / 0.31f, 0.0f, 1.0f), 1.0f / 0.86f);
+  return fmaxf(((bulge + halo) + disc) + polar, 0.0001f);
+}
+
+__device__ __forceinline__ float3 galaxy_cross(float3 a, float3 b) {
+  return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+__device__ __forceinline__ float galaxy_length(float3 v) { return sqrtf((v.x * v.x + v.y * v.y) + v.z * v.z); }
+
+__device__ __forceinline__ float3 galaxy_normalize(float3 v) {
+  const float l = galaxy_length(v);
+  return make_float3(v.x / l, v.y / l, v.z / l);
+}
+
+// Body i of the reference's default scene: a live halo (with a compact
+// bulge), a four-arm disc and four retrograde polar lanes.  The
+// transcendental functions are the device library's (the WGSL leaves their
+// accuracy to the implementation), so the seed matches the reference within
+// a tolerance, not bit for bit.
+__global__ __launch_bounds__(256) void k_seed_galaxy(SysDesc d) {
+  constexpr float kTau = 6.28318530718f;
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= d.n) return;
+  const float u0 = galaxy_random(i * 11u + 1u);
+  const float u1 = galaxy_random(i * 11u + 2u);
+  const float u2 = galaxy_random(i * 11u + 3u);
+  const float u3 = galaxy_random(i * 11u + 4u);
+  const float u4 = galaxy_random(i * 11u + 5u);
+  const float u5 = galaxy_random(i * 11u + 6u);
+  float3 pos, vel;
+  float excitation;
+  if (u5 < 0.75f) {
+    // Halo (bulge when u5 < 0.05): uniform ball, circular speed about a
+    // random axis.
+    const float z = u0 * 2.0f - 1.0f;
+    const float phi = kTau * u1;
+    const float outer_radius = u5 < 0.05f ? 0.11f : 0.70f;
+    const float radius = outer_radius * powf(u2, 0.3333333f);
+    const float ring = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    const float3 radial_direction = make_float3(ring * cosf(phi), z, ring * sinf(phi));
+    pos = make_float3(radius * radial_direction.x, radius * radial_direction.y, radius * radial_direction.z);
+    const float tangent_z = u3 * 2.0f - 1.0f;
+    const float tangent_phi = kTau * u4;
+    const float tangent_ring = sqrtf(fmaxf(0.0f, 1.0f - tangent_z * tangent_z));
+    const float3 random_direction =
+        make_float3(tangent_ring * cosf(tangent_phi), tangent_z, tangent_ring * sinf(tangent_phi));
+    float3 tangent = galaxy_cross(radial_direction, random_direction);
+    if (galaxy_length(tangent) < 0.001f) tangent = galaxy_cross(radial_direction, make_float3(0.0f, 1.0f, 0.0f));
+    if (galaxy_length(tangent) < 0.001f) tangent = galaxy_cross(radial_direction, make_float3(1.0f, 0.0f, 0.0f));
+    const float orbital = 0.985f * sqrtf(d.gravity * galaxy_enclosed_mass(radius) / (radius + 0.018f));
+    const float3 t = galaxy_normalize(tangent);
+    vel = make_float3(orbital * t.x, orbital * t.y, orbital * t.z);
+    excitation = u5 < 0.05f ? 0.09f : 0.025f + u4 * 0.025f;
+  } else if (u5 < 0.93f) {
+    // Four-arm disc, tilted by 0.2 about x.
+    const float radius = 0.055f + 0.60f * powf(u0, 0.72f);
+    const float arm = static_cast<float>(i & 3u) * (kTau / 4.0f);
+    const float angle = (arm + radius * 7.8f) + (u1 - 0.5f) * (0.045f + radius * 0.055f);
+    const float thickness = (((u2 + u3) + u4) - 1.5f) * (0.0035f + radius * 0.010f);
+    const float3 flat = make_float3(radius * cosf(angle), radius * sinf(angle) * 0.72f, thickness);
+    const float tilt = 0.20f;
+    pos = make_float3(flat.x, flat.y * cosf(tilt) - flat.z * sinf(tilt), flat.y * sinf(tilt) + flat.z * cosf(tilt));
+    const float orbital = 0.99f * sqrtf(d.gravity * galaxy_enclosed_mass(radius) / (radius + 0.018f));
+    const float3 tangent = make_float3(-sinf(angle), cosf(angle) * 0.72f * cosf(tilt), cosf(angle) * 0.72f * sinf(tilt));
+    const float3 pn = galaxy_normalize(pos);
+    const float wave = sinf(angle * 3.0f - radius * 13.0f);
+    const float3 radial = make_float3(((pn.x * orbital) * 0.035f) * wave, ((pn.y * orbital) * 0.035f) * wave,
+                                      ((pn.z * orbital) * 0.035f) * wave);
+    const float3 tn = galaxy_normalize(tangent);
+    vel = make_float3(orbital * tn.x + radial.x, orbital * tn.y + radial.y, orbital * tn.z + radial.z);
+    excitation = 0.08f + u3 * 0.05f;
+  } else {
+    // Four retrograde polar lanes, each in its own tilted plane.
+    const float lineage = static_cast<float>(i & 3u);
+    const float radius = (0.22f + 0.31f * powf(u0, 0.86f)) + (lineage - 1.5f) * 0.007f;
+    const float angle = kTau * u1 + radius * 3.5f;
+    const float thickness = (((u2 + u3) + u4) - 1.5f) * (0.004f + radius * 0.009f);
+    const float3 flat = make_float3(radius * cosf(angle), thickness, radius * sinf(angle));
+    const float tilt = -0.54f + lineage * 0.36f;
+    pos = make_float3(flat.x * cosf(tilt) - flat.y * sinf(tilt), flat.x * sinf(tilt) + flat.y * cosf(tilt), flat.z);
+    const float3 tangent = make_float3(-sinf(angle) * cosf(tilt), -sinf(angle) * sinf(tilt), cosf(angle));
+    const float orbital = 0.99f * sqrtf(d.gravity * galaxy_enclosed_mass(radius) / (radius + 0.018f));
+    const float3 tn = galaxy_normalize(tangent);
+    vel = make_float3(-orbital * tn.x, -orbital * tn.y, -orbital * tn.z);
+    excitation = 0.20f + u4 * 0.08f;
+  }
+  float hue = 0.0f;
+  if (u5 >= 0.75f) {
+    const float speed = fmaxf(galaxy_length(vel), 0.0001f);
+    const float radius = fmaxf(galaxy_length(pos), 0.0001f);
+    const float radial_motion =
+        clampf(((pos.x * vel.x + pos.y * vel.y) + pos.z * vel.z) / (radius * speed), -1.0f, 1.0f);
+    const float3 angular_momentum = galaxy_cross(pos, vel);
+    const float spin_signature = angular_momentum.y / fmaxf(galaxy_length(angular_momentum), 0.0001f);
+    const float h = ((((atan2f(vel.y, vel.x) / kTau + (vel.z / speed) * 0.14f) + spin_signature * 0.075f) +
+                      radial_motion * 0.065f) +
+                     excitation * 0.085f) +
+                    static_cast<float>(i & 3u) * 0.018f;
+    hue = h - floorf(h);
+  }
+  G(d.pos)[i] = make_float4(pos.x, pos.y, pos.z, hue);
+  G(d.vel)[i] = make_float4(vel.x, vel.y, vel.z, excitation);
+  G(d.perm)[i] = i;
+}
+
 __global__ __launch_bounds__(256) void k_join_particles(SysDesc d, float4* out) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   if (i >= d.n) return;
@@ -1792,6 +1919,12 @@ hipError_t launch_direct_f64(const SysDesc& d, const uint32_t* targets, uint32_t
                      chunk, partial);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(k_direct_final, dim3((nt + 255u) / 256u), dim3(256), 0, stream, d, nt, nchunks, partial, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_seed_galaxy(const SysDesc& d, hipStream_t stream) {
+  if (d.n == 0u) return hipSuccess;
+  hipLaunchKernelGGL(k_seed_galaxy, dim3((d.n + 255u) / 256u), dim3(256), 0, stream, d);
   return hipGetLastError();
 }
 

@@ -42,6 +42,9 @@ hipError_t launch_sort(const StepArgs& a, uint32_t nintegrate, hipStream_t strea
 // 4 nt nchunks doubles of scratch.
 hipError_t launch_direct_f64(const SysDesc& d, const uint32_t* targets, uint32_t nt, uint32_t* inv,
                              double* partial, uint32_t nchunks, double* out, hipStream_t stream);
+// The reference's default scene (initialize, src/rainbow_fmm.wgsl:232-374)
+// into a system's bodies, in caller order.
+hipError_t launch_seed_galaxy(const SysDesc& d, hipStream_t stream);
 // Interleaved particles (32 B each, caller order) to the pos/vel/perm
 // streams of a system, and back.
 hipError_t launch_split_particles(const SysDesc& d, const float4* in, hipStream_t stream);

@@ -103,6 +103,7 @@ class Engine {
   int time_steps(uint32_t steps, float* ms);
   int phase_ms(uint32_t steps, float* out, uint32_t n);
   int upload(uint32_t sys, const void* data, size_t bytes);
+  int seed_galaxy(uint32_t sys);
   int download(uint32_t sys, int which, void* dst, size_t bytes);
   int accelerations(uint32_t sys, void* dst, size_t bytes);
   int diagnostics(uint32_t sys, double out[8]);

@@ -776,6 +776,16 @@ int Engine::phase_ms(uint32_t steps, float* out, uint32_t n) {
   return static_cast<int>(nphase);
 }
 
+// The next deposit expects empty finest-level buffers and stragglers.
+static int clear_finest(const SysDesc& d, hipStream_t stream) {
+  const size_t fb = 16ull * level_cells(d.depth);
+  RB_HIP(hipMemsetAsync(d.moments + level_offset(d.depth), 0, fb, stream));
+  RB_HIP(hipMemsetAsync(d.finest_alt, 0, fb, stream));
+  RB_HIP(hipMemsetAsync(d.extra, 0, fb, stream));
+  RB_HIP(hipStreamSynchronize(stream));
+  return RAINBOW_OK;
+}
+
 int upload_system(const SysDesc& d, const void* data, size_t bytes, hipStream_t stream, DevBuf* io) {
   if (data == nullptr || bytes != 32ull * d.n)
     return fail(RAINBOW_E_INVALID, "upload expects %llu bytes, got %zu",
@@ -784,13 +794,7 @@ int upload_system(const SysDesc& d, const void* data, size_t bytes, hipStream_t 
   if (rc) return rc;
   RB_HIP(hipMemcpyAsync(io->p, data, bytes, hipMemcpyHostToDevice, stream));
   RB_HIP(launch_split_particles(d, static_cast<const float4*>(io->p), stream));
-  // The next deposit expects empty finest-level buffers and stragglers.
-  const size_t fb = 16ull * level_cells(d.depth);
-  RB_HIP(hipMemsetAsync(d.moments + level_offset(d.depth), 0, fb, stream));
-  RB_HIP(hipMemsetAsync(d.finest_alt, 0, fb, stream));
-  RB_HIP(hipMemsetAsync(d.extra, 0, fb, stream));
-  RB_HIP(hipStreamSynchronize(stream));
-  return RAINBOW_OK;
+  return clear_finest(d, stream);
 }
 
 int download_system(const SysDesc& d, int which, void* dst, size_t bytes, hipStream_t stream, DevBuf* io,
@@ -835,6 +839,17 @@ int Engine::upload(uint32_t s, const void* data, size_t bytes) {
   since_sort = sort_interval;
   pending_deposit = true;
   return upload_system(sys[s], data, bytes, stream, &io_buf);
+}
+
+// The reference's default scene into system s (initialize,
+// native/main.rs:423-434), in place of an upload.
+int Engine::seed_galaxy(uint32_t s) {
+  if (s >= sys.size()) return fail(RAINBOW_E_INVALID, "system %u out of range", s);
+  RB_HIP(hipStreamSynchronize(stream));
+  since_sort = sort_interval;
+  pending_deposit = true;
+  RB_HIP(launch_seed_galaxy(sys[s], stream));
+  return clear_finest(sys[s], stream);
 }
 
 int Engine::download(uint32_t s, int which, void* dst, size_t bytes) {
@@ -991,6 +1006,11 @@ int rainbow_hip_plan(rainbow_hip_t* h, uint32_t* depth, uint32_t* cells, float* 
 int rainbow_hip_upload_particles(rainbow_hip_t* h, const void* particles, size_t bytes) {
   RB_CHECK_HANDLE(h);
   return h->batch.engine.upload(0u, particles, bytes);
+}
+
+int rainbow_hip_seed_galaxy(rainbow_hip_t* h) {
+  RB_CHECK_HANDLE(h);
+  return h->batch.engine.seed_galaxy(0u);
 }
 
 int rainbow_hip_solve_fields(rainbow_hip_t* h) {
@@ -1183,6 +1203,11 @@ void rainbow_batch_destroy(rainbow_batch_t* b) { delete b; }
 int rainbow_batch_upload(rainbow_batch_t* b, uint32_t system, const void* particles, size_t bytes) {
   RB_CHECK_HANDLE(b);
   return b->batch.engine.upload(system, particles, bytes);
+}
+
+int rainbow_batch_seed_galaxy(rainbow_batch_t* b, uint32_t system) {
+  RB_CHECK_HANDLE(b);
+  return b->batch.engine.seed_galaxy(system);
 }
 
 int rainbow_batch_download(rainbow_batch_t* b, uint32_t system, int which, void* dst, size_t bytes) {

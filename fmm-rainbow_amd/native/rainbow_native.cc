@@ -4,11 +4,12 @@
 // GPU times, the amortised time of a full step and a finiteness check over
 // a particle slice -- the same report lines, read side by side.
 //
-//   rainbow_native [particles (default 2^20)] [steps (default 40)]
+//   rainbow_native [particles (default 2^20)] [steps (default 40)] [galaxy|plummer]
 //
-// The reference seeds its galaxy on the GPU (src/rainbow_fmm.wgsl:232-374);
-// this harness seeds a Plummer sphere (a = 0.1, cut at 1.5, seed 1) with the
-// harness physics constants of native/main.rs:57-82.
+// As the reference harness (native/main.rs:423-434), the default scene is
+// the galaxy seeded on the device (initialize, src/rainbow_fmm.wgsl:232-374);
+// `plummer` seeds a Plummer sphere on the host instead (a = 0.1, cut at
+// 1.5, seed 1).  Physics constants: native/main.rs:57-82.
 
 #include <hip/hip_runtime.h>
 
@@ -32,8 +33,9 @@ static int check(int rc, const char* what) {
 int main(int argc, char** argv) {
   const uint32_t count = argc > 1 ? static_cast<uint32_t>(std::strtoul(argv[1], nullptr, 10)) : (1u << 20);
   const uint32_t repeat = argc > 2 ? static_cast<uint32_t>(std::strtoul(argv[2], nullptr, 10)) : 40u;
-  if (count == 0u || repeat == 0u) {
-    std::fprintf(stderr, "usage: rainbow_native [particles] [steps]\n");
+  const bool plummer = argc > 3 && std::strcmp(argv[3], "plummer") == 0;
+  if (count == 0u || repeat == 0u || (argc > 3 && !plummer && std::strcmp(argv[3], "galaxy") != 0)) {
+    std::fprintf(stderr, "usage: rainbow_native [particles] [steps] [galaxy|plummer]\n");
     return 2;
   }
 
@@ -42,7 +44,7 @@ int main(int argc, char** argv) {
     std::fprintf(stderr, "no HIP device\n");
     return 1;
   }
-  std::printf("adapter: %s (HIP, %s)\n", prop.name, prop.gcnArchName);
+  std::printf("adapter: %s (HIP, %s)\n", prop.name[0] != '\0' ? prop.name : "AMD GPU", prop.gcnArchName);
 
   uint8_t slots[4096];
   const uint32_t depth = rainbow_build_plans(count, slots);
@@ -51,12 +53,16 @@ int main(int argc, char** argv) {
   std::printf("count=%u depth=%u cells=%u scale=%g\n", count, depth, params.total_cells,
               static_cast<double>(params.control_settings[2]));
 
-  std::vector<float> parts(8ull * count);
-  check(rainbow_ic_plummer(count, 1u, 0.1, 1.5, params.gravity, parts.data()), "seed");
-
   rainbow_hip_t* h = nullptr;
   check(rainbow_hip_create(count, &params, &h), "create");
-  check(rainbow_hip_upload_particles(h, parts.data(), parts.size() * sizeof(float)), "upload");
+  if (plummer) {
+    std::vector<float> parts(8ull * count);
+    check(rainbow_ic_plummer(count, 1u, 0.1, 1.5, params.gravity, parts.data()), "seed");
+    check(rainbow_hip_upload_particles(h, parts.data(), parts.size() * sizeof(float)), "upload");
+  } else {
+    check(rainbow_hip_seed_galaxy(h), "seed galaxy");
+  }
+  std::printf("scene: %s\n", plummer ? "Plummer sphere (host seed)" : "galaxy (device seed, initialize)");
 
   // Content digest of one field solve (native/main.rs:436-454).
   check(rainbow_hip_solve_fields(h), "solve");
@@ -82,18 +88,22 @@ int main(int argc, char** argv) {
     char name[32];
     if (i < 3)
       std::snprintf(name, sizeof(name), "%s", fixed[i]);
-    else if (i < 3 + ntree && lo_hi[2 * (i - 3)] == lo_hi[2 * (i - 3) + 1])
-      std::snprintf(name, sizeof(name), "tree_l%u", lo_hi[2 * (i - 3)]);
-    else if (i < 3 + ntree)
-      std::snprintf(name, sizeof(name), "tree_l%u-%u", lo_hi[2 * (i - 3)], lo_hi[2 * (i - 3) + 1]);
-    else if (i == nphase - 2)
+    else if (i < 3 + ntree) {
+      // lo | form << 8 (0: a run of levels, 1: M2L sums, 2: fields from them), hi.
+      const uint32_t lo = lo_hi[2 * (i - 3)] & 0xFFu, form = lo_hi[2 * (i - 3)] >> 8, hi = lo_hi[2 * (i - 3) + 1];
+      const char* pre = form == 1u ? "tree_m2l_" : (form == 2u ? "tree_finish_" : "tree_");
+      if (lo == hi)
+        std::snprintf(name, sizeof(name), "%sl%u", pre, lo);
+      else
+        std::snprintf(name, sizeof(name), "%sl%u-%u", pre, lo, hi);
+    } else if (i == nphase - 2)
       std::snprintf(name, sizeof(name), "integrate");
     else
       std::snprintf(name, sizeof(name), "sort");
-    std::printf("  ts %-10s %.4f ms\n", name, static_cast<double>(phase[i]));
+    std::printf("  ts %-16s %.4f ms\n", name, static_cast<double>(phase[i]));
     total += phase[i];
   }
-  std::printf("  ts TOTAL     %.4f ms\n", total);
+  std::printf("  ts %-16s %.4f ms\n", "TOTAL", total);
 
   // Amortised time of a full step (native/main.rs:522-534): graph replays.
   float ms = 0.0f;

@@ -123,6 +123,7 @@ _SIGNATURES = {
     "rainbow_hip_set_params": (_i32, [_vp, _P(Params)]),
     "rainbow_hip_plan": (_i32, [_vp, _P(_u32), _P(_u32), _P(_f32)]),
     "rainbow_hip_upload_particles": (_i32, [_vp, _vp, _sz]),
+    "rainbow_hip_seed_galaxy": (_i32, [_vp]),
     "rainbow_hip_solve_fields": (_i32, [_vp]),
     "rainbow_hip_step": (_i32, [_vp, _u32]),
     "rainbow_hip_synchronize": (_i32, [_vp]),
@@ -148,6 +149,7 @@ _SIGNATURES = {
     "rainbow_batch_create": (_i32, [_u32, _P(_u32), _P(Params), _u32, _P(_vp)]),
     "rainbow_batch_destroy": (None, [_vp]),
     "rainbow_batch_upload": (_i32, [_vp, _u32, _vp, _sz]),
+    "rainbow_batch_seed_galaxy": (_i32, [_vp, _u32]),
     "rainbow_batch_download": (_i32, [_vp, _u32, _i32, _vp, _sz]),
     "rainbow_batch_solve_fields": (_i32, [_vp]),
     "rainbow_batch_step": (_i32, [_vp, _u32]),
@@ -368,6 +370,11 @@ class Solver:
         assert a.shape == (self.n, 8)
         _check(lib().rainbow_hip_upload_particles(self._h, _ptr(a), a.nbytes))
 
+    def seed_galaxy(self):
+        """The reference's default scene (its initialize pass) on the device,
+        in place of an upload."""
+        _check(lib().rainbow_hip_seed_galaxy(self._h))
+
     def solve_fields(self):
         _check(lib().rainbow_hip_solve_fields(self._h))
 
@@ -504,6 +511,9 @@ class Batch:
     def upload(self, system: int, particles: np.ndarray):
         a = np.ascontiguousarray(particles, dtype=np.float32)
         _check(lib().rainbow_batch_upload(self._h, system, _ptr(a), a.nbytes))
+
+    def seed_galaxy(self, system: int):
+        _check(lib().rainbow_batch_seed_galaxy(self._h, system))
 
     def download(self, system: int, which: int) -> np.ndarray:
         n = self.counts[system]

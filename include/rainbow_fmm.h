@@ -203,6 +203,15 @@ int rainbow_hip_plan(rainbow_hip_t* h, uint32_t* depth, uint32_t* cells,
 int rainbow_hip_upload_particles(rainbow_hip_t* h, const void* particles,
                                  size_t bytes);
 
+/* Seeds the bodies on the device with the reference's default scene, in
+ * place of an upload: the initialize dispatch of native/main.rs:423-434
+ * (src/rainbow_fmm.wgsl:205-374: 5% bulge, 70% halo, 18% four-arm disc, 7%
+ * polar lanes, speeds from the solver's gravity).  Body i is a function of
+ * i alone.  The device's sin/cos/pow/atan2 are used, so the scene equals
+ * the reference's within a tolerance (the WGSL leaves their accuracy to
+ * the implementation), not bit for bit. */
+int rainbow_hip_seed_galaxy(rainbow_hip_t* h);
+
 /* One field solve without integrate: clear, deposit, upward, M2L, downward,
  * near (the digest point of native/main.rs:436-454).  Asynchronous. */
 int rainbow_hip_solve_fields(rainbow_hip_t* h);
@@ -329,6 +338,8 @@ int rainbow_batch_create(uint32_t systems, const uint32_t* counts,
 void rainbow_batch_destroy(rainbow_batch_t* b);
 int rainbow_batch_upload(rainbow_batch_t* b, uint32_t system,
                          const void* particles, size_t bytes);
+/* As rainbow_hip_seed_galaxy, for one system of the batch. */
+int rainbow_batch_seed_galaxy(rainbow_batch_t* b, uint32_t system);
 int rainbow_batch_download(rainbow_batch_t* b, uint32_t system, int which,
                            void* dst, size_t bytes);
 /* Solve without integrate, for every system. */

@@ -49,6 +49,9 @@ _SIGNATURES = {
     "oracle_pair_field": (None, [_vp, _vp, _vp, _f32, _vp]),
     "oracle_direct_sum_f64": (None, [_u32, _vp, _f64, _f64, _u32, _vp, _vp]),
     "oracle_conserved_f64": (None, [_u32, _vp, _vp]),
+    "oracle_seed_galaxy": (None, [_u32, _f32, _vp]),
+    "oracle_hash32": (_u32, [_u32]),
+    "oracle_enclosed_mass": (_f32, [_f32]),
 }
 
 _lib = None
@@ -222,3 +225,18 @@ def conserved_f64(particles) -> np.ndarray:
     out = np.zeros(8, dtype=np.float64)
     lib().oracle_conserved_f64(parts.shape[0], _p(parts), _p(out))
     return out
+
+
+def seed_galaxy(n: int, gravity: float) -> np.ndarray:
+    """initialize (src/rainbow_fmm.wgsl:232-374): particles float32[n, 8]."""
+    out = np.zeros((n, 8), dtype=np.float32)
+    lib().oracle_seed_galaxy(n, gravity, _p(out))
+    return out
+
+
+def hash32(x: int) -> int:
+    return int(lib().oracle_hash32(x))
+
+
+def enclosed_mass(r: float) -> float:
+    return float(lib().oracle_enclosed_mass(r))

@@ -22,8 +22,11 @@
 // independent numpy brute force and invariants; pair_field against its
 // closed form in f64, antisymmetry and trace; the M2L/near stencils against
 // the mass-partition invariant; the field solve against the f64 twin; the
-// integrator against closed-form kick-drift and boundary cases.  Functions
-// with no such check: the control ("shepherd") branch of integrate
+// integrator against closed-form kick-drift and boundary cases; the galaxy
+// seeding (initialize) by inverting hash32, the enclosed-mass model's
+// closed values and the geometry of each seeded population (shares, radii,
+// planes, arm phases, circular speeds, orthogonality).  Functions with no
+// such check: the control ("shepherd") branch of integrate
 // (src/rainbow_fmm.wgsl:659-689) -- parity unpinned (exp/normalize have no
 // fixed closed form check here).
 
@@ -730,6 +733,127 @@ void solve_fields(const uint8_t* slots, const Particle* particles, Moment* momen
   near_all<R, kComp>(slot(slots, 0), moments, fields);
 }
 
+// ---------------------------------------------------------------------------
+// initialize, src/rainbow_fmm.wgsl:205-374: the reference's default scene.
+// f32 throughout; sin/cos/pow/atan2 from the host libm (the WGSL leaves
+// their accuracy to the implementation, so this restatement is compared
+// within a tolerance).
+// ---------------------------------------------------------------------------
+
+struct V3 {
+  float x, y, z;
+};
+
+V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+V3 scale3(float s, V3 v) { return v3(s * v.x, s * v.y, s * v.z); }
+V3 cross3(V3 a, V3 b) { return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+float len3(V3 v) { return std::sqrt((v.x * v.x + v.y * v.y) + v.z * v.z); }
+V3 unit3(V3 v) {
+  const float l = len3(v);
+  return v3(v.x / l, v.y / l, v.z / l);
+}
+float clamp01f(float x) { return std::fmin(std::fmax(x, 0.0f), 1.0f); }
+
+// hash32 / random01, src/rainbow_fmm.wgsl:205-214.
+uint32_t hash32(uint32_t v) {
+  uint32_t x = v;
+  x = ((x >> 16u) ^ x) * 0x45d9f3bu;
+  x = ((x >> 16u) ^ x) * 0x45d9f3bu;
+  return (x >> 16u) ^ x;
+}
+float random01(uint32_t seed) { return static_cast<float>(hash32(seed)) * (1.0f / 4294967296.0f); }
+
+// seeded_enclosed_mass, src/rainbow_fmm.wgsl:216-230.
+float seeded_enclosed_mass(float r) {
+  const float bulge = 0.05f * std::pow(clamp01f(r / 0.11f), 3.0f);
+  const float halo = 0.70f * std::pow(clamp01f(r / 0.70f), 3.0f);
+  const float disc = 0.18f * std::pow(clamp01f((r - 0.055f) / 0.60f), 1.0f / 0.72f);
+  const float polar = 0.07f * std::pow(clamp01f((r - 0.22f) / 0.31f), 1.0f / 0.86f);
+  return std::fmax(((bulge + halo) + disc) + polar, 0.0001f);
+}
+
+// One body of initialize, src/rainbow_fmm.wgsl:232-374.
+Particle seed_body(uint32_t i, float gravity) {
+  const float tau = 6.28318530718f;
+  float u[6];
+  for (uint32_t k = 0; k < 6u; ++k) u[k] = random01(i * 11u + 1u + k);
+  V3 pos, vel;
+  float excitation;
+  if (u[5] < 0.75f) {  // :247-284
+    const float z = u[0] * 2.0f - 1.0f;
+    const float phi = tau * u[1];
+    const float radius = (u[5] < 0.05f ? 0.11f : 0.70f) * std::pow(u[2], 0.3333333f);
+    const float ring = std::sqrt(std::fmax(0.0f, 1.0f - z * z));
+    const V3 dir = v3(ring * std::cos(phi), z, ring * std::sin(phi));
+    pos = scale3(radius, dir);
+    const float tz = u[3] * 2.0f - 1.0f;
+    const float tphi = tau * u[4];
+    const float tring = std::sqrt(std::fmax(0.0f, 1.0f - tz * tz));
+    V3 tangent = cross3(dir, v3(tring * std::cos(tphi), tz, tring * std::sin(tphi)));
+    if (len3(tangent) < 0.001f) tangent = cross3(dir, v3(0.0f, 1.0f, 0.0f));
+    if (len3(tangent) < 0.001f) tangent = cross3(dir, v3(1.0f, 0.0f, 0.0f));
+    const float orbital = 0.985f * std::sqrt(gravity * seeded_enclosed_mass(radius) / (radius + 0.018f));
+    vel = scale3(orbital, unit3(tangent));
+    excitation = u[5] < 0.05f ? 0.09f : 0.025f + u[4] * 0.025f;
+  } else if (u[5] < 0.93f) {  // :285-318
+    const float radius = 0.055f + 0.60f * std::pow(u[0], 0.72f);
+    const float arm = static_cast<float>(i & 3u) * (tau / 4.0f);
+    const float angle = (arm + radius * 7.8f) + (u[1] - 0.5f) * (0.045f + radius * 0.055f);
+    const float thickness = (((u[2] + u[3]) + u[4]) - 1.5f) * (0.0035f + radius * 0.010f);
+    const V3 flat = v3(radius * std::cos(angle), radius * std::sin(angle) * 0.72f, thickness);
+    const float tilt = 0.20f;
+    pos = v3(flat.x, flat.y * std::cos(tilt) - flat.z * std::sin(tilt),
+             flat.y * std::sin(tilt) + flat.z * std::cos(tilt));
+    const float orbital = 0.99f * std::sqrt(gravity * seeded_enclosed_mass(radius) / (radius + 0.018f));
+    const V3 tangent = v3(-std::sin(angle), std::cos(angle) * 0.72f * std::cos(tilt),
+                          std::cos(angle) * 0.72f * std::sin(tilt));
+    const V3 pn = unit3(pos);
+    const float wave = std::sin(angle * 3.0f - radius * 13.0f);
+    const V3 radial = v3(((pn.x * orbital) * 0.035f) * wave, ((pn.y * orbital) * 0.035f) * wave,
+                         ((pn.z * orbital) * 0.035f) * wave);
+    const V3 tn = scale3(orbital, unit3(tangent));
+    vel = v3(tn.x + radial.x, tn.y + radial.y, tn.z + radial.z);
+    excitation = 0.08f + u[3] * 0.05f;
+  } else {  // :319-351
+    const float lineage = static_cast<float>(i & 3u);
+    const float radius = (0.22f + 0.31f * std::pow(u[0], 0.86f)) + (lineage - 1.5f) * 0.007f;
+    const float angle = tau * u[1] + radius * 3.5f;
+    const float thickness = (((u[2] + u[3]) + u[4]) - 1.5f) * (0.004f + radius * 0.009f);
+    const V3 flat = v3(radius * std::cos(angle), thickness, radius * std::sin(angle));
+    const float tilt = -0.54f + lineage * 0.36f;
+    pos = v3(flat.x * std::cos(tilt) - flat.y * std::sin(tilt), flat.x * std::sin(tilt) + flat.y * std::cos(tilt),
+             flat.z);
+    const V3 tangent = v3(-std::sin(angle) * std::cos(tilt), -std::sin(angle) * std::sin(tilt), std::cos(angle));
+    const float orbital = 0.99f * std::sqrt(gravity * seeded_enclosed_mass(radius) / (radius + 0.018f));
+    vel = scale3(-orbital, unit3(tangent));
+    excitation = 0.20f + u[4] * 0.08f;
+  }
+  float hue = 0.0f;
+  if (u[5] >= 0.75f) {  // :353-369
+    const float speed = std::fmax(len3(vel), 0.0001f);
+    const float radius = std::fmax(len3(pos), 0.0001f);
+    const float radial_motion =
+        std::fmin(std::fmax(((pos.x * vel.x + pos.y * vel.y) + pos.z * vel.z) / (radius * speed), -1.0f), 1.0f);
+    const V3 am = cross3(pos, vel);
+    const float spin = am.y / std::fmax(len3(am), 0.0001f);
+    const float h = ((((std::atan2(vel.y, vel.x) / tau + (vel.z / speed) * 0.14f) + spin * 0.075f) +
+                      radial_motion * 0.065f) +
+                     excitation * 0.085f) +
+                    static_cast<float>(i & 3u) * 0.018f;
+    hue = h - std::floor(h);
+  }
+  Particle out;
+  out.pos[0] = pos.x;
+  out.pos[1] = pos.y;
+  out.pos[2] = pos.z;
+  out.pos[3] = hue;
+  out.vel[0] = vel.x;
+  out.vel[1] = vel.y;
+  out.vel[2] = vel.z;
+  out.vel[3] = excitation;
+  return out;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1008,5 +1132,16 @@ void oracle_conserved_f64(uint32_t n, const void* particles, double* out) {
   out[6] = ly;
   out[7] = lz;
 }
+
+// initialize (src/rainbow_fmm.wgsl:232-374) for bodies 0..n-1 into a
+// reference-layout particle buffer.
+void oracle_seed_galaxy(uint32_t n, float gravity, void* particles) {
+  Particle* parts = static_cast<Particle*>(particles);
+  for (uint32_t i = 0; i < n; ++i) parts[i] = seed_body(i, gravity);
+}
+
+uint32_t oracle_hash32(uint32_t x) { return hash32(x); }
+
+float oracle_enclosed_mass(float r) { return seeded_enclosed_mass(r); }
 
 }  // extern "C"
