@@ -136,6 +136,15 @@ uint64_t rainbow_plan_linear_work(uint32_t particles, uint32_t depth) {
   return linear_work_bound(particles, depth);
 }
 
+// app.js:42-43 (BASE_DT, REFERENCE_FRAME_MS) and :716-719: JavaScript
+// arithmetic in double, stored as an f32 by DataView.setFloat32.
+float rainbow_frame_dt(double frame_interval_ms) {
+  const double base_dt = 0.0009;
+  const double reference_frame_ms = 1000.0 / 60.0;
+  const double interval = frame_interval_ms < 100.0 ? frame_interval_ms : 100.0;
+  return static_cast<float>(base_dt * (interval / reference_frame_ms));
+}
+
 void rainbow_write_params(uint32_t count, uint32_t depth, uint32_t level,
                           uint32_t level_offset_, uint32_t level_dim,
                           uint32_t parent_offset, uint32_t parent_dim,

@@ -1023,6 +1023,14 @@ int rainbow_hip_step(rainbow_hip_t* h, uint32_t steps) {
   return h->batch.engine.step(steps);
 }
 
+int rainbow_hip_step_frame(rainbow_hip_t* h, double frame_interval_ms) {
+  RB_CHECK_HANDLE(h);
+  const float dt = rainbow_frame_dt(frame_interval_ms);
+  for (auto& s : h->batch.engine.sys) s.dt = dt;
+  const int rc = h->batch.engine.push_descriptors();
+  return rc != RAINBOW_OK ? rc : h->batch.engine.step(1u);
+}
+
 int rainbow_hip_synchronize(rainbow_hip_t* h) {
   RB_CHECK_HANDLE(h);
   return h->batch.engine.sync();

@@ -124,6 +124,8 @@ _SIGNATURES = {
     "rainbow_hip_plan": (_i32, [_vp, _P(_u32), _P(_u32), _P(_f32)]),
     "rainbow_hip_upload_particles": (_i32, [_vp, _vp, _sz]),
     "rainbow_hip_seed_galaxy": (_i32, [_vp]),
+    "rainbow_hip_step_frame": (_i32, [_vp, ctypes.c_double]),
+    "rainbow_frame_dt": (_f32, [ctypes.c_double]),
     "rainbow_hip_solve_fields": (_i32, [_vp]),
     "rainbow_hip_step": (_i32, [_vp, _u32]),
     "rainbow_hip_synchronize": (_i32, [_vp]),
@@ -287,6 +289,11 @@ def build_plans(count: int) -> bytes:
     return bytes(buf)
 
 
+def frame_dt(frame_interval_ms: float) -> float:
+    """The browser's per-frame dt (app.js:42-43, 716-719)."""
+    return float(lib().rainbow_frame_dt(frame_interval_ms))
+
+
 def fnv1a(data) -> int:
     b = bytes(data) if not isinstance(data, np.ndarray) else np.ascontiguousarray(data).tobytes()
     return lib().rainbow_fnv1a(b, len(b))
@@ -369,6 +376,10 @@ class Solver:
         a = np.ascontiguousarray(particles, dtype=np.float32)
         assert a.shape == (self.n, 8)
         _check(lib().rainbow_hip_upload_particles(self._h, _ptr(a), a.nbytes))
+
+    def step_frame(self, frame_interval_ms: float):
+        """One step of a browser frame: dt from the frame interval."""
+        _check(lib().rainbow_hip_step_frame(self._h, frame_interval_ms))
 
     def seed_galaxy(self):
         """The reference's default scene (its initialize pass) on the device,

@@ -77,6 +77,11 @@ uint32_t rainbow_plan_depth(uint32_t particles);
 float rainbow_plan_fixed_point_scale(uint32_t particles);
 uint64_t rainbow_plan_linear_work(uint32_t particles, uint32_t depth);
 
+/* The browser's per-frame time step (app.js:42-43, 716-719): BASE_DT =
+ * 0.0009 scaled by the frame interval over a 60 Hz frame, the interval
+ * capped at 100 ms, computed in double and stored as the f32 Params.dt. */
+float rainbow_frame_dt(double frame_interval_ms);
+
 /* ------------------------------------------------------------------------ */
 /* Part 2. Params image, plan table and digest (native/main.rs).            */
 /* ------------------------------------------------------------------------ */
@@ -219,6 +224,11 @@ int rainbow_hip_solve_fields(rainbow_hip_t* h);
 /* `steps` full steps (native/main.rs:412-421), replayed from a captured HIP
  * graph.  Asynchronous. */
 int rainbow_hip_step(rainbow_hip_t* h, uint32_t steps);
+
+/* One step of a browser frame (app.js frame(), :711-722): dt =
+ * rainbow_frame_dt(frame_interval_ms) for this and later steps, the other
+ * physics unchanged, then one step. */
+int rainbow_hip_step_frame(rainbow_hip_t* h, double frame_interval_ms);
 
 /* Waits for all work on the handle's stream. */
 int rainbow_hip_synchronize(rainbow_hip_t* h);
