@@ -185,7 +185,9 @@ __device__ __forceinline__ uint32_t wave_prefix_sum(uint32_t x) {
 
 struct DepositLds {
   uint32_t keys[kDepositSlots];  // after the flush: rank base per slot
-  int vals[4][kDepositSlots];
+  int4 vals[kDepositSlots];      // {count, sum_x, sum_y, sum_z} per slot
+  uint32_t used[kDepositBodies]; // the slots that received a key
+  uint32_t nused;
 };
 
 // Deposit of one block's bodies first + k * kDepositThreads + thread
@@ -195,14 +197,12 @@ __device__ void deposit_block(const StepArgs& a, const SysDesc& d, uint32_t bloc
                               const float4 (&p)[kDepositPerThread], uint32_t parity, bool ranks,
                               DepositLds& L, const uint2* own_lds = nullptr) {
   uint32_t* const keys = L.keys;
-  int (*const vals)[kDepositSlots] = L.vals;
+  int4* const vals = L.vals;
   for (uint32_t s = threadIdx.x; s < kDepositSlots; s += kDepositThreads) {
     keys[s] = kEmptyKey;
-    vals[0][s] = 0;
-    vals[1][s] = 0;
-    vals[2][s] = 0;
-    vals[3][s] = 0;
+    vals[s] = make_int4(0, 0, 0, 0);
   }
+  if (threadIdx.x == 0u) L.nused = 0u;
   __syncthreads();
   const uint32_t dim = 1u << d.depth;
   const uint32_t lane = threadIdx.x & 63u;
@@ -226,20 +226,22 @@ __device__ void deposit_block(const StepArgs& a, const SysDesc& d, uint32_t bloc
     // Bodies are kept in Morton order, so equal keys form runs across the
     // wave.  A run's sums are the differences of the wave's inclusive prefix
     // sums at its last lane and before its first lane (wrapping i32 sums,
-    // exact in any order); its last lane adds them to the table.
+    // exact in any order); its last lane adds them to the table.  Its count
+    // is its length (lanes without a body form runs of their own key).
     const uint32_t prev_key = __shfl_up(key, 1u);
     const unsigned long long heads = __ballot(lane == 0u || key != prev_key);
     const uint32_t start = 63u - static_cast<uint32_t>(__clzll(heads & ((2ull << lane) - 1ull)));
     {
       uint32_t incl[4];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) incl[c] = wave_prefix_sum(static_cast<uint32_t>(v[c]));
+      for (int c = 1; c < 4; ++c) incl[c] = wave_prefix_sum(static_cast<uint32_t>(v[c]));
       const int src = static_cast<int>(start == 0u ? 0u : start - 1u);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
+      for (int c = 1; c < 4; ++c) {
         const uint32_t before = static_cast<uint32_t>(__shfl(static_cast<int>(incl[c]), src));
         v[c] = static_cast<int>(incl[c] - (start == 0u ? 0u : before));
       }
+      v[0] = valid ? static_cast<int>(lane - start + 1u) : 0;
     }
     // Heads after this lane; the lane before the next head ends the run.
     const unsigned long long later = lane == 63u ? 0ull : (heads >> (lane + 1u));
@@ -248,15 +250,21 @@ __device__ void deposit_block(const StepArgs& a, const SysDesc& d, uint32_t bloc
     if (valid && tail) {
       // Open addressing; at most kDepositBodies distinct keys in a table of
       // 2 kDepositBodies slots, so an empty slot always exists.
+      // The lane that places a key lists its slot for the flush.
       h = (key * 2654435761u) >> (32u - kDepositSlotBits);
       for (;;) {
         const uint32_t prev = atomicCAS(&keys[h], kEmptyKey, key);
-        if (prev == kEmptyKey || prev == key) break;
+        if (prev == kEmptyKey) {
+          L.used[atomicAdd(&L.nused, 1u)] = h;
+          break;
+        }
+        if (prev == key) break;
         h = (h + 1u) & (kDepositSlots - 1u);
       }
-      run_base = static_cast<uint32_t>(atomicAdd(&vals[0][h], v[0]));
-#pragma unroll
-      for (int c = 1; c < 4; ++c) atomicAdd(&vals[c][h], v[c]);
+      run_base = static_cast<uint32_t>(atomicAdd(&vals[h].x, v[0]));
+      atomicAdd(&vals[h].y, v[1]);
+      atomicAdd(&vals[h].z, v[2]);
+      atomicAdd(&vals[h].w, v[3]);
     }
     // The run's last lane holds its slot and its first index in the slot.
     const uint32_t tail_lane = later == 0ull ? 63u : lane + static_cast<uint32_t>(__ffsll(static_cast<long long>(later))) - 1u;
@@ -270,22 +278,24 @@ __device__ void deposit_block(const StepArgs& a, const SysDesc& d, uint32_t bloc
   // upward adds `extra` into the finest level.
   const uint2 own = own_lds != nullptr ? *own_lds : G(a.own)[block];
   const gptr<int4> finest = finest_of(d, parity);
-  for (uint32_t s = threadIdx.x; s < kDepositSlots; s += kDepositThreads) {
+  const uint32_t nused = L.nused;
+  for (uint32_t u = threadIdx.x; u < nused; u += kDepositThreads) {
+    const uint32_t s = L.used[u];
     const uint32_t key = keys[s];
-    if (key == kEmptyKey) continue;
+    const int4 sums = vals[s];
     uint32_t x, y, z;
     unlin(key, d.depth, x, y, z);
     const uint32_t morton = spread3(x) | (spread3(y) << 1) | (spread3(z) << 2);
     if (morton >= own.x && morton < own.y) {
-      finest[key] = make_int4(vals[0][s], vals[1][s], vals[2][s], vals[3][s]);
+      finest[key] = sums;
       keys[s] = 0u;
       continue;
     }
     // count and sum_x share one 64-bit atomic: the count never carries out
     // of its 32 bits, and the high word wraps as an i32 sum.  The old count
     // places this block's bodies after those already in `extra`.
-    const unsigned long long packed = static_cast<unsigned long long>(static_cast<uint32_t>(vals[0][s])) |
-                                      (static_cast<unsigned long long>(static_cast<uint32_t>(vals[1][s])) << 32);
+    const unsigned long long packed = static_cast<unsigned long long>(static_cast<uint32_t>(sums.x)) |
+                                      (static_cast<unsigned long long>(static_cast<uint32_t>(sums.y)) << 32);
     unsigned long long* const ce = reinterpret_cast<unsigned long long*>(G(d.extra) + key);
     if (ranks) {
       const unsigned long long old = atomicAdd(ce, packed);
@@ -293,8 +303,8 @@ __device__ void deposit_block(const StepArgs& a, const SysDesc& d, uint32_t bloc
     } else {
       (void)atomicAdd(ce, packed);  // no return value needed: the hardware need not wait for it
     }
-    atomicAdd(reinterpret_cast<int*>(G(d.extra) + key) + 2, vals[2][s]);
-    atomicAdd(reinterpret_cast<int*>(G(d.extra) + key) + 3, vals[3][s]);
+    atomicAdd(reinterpret_cast<int*>(G(d.extra) + key) + 2, sums.z);
+    atomicAdd(reinterpret_cast<int*>(G(d.extra) + key) + 3, sums.w);
   }
   if (!ranks) return;
   __syncthreads();
