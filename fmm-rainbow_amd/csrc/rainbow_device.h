@@ -8,7 +8,10 @@
 //   moments int4[C]     {count, sum_x, sum_y, sum_z}   reference Moment
 //   fields  float4[3C]  {a, phi}{Jxx,Jxy,Jxz,Jyy}{Jyz,Jzz,0,0}  reference Field
 //   source  float4[C]   {com.xyz, G m} per cell, G m = 0 for an empty cell
-//   active  uint8[C]    cell was an M2L target in the last solve
+//   active  uint8[C]    bit 0: cell was an M2L target in the last solve;
+//                       finest cells, bit 1: non-empty in the last solve
+//                       (only then are that solve's finest buffer cell and
+//                       the cell's source record non-zero)
 //   extra   int4[8^D]   finest-level moments of bodies deposited by a block
 //                       that does not own their cell (merged and zeroed by
 //                       upward)
@@ -16,6 +19,7 @@
 //                       between the moments array's finest level (parity 0)
 //                       and this one (parity 1); a step's upward zeroes the
 //                       buffer the next step deposits into
+//   xflag   uint8[8^D]  the deposit added stragglers of the cell into `extra`
 //   m2l     float4[3 level_offset(D)]  M2L sums of the cells above the
 //                       finest level, written by the tree pass's first
 //                       launch and read by its second
@@ -42,6 +46,7 @@ struct alignas(16) SysDesc {
   int4* extra;
   int4* finest_alt;
   float4* m2l;         // M2L sums of the levels below the finest, fields layout
+  uint8_t* xflag;      // per finest cell: stragglers were added into `extra`
   uint32_t n;          // bodies (particle_count)
   uint32_t depth;      // octree depth
   uint32_t body_base;  // first index of the system in the batch's body space
@@ -57,7 +62,7 @@ struct alignas(16) SysDesc {
   float control_point[4];
   float control_axis_mode[4];
 };
-static_assert(sizeof(SysDesc) == 160, "SysDesc layout");
+static_assert(sizeof(SysDesc) == 176, "SysDesc layout");
 
 // Work list of the particle passes: a block covers bodies [first, first +
 // block size) of one system.

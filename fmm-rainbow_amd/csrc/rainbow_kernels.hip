@@ -297,6 +297,7 @@ __device__ void deposit_block(const StepArgs& a, const SysDesc& d, uint32_t bloc
     const unsigned long long packed = static_cast<unsigned long long>(static_cast<uint32_t>(sums.x)) |
                                       (static_cast<unsigned long long>(static_cast<uint32_t>(sums.y)) << 32);
     unsigned long long* const ce = reinterpret_cast<unsigned long long*>(G(d.extra) + key);
+    G(d.xflag)[key] = 1u;
     if (ranks) {
       const unsigned long long old = atomicAdd(ce, packed);
       keys[s] = kRankExtra | static_cast<uint32_t>(old);
@@ -491,6 +492,23 @@ __device__ __forceinline__ void update_active(const SysDesc& d, uint32_t cell, i
   }
 }
 
+// Finest cells: bit 0 as update_active, bit 1 = non-empty now (the next
+// step's upward reads it to skip the cells empty in both solves).
+__device__ __forceinline__ void update_finest_flags(const SysDesc& d, uint32_t cell, int count, int parent_count,
+                                                    uint8_t prev) {
+  const uint8_t target = (count != 0 && parent_count > kLeafCapacity) ? 1u : 0u;
+  const uint8_t now = target | (count != 0 ? 2u : 0u);
+  if (prev != now) {
+    if ((prev & 1u) && !target) {
+      const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      G(d.fields)[3u * cell + 0u] = z;
+      G(d.fields)[3u * cell + 1u] = z;
+      G(d.fields)[3u * cell + 2u] = z;
+    }
+    G(d.active)[cell] = now;
+  }
+}
+
 // Internal cells (count > 16) are collected per level into kStripes striped
 // lists; a block reserves its items of one level with a single atomic on
 // the counter of its stripe, so no counter sees more than a few atomics.
@@ -579,18 +597,19 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
   const gptr<int4> next = finest_of(d, a.parity ^ 1u);
   int4 m0 = F[fcell];
   const uint32_t owned = static_cast<uint32_t>(m0.x);
-  {
+  if (G(d.xflag)[fcell] != 0u) {
     const int4 e = G(d.extra)[fcell];
-    if (e.x != 0) {
-      m0.x = static_cast<int>(static_cast<uint32_t>(m0.x) + static_cast<uint32_t>(e.x));
-      m0.y = static_cast<int>(static_cast<uint32_t>(m0.y) + static_cast<uint32_t>(e.y));
-      m0.z = static_cast<int>(static_cast<uint32_t>(m0.z) + static_cast<uint32_t>(e.z));
-      m0.w = static_cast<int>(static_cast<uint32_t>(m0.w) + static_cast<uint32_t>(e.w));
-      F[fcell] = m0;
-      G(d.extra)[fcell] = make_int4(0, 0, 0, 0);
-    }
+    m0.x = static_cast<int>(static_cast<uint32_t>(m0.x) + static_cast<uint32_t>(e.x));
+    m0.y = static_cast<int>(static_cast<uint32_t>(m0.y) + static_cast<uint32_t>(e.y));
+    m0.z = static_cast<int>(static_cast<uint32_t>(m0.z) + static_cast<uint32_t>(e.z));
+    m0.w = static_cast<int>(static_cast<uint32_t>(m0.w) + static_cast<uint32_t>(e.w));
+    F[fcell] = m0;
+    G(d.extra)[fcell] = make_int4(0, 0, 0, 0);
+    G(d.xflag)[fcell] = 0u;
   }
-  next[fcell] = make_int4(0, 0, 0, 0);
+  // The other buffer holds the last solve's counts: non-zero only where the
+  // cell's flag says it was non-empty (bit 1), so only those are zeroed.
+  if (prev_f & 2u) next[fcell] = make_int4(0, 0, 0, 0);
   lvl0[t] = m0;
   __syncthreads();
   if (a.sorting) {
@@ -678,9 +697,10 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
   if (t == 0u) reserve_items(a, ps, lev_of, 3u, stripe);
 
   {
+    // A cell empty now and in the last solve keeps its zero record.
     const int4 m = lvl0[t];
-    G(d.source)[level_off(D) + fcell] = source_record(m, d);
-    update_active(d, level_off(D) + fcell, m.x, lvl1[lin(ix >> 1, iy >> 1, iz >> 1, 4u)].x, prev_f);
+    if (m.x != 0 || (prev_f & 2u)) G(d.source)[level_off(D) + fcell] = source_record(m, d);
+    update_finest_flags(d, level_off(D) + fcell, m.x, lvl1[lin(ix >> 1, iy >> 1, iz >> 1, 4u)].x, prev_f);
   }
   if (mine) {
     G(d.source)[lcell] = source_record(lm, d);

@@ -94,7 +94,7 @@ class Engine {
   int push_descriptors();
   int enqueue(bool integrate, bool sorting, uint32_t step_parity, bool deposit_first, bool rank_next,
               bool record_counts = false);
-  hipError_t zero_finest(uint32_t parity);
+  hipError_t zero_finest();
   void plan_next(bool* sorting, bool* rank_next, uint32_t* since_after) const;
   int solve_fields();
   int step(uint32_t steps);

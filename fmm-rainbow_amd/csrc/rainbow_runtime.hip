@@ -99,7 +99,7 @@ size_t system_storage_bytes(uint32_t n) {
   const size_t above = level_offset(depth_for_particles(n));
   return 2u * align_up(16u * static_cast<size_t>(n)) + align_up(4u * static_cast<size_t>(n)) +
          align_up(16u * cells) + align_up(48u * cells) + align_up(16u * cells) + align_up(cells) +
-         2u * align_up(16u * finest) + align_up(48u * above);
+         2u * align_up(16u * finest) + align_up(48u * above) + align_up(finest);
 }
 
 int alloc_system(uint32_t n, const Physics& ph, float scale, hipStream_t stream, SystemStorage* out) {
@@ -139,6 +139,8 @@ int alloc_system(uint32_t n, const Physics& ph, float scale, hipStream_t stream,
   d.finest_alt = reinterpret_cast<int4*>(p);
   p += align_up(16u * level_cells(plan.depth));
   d.m2l = reinterpret_cast<float4*>(p);
+  p += align_up(48u * level_offset(plan.depth));
+  d.xflag = reinterpret_cast<uint8_t*>(p);
   d.n = n;
   d.depth = plan.depth;
   d.scale = scale != 0.0f ? scale : plan.scale;
@@ -459,7 +461,7 @@ int Engine::enqueue(bool integrate, bool sorting, uint32_t step_parity, bool dep
   a.parity = step_parity;
   a.rank_next = rank_next ? 1u : 0u;
   if (deposit_first) {
-    RB_HIP(zero_finest(step_parity));
+    RB_HIP(zero_finest());
     RB_HIP(launch_deposit(a, n_deposit, stream));
   }
   RB_HIP(launch_upward(a, n_subtree, stream));
@@ -479,12 +481,16 @@ int Engine::enqueue(bool integrate, bool sorting, uint32_t step_parity, bool dep
 
 // Zeroes every system's finest-level buffer `p` and straggler accumulator
 // (before a standalone deposit; not captured in graphs).
-hipError_t Engine::zero_finest(uint32_t p) {
+// Both finest buffers are cleared: upward then zeroes the other buffer's
+// cells only where the last solve's count was non-zero, which holds for a
+// cleared buffer whatever the flags say.
+hipError_t Engine::zero_finest() {
   for (const SysDesc& d : sys) {
     const size_t fb = 16ull * level_cells(d.depth);
-    void* f = p ? static_cast<void*>(d.finest_alt) : static_cast<void*>(d.moments + level_offset(d.depth));
-    hipError_t e = hipMemsetAsync(f, 0, fb, stream);
+    hipError_t e = hipMemsetAsync(d.finest_alt, 0, fb, stream);
+    if (e == hipSuccess) e = hipMemsetAsync(d.moments + level_offset(d.depth), 0, fb, stream);
     if (e == hipSuccess) e = hipMemsetAsync(d.extra, 0, fb, stream);
+    if (e == hipSuccess) e = hipMemsetAsync(d.xflag, 0, level_cells(d.depth), stream);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
@@ -740,7 +746,7 @@ int Engine::phase_ms(uint32_t steps, float* out, uint32_t n) {
     pending_deposit = false;
     uint32_t k = 0;
     if (deposit_first) {
-      if (err == hipSuccess) err = zero_finest(a.parity);
+      if (err == hipSuccess) err = zero_finest();
       if (err == hipSuccess) err = launch_deposit(a, n_deposit, stream, &tm[k++]);
     } else {
       skipped_deposit = true;
@@ -782,6 +788,7 @@ static int clear_finest(const SysDesc& d, hipStream_t stream) {
   RB_HIP(hipMemsetAsync(d.moments + level_offset(d.depth), 0, fb, stream));
   RB_HIP(hipMemsetAsync(d.finest_alt, 0, fb, stream));
   RB_HIP(hipMemsetAsync(d.extra, 0, fb, stream));
+  RB_HIP(hipMemsetAsync(d.xflag, 0, level_cells(d.depth), stream));
   RB_HIP(hipStreamSynchronize(stream));
   return RAINBOW_OK;
 }
