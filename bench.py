@@ -33,13 +33,15 @@ FP32_PEAK_TFLOPS = 157.3
 FLOPS_PER_PAIR = 50  # pair_field: 40 f32 operations (sqrt, div counted once); 10 additions into the field
 
 
-def algorithmic_bytes(n, depth, stats):
+def algorithmic_bytes(n, depth, stats, finest_nonempty):
     """Compulsory HBM bytes per launch of each kernel (each unique address once).
 
     deposit    16 N + 16 C_d             read positions, write finest moments
                (standalone; normally inside integrate)
-    upward     read the finest cells, zero the next step's finest buffer,
-               write levels depth-1..depth-3 and the source records
+    upward     read the finest cells and their flag bytes; for the cells
+               non-empty now or in the last solve (F) zero the other
+               buffer's cell and write the source record; write levels
+               depth-1..depth-3 and their source records
     top        read level depth-3, write the levels above
     tree(l)    16 T_l + 48 T_l + 48 P_l: read the level's target moments,
                write their fields, read their parents' fields (the 6^3
@@ -47,7 +49,8 @@ def algorithmic_bytes(n, depth, stats):
     integrate  64 N + 64 L + 16 C_d      read+write bodies, leaf moments and
                fields, deposit the next step's finest moments
     with C cells, C_d finest cells, T_l targets of level l, P_l their
-    parents, L leaves.
+    parents, L leaves; F is taken as the finest cells non-empty now (they
+    change little per step).
     """
     cells = (8 ** (depth + 1) - 1) // 7
     cd = 8 ** depth
@@ -55,7 +58,7 @@ def algorithmic_bytes(n, depth, stats):
     top = (8 ** (depth - 2) - 1) // 7  # cells of levels 0..depth-3
     mid = cells - cd - top + 8 ** (depth - 3)  # levels depth-1..depth-3
     out = {"deposit": 16 * n + 16 * cd,
-           "upward": 16 * cd + 16 * cd + 16 * mid + 16 * (cd + mid),
+           "upward": 18 * cd + 32 * finest_nonempty + 16 * mid + 16 * mid,
            "top": 16 * (8 ** (depth - 3)) + 16 * (top - 8 ** (depth - 3))}
     for l in range(2, depth + 1):
         t = stats[l]
@@ -389,6 +392,8 @@ def main():
               "decomposition is a design note, not built.", file=sys.stderr)
         sys.exit(2)
 
+    import numpy as np
+
     import rainbow_fmm as rf
     import workloads as wl
 
@@ -420,7 +425,9 @@ def main():
     phases = solver.phase_ms(args.phase_steps)
     names = solver.phase_names()
     per = dict(zip(names, phases))
-    bytes_per = algorithmic_bytes(n, solver.depth, stats)
+    moments = solver.moments()
+    finest_nonempty = int(np.count_nonzero(moments[rf.level_offset(solver.depth):, 0]))
+    bytes_per = algorithmic_bytes(n, solver.depth, stats, finest_nonempty)
     m2l, near = solver.pair_counts()
     flops_per = algorithmic_flops(solver.depth, m2l, near)
     # The dominant kernel: the phase with the largest mean duration (the
