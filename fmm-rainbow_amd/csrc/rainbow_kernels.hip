@@ -1032,6 +1032,34 @@ __device__ void ancestor_fields(const SysDesc& d, uint32_t lo, uint32_t level, u
   }
 }
 
+// Window offsets (2 qx + 12 qy + 72 qz) of the source-parent blocks q < 27
+// that wave w evaluates, q = r0 + w * per-wave + j, packed one byte per
+// (round r0 / chunks-per-round, j).
+struct TreeBlockOffsets {
+  uint64_t pack[kTreeThreads / 64u];
+};
+constexpr TreeBlockOffsets make_tree_block_offsets() {
+  TreeBlockOffsets t{};
+  constexpr uint32_t waves = kTreeThreads / 64u, per_wave = kTreeChunksPerRound / waves;
+  for (uint32_t w = 0; w < waves; ++w)
+    for (uint32_t r0 = 0; r0 < 32u; r0 += kTreeChunksPerRound)
+      for (uint32_t j = 0; j < per_wave; ++j) {
+        const uint32_t q = r0 + w * per_wave + j;
+        if (q < 27u) {
+          const uint64_t off = 2u * (q % 3u) + 12u * ((q / 3u) % 3u) + 72u * (q / 9u);
+          t.pack[w] |= off << (8u * (r0 / kTreeChunksPerRound * per_wave + j));
+        }
+      }
+  return t;
+}
+static_assert(32u / kTreeChunksPerRound * (kTreeChunksPerRound / (kTreeThreads / 64u)) <= 8u,
+              "a wave's block offsets fit in 64 bits");
+__device__ __forceinline__ uint64_t tree_block_offsets(uint32_t wave) {
+  constexpr TreeBlockOffsets kT = make_tree_block_offsets();
+  static_assert(kTreeThreads / 64u == 4u, "four waves per tree block");
+  return wave == 0u ? kT.pack[0] : wave == 1u ? kT.pack[1] : wave == 2u ? kT.pack[2] : kT.pack[3];
+}
+
 // Forms of the tree kernel:
 //   kTreeSingle  one level; the parent's field from memory.
 //   kTreeFused   levels lo..hi; the parent's field recomputed along the
@@ -1184,6 +1212,11 @@ __global__ __launch_bounds__(kTreeThreads, kKind == kTreeFused ? 5 : 6) void k_t
     near_q = (mx * ys) * zs;
   }
   const uint32_t wave_s = __builtin_amdgcn_readfirstlane(wave);
+  // A wave's chunks are the same for every item: q = r0 + wave * per-wave +
+  // j; the window offsets of their source-parent blocks come packed from a
+  // table built at compile time (one byte per round and j).
+  constexpr uint32_t kPerWave = kChunksPerRound / kWaves;
+  const uint64_t block_off = tree_block_offsets(wave_s);
   // Chain threads 0..79: target tid / 10, component tid % 10.
   const bool chain = tid < 80u;
   const uint32_t ct = tid / 10u, ck = tid % 10u;
@@ -1301,7 +1334,8 @@ __global__ __launch_bounds__(kTreeThreads, kKind == kTreeFused ? 5 : 6) void k_t
         float eps2 = eps2_m2l;
         if (q < 27u) {
           // Source parent q's block starts at (2 qx, 2 qy, 2 qz).
-          widx = (2u * (q % 3u) + 12u * ((q / 3u) % 3u) + 72u * (q / 9u)) + lane_src;
+          widx = static_cast<uint32_t>((block_off >> (8u * (r0 / kChunksPerRound * kPerWave + j))) & 0xFFu) +
+                 lane_src;
           active = tcount != 0 && ((near_q >> q) & 1u) == 0u;
         } else {
           // Near field of leaf targets (near_all, src/rainbow_fmm.wgsl:
