@@ -1319,8 +1319,15 @@ __global__ __launch_bounds__(kTreeThreads, kKind == kTreeFused ? 5 : 6) void k_t
     if (kKind == kTreeFinish && !full && chain) ch.reset(m2l_sum);
     __syncthreads();
 
-    // Items without M2L chunks start at the round of chunk 24 (chunks 24-26
-    // then hold zero terms).
+    // Chunks that can hold a pair: the M2L blocks but the parent's own (13,
+    // every pair adjacent) where this item evaluates M2L, and the near
+    // chunks 27-30 when a target is a leaf (chunk 31 is padding).  The
+    // other chunks' terms are zero.
+    auto chunk_has_terms = [&](uint32_t q) -> bool {
+      return q < 27u ? (full && q != 13u) : (kKind != kTreeM2l && any_leaf && q < 31u);
+    };
+    // Items without M2L chunks start at the round of chunk 24 (whose chunks
+    // 24-26 they skip).
 #pragma unroll 1
     for (uint32_t r0 = full ? 0u : 24u; r0 < 32u; r0 += kChunksPerRound) {
       // Each wave evaluates two chunks' 8 x 8 pair fields.
@@ -1328,6 +1335,11 @@ __global__ __launch_bounds__(kTreeThreads, kKind == kTreeFused ? 5 : 6) void k_t
       for (uint32_t j = 0; j < kChunksPerRound / kWaves; ++j) {
         const uint32_t slot = wave_s * (kChunksPerRound / kWaves) + j;
         const uint32_t q = r0 + slot;  // uniform
+        // The M2L form neither stores nor adds a chunk without terms (they
+        // would all be zero); the other forms store zeros and keep their
+        // chains' additions straight-line, which measured faster.
+        if (kKind == kTreeM2l && !chunk_has_terms(q)) continue;
+        const bool eval = chunk_has_terms(q);
         // This lane's source in the window and whether its pair counts.
         uint32_t widx = 0u;
         bool active = false;
@@ -1348,9 +1360,6 @@ __global__ __launch_bounds__(kTreeThreads, kKind == kTreeFused ? 5 : 6) void k_t
           }
           eps2 = eps2_near;
         }
-        // Chunk 13 (the parent's own block: every pair adjacent), chunk 31
-        // and, without a leaf target, the near chunks hold no pair.
-        const bool eval = q < 27u ? (full && q != 13u) : (kKind != kTreeM2l && any_leaf && q < 31u);
         float t[10];
         if (eval && fast) {
           // Every lane evaluates; lanes without a pair see a zero mass.
@@ -1377,6 +1386,7 @@ __global__ __launch_bounds__(kTreeThreads, kKind == kTreeFused ? 5 : 6) void k_t
 #pragma unroll
         for (uint32_t u = 0; u < kChunksPerRound; ++u) {
           if (kKind != kTreeM2l && r0 + u == 27u) ch.reset(ch.value() + inh);
+          if (kKind == kTreeM2l && !chunk_has_terms(r0 + u)) continue;
           const float4 v0 = *reinterpret_cast<const float4*>(&terms[((u * 8u + ct) * 10u + ck) * 8u]);
           const float4 v1 = *reinterpret_cast<const float4*>(&terms[((u * 8u + ct) * 10u + ck) * 8u + 4u]);
           ch.add(v0.x);
