@@ -75,6 +75,9 @@ def algorithmic_bytes(n, depth, stats, finest_nonempty):
     for lo, hi in tree_spans(depth):
         m2l_bytes = sum(16 * stats[l] + 48 * stats[l] for l in range(lo, hi + 1) if l < depth)
         out[f"tree_m2l_{span_name(lo, hi)}"] = m2l_bytes
+        # With the top levels inside: also levels 2..lo-1 and the top's bytes.
+        out[f"tree_m2l_top_{span_name(lo, hi)}"] = (
+            sum(16 * stats[l] + 48 * stats[l] for l in range(2, hi + 1) if l < depth) + out["top"])
         out[f"tree_finish_{span_name(lo, hi)}"] = sum(
             out[f"tree_l{l}"] + (48 * stats[l] if l < depth else 0) for l in range(lo, hi + 1))
     out["integrate"] = 64 * n + 64 * leaves + 16 * cd
@@ -104,6 +107,7 @@ def algorithmic_flops(depth, m2l, near):
             out[f"tree_l{lo}-{hi}"] = sum(out[f"tree_l{l}"] for l in range(lo, hi + 1))
     for lo, hi in tree_spans(depth):
         out[f"tree_m2l_{span_name(lo, hi)}"] = FLOPS_PER_PAIR * sum(m2l[l] for l in range(lo, hi + 1) if l < depth)
+        out[f"tree_m2l_top_{span_name(lo, hi)}"] = FLOPS_PER_PAIR * sum(m2l[l] for l in range(2, hi + 1) if l < depth)
         out[f"tree_finish_{span_name(lo, hi)}"] = FLOPS_PER_PAIR * sum(
             near[l] + (m2l[l] if l == depth else 0) for l in range(lo, hi + 1))
     return out

@@ -711,14 +711,23 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
   if (pushes) write_item(a, ps, my_level, ps.base[my_level] + my_slot, my_item);
 }
 
-// The levels above the subtrees, one block per system: levels r-1..0 from
-// the 8^r subtree roots, their source records and lists, the target flags
-// of levels 1..r, and the fields of level 1.
-template <bool kComp>
-__global__ __launch_bounds__(kTopThreads) void k_top(StepArgs a) {
-  __shared__ int4 lv[512 + 64 + 8 + 1];
-  __shared__ PushStage ps;
-  const uint32_t s = blockIdx.x;
+// LDS of the top levels' pass.
+struct TopLds {
+  int4 lv[512 + 64 + 8 + 1];
+  PushStage ps;
+  uint32_t wsum8[8];
+  uint8_t prev[1u + 8u + 64u + 512u];
+};
+
+// The levels above the subtrees of system s, by one block of kThreads
+// threads: levels r-1..0 from the 8^r subtree roots, their source records
+// and lists, the target flags of levels 1..r, the subtree offsets of a
+// sorting step, and the fields of level 1.
+template <bool kComp, uint32_t kThreads>
+__device__ void top_levels(const StepArgs& a, uint32_t s, TopLds& L) {
+  int4* const lv = L.lv;
+  PushStage& ps = L.ps;
+  uint8_t* const prev = L.prev;
   const SysDesc d = a.sys[s];
   if (!d.live) return;
   const uint32_t t = threadIdx.x;
@@ -734,10 +743,9 @@ __global__ __launch_bounds__(kTopThreads) void k_top(StepArgs a) {
   // Level r (8^r <= 512 cells) into buf(0), then levels r-1, r-2, ... into
   // buf(1), buf(2), buf(3).  The target flags of levels 0..r before this
   // step are loaded alongside.
-  __shared__ uint8_t prev[1u + 8u + 64u + 512u];
   const uint32_t nr = 1u << (3u * r);
-  for (uint32_t i = t; i < nr; i += kTopThreads) buf(0)[i] = M[level_off(r) + i];
-  for (uint32_t c = t; c < level_off(r + 1u); c += kTopThreads) prev[c] = G(d.active)[c];
+  for (uint32_t i = t; i < nr; i += kThreads) buf(0)[i] = M[level_off(r) + i];
+  for (uint32_t c = t; c < level_off(r + 1u); c += kThreads) prev[c] = G(d.active)[c];
   __syncthreads();
   for (uint32_t j = 1u; j <= r; ++j) {
     const uint32_t k = r - j;
@@ -752,22 +760,34 @@ __global__ __launch_bounds__(kTopThreads) void k_top(StepArgs a) {
     __syncthreads();
   }
   if (a.sorting) {
-    // Offsets of the subtrees (level r cells, Morton order) in the system.
+    // Offsets of the subtrees (level r cells, Morton order) in the system:
+    // 512 values (the absent ones zero), kThreads per pass; wave prefix
+    // sums, then the waves' totals.
     const uint32_t dr = 1u << r;
-    __shared__ uint32_t wsum8[8];
-    // One value per thread of the first 512: wave prefix sums, then the
-    // waves' totals.
-    uint32_t own = 0u, incl = 0u;
-    if (t < 512u) {
-      own = t < nr ? static_cast<uint32_t>(buf(0)[lin(compact3(t), compact3(t >> 1), compact3(t >> 2), dr)].x) : 0u;
-      incl = wave_prefix_sum(own);
-      if ((t & 63u) == 63u) wsum8[t >> 6] = incl;
+    uint32_t* const wsum8 = L.wsum8;
+    constexpr uint32_t kPasses = (512u + kThreads - 1u) / kThreads;
+    uint32_t own[kPasses], incl[kPasses];
+#pragma unroll
+    for (uint32_t k = 0; k < kPasses; ++k) {
+      const uint32_t e = k * kThreads + t;
+      own[k] = 0u;
+      incl[k] = 0u;
+      if (e < 512u) {
+        own[k] = e < nr ? static_cast<uint32_t>(buf(0)[lin(compact3(e), compact3(e >> 1), compact3(e >> 2), dr)].x)
+                        : 0u;
+        incl[k] = wave_prefix_sum(own[k]);
+        if ((e & 63u) == 63u) wsum8[e >> 6] = incl[k];
+      }
     }
     __syncthreads();
-    if (t < nr) {
-      uint32_t base = 0u;
-      for (uint32_t w = 0; w < (t >> 6); ++w) base += wsum8[w];
-      G(a.sstart)[(d.cell_base >> 9) + t] = base + incl - own;
+#pragma unroll
+    for (uint32_t k = 0; k < kPasses; ++k) {
+      const uint32_t e = k * kThreads + t;
+      if (e < nr) {
+        uint32_t base = 0u;
+        for (uint32_t w = 0; w < (e >> 6); ++w) base += wsum8[w];
+        G(a.sstart)[(d.cell_base >> 9) + e] = base + incl[k] - own[k];
+      }
     }
   }
   // Internal cells of levels 0..r-1 are staged first, so that thread 0's
@@ -778,7 +798,7 @@ __global__ __launch_bounds__(kTopThreads) void k_top(StepArgs a) {
     const uint32_t k = r - j;
     const uint32_t dk = 1u << k;
     const uint32_t ncell = dk * dk * dk;
-    for (uint32_t i = t; i < ncell; i += kTopThreads) {
+    for (uint32_t i = t; i < ncell; i += kThreads) {
       if (buf(j)[i].x > kLeafCapacity && nmine < 4u) {
         items[nmine] = pack_item(s, i);
         levs[nmine] = k;
@@ -801,7 +821,7 @@ __global__ __launch_bounds__(kTopThreads) void k_top(StepArgs a) {
     }
   }
   // Levels 0..r-1: source records; levels 1..r: flags; one cell per thread.
-  for (uint32_t cell = t; cell < level_off(r + 1u); cell += kTopThreads) {
+  for (uint32_t cell = t; cell < level_off(r + 1u); cell += kThreads) {
     uint32_t k = 0u;
     while (cell >= level_off(k + 1u)) ++k;
     const uint32_t j = r - k, dk = 1u << k, i = cell - level_off(k);
@@ -850,6 +870,13 @@ __global__ __launch_bounds__(kTopThreads) void k_top(StepArgs a) {
       f[2] = make_float4(ch[8].value(), ch[9].value(), 0.0f, 0.0f);
     }
   }
+}
+
+// One block per system.
+template <bool kComp>
+__global__ __launch_bounds__(kTopThreads) void k_top(StepArgs a) {
+  __shared__ TopLds L;
+  top_levels<kComp, kTopThreads>(a, blockIdx.x, L);
 }
 
 // ------------------------------------------------------------------------
@@ -906,6 +933,46 @@ __device__ __forceinline__ bool stage_window(float4* win, const SysDesc& d, uint
     float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (qx >= 0 && qy >= 0 && qz >= 0 && qx < dim_i && qy < dim_i && qz < dim_i)
       v = src[lin(static_cast<uint32_t>(qx), static_cast<uint32_t>(qy), static_cast<uint32_t>(qz), dim)];
+    win[w] = v;
+    bounded = bounded && fabsf(v.x) <= 0x1p60f && fabsf(v.y) <= 0x1p60f && fabsf(v.z) <= 0x1p60f;
+  }
+  return bounded;
+}
+
+// Moment of cell (x, y, z) of level l < depth from its 8 children (wrapping
+// i32 sums, as upward and the top levels form it), for a launch that runs
+// before the level's own moments are written.
+__device__ __forceinline__ int4 children_moment(const SysDesc& d, uint32_t l, uint32_t x, uint32_t y, uint32_t z) {
+  const uint32_t cdim = 2u << l;
+  const gptr<int4> M = G(d.moments) + level_off(l + 1u);
+  uint32_t s0 = 0u, s1 = 0u, s2 = 0u, s3 = 0u;
+#pragma unroll
+  for (uint32_t c = 0; c < 8u; ++c) {
+    const int4 m = M[lin(2u * x + (c & 1u), 2u * y + ((c >> 1) & 1u), 2u * z + (c >> 2), cdim)];
+    s0 += static_cast<uint32_t>(m.x);
+    s1 += static_cast<uint32_t>(m.y);
+    s2 += static_cast<uint32_t>(m.z);
+    s3 += static_cast<uint32_t>(m.w);
+  }
+  return make_int4(static_cast<int>(s0), static_cast<int>(s1), static_cast<int>(s2), static_cast<int>(s3));
+}
+
+// stage_window with the sources' records formed from their children's
+// moments (the same integer sums, so the same records).
+__device__ __forceinline__ bool stage_window_from_children(float4* win, const SysDesc& d, uint32_t level, uint32_t px,
+                                                           uint32_t py, uint32_t pz, uint32_t tid) {
+  bool bounded = true;
+  const int dim_i = static_cast<int>(1u << level);
+  for (uint32_t w = tid; w < 216u; w += kTreeThreads) {
+    const uint32_t wz = w / 36u, wy = (w / 6u) % 6u, wx = w % 6u;
+    const int qx = 2 * static_cast<int>(px) - 2 + static_cast<int>(wx);
+    const int qy = 2 * static_cast<int>(py) - 2 + static_cast<int>(wy);
+    const int qz = 2 * static_cast<int>(pz) - 2 + static_cast<int>(wz);
+    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (qx >= 0 && qy >= 0 && qz >= 0 && qx < dim_i && qy < dim_i && qz < dim_i)
+      v = source_record(children_moment(d, level, static_cast<uint32_t>(qx), static_cast<uint32_t>(qy),
+                                        static_cast<uint32_t>(qz)),
+                        d);
     win[w] = v;
     bounded = bounded && fabsf(v.x) <= 0x1p60f && fabsf(v.y) <= 0x1p60f && fabsf(v.z) <= 0x1p60f;
   }
@@ -1132,7 +1199,7 @@ __device__ __forceinline__ float path_field(const PathSums& p, uint32_t level, u
 // blocks per CU), 5 for fused launches, which carry the ancestor path.
 template <bool kComp, uint32_t kKind>
 __global__ __launch_bounds__(kTreeThreads, kKind == kTreeFused ? 5 : 6) void k_tree(StepArgs a, uint32_t lo,
-                                                                                    uint32_t hi) {
+                                                                                    uint32_t hi, uint32_t lead) {
   constexpr bool kFused = kKind == kTreeFused;
   constexpr uint32_t kWaves = kTreeThreads / 64u;
   constexpr uint32_t kChunksPerRound = kTreeChunksPerRound;
@@ -1140,7 +1207,7 @@ __global__ __launch_bounds__(kTreeThreads, kKind == kTreeFused ? 5 : 6) void k_t
   // max(kItemBytes, (hi - lo) * kAncestorBytes) bytes, used by the
   // ancestor path and then by the item.
   __shared__ float4 win_s[kFused ? 1 : 216];
-  __shared__ float terms_s[kFused ? 1 : kChunksPerRound * 10 * 64];
+  __shared__ __attribute__((aligned(16))) float terms_s[kFused ? 1 : kChunksPerRound * 10 * 64];
   extern __shared__ float4 dyn[];
   __shared__ float pf[12];
   __shared__ uint32_t votes[kTreeThreads / 64u];
@@ -1149,6 +1216,22 @@ __global__ __launch_bounds__(kTreeThreads, kKind == kTreeFused ? 5 : 6) void k_t
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63u;
   const uint32_t wave = tid >> 6;
+
+  // The M2L form of a one-system step can carry the top levels (lead > 0):
+  // block 0 runs the top levels' pass, blocks 1..lead-1 take the items of
+  // levels 2..r (r = depth - 3; their parents, at levels 1..r-1, are the
+  // ones that pass lists) directly, one parent each, and the other blocks
+  // the listed items of levels lo..hi (parents pushed by upward).
+  if constexpr (kKind == kTreeM2l) {
+    static_assert(sizeof(TopLds) <= sizeof(terms_s), "the top levels' LDS fits the terms array");
+    if (lead != 0u && blockIdx.x == 0u) {
+      top_levels<kComp, kTreeThreads>(a, 0u, *reinterpret_cast<TopLds*>(terms_s));
+      return;
+    }
+  }
+  const bool direct = kKind == kTreeM2l && blockIdx.x < lead;
+  const uint32_t first_block = blockIdx.x - (kKind == kTreeM2l ? lead : 0u);
+  const uint32_t nlisted = gridDim.x - (kKind == kTreeM2l ? lead : 0u);
 
   // Items of levels hi, hi-1, ..., lo in that order.  The work table holds,
   // per level and stripe, the inclusive prefix of the stripe counts and the
@@ -1224,13 +1307,28 @@ __global__ __launch_bounds__(kTreeThreads, kKind == kTreeFused ? 5 : 6) void k_t
   // Each item's list entry is loaded one item ahead; the system descriptor
   // is reloaded only when the system changes.
   uint32_t next_level = lo, next_item = 0u;
-  if (blockIdx.x < total) next_item = G(a.lists)[locate(blockIdx.x, next_level)];
+  uint32_t it0 = first_block, stride = nlisted, items = total;
+  if (direct) {
+    // Block 1 + k: parent k of level 1, then k - 8 of level 2, ...
+    uint32_t k = blockIdx.x - 1u, pl = 1u;
+    while (k >= (1u << (3u * pl))) {
+      k -= 1u << (3u * pl);
+      ++pl;
+    }
+    next_level = pl + 1u;
+    next_item = pack_item(0u, k);
+    it0 = 0u;
+    stride = 1u;
+    items = 1u;
+  } else if (first_block < total) {
+    next_item = G(a.lists)[locate(first_block, next_level)];
+  }
   uint32_t cur_sys = 0xFFFFFFFFu;
   SysDesc d;
-  for (uint32_t it = blockIdx.x; it < total; it += gridDim.x) {
+  for (uint32_t it = it0; it < items; it += stride) {
     const uint32_t level = __builtin_amdgcn_readfirstlane(next_level);
     const uint32_t item = __builtin_amdgcn_readfirstlane(next_item);
-    if (it + gridDim.x < total) next_item = G(a.lists)[locate(it + gridDim.x, next_level)];
+    if (!direct && it + stride < items) next_item = G(a.lists)[locate(it + stride, next_level)];
     const uint32_t s = item >> 16;
     if (s >= a.nsys) continue;
     const uint32_t pidx = item & 0xFFFFu;
@@ -1238,6 +1336,7 @@ __global__ __launch_bounds__(kTreeThreads, kKind == kTreeFused ? 5 : 6) void k_t
       d = a.sys[s];
       cur_sys = s;
     }
+    if (direct && !d.live) continue;
     // The M2L launch leaves each system's finest level to the finish launch;
     // there a target's M2L sum is evaluated at the finest level only.
     if (kKind == kTreeM2l && level >= d.depth) continue;
@@ -1263,8 +1362,20 @@ __global__ __launch_bounds__(kTreeThreads, kKind == kTreeFused ? 5 : 6) void k_t
       m2l_sum = reinterpret_cast<gptr<float>>(G(d.m2l) + 3u * (off + lin(x, y, z, dim)))[ck];
     }
     const uint32_t tx = 2u * px + (tt & 1u), ty = 2u * py + ((tt >> 1) & 1u), tz = 2u * pz + (tt >> 2);
-    const int tcount = cell_count(d, a.parity, level, lin(tx, ty, tz, dim));
-    const bool bounded = stage_window(win, d, level, px, py, pz, tid);
+    // Direct items below level r read their level before the top levels'
+    // pass writes it: counts and records from the children's moments.
+    const bool derived = direct && level + 3u < d.depth;
+    const int tcount = derived ? children_moment(d, level, tx, ty, tz).x : cell_count(d, a.parity, level, lin(tx, ty, tz, dim));
+    if (direct) {
+      // A direct item exists when its parent holds more than 16 bodies:
+      // the sum of its 8 targets' counts (lanes 0, 8, ..., 56).
+      uint32_t parent_count = 0u;
+#pragma unroll
+      for (int l = 0; l < 64; l += 8) parent_count += static_cast<uint32_t>(__builtin_amdgcn_readlane(tcount, l));
+      if (parent_count <= static_cast<uint32_t>(kLeafCapacity)) continue;
+    }
+    const bool bounded =
+        derived ? stage_window_from_children(win, d, level, px, py, pz, tid) : stage_window(win, d, level, px, py, pz, tid);
     const float tcx = center1(tx, dim), tcy = center1(ty, dim), tcz = center1(tz, dim);
     const bool leaf = tcount != 0 && (tcount <= kLeafCapacity || level == d.depth);
     // 32 chunks of 8 sources: the 27 source-parent blocks in pz, py, px
@@ -1406,7 +1517,7 @@ __global__ __launch_bounds__(kTreeThreads, kKind == kTreeFused ? 5 : 6) void k_t
     if (chain) {
       const uint32_t x = 2u * px + (ct & 1u), y = 2u * py + ((ct >> 1) & 1u), z = 2u * pz + (ct >> 2);
       const uint32_t g = off + lin(x, y, z, dim);
-      if (cell_count(d, a.parity, level, lin(x, y, z, dim)) != 0) {
+      if ((derived ? children_moment(d, level, x, y, z).x : cell_count(d, a.parity, level, lin(x, y, z, dim))) != 0) {
         const gptr<float> f = reinterpret_cast<gptr<float>>(G(kKind == kTreeM2l ? d.m2l : d.fields) + 3u * g);
         f[ck] = ch.value();
         if (kKind != kTreeM2l && ck == 9u) {
@@ -2085,35 +2196,36 @@ uint32_t tree_single_level_capacity(bool compensated) {
 }
 
 template <bool kComp, uint32_t kKind>
-static hipError_t launch_tree_form(const StepArgs& a, uint32_t lo, uint32_t hi, uint32_t nblocks, uint32_t lds,
-                                   hipStream_t stream, const Timing* t) {
-  nblocks = std::min(nblocks, tree_resident_blocks<kComp, kKind>(hi - lo, lds));
+static hipError_t launch_tree_form(const StepArgs& a, uint32_t lo, uint32_t hi, uint32_t nblocks, uint32_t lead,
+                                   uint32_t lds, hipStream_t stream, const Timing* t) {
+  nblocks = std::min(nblocks, tree_resident_blocks<kComp, kKind>(hi - lo, lds)) + lead;
   if (lds != 0u)
-    RB_LAUNCH_LDS(t, (k_tree<kComp, kKind>), dim3(nblocks), dim3(kTreeThreads), lds, stream, a, lo, hi);
+    RB_LAUNCH_LDS(t, (k_tree<kComp, kKind>), dim3(nblocks), dim3(kTreeThreads), lds, stream, a, lo, hi, lead);
   else
-    RB_LAUNCH(t, (k_tree<kComp, kKind>), dim3(nblocks), dim3(kTreeThreads), stream, a, lo, hi);
+    RB_LAUNCH(t, (k_tree<kComp, kKind>), dim3(nblocks), dim3(kTreeThreads), stream, a, lo, hi, lead);
   return hipGetLastError();
 }
 
 template <bool kComp>
 static hipError_t launch_tree_comp(const StepArgs& a, uint32_t kind, uint32_t lo, uint32_t hi, uint32_t nblocks,
-                                   hipStream_t stream, const Timing* t) {
+                                   uint32_t lead, hipStream_t stream, const Timing* t) {
   switch (kind) {
-    case kTreePassM2l: return launch_tree_form<kComp, kTreeM2l>(a, lo, hi, nblocks, 0u, stream, t);
-    case kTreePassFinish: return launch_tree_form<kComp, kTreeFinish>(a, lo, hi, nblocks, 0u, stream, t);
+    case kTreePassM2l: return launch_tree_form<kComp, kTreeM2l>(a, lo, hi, nblocks, lead, 0u, stream, t);
+    case kTreePassFinish: return launch_tree_form<kComp, kTreeFinish>(a, lo, hi, nblocks, 0u, 0u, stream, t);
     default:
       if (lo < hi)
-        return launch_tree_form<kComp, kTreeFused>(a, lo, hi, nblocks, std::max(kItemBytes, (hi - lo) * kAncestorBytes),
-                                                   stream, t);
-      return launch_tree_form<kComp, kTreeSingle>(a, lo, hi, nblocks, 0u, stream, t);
+        return launch_tree_form<kComp, kTreeFused>(a, lo, hi, nblocks, 0u,
+                                                   std::max(kItemBytes, (hi - lo) * kAncestorBytes), stream, t);
+      return launch_tree_form<kComp, kTreeSingle>(a, lo, hi, nblocks, 0u, 0u, stream, t);
   }
 }
 
-hipError_t launch_tree(const StepArgs& a, uint32_t kind, uint32_t lo, uint32_t hi, uint32_t nblocks,
+hipError_t launch_tree(const StepArgs& a, uint32_t kind, uint32_t lo, uint32_t hi, uint32_t nblocks, uint32_t lead,
                        hipStream_t stream, const Timing* t) {
-  if (nblocks == 0u || lo < 2u || hi < lo || hi >= kLevels) return no_launch(stream, t);
-  return a.compensated ? launch_tree_comp<true>(a, kind, lo, hi, nblocks, stream, t)
-                       : launch_tree_comp<false>(a, kind, lo, hi, nblocks, stream, t);
+  if ((nblocks == 0u && lead == 0u) || lo < 2u || hi < lo || hi >= kLevels) return no_launch(stream, t);
+  if (lead != 0u && (kind != kTreePassM2l || a.nsys != 1u)) return hipErrorInvalidValue;
+  return a.compensated ? launch_tree_comp<true>(a, kind, lo, hi, nblocks, lead, stream, t)
+                       : launch_tree_comp<false>(a, kind, lo, hi, nblocks, lead, stream, t);
 }
 
 hipError_t launch_sort(const StepArgs& a, uint32_t nintegrate, hipStream_t stream) {

@@ -28,7 +28,10 @@ hipError_t launch_top(const StepArgs& a, hipStream_t stream, const Timing* t = n
 constexpr uint32_t kTreePassRun = 0u, kTreePassM2l = 1u, kTreePassFinish = 2u;
 // Blocks of a single-level tree launch resident on the device at once.
 uint32_t tree_single_level_capacity(bool compensated);
-hipError_t launch_tree(const StepArgs& a, uint32_t kind, uint32_t lo, uint32_t hi, uint32_t nblocks,
+// lead > 0 (kTreePassM2l, one system): the launch also runs the top
+// levels (block 0) and the items of levels 2..depth-3 (blocks 1..lead-1);
+// lo..hi are then the levels whose parents upward lists.
+hipError_t launch_tree(const StepArgs& a, uint32_t kind, uint32_t lo, uint32_t hi, uint32_t nblocks, uint32_t lead,
                        hipStream_t stream, const Timing* t = nullptr);
 // Integrate plus the next step's deposit (into finest buffer parity ^ 1),
 // one block per deposit range.

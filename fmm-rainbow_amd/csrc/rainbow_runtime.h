@@ -43,9 +43,10 @@ constexpr uint32_t kDefaultSortInterval = 8;
 // 2..5 together and 6 alone (tunable per engine).
 constexpr uint32_t kDefaultTreeSplit = (1u << 2) | (1u << 6);
 
-// One launch of the tree pass: form (kTreePass*) and levels lo..hi.
+// One launch of the tree pass: form (kTreePass*) and levels lo..hi; lead
+// > 0: the launch also runs the top levels (see launch_tree).
 struct TreeLaunch {
-  uint32_t kind, lo, hi, blocks;
+  uint32_t kind, lo, hi, blocks, lead;
 };
 
 // One device allocation holding a system's bodies and cells, and its

@@ -368,15 +368,25 @@ void Engine::plan_tree() {
     return static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(blocks, 1u), kTreeMaxBlocks));
   };
   if (tree_split & 1u) {
-    if (dmax >= 3u) tree_launches.push_back(TreeLaunch{kTreePassM2l, 2u, dmax - 1u, blocks_of(2u, dmax - 1u)});
-    if (dmax >= 2u) tree_launches.push_back(TreeLaunch{kTreePassFinish, 2u, dmax, blocks_of(2u, dmax)});
+    if (sys.size() == 1u && dmax >= 4u) {
+      // One system: the M2L launch also runs the top levels and the items
+      // of levels 2..dmax-3 (one block per possible parent at levels
+      // 1..dmax-4), so the step has no launch of its own for them.
+      uint32_t lead = 1u;
+      for (uint32_t l = 1u; l + 4u <= dmax; ++l) lead += 1u << (3u * l);
+      tree_launches.push_back(
+          TreeLaunch{kTreePassM2l, dmax - 2u, dmax - 1u, blocks_of(dmax - 2u, dmax - 1u), lead});
+    } else if (dmax >= 3u) {
+      tree_launches.push_back(TreeLaunch{kTreePassM2l, 2u, dmax - 1u, blocks_of(2u, dmax - 1u), 0u});
+    }
+    if (dmax >= 2u) tree_launches.push_back(TreeLaunch{kTreePassFinish, 2u, dmax, blocks_of(2u, dmax), 0u});
     return;
   }
   uint32_t l = 2u;
   while (l <= dmax) {
     uint32_t hi = l;
     while (hi + 1u <= dmax && !((tree_split >> (hi + 1u)) & 1u)) ++hi;
-    tree_launches.push_back(TreeLaunch{kTreePassRun, l, hi, blocks_of(l, hi)});
+    tree_launches.push_back(TreeLaunch{kTreePassRun, l, hi, blocks_of(l, hi), 0u});
     l = hi + 1u;
   }
 }
@@ -465,9 +475,18 @@ int Engine::enqueue(bool integrate, bool sorting, uint32_t step_parity, bool dep
     RB_HIP(launch_deposit(a, n_deposit, stream));
   }
   RB_HIP(launch_upward(a, n_subtree, stream));
-  RB_HIP(launch_top(a, stream));
-  if (sorting) RB_HIP(launch_sort(a, n_integrate, stream));
-  for (const TreeLaunch& t : tree_launches) RB_HIP(launch_tree(a, t.kind, t.lo, t.hi, t.blocks, stream));
+  // The top levels run in their own launch, or inside the first tree
+  // launch; the sort's scatter follows whichever runs them.
+  const bool top_in_tree = !tree_launches.empty() && tree_launches[0].lead != 0u;
+  if (!top_in_tree) {
+    RB_HIP(launch_top(a, stream));
+    if (sorting) RB_HIP(launch_sort(a, n_integrate, stream));
+  }
+  for (size_t i = 0; i < tree_launches.size(); ++i) {
+    const TreeLaunch& t = tree_launches[i];
+    RB_HIP(launch_tree(a, t.kind, t.lo, t.hi, t.blocks, t.lead, stream));
+    if (top_in_tree && i == 0u && sorting) RB_HIP(launch_sort(a, n_integrate, stream));
+  }
   if (record_counts) {
     // The step's internal-cell counts, for the measured tree split (the
     // integrate kernel clears them for the next step).
@@ -741,6 +760,7 @@ int Engine::phase_ms(uint32_t steps, float* out, uint32_t n) {
     a.rank_next = rank_next ? 1u : 0u;
     const bool deposit_first = pending_deposit;
     bool skipped_deposit = false;
+    bool top_slot_empty = false;
     parity ^= 1u;
     since_sort = after;
     pending_deposit = false;
@@ -753,14 +773,26 @@ int Engine::phase_ms(uint32_t steps, float* out, uint32_t n) {
       ++k;
     }
     if (err == hipSuccess) err = launch_upward(a, n_subtree, stream, &tm[k++]);
-    if (err == hipSuccess) err = launch_top(a, stream, &tm[k++]);
-    if (sorting) {
+    const bool top_in_tree = !tree_launches.empty() && tree_launches[0].lead != 0u;
+    auto sort_timed = [&]() {
       if (err == hipSuccess) err = hipEventRecord(sort_ev[0], stream);
       if (err == hipSuccess) err = launch_sort(a, n_integrate, stream);
       if (err == hipSuccess) err = hipEventRecord(sort_ev[1], stream);
+    };
+    if (top_in_tree) {
+      // The top levels run inside the first tree launch: their entry stays
+      // zero (no launch, no events).
+      top_slot_empty = true;
+      ++k;
+    } else {
+      if (err == hipSuccess) err = launch_top(a, stream, &tm[k++]);
+      if (sorting) sort_timed();
     }
-    for (const TreeLaunch& t : tree_launches)
-      if (err == hipSuccess) err = launch_tree(a, t.kind, t.lo, t.hi, t.blocks, stream, &tm[k++]);
+    for (size_t i = 0; i < tree_launches.size(); ++i) {
+      const TreeLaunch& t = tree_launches[i];
+      if (err == hipSuccess) err = launch_tree(a, t.kind, t.lo, t.hi, t.blocks, t.lead, stream, &tm[k++]);
+      if (top_in_tree && i == 0u && sorting) sort_timed();
+    }
     if (err == hipSuccess) err = launch_integrate_deposit(a, n_deposit, stream, &tm[k++]);
     if (sorting && err == hipSuccess) {
       err = hipEventSynchronize(sort_ev[1]);
@@ -770,6 +802,7 @@ int Engine::phase_ms(uint32_t steps, float* out, uint32_t n) {
     }
     if (err == hipSuccess) err = hipStreamSynchronize(stream);
     for (uint32_t i = skipped_deposit ? 1u : 0u; i < nkern && err == hipSuccess; ++i) {
+      if (i == 2u && top_slot_empty) continue;
       float ms = 0.0f;
       err = hipEventElapsedTime(&ms, tm[i].start, tm[i].stop);
       acc[i] += ms;
@@ -1100,7 +1133,8 @@ int rainbow_hip_tree_launches(rainbow_hip_t* h, uint32_t* lo_hi, uint32_t n) {
   const auto& tl = h->batch.engine.tree_launches;
   if (lo_hi == nullptr || n < 2u * tl.size()) return fail(RAINBOW_E_INVALID, "tree_launches needs %zu entries", 2u * tl.size());
   for (size_t i = 0; i < tl.size(); ++i) {
-    lo_hi[2u * i] = tl[i].lo | (tl[i].kind << 8);
+    const uint32_t form = tl[i].lead != 0u ? 3u : tl[i].kind;
+    lo_hi[2u * i] = tl[i].lo | (form << 8);
     lo_hi[2u * i + 1u] = tl[i].hi;
   }
   return static_cast<int>(tl.size());

@@ -89,9 +89,10 @@ int main(int argc, char** argv) {
     if (i < 3)
       std::snprintf(name, sizeof(name), "%s", fixed[i]);
     else if (i < 3 + ntree) {
-      // lo | form << 8 (0: a run of levels, 1: M2L sums, 2: fields from them), hi.
+      // lo | form << 8 (0: a run of levels, 1: M2L sums, 2: fields from them,
+      // 3: M2L sums with the top levels), hi.
       const uint32_t lo = lo_hi[2 * (i - 3)] & 0xFFu, form = lo_hi[2 * (i - 3)] >> 8, hi = lo_hi[2 * (i - 3) + 1];
-      const char* pre = form == 1u ? "tree_m2l_" : (form == 2u ? "tree_finish_" : "tree_");
+      const char* pre = form == 1u ? "tree_m2l_" : (form == 2u ? "tree_finish_" : (form == 3u ? "tree_m2l_top_" : "tree_"));
       if (lo == hi)
         std::snprintf(name, sizeof(name), "%sl%u", pre, lo);
       else
@@ -100,10 +101,10 @@ int main(int argc, char** argv) {
       std::snprintf(name, sizeof(name), "integrate");
     else
       std::snprintf(name, sizeof(name), "sort");
-    std::printf("  ts %-16s %.4f ms\n", name, static_cast<double>(phase[i]));
+    std::printf("  ts %-20s %.4f ms\n", name, static_cast<double>(phase[i]));
     total += phase[i];
   }
-  std::printf("  ts %-16s %.4f ms\n", "TOTAL", total);
+  std::printf("  ts %-20s %.4f ms\n", "TOTAL", total);
 
   // Amortised time of a full step (native/main.rs:522-534): graph replays.
   float ms = 0.0f;

@@ -455,11 +455,13 @@ class Solver:
 
     def tree_launch_forms(self) -> list[tuple[str, int, int]]:
         """(form, lo, hi) of each tree launch: form "run" (M2L, L2L and near
-        of levels lo..hi), "m2l" (the M2L sums below the finest level) or
+        of levels lo..hi), "m2l" (the M2L sums below the finest level),
+        "m2l_top" (the same for one system, also running the top levels and
+        the levels 2..depth-3; lo..hi are the levels upward lists) or
         "finish" (every level's fields from those sums)."""
         buf = (_u32 * 16)()
         n = _check(lib().rainbow_hip_tree_launches(self._h, buf, 16))
-        forms = {0: "run", 1: "m2l", 2: "finish"}
+        forms = {0: "run", 1: "m2l", 2: "finish", 3: "m2l_top"}
         return [(forms[buf[2 * i] >> 8], buf[2 * i] & 0xFF, buf[2 * i + 1]) for i in range(n)]
 
     def phase_names(self) -> list[str]:

@@ -288,7 +288,10 @@ int rainbow_hip_set_tree_fusion(rainbow_hip_t* h, uint32_t hi);
 
 /* The tree launches of a step, as (lo | form << 8, hi) pairs in launch
  * order; form 0: a run of levels lo..hi, 1: the M2L sums of levels lo..hi
- * (two-launch form), 2: the fields of levels lo..hi from them.  `n` is the
+ * (two-launch form), 2: the fields of levels lo..hi from them, 3: form 1
+ * for a single system, also running the top levels (so the step has no
+ * launch of its own for them) and the levels 2..depth-3 whose parents
+ * those list (lo..hi are the levels listed by upward).  `n` is the
  * capacity of lo_hi (2 entries per launch).  Returns the number of
  * launches or a negative status. */
 int rainbow_hip_tree_launches(rainbow_hip_t* h, uint32_t* lo_hi, uint32_t n);

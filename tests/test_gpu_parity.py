@@ -432,7 +432,7 @@ def test_measured_split(cfg2_reference):
     s.step(2)
     assert_bits(s.particles(), op, "particles")
     s.step(16)
-    assert s.tree_launch_forms() == [("m2l", 2, 5), ("finish", 2, 6)]
+    assert s.tree_launch_forms() == [("m2l_top", 4, 5), ("finish", 2, 6)]
     t = rf.Solver(c["n"], wl.params_for(c))
     t.set_tree_fusion(6)
     t.upload(parts)
@@ -451,7 +451,7 @@ def test_two_launch_tree_matches_oracle(cfg2_reference):
     c = wl.CFG2
     s = rf.Solver(c["n"], wl.params_for(c))
     s.set_tree_split(1)
-    assert s.tree_launch_forms() == [("m2l", 2, 5), ("finish", 2, 6)]
+    assert s.tree_launch_forms() == [("m2l_top", 4, 5), ("finish", 2, 6)]
     s.upload(parts)
     s.solve_fields()
     assert_bits(s.moments(), om, "moments")
@@ -459,6 +459,31 @@ def test_two_launch_tree_matches_oracle(cfg2_reference):
     s.upload(parts)
     s.step(2)
     assert_bits(s.particles(), op, "particles")
+    s.close()
+
+
+@pytest.mark.parametrize("n,sort_interval", [(4096, 1), (70000, 1), (400000, 2)])
+def test_two_launch_tree_with_top_levels_inside(n, sort_interval):
+    # A one-system step in the two-launch form runs the top levels inside
+    # the M2L launch (block 0), with the items of levels 2..depth-3 taken
+    # directly (their level-2 counts and records from the level-3 moments at
+    # depth 6); the body sort follows that launch.  Depths 4, 5 and 6.
+    say(f"two-launch tree with the top levels inside, n={n}, sort every {sort_interval}")
+    parts = rf.ic_plummer(n, 90 + n % 7, 0.1, 1.5, 1.0)
+    dt, g, soft = 5e-4, 1.0, 1e-3
+    s = rf.Solver(n, rf.physics(dt=dt, gravity=g, softening=soft))
+    s.set_tree_split(1)
+    s.set_sort_interval(sort_interval)
+    s.upload(parts)
+    s.solve_fields()
+    om, of = orc.solve_fields(slots_for(n, dt, g, soft), parts)
+    assert_bits(s.moments(), om, "moments")
+    assert_bits(s.fields(), of, "fields")
+    s.upload(parts)
+    s.step(5)
+    op, _, of = orc.step(slots_for(n, dt, g, soft), parts, 5)
+    assert_bits(s.particles(), op, "particles")
+    assert_bits(s.fields(), of, "fields after 5 steps")
     s.close()
 
 
