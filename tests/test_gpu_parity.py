@@ -487,6 +487,34 @@ def test_two_launch_tree_with_top_levels_inside(n, sort_interval):
     s.close()
 
 
+def test_two_launch_tree_top_levels_sparse_octants():
+    # Depth 6 with a dense clump in one octant and exactly 16 and 17 bodies
+    # in two others: the direct items of levels 2 and 3 meet parents at the
+    # 16-body boundary (skipped at 16, taken at 17) and level-2 windows that
+    # are mostly empty, their records formed from the level-3 moments.
+    say("two-launch tree with the top levels inside, sparse octants at the 16-body boundary")
+    rng = np.random.default_rng(123)
+    clump = rf.ic_plummer(400000, 77, 0.05, 0.4, 1.0)
+    clump[:, 0:3] += np.float32(0.6)
+    def few(k, centre):
+        p = np.zeros((k, 8), np.float32)
+        p[:, 0:3] = (np.asarray(centre, np.float32) + rng.uniform(-0.3, 0.3, (k, 3))).astype(np.float32)
+        return p
+    parts = np.concatenate([clump, few(16, (-0.8, -0.8, -0.8)), few(17, (-0.8, 0.8, -0.8))])
+    n = parts.shape[0]
+    dt, g, soft = 5e-4, 1.0, 1e-3
+    s = rf.Solver(n, rf.physics(dt=dt, gravity=g, softening=soft))
+    assert s.depth == 6
+    s.set_tree_split(1)
+    assert s.tree_launch_forms()[0][0] == "m2l_top"
+    s.upload(parts)
+    s.step(3)
+    op, _, of = orc.step(slots_for(n, dt, g, soft), parts, 3)
+    assert_bits(s.particles(), op, "particles")
+    assert_bits(s.fields(), of, "fields")
+    s.close()
+
+
 @pytest.mark.parametrize("compensated", [False, True])
 def test_two_launch_tree_ball_and_uneven_batch(compensated):
     say(f"two-launch tree, uniform ball 400k and an uneven batch, compensated={compensated}")
