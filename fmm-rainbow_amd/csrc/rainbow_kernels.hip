@@ -1565,6 +1565,12 @@ __device__ __forceinline__ Leaf leaf_field(const SysDesc& d, uint32_t parity, fl
       lc[level] = level == depth ? finest[idx].x : G(d.moments)[level_off(level) + idx].x;
     }
   }
+  // The finest cell is the usual leaf: its field and record are loaded
+  // with the counts, and used when the search ends there.
+  float4 sa, sj0, sj1;
+  const uint32_t fcell = level_off(depth) + lin(fx, fy, fz, 1u << depth);
+  load_field(G(d.fields), fcell, sa, sj0, sj1);
+  const float4 scom = G(d.source)[fcell];
   uint32_t chosen = depth;
 #pragma unroll
   for (uint32_t level = kLevels - 1u; level >= 1u; --level)
@@ -1574,8 +1580,8 @@ __device__ __forceinline__ Leaf leaf_field(const SysDesc& d, uint32_t parity, fl
   out.dim = 1u << chosen;
   out.cell = level_off(chosen) + lin(c[0], c[1], c[2], out.dim);
   const uint32_t cell = out.cell;
-  float4 fa, fj0, fj1;
-  load_field(G(d.fields), cell, fa, fj0, fj1);
+  float4 fa = sa, fj0 = sj0, fj1 = sj1;
+  if (chosen != depth) load_field(G(d.fields), cell, fa, fj0, fj1);
   const float dx = px - center1(c[0], out.dim);
   const float dy = py - center1(c[1], out.dim);
   const float dz = pz - center1(c[2], out.dim);
@@ -1589,7 +1595,7 @@ __device__ __forceinline__ Leaf leaf_field(const SysDesc& d, uint32_t parity, fl
   float self_phi = 0.0f;
   const bool has_self = count > 1u;
   if (has_self) {
-    const float4 com = G(d.source)[cell];
+    const float4 com = chosen == depth ? scom : G(d.source)[cell];
     const float ex = com.x - px, ey = com.y - py, ez = com.z - pz;
     const float epsilon =
         fmaxf(d.softening, (0.62f * kWorldHalfExtent) * inv_pow2(out.dim));
