@@ -25,10 +25,11 @@
 // integrator against closed-form kick-drift and boundary cases; the galaxy
 // seeding (initialize) by inverting hash32, the enclosed-mass model's
 // closed values and the geometry of each seeded population (shares, radii,
-// planes, arm phases, circular speeds, orthogonality).  Functions with no
-// such check: the control ("shepherd") branch of integrate
-// (src/rainbow_fmm.wgsl:659-689) -- parity unpinned (exp/normalize have no
-// fixed closed form check here).
+// planes, arm phases, circular speeds, orthogonality); integrate's
+// excitation and hue lanes and the control ("shepherd") branch
+// (src/rainbow_fmm.wgsl:659-768) by one-term-at-a-time targets, geometric
+// relaxation, the smoothstep's Hermite values, a Gaussian influence against
+// exp in f64, exact attract/repel negation and swirl orthogonality.
 
 #include <cmath>
 #include <cstdint>
