@@ -1,5 +1,6 @@
 """pair_field (src/rainbow_fmm.wgsl:170-203) of the oracle against the
-softened-monopole closed form in f64 and its exact symmetries."""
+softened-monopole closed form in f64, its exact symmetries, and the
+derivative relations between its lanes (a = -grad phi, J = d a / d target)."""
 
 import numpy as np
 import pytest
@@ -89,3 +90,36 @@ def test_mass_scales_exactly():
     f1 = orc.pair_field(slot, m1, t, 0.02)
     f2 = orc.pair_field(slot, m2, t, 0.02)
     np.testing.assert_array_equal(f2, 2 * f1)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_field_is_the_gradient_of_the_potential(seed):
+    # central differences of the oracle's own phi and a (target shifted by
+    # h = 2^-10, exact in f32): a = -grad phi and J = d a / d target, the
+    # Jacobian integrate applies to a body's offset from the cell centre
+    rng = np.random.default_rng(100 + seed)
+    n = 1 << 20
+    slot = slot0(n, 1.0)
+    scale = orc.plan(n)[2]
+    h = 2.0 ** -10
+    for _ in range(20):
+        count = int(rng.integers(1, 4000))
+        com = rng.uniform(-1.0, 1.0, 3)
+        moment = np.array([count, *np.rint(com * scale * count)], dtype=np.int64).astype(np.int32)
+        while True:
+            target = rng.uniform(-1.2, 1.2, 3).astype(np.float32)
+            if np.linalg.norm(com - target) > 0.3:
+                break
+        eps = float(np.float32(rng.choice([1e-3, 0.05])))
+        a, _, J = unpack(orc.pair_field(slot, moment, target, eps))
+        grad = np.zeros(3)
+        jac = np.zeros((3, 3))
+        for k in range(3):
+            e = np.zeros(3, np.float32)
+            e[k] = h
+            fp = orc.pair_field(slot, moment, target + e, eps)
+            fm = orc.pair_field(slot, moment, target - e, eps)
+            grad[k] = (float(fp[3]) - float(fm[3])) / (2 * h)
+            jac[:, k] = (unpack(fp)[0] - unpack(fm)[0]) / (2 * h)
+        assert np.linalg.norm(a + grad) <= 2e-3 * np.linalg.norm(a)
+        assert np.linalg.norm(J - jac) <= 2e-3 * np.linalg.norm(J)
