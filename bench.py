@@ -157,8 +157,11 @@ def cpu_baseline(parts, cfg, budget_s):
     steps = 0
     t0 = time.perf_counter()
     p = parts
+    phases = dict.fromkeys(orc.PHASES, 0.0)
     while True:
-        p, _, _ = orc.step(slots, p, 1)
+        p, sec = orc.step_timed(slots, p, 1)
+        for k, v in sec.items():
+            phases[k] += v
         steps += 1
         el = time.perf_counter() - t0
         if el >= budget_s and steps >= 3:
@@ -170,6 +173,7 @@ def cpu_baseline(parts, cfg, budget_s):
         "kind": "port",
         "sample": f"{steps} steps of the same {parts.shape[0]}-body workload on the host "
                   f"({el:.1f} s, OpenMP threads={orc.threads()})",
+        "phases_ms": {k: round(v * 1e3 / steps, 3) for k, v in phases.items()},
     }
 
 

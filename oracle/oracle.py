@@ -40,6 +40,7 @@ _SIGNATURES = {
     "oracle_integrate": (None, [_vp, _vp, _vp, _vp]),
     "oracle_solve_fields": (None, [_vp, _vp, _vp, _vp, _i32]),
     "oracle_step": (None, [_vp, _vp, _vp, _vp, _u32, _i32]),
+    "oracle_step_timed": (None, [_vp, _vp, _vp, _vp, _u32, _vp]),
     "oracle_solve_fields_f64": (None, [_vp, _vp, _vp, _vp]),
     "oracle_accelerations": (None, [_vp, _vp, _vp, _vp, _vp, _i32]),
     "oracle_accelerations_f64": (None, [_vp, _vp, _vp, _vp, _vp]),
@@ -155,6 +156,21 @@ def step(slots: np.ndarray, particles: np.ndarray, steps: int, compensated: bool
     parts = np.ascontiguousarray(particles, dtype=np.float32).copy()
     lib().oracle_step(_p(slots), _p(parts), _p(m), _p(f), steps, 1 if compensated else 0)
     return parts, m, f
+
+
+PHASES = ("clear", "deposit", "upward", "m2l", "downward", "near", "integrate")
+
+
+def step_timed(slots: np.ndarray, particles: np.ndarray, steps: int):
+    """`step` (plain sums) with the wall seconds of each phase, summed over
+    the steps: returns (particles, {phase: seconds})."""
+    c = cells_of(slots)
+    m = np.zeros((c, 4), dtype=np.int32)
+    f = np.zeros((c, 12), dtype=np.float32)
+    parts = np.ascontiguousarray(particles, dtype=np.float32).copy()
+    sec = np.zeros(len(PHASES), dtype=np.float64)
+    lib().oracle_step_timed(_p(slots), _p(parts), _p(m), _p(f), steps, _p(sec))
+    return parts, dict(zip(PHASES, sec.tolist()))
 
 
 def solve_fields_f64(slots: np.ndarray, particles: np.ndarray):

@@ -31,6 +31,7 @@
 // relaxation, the smoothstep's Hermite values, a Gaussian influence against
 // exp in f64, exact attract/repel negation and swirl orthogonality.
 
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -964,6 +965,39 @@ void oracle_step(const uint8_t* slots, void* particles, void* moments, void* fie
     else
       integrate<false>(slot(slots, depth), static_cast<const Moment*>(moments),
                        static_cast<const FieldT<float>*>(fields), static_cast<Particle*>(particles));
+  }
+}
+
+// oracle_step (plain sums) with each phase's wall time added into
+// seconds[7]: clear, deposit, upward, m2l_all, downward, near_all, integrate.
+void oracle_step_timed(const uint8_t* slots, void* particles, void* moments, void* fields, uint32_t steps,
+                       double* seconds) {
+  using Clock = std::chrono::steady_clock;
+  const uint32_t depth = slot(slots, 0).depth;
+  Moment* m = static_cast<Moment*>(moments);
+  FieldT<float>* f = static_cast<FieldT<float>*>(fields);
+  Particle* parts = static_cast<Particle*>(particles);
+  for (uint32_t s = 0; s < steps; ++s) {
+    Clock::time_point t = Clock::now();
+    auto lap = [&](int k) {
+      const Clock::time_point now = Clock::now();
+      seconds[k] += std::chrono::duration<double>(now - t).count();
+      t = now;
+    };
+    clear_cells(slot(slots, depth), m);
+    lap(0);
+    deposit(slot(slots, depth), parts, m);
+    lap(1);
+    for (uint32_t l = depth; l-- > 0u;) upward(slot(slots, 8u + l), m);
+    lap(2);
+    m2l_all<float, false>(slot(slots, 0), m, f);
+    lap(3);
+    for (uint32_t l = 1u; l <= depth; ++l) downward<float>(slot(slots, l), m, f);
+    lap(4);
+    near_all<float, false>(slot(slots, 0), m, f);
+    lap(5);
+    integrate<false>(slot(slots, depth), m, f, parts);
+    lap(6);
   }
 }
 

@@ -106,3 +106,18 @@ def test_round_half_to_even():
     m, _ = orc.solve_fields(orc.build_plans(n), parts)
     # every other body sits at 0 and adds 0
     assert m[0, 1] == sum([0, 2, 2, 0, -2, -2, 8])
+
+
+def test_timed_step_is_the_step():
+    # the CPU baseline's per-phase timing runs the same phases in the same
+    # order: identical bodies, and a time for each of the seven phases
+    n = 20000
+    parts = np.zeros((n, 8), np.float32)
+    rng = np.random.default_rng(17)
+    parts[:, :3] = rng.normal(0, 0.3, (n, 3)).clip(-1.5, 1.5)
+    parts[:, 4:7] = rng.normal(0, 0.1, (n, 3))
+    slots = orc.build_plans(n, dt=1e-3, gravity=1.0, softening=1e-3)
+    a, _, _ = orc.step(slots, parts, 2)
+    b, sec = orc.step_timed(slots, parts, 2)
+    np.testing.assert_array_equal(a, b)
+    assert tuple(sec) == orc.PHASES and all(v >= 0.0 for v in sec.values())
