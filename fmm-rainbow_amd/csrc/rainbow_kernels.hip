@@ -1763,13 +1763,15 @@ __device__ __forceinline__ void integrate_body(const SysDesc& d, uint32_t parity
 // caller indices), and each block takes the Morton range it owns from then
 // on: [cell of its first body, cell of the next block's first body); a
 // system's first block starts at 0 and its last ends at 2^18.
-template <bool kComp>
+// kOne: the step holds one system, whose descriptor is then loaded with the
+// block's entry instead of after it (one dependent load less per block).
+template <bool kComp, bool kOne>
 __global__ __launch_bounds__(kDepositThreads) void k_integrate_deposit(StepArgs a) {
   __shared__ DepositLds L;
   __shared__ uint2 own_new;
   reset_lists(a);
   const BodyBlock blk = a.deposit_blocks[blockIdx.x];
-  const SysDesc d = a.sys[blk.sys];
+  const SysDesc d = a.sys[kOne ? 0u : blk.sys];
   if (!d.live) return;
   const bool staged = a.sorting != 0u;
   const uint32_t dim = 1u << d.depth;
@@ -2242,10 +2244,17 @@ hipError_t launch_sort(const StepArgs& a, uint32_t nintegrate, hipStream_t strea
 
 hipError_t launch_integrate_deposit(const StepArgs& a, uint32_t ndeposit, hipStream_t stream, const Timing* t) {
   if (ndeposit == 0u) return no_launch(stream, t);
-  if (a.compensated)
-    RB_LAUNCH(t, k_integrate_deposit<true>, dim3(ndeposit), dim3(kDepositThreads), stream, a);
-  else
-    RB_LAUNCH(t, k_integrate_deposit<false>, dim3(ndeposit), dim3(kDepositThreads), stream, a);
+  const dim3 grid(ndeposit), block(kDepositThreads);
+  if (a.compensated) {
+    if (a.nsys == 1u)
+      RB_LAUNCH(t, (k_integrate_deposit<true, true>), grid, block, stream, a);
+    else
+      RB_LAUNCH(t, (k_integrate_deposit<true, false>), grid, block, stream, a);
+  } else if (a.nsys == 1u) {
+    RB_LAUNCH(t, (k_integrate_deposit<false, true>), grid, block, stream, a);
+  } else {
+    RB_LAUNCH(t, (k_integrate_deposit<false, false>), grid, block, stream, a);
+  }
   return hipGetLastError();
 }
 
