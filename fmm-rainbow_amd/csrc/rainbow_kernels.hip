@@ -115,9 +115,15 @@ __device__ __forceinline__ float div_corrected(float x, float d, float r) {
   return __builtin_fmaf(__builtin_fmaf(-q0, d, x), r, q0);
 }
 
+// The first estimate p * 0.625 is clamped to [-2, 2] before the correction:
+// that leaves the clamped result of every finite p unchanged and keeps
+// infinities out of the remainder (inf - inf), so +inf lands in the last
+// cell and -inf and NaN (quieted by the product; fmax/fmin then return the
+// other operand) in the first, as with the division.
 __device__ __forceinline__ float unit_coord(float p) {
   constexpr float kInv = 0.625f;  // RN(1 / 1.6f)
-  float n = div_corrected(p, kWorldHalfExtent, kInv);
+  const float q0 = fminf(fmaxf(p * kInv, -2.0f), 2.0f);
+  float n = __builtin_fmaf(__builtin_fmaf(-q0, kWorldHalfExtent, p), kInv, q0);
   n = fminf(fmaxf(n, -0.99999f), 0.99999f);
   return n * 0.5f + 0.5f;
 }

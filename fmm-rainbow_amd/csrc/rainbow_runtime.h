@@ -93,6 +93,7 @@ class Engine {
   // work tables and drops captured graphs.  Waits for the stream first.
   int configure(const std::vector<SysDesc>& systems);
   int push_descriptors();
+  int set_physics(const Physics& ph);
   int enqueue(bool integrate, bool sorting, uint32_t step_parity, bool deposit_first, bool rank_next,
               bool record_counts = false);
   hipError_t zero_finest();

@@ -348,6 +348,27 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
   return RAINBOW_OK;
 }
 
+// New physics for every system.  A nonzero scale replaces the systems'
+// fixed-point scale; the finest moments the last fused deposit wrote are at
+// the old scale, so the next step deposits again (as after an upload; the
+// reference re-deposits every frame).
+int Engine::set_physics(const Physics& ph) {
+  bool rescaled = false;
+  for (SysDesc& d : sys) {
+    apply_physics(d, ph);
+    if (ph.scale != 0.0f && ph.scale != d.scale) {
+      d.scale = ph.scale;
+      rescaled = true;
+    }
+  }
+  const int rc = push_descriptors();
+  if (rc == RAINBOW_OK && rescaled) {
+    since_sort = sort_interval;
+    pending_deposit = true;
+  }
+  return rc;
+}
+
 int Engine::push_descriptors() {
   RB_HIP(hipStreamSynchronize(stream));
   if (!sys.empty())
@@ -1026,12 +1047,7 @@ void rainbow_hip_destroy(rainbow_hip_t* h) { delete h; }
 
 int rainbow_hip_set_params(rainbow_hip_t* h, const rainbow_params_t* params) {
   RB_CHECK_HANDLE(h);
-  const rainbow::Physics ph = rainbow::physics_from(params);
-  for (auto& s : h->batch.engine.sys) {
-    rainbow::apply_physics(s, ph);
-    if (ph.scale != 0.0f) s.scale = ph.scale;
-  }
-  return h->batch.engine.push_descriptors();
+  return h->batch.engine.set_physics(rainbow::physics_from(params));
 }
 
 int rainbow_hip_plan(rainbow_hip_t* h, uint32_t* depth, uint32_t* cells, float* scale) {

@@ -4,12 +4,14 @@
 // GPU times, the amortised time of a full step and a finiteness check over
 // a particle slice -- the same report lines, read side by side.
 //
-//   rainbow_native [particles (default 2^20)] [steps (default 40)] [galaxy|plummer]
+//   rainbow_native [particles (default 2^20)] [steps (default 40)]
+//                  [galaxy|plummer|file:<path>]
 //
 // As the reference harness (native/main.rs:423-434), the default scene is
 // the galaxy seeded on the device (initialize, src/rainbow_fmm.wgsl:232-374);
 // `plummer` seeds a Plummer sphere on the host instead (a = 0.1, cut at
-// 1.5, seed 1).  Physics constants: native/main.rs:57-82.
+// 1.5, seed 1); `file:<path>` reads `particles` reference-layout bodies
+// (32 bytes each) from a file.  Physics constants: native/main.rs:57-82.
 
 #include <hip/hip_runtime.h>
 
@@ -34,9 +36,22 @@ int main(int argc, char** argv) {
   const uint32_t count = argc > 1 ? static_cast<uint32_t>(std::strtoul(argv[1], nullptr, 10)) : (1u << 20);
   const uint32_t repeat = argc > 2 ? static_cast<uint32_t>(std::strtoul(argv[2], nullptr, 10)) : 40u;
   const bool plummer = argc > 3 && std::strcmp(argv[3], "plummer") == 0;
-  if (count == 0u || repeat == 0u || (argc > 3 && !plummer && std::strcmp(argv[3], "galaxy") != 0)) {
-    std::fprintf(stderr, "usage: rainbow_native [particles] [steps] [galaxy|plummer]\n");
+  const char* file = argc > 3 && std::strncmp(argv[3], "file:", 5) == 0 ? argv[3] + 5 : nullptr;
+  if (count == 0u || repeat == 0u ||
+      (argc > 3 && !plummer && file == nullptr && std::strcmp(argv[3], "galaxy") != 0)) {
+    std::fprintf(stderr, "usage: rainbow_native [particles] [steps] [galaxy|plummer|file:<path>]\n");
     return 2;
+  }
+  std::vector<float> from_file;
+  if (file != nullptr) {
+    FILE* f = std::fopen(file, "rb");
+    from_file.resize(8ull * count);
+    const size_t got = f != nullptr ? std::fread(from_file.data(), 32, count, f) : 0u;
+    if (f != nullptr) std::fclose(f);
+    if (got != count) {
+      std::fprintf(stderr, "%s: expected %u particles of 32 bytes\n", file, count);
+      return 2;
+    }
   }
 
   hipDeviceProp_t prop;
@@ -59,10 +74,13 @@ int main(int argc, char** argv) {
     std::vector<float> parts(8ull * count);
     check(rainbow_ic_plummer(count, 1u, 0.1, 1.5, params.gravity, parts.data()), "seed");
     check(rainbow_hip_upload_particles(h, parts.data(), parts.size() * sizeof(float)), "upload");
+  } else if (file != nullptr) {
+    check(rainbow_hip_upload_particles(h, from_file.data(), from_file.size() * sizeof(float)), "upload");
   } else {
     check(rainbow_hip_seed_galaxy(h), "seed galaxy");
   }
-  std::printf("scene: %s\n", plummer ? "Plummer sphere (host seed)" : "galaxy (device seed, initialize)");
+  std::printf("scene: %s\n", plummer ? "Plummer sphere (host seed)"
+                                     : (file != nullptr ? "particles from file" : "galaxy (device seed, initialize)"));
 
   // Content digest of one field solve (native/main.rs:436-454).
   check(rainbow_hip_solve_fields(h), "solve");

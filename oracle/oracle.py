@@ -11,6 +11,7 @@ Params slots are the 16 x 256-byte table of native/main.rs:89-128.
 
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import struct
 from pathlib import Path
@@ -53,9 +54,19 @@ _SIGNATURES = {
     "oracle_seed_galaxy": (None, [_u32, _f32, _vp]),
     "oracle_hash32": (_u32, [_u32]),
     "oracle_enclosed_mass": (_f32, [_f32]),
+    "oracle_comp_sum_f32": (_f32, [_vp, _u32, _f32]),
 }
 
 _lib = None
+
+
+def _load(path: Path) -> ctypes.CDLL:
+    h = ctypes.CDLL(str(path))
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(h, name)
+        fn.restype = res
+        fn.argtypes = args
+    return h
 
 
 def lib() -> ctypes.CDLL:
@@ -63,13 +74,21 @@ def lib() -> ctypes.CDLL:
     if _lib is None:
         if not LIB_PATH.exists():
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C {ORACLE_DIR}`")
-        h = ctypes.CDLL(str(LIB_PATH))
-        for name, (res, args) in _SIGNATURES.items():
-            fn = getattr(h, name)
-            fn.restype = res
-            fn.argtypes = args
-        _lib = h
+        _lib = _load(LIB_PATH)
     return _lib
+
+
+@contextlib.contextmanager
+def use_library(path):
+    """Temporarily route every call through another build of the oracle
+    (the mutation tests load builds with a planted mistake)."""
+    global _lib
+    saved = lib()
+    _lib = _load(Path(path))
+    try:
+        yield
+    finally:
+        _lib = saved
 
 
 def _p(a: np.ndarray) -> int:
@@ -96,8 +115,10 @@ _OFF = {"dt": 32, "gravity": 36, "softening": 40, "speed": 76, "control_point": 
         "control_axis_mode": 96, "control_settings": 112}
 
 
-def build_plans(n: int, dt=None, gravity=None, softening=None, speed=None, control=None) -> np.ndarray:
-    """Slot table of native/main.rs:89-128 with the physics overridden."""
+def build_plans(n: int, dt=None, gravity=None, softening=None, speed=None, control=None,
+                scale=None) -> np.ndarray:
+    """Slot table of native/main.rs:89-128 with the physics (and optionally
+    the fixed-point scale, control_settings.z) overridden."""
     slots = np.zeros(4096, dtype=np.uint8)
     depth = lib().oracle_build_plans(n, _p(slots))
     raw = bytearray(slots.tobytes())
@@ -112,6 +133,8 @@ def build_plans(n: int, dt=None, gravity=None, softening=None, speed=None, contr
             raw[base + 80:base + 96] = struct.pack("<4f", *point)
             raw[base + 96:base + 112] = struct.pack("<4f", *axis_mode)
             raw[base + 112:base + 120] = struct.pack("<2f", strength, radius)
+        if scale is not None:
+            raw[base + 120:base + 124] = struct.pack("<f", scale)
     return np.frombuffer(bytes(raw), dtype=np.uint8).copy()
 
 
@@ -256,3 +279,9 @@ def hash32(x: int) -> int:
 
 def enclosed_mass(r: float) -> float:
     return float(lib().oracle_enclosed_mass(r))
+
+
+def comp_sum_f32(terms, init: float = 0.0) -> np.float32:
+    """The Neumaier accumulator of the compensated solve over f32 terms."""
+    t = np.ascontiguousarray(terms, dtype=np.float32)
+    return np.float32(lib().oracle_comp_sum_f32(_p(t), t.shape[0], init))

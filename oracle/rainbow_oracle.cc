@@ -345,14 +345,24 @@ void clear_cells(const Params& p, Moment* moments) {
 }
 
 // deposit, src/rainbow_fmm.wgsl:408-422.  Integer sums are order-free, so
-// per-thread partial sums reduced afterwards give the atomics' result.
+// per-thread partial sums merged afterwards give the atomics' result.  Each
+// thread keeps its partial finest level between calls and lists the cells
+// its bodies touched; only those are merged (relaxed atomic adds, which
+// wrap as WGSL's atomicAdd does) and cleared again.
+struct DepositScratch {
+  std::vector<Moment> cells;
+  std::vector<uint32_t> touched;
+};
+
 void deposit(const Params& p, const Particle* particles, Moment* moments) {
+  static std::vector<DepositScratch> scratch;
   const uint32_t dim = p.level_dim;
   const uint32_t ncell = dim * dim * dim;
   const float fixed_point = p.control_settings[2];
   const int nt = nthreads();
-  std::vector<Moment> partial(static_cast<size_t>(nt) * ncell, Moment{0u, 0, 0, 0});
+  if (scratch.size() < static_cast<size_t>(nt)) scratch.resize(static_cast<size_t>(nt));
   const int64_t n = p.particle_count;
+  Moment* finest = moments + p.level_offset;
 #pragma omp parallel num_threads(nt)
   {
 #ifdef _OPENMP
@@ -360,28 +370,29 @@ void deposit(const Params& p, const Particle* particles, Moment* moments) {
 #else
     const int t = 0;
 #endif
-    Moment* mine = partial.data() + static_cast<size_t>(t) * ncell;
+    DepositScratch& mine = scratch[static_cast<size_t>(t)];
+    if (mine.cells.size() < ncell) mine.cells.assign(ncell, Moment{0u, 0, 0, 0});
+    mine.touched.clear();
 #pragma omp for schedule(static)
     for (int64_t i = 0; i < n; ++i) {
       const float* pos = particles[i].pos;
       const uint32_t c = linear_index(position_cell1(pos[0], dim), position_cell1(pos[1], dim),
                                       position_cell1(pos[2], dim), dim);
-      Moment& m = mine[c];
+      Moment& m = mine.cells[c];
+      if (m.count == 0u) mine.touched.push_back(c);
       m.count += 1u;
       m.sx = wrap_add(m.sx, to_i32_sat(std::nearbyint(pos[0] * fixed_point)));
       m.sy = wrap_add(m.sy, to_i32_sat(std::nearbyint(pos[1] * fixed_point)));
       m.sz = wrap_add(m.sz, to_i32_sat(std::nearbyint(pos[2] * fixed_point)));
     }
-  }
-  ORACLE_PARALLEL_FOR
-  for (int64_t c = 0; c < static_cast<int64_t>(ncell); ++c) {
-    Moment& dst = moments[p.level_offset + c];
-    for (int t = 0; t < nt; ++t) {
-      const Moment& s = partial[static_cast<size_t>(t) * ncell + c];
-      dst.count += s.count;
-      dst.sx = wrap_add(dst.sx, s.sx);
-      dst.sy = wrap_add(dst.sy, s.sy);
-      dst.sz = wrap_add(dst.sz, s.sz);
+    for (const uint32_t c : mine.touched) {
+      Moment& s = mine.cells[c];
+      Moment& dst = finest[c];
+      __atomic_fetch_add(&dst.count, s.count, __ATOMIC_RELAXED);
+      __atomic_fetch_add(reinterpret_cast<uint32_t*>(&dst.sx), static_cast<uint32_t>(s.sx), __ATOMIC_RELAXED);
+      __atomic_fetch_add(reinterpret_cast<uint32_t*>(&dst.sy), static_cast<uint32_t>(s.sy), __ATOMIC_RELAXED);
+      __atomic_fetch_add(reinterpret_cast<uint32_t*>(&dst.sz), static_cast<uint32_t>(s.sz), __ATOMIC_RELAXED);
+      s = Moment{0u, 0, 0, 0};
     }
   }
 }
@@ -1173,6 +1184,16 @@ void oracle_conserved_f64(uint32_t n, const void* particles, double* out) {
 void oracle_seed_galaxy(uint32_t n, float gravity, void* particles) {
   Particle* parts = static_cast<Particle*>(particles);
   for (uint32_t i = 0; i < n; ++i) parts[i] = seed_body(i, gravity);
+}
+
+// The Neumaier accumulator the compensated field solve uses, over `n` f32
+// terms starting from `init` (the value already in the field, as near_all
+// starts from the M2L and L2L result): returns s + c.
+float oracle_comp_sum_f32(const float* terms, uint32_t n, float init) {
+  Comp<float> acc;
+  acc.s = init;
+  for (uint32_t i = 0; i < n; ++i) acc.add(terms[i]);
+  return acc.value();
 }
 
 uint32_t oracle_hash32(uint32_t x) { return hash32(x); }

@@ -6,6 +6,9 @@
 // q can differ from x / d in the last bit for some x; what is checked here is
 // the quantity the kernel actually uses after the division:
 //   unit  : clamp(q, -0.99999, 0.99999) * 0.5 + 0.5   (position -> unit cube)
+//   position: the same for every float bit pattern, infinities and NaN
+//           included, with the first estimate x * r clamped to [-2, 2] as
+//           the kernel does (fmax/fmin return the non-NaN operand)
 //   clamp01: clamp(q, 0, 1)                            (ramps and weights)
 // and two ramps of the integrate step in the exact form the kernel uses,
 // where the dividend is first clamped to a range that leaves the clamped
@@ -31,9 +34,10 @@ float clampf(float v, float lo, float hi) { return std::fmin(std::fmax(v, lo), h
 }  // namespace
 
 int main(int argc, char** argv) {
-  if (argc != 3) { std::fprintf(stderr, "usage: %s <d> <unit|clamp01>\n", argv[0]); return 2; }
+  if (argc != 3) { std::fprintf(stderr, "usage: %s <d> <unit|position|clamp01|excitation|density>\n", argv[0]); return 2; }
   const float d = std::strtof(argv[1], nullptr);
-  const bool unit = std::strcmp(argv[2], "unit") == 0;
+  const bool position = std::strcmp(argv[2], "position") == 0;
+  const bool unit = position || std::strcmp(argv[2], "unit") == 0;
   const bool excitation = std::strcmp(argv[2], "excitation") == 0;
   const bool density = std::strcmp(argv[2], "density") == 0;
   if (!unit && !excitation && !density && std::strcmp(argv[2], "clamp01") != 0) return 2;
@@ -58,10 +62,11 @@ int main(int argc, char** argv) {
   }
 #pragma omp parallel for reduction(+ : raw, kept) schedule(static, 1 << 20)
   for (long long i = 0; i < (1ll << 32); ++i) {
-    const float x = from_bits(static_cast<uint32_t>(i));
-    if (!std::isfinite(x)) continue;
-    const float exact = x / dv;
-    const float q0 = x * r;
+    const float xi = from_bits(static_cast<uint32_t>(i));
+    if (!position && !std::isfinite(xi)) continue;
+    const float exact = xi / dv;
+    const float x = xi;
+    const float q0 = position ? clampf(x * r, -2.0f, 2.0f) : x * r;
     const float rem = std::fmaf(-q0, d, x);
     const float q = std::fmaf(rem, r, q0);
     if (to_bits(q) == to_bits(exact)) continue;

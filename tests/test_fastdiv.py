@@ -35,6 +35,15 @@ def test_unit_coord_division_exact_for_all_floats(checker):
     assert "kept values differing 0" in r.stdout
 
 
+def test_position_map_exact_for_every_bit_pattern(checker):
+    # The kernel's full form, dividend clamped to [-2, 2] first, over all
+    # 2^32 bit patterns: +inf must land where the division puts it (the last
+    # cell), -inf and NaN in the first.
+    r = subprocess.run([str(checker), "1.6", "position"], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "kept values differing 0" in r.stdout
+
+
 def test_integrate_ramps_exact_for_all_inputs(checker):
     # The excitation ramp over every float excitation, the density ramp over
     # every uint32 body count (divisors as the kernel forms them).
