@@ -1885,19 +1885,35 @@ __global__ __launch_bounds__(64) void k_diag_final(const double* partial, uint32
 // added in chunk order, so the result does not depend on scheduling.
 // ------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_invert_perm(SysDesc d, uint32_t* inv) {
+// The slot holding each target body (caller index targets[t]): every slot
+// compares its caller index with the targets staged in LDS, so a target
+// listed twice gets its slot twice.  slot_of is filled with ~0 beforehand; a
+// slot that is never found stays ~0 and is clamped by the sum.
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-  if (i >= d.n) return;
-  const uint32_t c = G(d.perm)[i];
-  if (c < d.n) inv[c] = i;
+  if (i < n) p[i] = v;
 }
-
-__global__ __launch_bounds__(256) void k_direct_f64(SysDesc d, const uint32_t* inv, const uint32_t* targets,
-                                                    uint32_t nt, uint32_t chunk, double* partial) {
+__global__ __launch_bounds__(256) void k_find_slots(SysDesc d, const uint32_t* targets, uint32_t nt,
+                                                    uint32_t* slot_of) {
+  __shared__ uint32_t tl[kDirectMaxTargetsPerPass];
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t c = i < d.n ? G(d.perm)[i] : 0xFFFFFFFFu;
+  for (uint32_t base = 0; base < nt; base += kDirectMaxTargetsPerPass) {
+    const uint32_t m = min(kDirectMaxTargetsPerPass, nt - base);
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < m; k += 256u) tl[k] = targets[base + k];
+    __syncthreads();
+    if (i < d.n)
+      for (uint32_t k = 0; k < m; ++k)
+        if (tl[k] == c) slot_of[base + k] = i;
+  }
+}
+__global__ __launch_bounds__(256) void k_direct_f64(SysDesc d, const uint32_t* slot_of, uint32_t nt,
+                                                    uint32_t chunk, double* partial) {
   __shared__ double sx[256], sy[256], sz[256];
   const uint32_t t = blockIdx.x * 256u + threadIdx.x;
   const bool valid = t < nt;
-  const uint32_t self = valid ? min(inv[min(targets[t], d.n - 1u)], d.n - 1u) : 0xFFFFFFFFu;
+  const uint32_t self = valid ? min(slot_of[t], d.n - 1u) : 0xFFFFFFFFu;
   double xi = 0.0, yi = 0.0, zi = 0.0;
   if (valid) {
     const float4 p = G(d.pos)[self];
@@ -2107,14 +2123,22 @@ __global__ __launch_bounds__(256) void k_join_particles(SysDesc d, float4* out) 
 // launch wrappers
 // ------------------------------------------------------------------------
 
-hipError_t launch_direct_f64(const SysDesc& d, const uint32_t* targets, uint32_t nt, uint32_t* inv,
-                             double* partial, uint32_t nchunks, double* out, hipStream_t stream) {
+hipError_t launch_direct_f64(const SysDesc& d, uint32_t nt, void* scratch, hipStream_t stream) {
   if (nt == 0u || d.n == 0u) return hipSuccess;
-  hipLaunchKernelGGL(k_invert_perm, dim3((d.n + 255u) / 256u), dim3(256), 0, stream, d, inv);
+  const DirectLayout L = direct_layout(nt);
+  char* base = static_cast<char*>(scratch);
+  const uint32_t* targets = reinterpret_cast<const uint32_t*>(base + L.targets);
+  uint32_t* slot_of = reinterpret_cast<uint32_t*>(base + L.slots);
+  double* partial = reinterpret_cast<double*>(base + L.partial);
+  double* out = reinterpret_cast<double*>(base + L.out);
+  hipLaunchKernelGGL(k_fill_u32, dim3((nt + 255u) / 256u), dim3(256), 0, stream, slot_of, nt, 0xFFFFFFFFu);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_find_slots, dim3((d.n + 255u) / 256u), dim3(256), 0, stream, d, targets, nt, slot_of);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  const uint32_t nchunks = L.nchunks;
   const uint32_t chunk = ((d.n + nchunks - 1u) / nchunks + 255u) / 256u * 256u;
-  hipLaunchKernelGGL(k_direct_f64, dim3((nt + 255u) / 256u, nchunks), dim3(256), 0, stream, d, inv, targets, nt,
+  hipLaunchKernelGGL(k_direct_f64, dim3((nt + 255u) / 256u, nchunks), dim3(256), 0, stream, d, slot_of, nt,
                      chunk, partial);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   hipLaunchKernelGGL(k_direct_final, dim3((nt + 255u) / 256u), dim3(256), 0, stream, d, nt, nchunks, partial, out);

@@ -40,11 +40,33 @@ hipError_t launch_integrate_deposit(const StepArgs& a, uint32_t ndeposit, hipStr
 // The counting sort's scatter into the staging arrays (after upward and top
 // of a step run with a.sorting set; that step's integrate reads them).
 hipError_t launch_sort(const StepArgs& a, uint32_t nintegrate, hipStream_t stream);
-// FP64 direct sum at `nt` target bodies (caller indices, device array):
-// out[4 t ..] = {ax, ay, az, phi}.  inv: n words of scratch; partial:
-// 4 nt nchunks doubles of scratch.
-hipError_t launch_direct_f64(const SysDesc& d, const uint32_t* targets, uint32_t nt, uint32_t* inv,
-                             double* partial, uint32_t nchunks, double* out, hipStream_t stream);
+// FP64 direct sum at `nt` target bodies given by caller index.  It works in
+// one scratch buffer laid out by direct_layout(nt): the target list (the
+// caller copies it in), each target's slot, the partial sums of the body
+// chunks (direct_chunks(nt) of them, a fixed rule so that every caller gets
+// the same summation order) and the result out[4 t ..] = {ax, ay, az, phi}.
+constexpr uint32_t kDirectMaxTargetsPerPass = 4096u;  // targets staged in LDS per pass
+constexpr uint32_t kDirectMaxChunks = 128u;
+constexpr uint32_t kDirectBlocks = 2048u;  // target tiles x body chunks
+constexpr uint32_t direct_chunks(uint32_t nt) {
+  const uint32_t tiles = (nt + 255u) / 256u;
+  const uint32_t c = tiles == 0u ? kDirectMaxChunks : kDirectBlocks / tiles;
+  return c < 1u ? 1u : (c > kDirectMaxChunks ? kDirectMaxChunks : c);
+}
+struct DirectLayout {
+  size_t targets, slots, partial, out, bytes;
+  uint32_t nchunks;
+};
+constexpr size_t direct_align(size_t x) { return (x + 255u) & ~static_cast<size_t>(255u); }
+constexpr DirectLayout direct_layout(uint32_t nt) {
+  const uint32_t nchunks = direct_chunks(nt);
+  const size_t slots = direct_align(4ull * nt);
+  const size_t out = slots + direct_align(4ull * nt);
+  const size_t partial = out + direct_align(32ull * nt);
+  return DirectLayout{0u, slots, partial, out, partial + direct_align(32ull * nt * nchunks), nchunks};
+}
+constexpr size_t direct_scratch_bytes(uint32_t nt) { return direct_layout(nt).bytes; }
+hipError_t launch_direct_f64(const SysDesc& d, uint32_t nt, void* scratch, hipStream_t stream);
 // The reference's default scene (initialize, src/rainbow_fmm.wgsl:232-374)
 // into a system's bodies, in caller order.
 hipError_t launch_seed_galaxy(const SysDesc& d, hipStream_t stream);

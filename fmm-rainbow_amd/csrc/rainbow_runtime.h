@@ -110,6 +110,8 @@ class Engine {
   int accelerations(uint32_t sys, void* dst, size_t bytes);
   int diagnostics(uint32_t sys, double out[8]);
   int direct_sum_f64(uint32_t sys, const uint32_t* targets, uint32_t nt, double* out);
+  int check_direct_args(uint32_t sys, const uint32_t* targets, uint32_t nt, const double* out) const;
+  int direct_sum_f64_in(uint32_t sys, const uint32_t* targets, uint32_t nt, double* out, void* scratch);
   int set_sort_interval(uint32_t interval);
   int set_tree_fusion(uint32_t hi);
   int set_tree_split(uint32_t starts);  // fixes the split (no measured choice)
@@ -117,6 +119,7 @@ class Engine {
   int snapshot(bool restore);
 
   hipStream_t stream = nullptr;
+  int device = 0;  // current device at open(); the stream belongs to it
   uint32_t options = 0;
   uint32_t dmax = 0;
   uint64_t total_bodies = 0;
@@ -180,3 +183,12 @@ class Batch {
 };
 
 }  // namespace rainbow
+
+// The C ABI handles of include/rainbow_fmm.h: a solver is a batch of one.
+struct rainbow_hip {
+  rainbow::Batch batch;
+};
+
+struct rainbow_batch {
+  rainbow::Batch batch;
+};

@@ -205,6 +205,7 @@ void Engine::drop_graphs() {
 
 int Engine::open(uint32_t opts) {
   options = opts;
+  RB_HIP(hipGetDevice(&device));
   RB_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   RB_HIP(hipEventCreate(&ev_a));
   RB_HIP(hipEventCreate(&ev_b));
@@ -936,24 +937,31 @@ int Engine::accelerations(uint32_t s, void* dst, size_t bytes) {
 }
 
 int Engine::direct_sum_f64(uint32_t s, const uint32_t* targets, uint32_t nt, double* out) {
+  int rc = check_direct_args(s, targets, nt, out);
+  if (rc != RAINBOW_OK || nt == 0u) return rc;
+  if ((rc = io_buf.ensure(direct_scratch_bytes(nt))) != RAINBOW_OK) return rc;
+  return direct_sum_f64_in(s, targets, nt, out, io_buf.p);
+}
+
+int Engine::check_direct_args(uint32_t s, const uint32_t* targets, uint32_t nt, const double* out) const {
   if (s >= sys.size()) return fail(RAINBOW_E_INVALID, "system %u out of range", s);
   if (nt == 0u) return RAINBOW_OK;
   if (targets == nullptr || out == nullptr) return fail(RAINBOW_E_INVALID, "null argument");
   const SysDesc& d = sys[s];
   for (uint32_t t = 0; t < nt; ++t)
     if (targets[t] >= d.n) return fail(RAINBOW_E_INVALID, "target %u out of range", targets[t]);
-  const uint32_t nchunks = std::max(1u, std::min(256u, 4096u / ((nt + 255u) / 256u)));
-  const size_t need = 4ull * d.n + 4ull * nt + 8ull * 4u * nt * (nchunks + 1u) + 64u;
-  int rc = io_buf.ensure(need);
-  if (rc) return rc;
-  char* p = static_cast<char*>(io_buf.p);
-  uint32_t* inv = reinterpret_cast<uint32_t*>(p);
-  uint32_t* dt = inv + d.n;
-  double* partial = reinterpret_cast<double*>(p + ((4ull * d.n + 4ull * nt + 7u) & ~7ull));
-  double* dout = partial + 4ull * nt * nchunks;
-  RB_HIP(hipMemcpyAsync(dt, targets, 4ull * nt, hipMemcpyHostToDevice, stream));
-  RB_HIP(launch_direct_f64(d, dt, nt, inv, partial, nchunks, dout, stream));
-  RB_HIP(hipMemcpyAsync(out, dout, 32ull * nt, hipMemcpyDeviceToHost, stream));
+  return RAINBOW_OK;
+}
+
+// The direct sum in a scratch buffer of direct_scratch_bytes(nt) bytes
+// (arguments already checked): targets in, kernels, result out, then a
+// wait for the stream.
+int Engine::direct_sum_f64_in(uint32_t s, const uint32_t* targets, uint32_t nt, double* out, void* scratch) {
+  const DirectLayout L = direct_layout(nt);
+  char* base = static_cast<char*>(scratch);
+  RB_HIP(hipMemcpyAsync(base + L.targets, targets, 4ull * nt, hipMemcpyHostToDevice, stream));
+  RB_HIP(launch_direct_f64(sys[s], nt, scratch, stream));
+  RB_HIP(hipMemcpyAsync(out, base + L.out, 32ull * nt, hipMemcpyDeviceToHost, stream));
   RB_HIP(hipStreamSynchronize(stream));
   return RAINBOW_OK;
 }
@@ -1003,13 +1011,6 @@ int Batch::init(const std::vector<uint32_t>& counts, const Physics& ph, uint32_t
 using rainbow::Batch;
 using rainbow::fail;
 
-struct rainbow_hip {
-  Batch batch;
-};
-
-struct rainbow_batch {
-  Batch batch;
-};
 
 #define RB_CHECK_HANDLE(h) \
   if ((h) == nullptr) return fail(RAINBOW_E_INVALID, "null handle")

@@ -137,6 +137,7 @@ _SIGNATURES = {
     "rainbow_hip_set_tree_split": (_i32, [_vp, _u32]),
     "rainbow_batch_set_tree_split": (_i32, [_vp, _u32]),
     "rainbow_hip_direct_sum_f64": (_i32, [_vp, _P(_u32), _u32, _P(_f64)]),
+    "rainbow_hip_direct_sum_f64_noalloc": (_i32, [_vp, _P(_u32), _u32, _P(_f64)]),
     "rainbow_hip_tree_launches": (_i32, [_vp, _P(_u32), _u32]),
     "rainbow_batch_set_tree_fusion": (_i32, [_vp, _u32]),
     "rainbow_hip_tree_stats": (_i32, [_vp, _P(_u32)]),

@@ -328,6 +328,17 @@ int rainbow_hip_diagnostics(rainbow_hip_t* h, double out[8]);
  * measurements; O(N ntargets) work on the device. */
 int rainbow_hip_direct_sum_f64(rainbow_hip_t* h, const uint32_t* targets, uint32_t ntargets, double* out);
 
+/* The same FP64 direct sum without allocating: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It runs the kernels of rainbow_hip_direct_sum_f64 in a fixed 17 MiB
+ * device array that each GPU the library runs on holds until the process
+ * ends, so its output equals that function's bit for bit.  At most 4096
+ * targets (more: RAINBOW_E_INVALID).  Calls that overlap on one GPU share
+ * the array and race; the call returns after its own stream has finished. */
+int rainbow_hip_direct_sum_f64_noalloc(rainbow_hip_t* h, const uint32_t* targets, uint32_t ntargets,
+                                       double* out);
+
 /* Per-body acceleration the integrator applies (local expansion plus the
  * own-leaf monopole), as float4 {ax, ay, az, phi} per body, for the current
  * particles and the fields of the last solve.  `bytes` = 16 N. */
