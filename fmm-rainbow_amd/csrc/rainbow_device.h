@@ -23,10 +23,29 @@
 //   m2l     float4[3 level_offset(D)]  M2L sums of the cells above the
 //                       finest level, written by the tree pass's first
 //                       launch and read by its second
+// and the system's counting-sort state (bodies are re-sorted into Morton
+// order of their finest cell every few steps):
+//   rank    uint32[n]   each body's index within its cell's bodies, from
+//                       the deposit (bit 31: after the cell's bodies stored
+//                       by its owning block)
+//   tpos, tvel float4[n] each, contiguous: the sorted bodies, written by the
+//                       scatter and read by that step's integrate; between
+//                       steps the 32 n bytes stage particles in the
+//                       reference's interleaved layout for uploads and
+//                       downloads
+//   tperm   uint32[n]   caller index of each sorted body
+//   cstart, xstart uint32[8^D]  offset of each finest cell (Morton order)
+//                       within its 8^3 subtree, without and with the cell's
+//                       owned bodies
+//   sstart  uint32[8^(D-3)]  offset of each subtree within the system
+//   own     uint2[ceil(n / 1024)]  Morton range of finest cells each
+//                       deposit range of 1024 bodies owns (stores without
+//                       atomics); (0, 0) owns nothing
 // with the reference's cell numbering level_offset(l) + x + 2^l (y + 2^l z).
 // A step works on a set of systems (a plain solve is a set of one); each
-// system's descriptor carries its own buffer pointers, so systems can join
-// and leave the set without moving memory.
+// system's descriptor carries its own buffer pointers and nothing of a
+// system is indexed by its place in the set, so systems join and leave the
+// set between steps without moving memory or losing state.
 #pragma once
 
 #include <cstdint>
@@ -47,10 +66,16 @@ struct alignas(16) SysDesc {
   int4* finest_alt;
   float4* m2l;         // M2L sums of the levels below the finest, fields layout
   uint8_t* xflag;      // per finest cell: stragglers were added into `extra`
+  uint32_t* rank;      // counting sort: see the layout above
+  float4* tpos;
+  float4* tvel;
+  uint32_t* tperm;
+  uint32_t* cstart;
+  uint32_t* xstart;
+  uint32_t* sstart;
+  uint2* own;
   uint32_t n;          // bodies (particle_count)
   uint32_t depth;      // octree depth
-  uint32_t body_base;  // first index of the system in the batch's body space
-  uint32_t cell_base;  // first finest cell of the system in the batch's cell space
   float scale;        // fixed-point scale (control_settings.z)
   float dt;
   float gravity;
@@ -62,7 +87,7 @@ struct alignas(16) SysDesc {
   float control_point[4];
   float control_axis_mode[4];
 };
-static_assert(sizeof(SysDesc) == 176, "SysDesc layout");
+static_assert(sizeof(SysDesc) == 224, "SysDesc layout");
 
 // Work list of the particle passes: a block covers bodies [first, first +
 // block size) of one system.
@@ -90,24 +115,16 @@ struct StepArgs {
   const BodyBlock* deposit_blocks;
   const BodyBlock* integrate_blocks;
   const SubtreeBlock* subtree_blocks;
-  // Per deposit block: the Morton range [lo, hi) of finest cells it owns
-  // (stores without atomics); bodies of other cells go to `extra`.  Ranges
-  // of one system's blocks are disjoint; (0, 0) owns nothing.
-  uint2* own;
   // Counting sort of the bodies by the Morton code of their finest cell,
-  // run within a step when `sorting` is set: the deposit writes each body's
-  // index within its cell to rank (bit 31 set: after the cell's owned
-  // bodies), upward writes each cell's offset within its subtree to cstart
-  // and top each subtree's offset within its system to sstart (both in
-  // Morton order, indexed from the system's cell_base), and the scatter
-  // moves the bodies to tpos/tvel/tperm, copied back before integrate.
-  uint32_t* rank;
-  uint32_t* cstart;
-  uint32_t* xstart;  // cstart + the cell's bodies stored by its owning block
-  uint32_t* sstart;
-  float4* tpos;
-  float4* tvel;
-  uint32_t* tperm;
+  // run within a step when `sorting` is set (each system's rank, offsets,
+  // staging and owned ranges are in its descriptor): the deposit writes
+  // each body's index within its cell to rank (bit 31 set: after the cell's
+  // owned bodies), upward writes each cell's offset within its subtree to
+  // cstart/xstart and top each subtree's offset to sstart (both in Morton
+  // order), the scatter moves the bodies to tpos/tvel/tperm, and the step's
+  // integrate reads them from there and sets the owned ranges.  A deposit
+  // block owns the Morton range [lo, hi) of finest cells (stores without
+  // atomics); bodies of other cells go to `extra`.
   uint32_t* lists;             // internal cells (count > 16)
   uint32_t* list_count;        // [kLevels][kStripes] items per stripe
   const uint32_t* stripe_off;  // [kLevels][kStripes] first slot of a stripe

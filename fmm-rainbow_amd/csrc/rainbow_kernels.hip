@@ -199,7 +199,7 @@ struct DepositLds {
 // Deposit of one block's bodies first + k * kDepositThreads + thread
 // (positions p[k]) into the finest buffer `parity` of system d; writes the
 // bodies' ranks when `ranks`.  All threads of the block call it.
-__device__ void deposit_block(const StepArgs& a, const SysDesc& d, uint32_t block, uint32_t first,
+__device__ void deposit_block(const StepArgs& a, const SysDesc& d, uint32_t first,
                               const float4 (&p)[kDepositPerThread], uint32_t parity, bool ranks,
                               DepositLds& L, const uint2* own_lds = nullptr) {
   uint32_t* const keys = L.keys;
@@ -282,7 +282,7 @@ __device__ void deposit_block(const StepArgs& a, const SysDesc& d, uint32_t bloc
   // stored.  Any other cell's sums go to `extra` with atomics (bodies that
   // moved since the last sort, and cells shared with other blocks);
   // upward adds `extra` into the finest level.
-  const uint2 own = own_lds != nullptr ? *own_lds : G(a.own)[block];
+  const uint2 own = own_lds != nullptr ? *own_lds : G(d.own)[first / kDepositBodies];
   const gptr<int4> finest = finest_of(d, parity);
   const uint32_t nused = L.nused;
   for (uint32_t u = threadIdx.x; u < nused; u += kDepositThreads) {
@@ -318,7 +318,7 @@ __device__ void deposit_block(const StepArgs& a, const SysDesc& d, uint32_t bloc
 #pragma unroll
   for (uint32_t k = 0; k < kDepositPerThread; ++k) {
     const uint32_t i = first + k * kDepositThreads + threadIdx.x;
-    if (i < d.n) G(a.rank)[d.body_base + i] = keys[my_slot[k]] + my_rank[k];
+    if (i < d.n) G(d.rank)[i] = keys[my_slot[k]] + my_rank[k];
   }
 }
 
@@ -343,7 +343,7 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(StepArgs a) {
     const uint32_t i = blk.first + k * kDepositThreads + threadIdx.x;
     p[k] = i < d.n ? G(d.pos)[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
-  deposit_block(a, d, blockIdx.x, blk.first, p, a.parity, a.sorting != 0u, L);
+  deposit_block(a, d, blk.first, p, a.parity, a.sorting != 0u, L);
 }
 
 // Morton code of the finest cell of a position.
@@ -375,16 +375,15 @@ __global__ __launch_bounds__(kIntegrateThreads) void k_scatter(StepArgs a) {
   const uint32_t dim = 1u << d.depth;
   const uint32_t cx = cell1(unit_coord(p.x), dim), cy = cell1(unit_coord(p.y), dim), cz = cell1(unit_coord(p.z), dim);
   const uint32_t m = spread3(cx) | (spread3(cy) << 1) | (spread3(cz) << 2);
-  const uint32_t r = G(a.rank)[d.body_base + i];
+  const uint32_t r = G(d.rank)[i];
   // Bodies flagged in the rank follow the cell's bodies stored by its
   // owning block.
-  const gptr<uint32_t> cs = (r & kRankExtra) ? G(a.xstart) : G(a.cstart);
-  const uint32_t dst = G(a.sstart)[(d.cell_base >> 9) + (m >> 9)] + cs[d.cell_base + m] + (r & ~kRankExtra);
+  const gptr<uint32_t> cs = (r & kRankExtra) ? G(d.xstart) : G(d.cstart);
+  const uint32_t dst = G(d.sstart)[m >> 9] + cs[m] + (r & ~kRankExtra);
   if (dst >= d.n) return;  // unreachable when the counts are consistent
-  const uint32_t g = d.body_base + dst;
-  G(a.tpos)[g] = p;
-  G(a.tvel)[g] = G(d.vel)[i];
-  G(a.tperm)[g] = G(d.perm)[i];
+  G(d.tpos)[dst] = p;
+  G(d.tvel)[dst] = G(d.vel)[i];
+  G(d.tperm)[dst] = G(d.perm)[i];
 }
 
 // ------------------------------------------------------------------------
@@ -631,13 +630,13 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
     for (uint32_t w = 0; w < (t >> 6); ++w) base += wsum8[w];
     const uint32_t start = base + incl - cnt;
     const uint32_t root_m = spread3(rx) | (spread3(ry) << 1) | (spread3(rz) << 2);
-    const uint32_t g = d.cell_base + (root_m << 9) + t;
+    const uint32_t g = (root_m << 9) + t;
     // The owned count of Morton cell t lives with the thread of its lin index.
     __shared__ uint32_t owned_s[512];
     owned_s[t] = owned;
     __syncthreads();
-    G(a.cstart)[g] = start;
-    G(a.xstart)[g] = start + owned_s[mi];
+    G(d.cstart)[g] = start;
+    G(d.xstart)[g] = start + owned_s[mi];
   }
   if (t < 64u) {
     const uint32_t cx = t & 3u, cy = (t >> 2) & 3u, cz = t >> 4;
@@ -792,7 +791,7 @@ __device__ void top_levels(const StepArgs& a, uint32_t s, TopLds& L) {
       if (e < nr) {
         uint32_t base = 0u;
         for (uint32_t w = 0; w < (e >> 6); ++w) base += wsum8[w];
-        G(a.sstart)[(d.cell_base >> 9) + e] = base + incl[k] - own[k];
+        G(d.sstart)[e] = base + incl[k] - own[k];
       }
     }
   }
@@ -1787,17 +1786,16 @@ __global__ __launch_bounds__(kDepositThreads) void k_integrate_deposit(StepArgs 
     const uint32_t i = blk.first + k * kDepositThreads + threadIdx.x;
     p[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (i < d.n) {
-      const uint32_t g = d.body_base + i;
-      float4 p4 = staged ? G(a.tpos)[g] : G(d.pos)[i];
-      float4 v4 = staged ? G(a.tvel)[g] : G(d.vel)[i];
+      float4 p4 = staged ? G(d.tpos)[i] : G(d.pos)[i];
+      float4 v4 = staged ? G(d.tvel)[i] : G(d.vel)[i];
       if (staged) {
-        G(d.perm)[i] = G(a.tperm)[g];
+        G(d.perm)[i] = G(d.tperm)[i];
         if (k == 0u && threadIdx.x == 0u) {
           const uint32_t next = blk.first + kDepositBodies;
           const uint32_t lo = blk.first == 0u ? 0u : morton_of(p4, dim);
-          const uint32_t hi = next >= d.n ? (1u << 18) : morton_of(G(a.tpos)[d.body_base + next], dim);
+          const uint32_t hi = next >= d.n ? (1u << 18) : morton_of(G(d.tpos)[next], dim);
           own_new = make_uint2(lo, hi);
-          G(a.own)[blockIdx.x] = own_new;
+          G(d.own)[blk.first / kDepositBodies] = own_new;
         }
       }
       // Depth as a constant: the level loops of the leaf search unroll to
@@ -1813,7 +1811,7 @@ __global__ __launch_bounds__(kDepositThreads) void k_integrate_deposit(StepArgs 
       p[k] = p4;
     }
   }
-  deposit_block(a, d, blockIdx.x, blk.first, p, a.parity ^ 1u, a.rank_next != 0u, L, staged ? &own_new : nullptr);
+  deposit_block(a, d, blk.first, p, a.parity ^ 1u, a.rank_next != 0u, L, staged ? &own_new : nullptr);
 }
 
 // ------------------------------------------------------------------------

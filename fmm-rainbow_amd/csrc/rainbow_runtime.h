@@ -62,14 +62,16 @@ size_t system_storage_bytes(uint32_t n);
 // Allocates and zeroes a system's storage; scale 0 = planner scale.
 int alloc_system(uint32_t n, const Physics& ph, float scale, hipStream_t stream, SystemStorage* out);
 void free_system(SystemStorage* s);
+// The descriptor of a system of n bodies whose storage starts at `base`
+// (system_storage_bytes(n) bytes); scale 0 = planner scale.
+SysDesc describe_system(void* base, uint32_t n, const Physics& ph, float scale);
 // Copies reference-layout particles in / a buffer out (synchronous on stream).
-struct DevBuf;
 // Particles cross the bus in the reference's interleaved layout through the
-// device staging buffer `io` (grown as needed); kernels split them into the
-// pos/vel streams and restore caller order on the way out.
-int upload_system(const SysDesc& d, const void* data, size_t bytes, hipStream_t stream, DevBuf* io);
+// system's own staging area (tpos/tvel, 32 n bytes); kernels split them into
+// the pos/vel streams and restore caller order on the way out.
+int upload_system(const SysDesc& d, const void* data, size_t bytes, hipStream_t stream);
 // finest_parity: the finest-level buffer holding the last solve's moments.
-int download_system(const SysDesc& d, int which, void* dst, size_t bytes, hipStream_t stream, DevBuf* io,
+int download_system(const SysDesc& d, int which, void* dst, size_t bytes, hipStream_t stream,
                     uint32_t finest_parity = 0u);
 
 // A device buffer that only grows.
@@ -90,8 +92,17 @@ class Engine {
 
   int open(uint32_t options);
   // Installs the system set (descriptors with device pointers).  Rebuilds the
-  // work tables and drops captured graphs.  Waits for the stream first.
+  // work tables and drops captured graphs.  Waits for the stream first; the
+  // next step deposits every system again.
   int configure(const std::vector<SysDesc>& systems);
+  // The same between steps without waiting: systems keep their state, and
+  // `fresh` (indices into `systems`, storage zeroed, bodies in place) are
+  // deposited by the next step.
+  int configure_async(const std::vector<SysDesc>& systems, const std::vector<uint32_t>& fresh);
+  // Table capacity for later installs of up to these totals.
+  int reserve(uint64_t bodies, uint32_t systems, uint64_t finest);
+  // Systems join only through configure_async from now on (the server).
+  void start_incremental();
   int push_descriptors();
   int set_physics(const Physics& ph);
   int enqueue(bool integrate, bool sorting, uint32_t step_parity, bool deposit_first, bool rank_next,
@@ -128,6 +139,7 @@ class Engine {
   uint32_t n_deposit = 0;
   uint32_t n_integrate = 0;
   uint32_t n_subtree = 0;
+  uint32_t n_fresh = 0;  // deposit blocks of systems that joined and await their first deposit
   uint32_t tree_blocks[kLevels + 1] = {};
   std::vector<TreeLaunch> tree_launches;
   uint32_t tree_split = kDefaultTreeSplit;
@@ -148,11 +160,26 @@ class Engine {
   int adapt_split(bool wait);
   void drop_graphs();
 
-  DevBuf sys_buf, deposit_buf, integrate_buf, subtree_buf, list_buf, count_buf,
-      stripe_buf, diag_buf, body_buf, snap_buf, own_buf, rank_buf, cstart_buf, sstart_buf;
+  int install(const std::vector<SysDesc>& systems, const std::vector<uint32_t>& fresh);
+
+  DevBuf sys_buf, deposit_buf, fresh_buf, integrate_buf, subtree_buf, list_buf, count_buf, stripe_buf, diag_buf,
+      snap_buf;
+  // Host copies of the work tables (kept to reuse their storage).
+  std::vector<BodyBlock> dep_, integ_, fresh_;
+  std::vector<SubtreeBlock> sub_;
+  // Pinned staging of the table copies, a ring: a buffer is refilled once
+  // the copies that read it have run (its event).
+  struct TableStage {
+    void* host = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+  };
+  static constexpr uint32_t kTableStages = 4;
+  TableStage stages_[kTableStages];
+  uint32_t stage_next_ = 0;
 
  public:
-  DevBuf io_buf;  // particle staging for uploads and downloads
+  DevBuf io_buf;  // staging of the direct sum
 
  private:
   uint32_t sort_interval = kDefaultSortInterval;

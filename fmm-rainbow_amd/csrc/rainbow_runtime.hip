@@ -93,18 +93,86 @@ void apply_physics(SysDesc& s, const Physics& ph) {
 // Per-system storage: pos, vel, moments, fields, source, active in one block.
 // ---------------------------------------------------------------------------
 
-size_t system_storage_bytes(uint32_t n) {
-  const size_t cells = total_cells(depth_for_particles(n));
-  const size_t finest = level_cells(depth_for_particles(n));
-  const size_t above = level_offset(depth_for_particles(n));
-  return 2u * align_up(16u * static_cast<size_t>(n)) + align_up(4u * static_cast<size_t>(n)) +
-         align_up(16u * cells) + align_up(48u * cells) + align_up(16u * cells) + align_up(cells) +
-         2u * align_up(16u * finest) + align_up(48u * above) + align_up(finest);
+namespace {
+// Byte offsets of a system's buffers in its allocation (each 256-aligned).
+struct StorageLayout {
+  size_t pos, vel, perm, moments, fields, source, active, extra, finest_alt, m2l, xflag;
+  size_t rank, staging, tperm, cstart, xstart, sstart, own, bytes;
+};
+
+StorageLayout storage_layout(uint32_t n) {
+  const uint32_t depth = depth_for_particles(n);
+  const size_t cells = total_cells(depth);
+  const size_t finest = level_cells(depth);
+  const size_t above = level_offset(depth);
+  const size_t nb = static_cast<size_t>(n);
+  const size_t blocks = (nb + kDepositBodies - 1u) / kDepositBodies;
+  StorageLayout L{};
+  size_t at = 0;
+  auto take = [&](size_t bytes) {
+    const size_t here = at;
+    at += align_up(bytes);
+    return here;
+  };
+  L.pos = take(16u * nb);
+  L.vel = take(16u * nb);
+  L.perm = take(4u * nb);
+  L.moments = take(16u * cells);
+  L.fields = take(48u * cells);
+  L.source = take(16u * cells);
+  L.active = take(cells);
+  L.extra = take(16u * finest);
+  L.finest_alt = take(16u * finest);
+  L.m2l = take(48u * above);
+  L.xflag = take(finest);
+  L.rank = take(4u * nb);
+  L.staging = take(32u * nb);  // tpos then tvel, contiguous
+  L.tperm = take(4u * nb);
+  L.cstart = take(4u * finest);
+  L.xstart = take(4u * finest);
+  L.sstart = take(4u * (finest / 512u));
+  L.own = take(8u * blocks);
+  L.bytes = at;
+  return L;
+}
+}  // namespace
+
+size_t system_storage_bytes(uint32_t n) { return storage_layout(n).bytes; }
+
+SysDesc describe_system(void* base, uint32_t n, const Physics& ph, float scale) {
+  const Plan plan = make_plan(n);
+  const StorageLayout L = storage_layout(n);
+  char* p = static_cast<char*>(base);
+  SysDesc d{};
+  d.pos = reinterpret_cast<float4*>(p + L.pos);
+  d.vel = reinterpret_cast<float4*>(p + L.vel);
+  d.perm = reinterpret_cast<uint32_t*>(p + L.perm);
+  d.moments = reinterpret_cast<int4*>(p + L.moments);
+  d.fields = reinterpret_cast<float4*>(p + L.fields);
+  d.source = reinterpret_cast<float4*>(p + L.source);
+  d.active = reinterpret_cast<uint8_t*>(p + L.active);
+  d.extra = reinterpret_cast<int4*>(p + L.extra);
+  d.finest_alt = reinterpret_cast<int4*>(p + L.finest_alt);
+  d.m2l = reinterpret_cast<float4*>(p + L.m2l);
+  d.xflag = reinterpret_cast<uint8_t*>(p + L.xflag);
+  d.rank = reinterpret_cast<uint32_t*>(p + L.rank);
+  d.tpos = reinterpret_cast<float4*>(p + L.staging);
+  d.tvel = d.tpos + n;
+  d.tperm = reinterpret_cast<uint32_t*>(p + L.tperm);
+  d.cstart = reinterpret_cast<uint32_t*>(p + L.cstart);
+  d.xstart = reinterpret_cast<uint32_t*>(p + L.xstart);
+  d.sstart = reinterpret_cast<uint32_t*>(p + L.sstart);
+  d.own = reinterpret_cast<uint2*>(p + L.own);
+  d.n = n;
+  d.depth = plan.depth;
+  d.scale = scale != 0.0f ? scale : plan.scale;
+  d.live = 1u;
+  apply_physics(d, ph);
+  return d;
 }
 
 int alloc_system(uint32_t n, const Physics& ph, float scale, hipStream_t stream, SystemStorage* out) {
   if (n == 0u) return fail(RAINBOW_E_INVALID, "a system needs at least one body");
-  const Plan plan = make_plan(n);
   const size_t bytes = system_storage_bytes(n);
   void* base = nullptr;
   hipError_t e = hipMalloc(&base, bytes);
@@ -117,38 +185,9 @@ int alloc_system(uint32_t n, const Physics& ph, float scale, hipStream_t stream,
     (void)hipFree(base);
     return fail(RAINBOW_E_HIP, "hipMemsetAsync failed: %s", hipGetErrorString(e));
   }
-  const size_t cells = plan.cells;
-  char* p = static_cast<char*>(base);
-  SysDesc d{};
-  d.pos = reinterpret_cast<float4*>(p);
-  p += align_up(16u * static_cast<size_t>(n));
-  d.vel = reinterpret_cast<float4*>(p);
-  p += align_up(16u * static_cast<size_t>(n));
-  d.perm = reinterpret_cast<uint32_t*>(p);
-  p += align_up(4u * static_cast<size_t>(n));
-  d.moments = reinterpret_cast<int4*>(p);
-  p += align_up(16u * cells);
-  d.fields = reinterpret_cast<float4*>(p);
-  p += align_up(48u * cells);
-  d.source = reinterpret_cast<float4*>(p);
-  p += align_up(16u * cells);
-  d.active = reinterpret_cast<uint8_t*>(p);
-  p += align_up(cells);
-  d.extra = reinterpret_cast<int4*>(p);
-  p += align_up(16u * level_cells(plan.depth));
-  d.finest_alt = reinterpret_cast<int4*>(p);
-  p += align_up(16u * level_cells(plan.depth));
-  d.m2l = reinterpret_cast<float4*>(p);
-  p += align_up(48u * level_offset(plan.depth));
-  d.xflag = reinterpret_cast<uint8_t*>(p);
-  d.n = n;
-  d.depth = plan.depth;
-  d.scale = scale != 0.0f ? scale : plan.scale;
-  d.live = 1u;
-  apply_physics(d, ph);
   out->base = base;
   out->bytes = bytes;
-  out->desc = d;
+  out->desc = describe_system(base, n, ph, scale);
   return RAINBOW_OK;
 }
 
@@ -186,10 +225,13 @@ void DevBuf::release() {
 Engine::~Engine() {
   if (stream) (void)hipStreamSynchronize(stream);
   drop_graphs();
-  for (DevBuf* b : {&sys_buf, &deposit_buf, &integrate_buf, &subtree_buf, &list_buf, &count_buf,
-                    &stripe_buf, &diag_buf, &body_buf, &snap_buf, &own_buf, &rank_buf, &cstart_buf, &sstart_buf,
-                    &io_buf})
+  for (DevBuf* b : {&sys_buf, &deposit_buf, &fresh_buf, &integrate_buf, &subtree_buf, &list_buf, &count_buf,
+                    &stripe_buf, &diag_buf, &snap_buf, &io_buf})
     b->release();
+  for (TableStage& st : stages_) {
+    if (st.done) (void)hipEventDestroy(st.done);
+    if (st.host) (void)hipHostFree(st.host);
+  }
   for (hipEvent_t e : {ev_a, ev_b, ev_counts})
     if (e) (void)hipEventDestroy(e);
   if (counts_host) (void)hipHostFree(counts_host);
@@ -215,29 +257,66 @@ int Engine::open(uint32_t opts) {
   return RAINBOW_OK;
 }
 
+// Installs the system set and waits for it: the next step deposits every
+// system again into zeroed finest buffers (bodies may have changed).
 int Engine::configure(const std::vector<SysDesc>& systems) {
-  if (systems.size() > 65535u) return fail(RAINBOW_E_INVALID, "at most 65535 systems");
   RB_HIP(hipStreamSynchronize(stream));
   drop_graphs();
-  // Owned deposit ranges of the systems that stay (same body buffer and
-  // count) carry over; the others start owning nothing until their sort.
-  std::vector<uint2> old_own(n_deposit);
-  if (n_deposit > 0u)
-    RB_HIP(hipMemcpy(old_own.data(), own_buf.p, sizeof(uint2) * n_deposit, hipMemcpyDeviceToHost));
-  std::vector<std::pair<const float4*, uint32_t>> old_first;  // (bodies, first deposit block)
-  {
-    uint32_t f = 0;
-    for (const SysDesc& d : sys) {
-      old_first.push_back({d.pos, f});
-      f += (d.n + kDepositBodies - 1u) / kDepositBodies;
-    }
+  int rc = install(systems, std::vector<uint32_t>());
+  if (rc != RAINBOW_OK) return rc;
+  RB_HIP(hipMemsetAsync(count_buf.p, 0, sizeof(uint32_t) * kLevels * kStripes, stream));
+  RB_HIP(hipStreamSynchronize(stream));
+  since_sort = sort_interval;
+  pending_deposit = true;
+  return RAINBOW_OK;
+}
+
+// Installs a new system set between steps without waiting for the device:
+// the steps already enqueued keep the old tables (the table copies run after
+// them in stream order), and every system's own state (finest buffers,
+// ranks, owned ranges, fields) carries over, since nothing of a system is
+// indexed by its place in the set.  `fresh` lists the systems (indices into
+// `systems`) whose storage is zeroed and whose bodies are not deposited yet:
+// the next step deposits them, alone, before its upward.  Captured graphs
+// are dropped (the only wait, and only when a graph exists).
+int Engine::configure_async(const std::vector<SysDesc>& systems, const std::vector<uint32_t>& fresh) {
+  if (graph_chunk[0] != nullptr || graph_chunk[1] != nullptr) {
+    RB_HIP(hipStreamSynchronize(stream));
+    drop_graphs();
   }
-  const std::vector<SysDesc> old_sys = sys;
-  std::vector<uint2> own;  // uploaded below; lives until the final synchronize
+  return install(systems, fresh);
+}
+
+// Device table capacity for a set of up to `bodies` bodies, `systems`
+// systems and `finest` finest cells in all, so that later installs within
+// those bounds never reallocate (which would wait for the device).
+int Engine::reserve(uint64_t bodies, uint32_t systems, uint64_t finest) {
+  const uint64_t dep = bodies / kDepositBodies + systems;
+  const uint64_t integ = bodies / kIntegrateThreads + systems;
+  const uint64_t sub = finest / 512u + systems;
+  // Lists: a system lists at most level_cells(k) internal cells at level
+  // k < depth, (8^depth - 1) / 7 in all, fewer than finest / 7 + 1.
+  const uint64_t lists = finest / 7u + systems + kLevels * kStripes;
+  int rc;
+  if ((rc = sys_buf.ensure(sizeof(SysDesc) * systems)) || (rc = deposit_buf.ensure(sizeof(BodyBlock) * dep)) ||
+      (rc = fresh_buf.ensure(sizeof(BodyBlock) * dep)) || (rc = integrate_buf.ensure(sizeof(BodyBlock) * integ)) ||
+      (rc = subtree_buf.ensure(sizeof(SubtreeBlock) * sub)) || (rc = list_buf.ensure(sizeof(uint32_t) * lists)))
+    return rc;
+  return RAINBOW_OK;
+}
+
+// Builds the work tables of `systems` on the host and copies them (and the
+// descriptors) to the device in stream order, through a ring of pinned
+// staging buffers; nothing here waits for the device unless a buffer must
+// grow or every staging buffer is still being read.
+int Engine::install(const std::vector<SysDesc>& systems, const std::vector<uint32_t>& fresh) {
+  if (systems.size() > 65535u) return fail(RAINBOW_E_INVALID, "at most 65535 systems");
   sys = systems;
   const uint32_t nsys = static_cast<uint32_t>(sys.size());
-  std::vector<BodyBlock> dep, integ;
-  std::vector<SubtreeBlock> sub;
+  dep_.clear();
+  integ_.clear();
+  sub_.clear();
+  fresh_.clear();
   // Capacity of every (level, stripe) list: a subtree block appends at most
   // 64 / 8 / 1 internal cells of levels depth-1 / -2 / -3 to stripe (block
   // index mod kStripes), the top block of system s appends the levels above
@@ -248,28 +327,31 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
   total_bodies = 0;
   uint64_t total_finest = 0;
   for (uint32_t s = 0; s < nsys; ++s) {
-    sys[s].body_base = static_cast<uint32_t>(total_bodies);
-    sys[s].cell_base = static_cast<uint32_t>(total_finest);
-    total_finest += level_cells(sys[s].depth);
     const SysDesc& d = sys[s];
-    const uint32_t finest = level_cells(d.depth);
+    total_finest += level_cells(d.depth);
     const uint32_t ib = d.n / (kAdaptiveLeafCapacity + 1u);
-    for (uint32_t f = 0; f < d.n; f += kDepositBodies) dep.push_back(BodyBlock{s, f});
-    for (uint32_t f = 0; f < d.n; f += kIntegrateThreads) integ.push_back(BodyBlock{s, f});
+    for (uint32_t f = 0; f < d.n; f += kDepositBodies) dep_.push_back(BodyBlock{s, f});
+    for (uint32_t f = 0; f < d.n; f += kIntegrateThreads) integ_.push_back(BodyBlock{s, f});
     for (uint32_t r = 0; r < level_cells(d.depth - 3u); ++r) {
-      const uint32_t stripe = static_cast<uint32_t>(sub.size()) % kStripes;
+      const uint32_t stripe = static_cast<uint32_t>(sub_.size()) % kStripes;
       caps[(d.depth - 1u) * kStripes + stripe] += std::min(64u, ib);
       caps[(d.depth - 2u) * kStripes + stripe] += std::min(8u, ib);
       caps[(d.depth - 3u) * kStripes + stripe] += std::min(1u, ib);
-      sub.push_back(SubtreeBlock{s, r});
+      sub_.push_back(SubtreeBlock{s, r});
     }
     for (uint32_t k = 0; k + 3u < d.depth; ++k) caps[k * kStripes + s % kStripes] += std::min(level_cells(k), ib);
     dmax = std::max(dmax, d.depth);
     total_bodies += d.n;
   }
-  n_deposit = static_cast<uint32_t>(dep.size());
-  n_integrate = static_cast<uint32_t>(integ.size());
-  n_subtree = static_cast<uint32_t>(sub.size());
+  for (uint32_t s : fresh) {
+    if (s >= nsys) return fail(RAINBOW_E_INVALID, "fresh system %u out of range", s);
+    for (uint32_t f = 0; f < sys[s].n; f += kDepositBodies) fresh_.push_back(BodyBlock{s, f});
+  }
+  if (total_bodies > 0xFFFFFFFFull) return fail(RAINBOW_E_INVALID, "at most 2^32 - 1 bodies per batch");
+  n_deposit = static_cast<uint32_t>(dep_.size());
+  n_integrate = static_cast<uint32_t>(integ_.size());
+  n_subtree = static_cast<uint32_t>(sub_.size());
+  n_fresh = static_cast<uint32_t>(fresh_.size());
   std::vector<uint32_t> stripes(2u * kLevels * kStripes);
   uint32_t list_total = 0;
   for (uint32_t i = 0; i < kLevels * kStripes; ++i) {
@@ -281,29 +363,15 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
   counts_pending = false;
   split_wanted = split_auto;
   int rc;
-  if ((rc = sys_buf.ensure(sizeof(SysDesc) * nsys)) ||
-      (rc = deposit_buf.ensure(sizeof(BodyBlock) * n_deposit)) ||
+  if ((rc = sys_buf.ensure(sizeof(SysDesc) * nsys)) || (rc = deposit_buf.ensure(sizeof(BodyBlock) * n_deposit)) ||
+      (rc = fresh_buf.ensure(sizeof(BodyBlock) * n_fresh)) ||
       (rc = integrate_buf.ensure(sizeof(BodyBlock) * n_integrate)) ||
       (rc = subtree_buf.ensure(sizeof(SubtreeBlock) * n_subtree)) ||
       (rc = list_buf.ensure(sizeof(uint32_t) * list_total)) ||
       (rc = count_buf.ensure(sizeof(uint32_t) * kLevels * kStripes)) ||
-      (rc = stripe_buf.ensure(sizeof(uint32_t) * stripes.size())) ||
-      (rc = diag_buf.ensure(sizeof(double) * (8u * static_cast<size_t>(n_integrate) + 8u))) ||
-      (rc = own_buf.ensure(sizeof(uint2) * n_deposit)) || (rc = rank_buf.ensure(4ull * total_bodies)) ||
-      (rc = body_buf.ensure(36ull * total_bodies)) || (rc = cstart_buf.ensure(8ull * total_finest)) ||
-      (rc = sstart_buf.ensure(4ull * (total_finest / 512u + 1u))))
+      (rc = stripe_buf.ensure(sizeof(uint32_t) * stripes.size())))
     return rc;
-  if (total_bodies > 0xFFFFFFFFull || total_finest > 0xFFFFFFFFull)
-    return fail(RAINBOW_E_INVALID, "at most 2^32 - 1 bodies and finest cells per batch");
   args.sys = static_cast<const SysDesc*>(sys_buf.p);
-  args.own = static_cast<uint2*>(own_buf.p);
-  args.rank = static_cast<uint32_t*>(rank_buf.p);
-  args.cstart = static_cast<uint32_t*>(cstart_buf.p);
-  args.xstart = args.cstart + total_finest;
-  args.sstart = static_cast<uint32_t*>(sstart_buf.p);
-  args.tpos = static_cast<float4*>(body_buf.p);
-  args.tvel = args.tpos + total_bodies;
-  args.tperm = reinterpret_cast<uint32_t*>(args.tvel + total_bodies);
   args.sorting = 0u;
   args.deposit_blocks = static_cast<const BodyBlock*>(deposit_buf.p);
   args.integrate_blocks = static_cast<const BodyBlock*>(integrate_buf.p);
@@ -320,32 +388,38 @@ int Engine::configure(const std::vector<SysDesc>& systems) {
     tree_blocks[l] = static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(items, 1u), kTreeMaxBlocks));
   }
   plan_tree();
-  if (nsys) {
-    RB_HIP(hipMemcpyAsync(sys_buf.p, sys.data(), sizeof(SysDesc) * nsys, hipMemcpyHostToDevice, stream));
-    RB_HIP(hipMemcpyAsync(deposit_buf.p, dep.data(), sizeof(BodyBlock) * n_deposit, hipMemcpyHostToDevice, stream));
-    RB_HIP(hipMemcpyAsync(integrate_buf.p, integ.data(), sizeof(BodyBlock) * n_integrate, hipMemcpyHostToDevice,
-                          stream));
-    RB_HIP(hipMemcpyAsync(subtree_buf.p, sub.data(), sizeof(SubtreeBlock) * n_subtree, hipMemcpyHostToDevice,
-                          stream));
-    own.assign(n_deposit, make_uint2(0u, 0u));
-    uint32_t f = 0;
-    for (const SysDesc& d : sys) {
-      const uint32_t nb = (d.n + kDepositBodies - 1u) / kDepositBodies;
-      for (size_t o = 0; o < old_sys.size(); ++o)
-        if (old_sys[o].pos == d.pos && old_sys[o].n == d.n) {
-          for (uint32_t b = 0; b < nb; ++b) own[f + b] = old_own[old_first[o].second + b];
-          break;
-        }
-      f += nb;
-    }
-    RB_HIP(hipMemcpyAsync(own_buf.p, own.data(), sizeof(uint2) * n_deposit, hipMemcpyHostToDevice, stream));
+  // Stage every table in one pinned buffer of the ring and copy it out.
+  const size_t b_sys = sizeof(SysDesc) * nsys, b_dep = sizeof(BodyBlock) * n_deposit,
+               b_int = sizeof(BodyBlock) * n_integrate, b_sub = sizeof(SubtreeBlock) * n_subtree,
+               b_str = sizeof(uint32_t) * stripes.size(), b_fr = sizeof(BodyBlock) * n_fresh;
+  const size_t total = b_sys + b_dep + b_int + b_sub + b_str + b_fr;
+  TableStage& st = stages_[stage_next_];
+  stage_next_ = (stage_next_ + 1u) % kTableStages;
+  if (st.done != nullptr) RB_HIP(hipEventSynchronize(st.done));
+  if (st.cap < total) {
+    if (st.host != nullptr) (void)hipHostFree(st.host);
+    st.host = nullptr;
+    st.cap = 0;
+    const size_t want = std::max<size_t>(total + total / 4u, 1u << 16);
+    RB_HIP(hipHostMalloc(&st.host, want, hipHostMallocDefault));
+    st.cap = want;
   }
-  RB_HIP(hipMemcpyAsync(stripe_buf.p, stripes.data(), sizeof(uint32_t) * stripes.size(), hipMemcpyHostToDevice,
-                        stream));
-  RB_HIP(hipMemsetAsync(count_buf.p, 0, sizeof(uint32_t) * kLevels * kStripes, stream));
-  RB_HIP(hipStreamSynchronize(stream));
-  since_sort = sort_interval;
-  pending_deposit = true;
+  if (st.done == nullptr) RB_HIP(hipEventCreateWithFlags(&st.done, hipEventDisableTiming));
+  char* h = static_cast<char*>(st.host);
+  auto put = [&](void* dev, const void* src, size_t bytes) -> hipError_t {
+    if (bytes == 0u) return hipSuccess;
+    std::memcpy(h, src, bytes);
+    const hipError_t e = hipMemcpyAsync(dev, h, bytes, hipMemcpyHostToDevice, stream);
+    h += bytes;
+    return e;
+  };
+  RB_HIP(put(sys_buf.p, sys.data(), b_sys));
+  RB_HIP(put(deposit_buf.p, dep_.data(), b_dep));
+  RB_HIP(put(integrate_buf.p, integ_.data(), b_int));
+  RB_HIP(put(subtree_buf.p, sub_.data(), b_sub));
+  RB_HIP(put(stripe_buf.p, stripes.data(), b_str));
+  RB_HIP(put(fresh_buf.p, fresh_.data(), b_fr));
+  RB_HIP(hipEventRecord(st.done, stream));
   return RAINBOW_OK;
 }
 
@@ -420,13 +494,24 @@ int Engine::set_tree_split(uint32_t starts) {
   return apply_tree_split(starts);
 }
 
+// Launches already enqueued keep their own split (they were launched with
+// it); only captured graphs must wait before they are dropped.
 int Engine::apply_tree_split(uint32_t starts) {
   starts |= 1u << 2;
-  RB_HIP(hipStreamSynchronize(stream));
-  if (starts != tree_split) drop_graphs();
+  if (starts != tree_split && (graph_chunk[0] != nullptr || graph_chunk[1] != nullptr)) {
+    RB_HIP(hipStreamSynchronize(stream));
+    drop_graphs();
+  }
   tree_split = starts;
   plan_tree();
   return RAINBOW_OK;
+}
+
+// From now on systems join only through configure_async (fresh, zeroed
+// storage): nothing is pending, and the next step sorts.
+void Engine::start_incremental() {
+  pending_deposit = false;
+  since_sort = sort_interval;
 }
 
 // The measured split: a level starts a launch of its own when its internal
@@ -495,7 +580,14 @@ int Engine::enqueue(bool integrate, bool sorting, uint32_t step_parity, bool dep
   if (deposit_first) {
     RB_HIP(zero_finest());
     RB_HIP(launch_deposit(a, n_deposit, stream));
+  } else if (n_fresh > 0u) {
+    // Systems that joined since the last step: their own first deposit
+    // (ranks too when this step sorts), into this step's finest buffer.
+    StepArgs f = a;
+    f.deposit_blocks = static_cast<const BodyBlock*>(fresh_buf.p);
+    RB_HIP(launch_deposit(f, n_fresh, stream));
   }
+  n_fresh = 0u;
   RB_HIP(launch_upward(a, n_subtree, stream));
   // The top levels run in their own launch, or inside the first tree
   // launch; the sort's scatter follows whichever runs them.
@@ -599,7 +691,7 @@ int Engine::step(uint32_t steps) {
   uint32_t done = 0;
   while (done < steps) {
     const uint32_t p = parity;
-    bool use_chunk = !pending_deposit && chunk > 1u && steps - done >= chunk;
+    bool use_chunk = !pending_deposit && n_fresh == 0u && chunk > 1u && steps - done >= chunk;
     if (use_chunk) {
       if (sort_interval != 0u && sort_interval <= kGraphSteps)
         use_chunk = since_sort >= sort_interval;  // starts with a sort
@@ -662,10 +754,9 @@ int Engine::snapshot(bool restore) {
     const int rc = snap_buf.ensure(36ull * total_bodies);
     if (rc) return rc;
   }
-  char* p = static_cast<char*>(snap_buf.p);
+  char* sp = static_cast<char*>(snap_buf.p);
   for (const SysDesc& d : sys) {
     const size_t b16 = 16ull * d.n, b4 = 4ull * d.n;
-    char* sp = p + 36ull * d.body_base;
     if (restore) {
       RB_HIP(hipMemcpyAsync(d.pos, sp, b16, hipMemcpyDeviceToDevice, stream));
       RB_HIP(hipMemcpyAsync(d.vel, sp + b16, b16, hipMemcpyDeviceToDevice, stream));
@@ -675,6 +766,7 @@ int Engine::snapshot(bool restore) {
       RB_HIP(hipMemcpyAsync(sp + b16, d.vel, b16, hipMemcpyDeviceToDevice, stream));
       RB_HIP(hipMemcpyAsync(sp + 2 * b16, d.perm, b4, hipMemcpyDeviceToDevice, stream));
     }
+    sp += 36ull * d.n;
   }
   RB_HIP(hipStreamSynchronize(stream));
   if (restore) pending_deposit = true;  // the last fused deposit was of other positions
@@ -848,28 +940,24 @@ static int clear_finest(const SysDesc& d, hipStream_t stream) {
   return RAINBOW_OK;
 }
 
-int upload_system(const SysDesc& d, const void* data, size_t bytes, hipStream_t stream, DevBuf* io) {
+int upload_system(const SysDesc& d, const void* data, size_t bytes, hipStream_t stream) {
   if (data == nullptr || bytes != 32ull * d.n)
     return fail(RAINBOW_E_INVALID, "upload expects %llu bytes, got %zu",
                 static_cast<unsigned long long>(32ull * d.n), bytes);
-  int rc = io->ensure(bytes);
-  if (rc) return rc;
-  RB_HIP(hipMemcpyAsync(io->p, data, bytes, hipMemcpyHostToDevice, stream));
-  RB_HIP(launch_split_particles(d, static_cast<const float4*>(io->p), stream));
+  RB_HIP(hipMemcpyAsync(d.tpos, data, bytes, hipMemcpyHostToDevice, stream));
+  RB_HIP(launch_split_particles(d, d.tpos, stream));
   return clear_finest(d, stream);
 }
 
-int download_system(const SysDesc& d, int which, void* dst, size_t bytes, hipStream_t stream, DevBuf* io,
+int download_system(const SysDesc& d, int which, void* dst, size_t bytes, hipStream_t stream,
                     uint32_t finest_parity) {
   const uint64_t cells = total_cells(d.depth);
   if (dst == nullptr) return fail(RAINBOW_E_INVALID, "null destination");
   if (which == RAINBOW_PARTICLES) {
     if (bytes != 32ull * d.n)
       return fail(RAINBOW_E_INVALID, "particles need %llu bytes", static_cast<unsigned long long>(32ull * d.n));
-    int rc = io->ensure(bytes);
-    if (rc) return rc;
-    RB_HIP(launch_join_particles(d, static_cast<float4*>(io->p), stream));
-    RB_HIP(hipMemcpyAsync(dst, io->p, bytes, hipMemcpyDeviceToHost, stream));
+    RB_HIP(launch_join_particles(d, d.tpos, stream));
+    RB_HIP(hipMemcpyAsync(dst, d.tpos, bytes, hipMemcpyDeviceToHost, stream));
     RB_HIP(hipStreamSynchronize(stream));
     return RAINBOW_OK;
   }
@@ -900,7 +988,7 @@ int Engine::upload(uint32_t s, const void* data, size_t bytes) {
   RB_HIP(hipStreamSynchronize(stream));
   since_sort = sort_interval;
   pending_deposit = true;
-  return upload_system(sys[s], data, bytes, stream, &io_buf);
+  return upload_system(sys[s], data, bytes, stream);
 }
 
 // The reference's default scene into system s (initialize,
@@ -916,7 +1004,7 @@ int Engine::seed_galaxy(uint32_t s) {
 
 int Engine::download(uint32_t s, int which, void* dst, size_t bytes) {
   if (s >= sys.size()) return fail(RAINBOW_E_INVALID, "system %u out of range", s);
-  return download_system(sys[s], which, dst, bytes, stream, &io_buf, parity ^ 1u);
+  return download_system(sys[s], which, dst, bytes, stream, parity ^ 1u);
 }
 
 int Engine::accelerations(uint32_t s, void* dst, size_t bytes) {
@@ -968,6 +1056,8 @@ int Engine::direct_sum_f64_in(uint32_t s, const uint32_t* targets, uint32_t nt, 
 
 int Engine::diagnostics(uint32_t s, double out[8]) {
   if (s >= sys.size()) return fail(RAINBOW_E_INVALID, "system %u out of range", s);
+  const int rc = diag_buf.ensure(sizeof(double) * (8u * static_cast<size_t>(n_integrate) + 8u));
+  if (rc) return rc;
   double* partial = static_cast<double*>(diag_buf.p);
   double* result = partial + 8u * static_cast<size_t>(n_integrate);
   StepArgs a = args;

@@ -1,14 +1,21 @@
 // Continuous-batching server: N-body solve requests of varying size share one
-// batched step.  A request is admitted into a resident slot when a slot and
-// device memory (counted against a byte budget) are free; its trees and
-// particles stay on the device until its last step; every scheduling
-// iteration runs batched steps over all resident systems (up to
-// kServeSteps, and no further than the next request's last step), so
-// requests that arrive while others run join at the next iteration and
-// their work overlaps on the device.  Each system's arithmetic depends only
-// on its own data, so a request's result is the same alone and in any mix.
-// The set of systems changes at most iterations, so steps are launched
-// directly rather than captured.
+// batched step.
+//
+// A request is admitted when a slot and device memory (counted against a
+// byte budget) are free: its storage is carved from an arena of large
+// device chunks (no allocation per request once the arena has grown to the
+// working set), zeroed and filled by a copy stream, and it joins the
+// resident set at the next step boundary, where the step deposits it
+// alone.  Every step advances all resident systems; a request leaves the set
+// right after its own last step (each runs exactly its step count), its
+// bodies are gathered into caller order behind that step and copied out on
+// the copy stream, and its memory returns to the arena once that copy has
+// finished.  The host enqueues up to kWindow steps ahead of the device and
+// waits only when that window is full, so uploads, downloads, admissions
+// and departures overlap the running steps: the set changes between steps
+// without waiting for the device (Engine::configure_async).  Each system's
+// arithmetic depends only on its own data, so a request's result is the
+// same alone and in any mix.
 //
 // The reference has no serving loop; it advances one system per frame
 // (app.js:712-741).
@@ -17,25 +24,138 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstring>
 #include <deque>
 #include <map>
-#include <cstring>
 #include <new>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "rainbow_fmm.h"
+#include "rainbow_launch.h"
 #include "rainbow_runtime.h"
 
 namespace rainbow {
 namespace {
 
-constexpr uint32_t kServeSteps = 8;
+constexpr uint32_t kWindow = 3;          // steps enqueued ahead of the device
+constexpr uint32_t kStepsPerPoll = 16;   // steps one poll may enqueue
+constexpr size_t kChunkBytes = 4ull << 30;
+constexpr size_t kArenaAlign = 256;
 
 double now_s() {
   using namespace std::chrono;
   return duration_cast<duration<double>>(steady_clock::now().time_since_epoch()).count();
 }
+
+#define RB_HIP(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return fail(RAINBOW_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),       \
+                  __FILE__, __LINE__);                                                        \
+  } while (0)
+
+// Device memory of the resident requests: chunks of at least kChunkBytes
+// (less when the budget is smaller), allocated as the working set grows and
+// kept until the server is destroyed, sub-allocated first-fit with
+// neighbouring free ranges merged.
+class Arena {
+ public:
+  explicit Arena(size_t limit) : limit_(limit) {}
+  ~Arena() {
+    for (Chunk& c : chunks_)
+      if (c.base != nullptr) (void)hipFree(c.base);
+  }
+  // nullptr when the request does not fit (the caller waits for frees).
+  // *grew is set when a chunk had to be allocated.
+  void* alloc(size_t bytes, bool* grew) {
+    bytes = (bytes + kArenaAlign - 1u) & ~(kArenaAlign - 1u);
+    *grew = false;
+    for (size_t ci = 0; ci < chunks_.size(); ++ci)
+      if (void* p = take(ci, bytes)) return p;
+    const size_t want = std::max(bytes, std::min(kChunkBytes, limit_ - std::min(limit_, reserved_)));
+    if (reserved_ + want > limit_ || want < bytes) return nullptr;
+    void* base = nullptr;
+    if (hipMalloc(&base, want) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    *grew = true;
+    reserved_ += want;
+    // Reuse a trimmed entry: live blocks hold chunk indices.
+    size_t ci = chunks_.size();
+    for (size_t k = 0; k < chunks_.size(); ++k)
+      if (chunks_[k].base == nullptr) {
+        ci = k;
+        break;
+      }
+    if (ci == chunks_.size()) chunks_.push_back(Chunk{});
+    chunks_[ci] = Chunk{static_cast<char*>(base), want, {{0u, want}}};
+    return take(ci, bytes);
+  }
+  void release(void* p) {
+    auto it = live_.find(p);
+    if (it == live_.end()) return;
+    Chunk& c = chunks_[it->second.first];
+    size_t off = static_cast<size_t>(static_cast<char*>(p) - c.base), len = it->second.second;
+    live_.erase(it);
+    auto next = c.free_.lower_bound(off);
+    if (next != c.free_.end() && off + len == next->first) {
+      len += next->second;
+      next = c.free_.erase(next);
+    }
+    if (next != c.free_.begin()) {
+      auto prev = std::prev(next);
+      if (prev->first + prev->second == off) {
+        prev->second += len;
+        return;
+      }
+    }
+    c.free_.emplace(off, len);
+  }
+  size_t reserved() const { return reserved_; }
+  // Returns the chunks that hold no block to the device (a request larger
+  // than every chunk's free space can then get a chunk of its own); true if
+  // any was freed.  hipFree waits for the device.
+  bool trim() {
+    bool any = false;
+    for (Chunk& c : chunks_) {
+      if (c.base == nullptr || c.free_.size() != 1u || c.free_.begin()->second != c.bytes) continue;
+      (void)hipFree(c.base);
+      reserved_ -= c.bytes;
+      c.base = nullptr;
+      c.free_.clear();
+      any = true;
+    }
+    return any;
+  }
+
+ private:
+  struct Chunk {
+    char* base;
+    size_t bytes;
+    std::map<size_t, size_t> free_;  // offset -> length
+  };
+  void* take(size_t ci, size_t bytes) {
+    Chunk& c = chunks_[ci];
+    for (auto it = c.free_.begin(); it != c.free_.end(); ++it) {
+      if (it->second < bytes) continue;
+      const size_t off = it->first, rest = it->second - bytes;
+      c.free_.erase(it);
+      if (rest > 0u) c.free_.emplace(off + bytes, rest);
+      void* p = c.base + off;
+      live_[p] = {ci, bytes};
+      return p;
+    }
+    return nullptr;
+  }
+  size_t limit_;
+  size_t reserved_ = 0;
+  std::vector<Chunk> chunks_;
+  std::unordered_map<void*, std::pair<size_t, size_t>> live_;  // block -> (chunk, length)
+};
 
 // Page-locked host buffers, reused: requests are copied into one at submit
 // and results land in one, so transfers run as DMA at full bus speed.
@@ -43,6 +163,7 @@ class PinnedPool {
  public:
   ~PinnedPool() {
     for (auto& kv : free_) (void)hipHostFree(kv.second);
+    for (auto& kv : size_) (void)hipHostFree(kv.first);
   }
   void* get(size_t bytes) {
     auto it = free_.lower_bound(bytes);
@@ -73,12 +194,66 @@ class PinnedPool {
   std::unordered_map<void*, size_t> size_;  // buffers in use
 };
 
+// Events, reused.
+class EventPool {
+ public:
+  ~EventPool() {
+    for (hipEvent_t e : free_) (void)hipEventDestroy(e);
+  }
+  hipError_t get(hipEvent_t* out, bool timing = false) {
+    std::vector<hipEvent_t>& pool = timing ? timed_ : free_;
+    if (!pool.empty()) {
+      *out = pool.back();
+      pool.pop_back();
+      return hipSuccess;
+    }
+    return hipEventCreateWithFlags(out, timing ? hipEventDefault : hipEventDisableTiming);
+  }
+  void put(hipEvent_t e, bool timing = false) {
+    if (e != nullptr) (timing ? timed_ : free_).push_back(e);
+  }
+  void destroy_timed() {
+    for (hipEvent_t e : timed_) (void)hipEventDestroy(e);
+    timed_.clear();
+  }
+
+ private:
+  std::vector<hipEvent_t> free_, timed_;
+};
+
 struct Pending {
   uint64_t id;
   uint32_t n;
   uint32_t steps;
   double submit_s;
-  void* data;  // pinned, 32 n bytes
+  double deadline;  // 0: none
+  void* data;       // pinned, 32 n bytes
+};
+
+enum class SlotState { kResident, kDraining };
+
+struct Slot {
+  uint64_t id;
+  uint32_t n;
+  uint32_t steps;
+  uint32_t run = 0;  // steps enqueued
+  double submit_s;
+  double admit_s;
+  double deadline;
+  uint64_t bytes;    // counted against the budget
+  void* block;       // arena storage
+  void* upload;      // pinned request data, returned once `loaded` has passed
+  void* result = nullptr;  // pinned, 32 n bytes (a completed request)
+  hipEvent_t loaded = nullptr;  // copy stream: storage zeroed and bodies copied in
+  hipEvent_t stepped = nullptr; // compute stream: the last step and the gather into caller order
+  hipEvent_t done = nullptr;    // the last use of the storage (download, or last step)
+  SlotState state = SlotState::kResident;
+  int32_t status = RAINBOW_OK;
+  SysDesc desc;
+};
+
+struct InFlight {
+  hipEvent_t start, stop;
 };
 
 struct Result {
@@ -86,33 +261,12 @@ struct Result {
   size_t bytes;
 };
 
-struct Slot {
-  uint64_t id;
-  uint32_t n;
-  uint32_t steps;
-  uint32_t left;
-  double submit_s;
-  double admit_s;
-  uint64_t bytes;
-  SystemStorage storage;
-};
-
 }  // namespace
 
-// Device bytes a resident request holds: its storage plus its share of the
-// engine's work tables, lists and sort buffers (ranks and staging: 40 bytes
-// per body; cell offsets: 8 per finest cell; owned ranges: 8 per deposit
-// block).
-uint64_t request_bytes(uint32_t n) {
-  const Plan p = make_plan(n);
-  uint64_t lists = 0;
-  for (uint32_t k = 0; k < p.depth; ++k) lists += std::min<uint64_t>(level_cells(k), n / 17u);
-  const uint64_t deposit_blocks = (n + kDepositBodies - 1u) / kDepositBodies;
-  const uint64_t blocks = deposit_blocks + (n + kIntegrateThreads - 1u) / kIntegrateThreads + level_cells(p.depth - 3u);
-  return system_storage_bytes(n) + 4u * lists + 8u * blocks +
-         64u * ((n + kIntegrateThreads - 1u) / kIntegrateThreads) + sizeof(SysDesc) + 4u + 40ull * n +
-         8ull * level_cells(p.depth) + 4ull * level_cells(p.depth - 3u) + 8ull * deposit_blocks;
-}
+// Device bytes a resident request holds: its storage (bodies, cells and
+// sort state).  The work tables of the batched step are reserved once for
+// the whole budget when the server is created.
+uint64_t request_bytes(uint32_t n) { return system_storage_bytes(n); }
 
 }  // namespace rainbow
 
@@ -121,88 +275,270 @@ using rainbow::fail;
 struct rainbow_server {
   rainbow::Engine engine;
   rainbow::Physics physics;
+  hipStream_t copy = nullptr;
   uint64_t budget = 0;
   uint64_t used = 0;
   uint32_t max_slots = 0;
   bool dirty = false;
+  rainbow::Arena* arena = nullptr;
   std::deque<rainbow::Pending> queue;
-  std::vector<rainbow::Slot> slots;
+  std::vector<rainbow::Slot> slots;  // resident and draining
+  std::vector<uint32_t> fresh;       // indices into the next install's set
   std::deque<rainbow_completion_t> done;
   rainbow::PinnedPool pinned;
+  rainbow::EventPool events;
+  std::deque<rainbow::InFlight> inflight;
   std::unordered_map<uint64_t, rainbow::Result> results;
-  // Host seconds spent admitting (allocation + upload), reconfiguring,
-  // stepping (launch + wait) and retiring (download + free); steps; polls.
-  double stats[6] = {0, 0, 0, 0, 0, 0};
+  std::unordered_set<uint64_t> ids;  // queued, resident or with a result not yet taken
+  std::unordered_set<uint64_t> cancel_wanted;
+  // Host seconds admitting, reconfiguring, enqueuing steps, retiring;
+  // steps; polls; device seconds of the steps (events); host seconds
+  // blocked on the device.
+  double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 
   ~rainbow_server() {
     if (engine.stream) (void)hipStreamSynchronize(engine.stream);
-    for (auto& s : slots) rainbow::free_system(&s.storage);
+    if (copy) (void)hipStreamSynchronize(copy);
+    for (auto& s : slots) {
+      pinned.put(s.upload);
+      pinned.put(s.result);
+      events.put(s.loaded);
+      events.put(s.stepped);
+      events.put(s.done);
+    }
+    for (auto& f : inflight) {
+      events.put(f.start, true);
+      events.put(f.stop, true);
+    }
+    events.destroy_timed();
     for (auto& p : queue) pinned.put(p.data);
     for (auto& r : results) pinned.put(r.second.data);
+    delete arena;
+    if (copy) (void)hipStreamDestroy(copy);
   }
 
-  int reconfigure() {
-    std::vector<rainbow::SysDesc> descs;
-    descs.reserve(slots.size());
-    for (const auto& s : slots) descs.push_back(s.storage.desc);
-    dirty = false;
-    return engine.configure(descs);
-  }
-
-  int retire(size_t i, double t) {
-    rainbow::Slot& s = slots[i];
-    const size_t bytes = 32ull * s.n;
-    void* out = pinned.get(bytes);
-    if (out == nullptr) return fail(RAINBOW_E_NOMEM, "no pinned host memory for a %zu-byte result", bytes);
-    const int rc = rainbow::download_system(s.storage.desc, RAINBOW_PARTICLES, out, bytes, engine.stream,
-                                            &engine.io_buf);
-    if (rc) {
-      pinned.put(out);
-      return rc;
-    }
-    rainbow_completion_t c;
+  void complete(const rainbow::Slot& s, double t) {
+    rainbow_completion_t c{};
     c.id = s.id;
     c.particles = s.n;
     c.steps = s.steps;
     c.submit_s = s.submit_s;
     c.admit_s = s.admit_s;
     c.finish_s = t;
+    c.status = s.status;
+    c.steps_run = s.run;
     done.push_back(c);
-    auto old = results.find(s.id);
-    if (old != results.end()) pinned.put(old->second.data);
-    results[s.id] = rainbow::Result{out, bytes};
-    rainbow::free_system(&s.storage);
-    used -= s.bytes;
-    slots.erase(slots.begin() + static_cast<std::ptrdiff_t>(i));
+  }
+
+  // Completed step events: their device time, and the window they free.
+  int reap_steps(bool wait_one) {
+    while (!inflight.empty()) {
+      const rainbow::InFlight f = inflight.front();
+      hipError_t q = hipEventQuery(f.stop);
+      if (q == hipErrorNotReady && wait_one) {
+        const double t0 = rainbow::now_s();
+        q = hipEventSynchronize(f.stop);
+        stats[7] += rainbow::now_s() - t0;
+        wait_one = false;
+      }
+      if (q == hipErrorNotReady) return RAINBOW_OK;
+      if (q != hipSuccess) return fail(RAINBOW_E_HIP, "step event: %s", hipGetErrorString(q));
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, f.start, f.stop) == hipSuccess) stats[6] += 1e-3 * ms;
+      events.put(f.start, true);
+      events.put(f.stop, true);
+      inflight.pop_front();
+    }
+    return RAINBOW_OK;
+  }
+
+  // Slots whose storage is no longer used: completion, result, memory back.
+  int retire(double t) {
+    for (size_t i = slots.size(); i-- > 0;) {
+      rainbow::Slot& s = slots[i];
+      if (s.upload != nullptr && s.loaded != nullptr) {
+        const hipError_t q = hipEventQuery(s.loaded);
+        if (q == hipSuccess) {
+          pinned.put(s.upload);
+          s.upload = nullptr;
+        } else if (q != hipErrorNotReady) {
+          return fail(RAINBOW_E_HIP, "upload event: %s", hipGetErrorString(q));
+        }
+      }
+      if (s.state != rainbow::SlotState::kDraining) continue;
+      const hipError_t q = hipEventQuery(s.done);
+      if (q == hipErrorNotReady) continue;
+      if (q != hipSuccess) return fail(RAINBOW_E_HIP, "retire event: %s", hipGetErrorString(q));
+      complete(s, t);
+      if (s.status == RAINBOW_OK) {
+        auto old = results.find(s.id);
+        if (old != results.end()) pinned.put(old->second.data);
+        results[s.id] = rainbow::Result{s.result, 32ull * s.n};
+      } else {
+        pinned.put(s.result);
+        ids.erase(s.id);
+      }
+      pinned.put(s.upload);
+      events.put(s.loaded);
+      events.put(s.stepped);
+      events.put(s.done);
+      arena->release(s.block);
+      used -= s.bytes;
+      slots.erase(slots.begin() + static_cast<std::ptrdiff_t>(i));
+    }
+    return RAINBOW_OK;
+  }
+
+  // A resident slot stops (cancelled or timed out) after the steps already
+  // enqueued: its storage is free once they have run.
+  int stop(rainbow::Slot& s, int32_t status) {
+    s.status = status;
+    s.state = rainbow::SlotState::kDraining;
+    RB_HIP(events.get(&s.done));
+    RB_HIP(hipEventRecord(s.done, engine.stream));
     dirty = true;
     return RAINBOW_OK;
   }
 
+  // Queued requests past their deadline or cancelled leave without running;
+  // resident ones stop at the next step boundary.
+  int expire(double t) {
+    for (auto it = queue.begin(); it != queue.end();) {
+      const bool cancelled = cancel_wanted.erase(it->id) > 0u;
+      if (!cancelled && !(it->deadline > 0.0 && t >= it->deadline)) {
+        ++it;
+        continue;
+      }
+      rainbow_completion_t c{};
+      c.id = it->id;
+      c.particles = it->n;
+      c.steps = it->steps;
+      c.submit_s = it->submit_s;
+      c.admit_s = 0.0;
+      c.finish_s = t;
+      c.status = cancelled ? RAINBOW_E_CANCELLED : RAINBOW_E_TIMEOUT;
+      c.steps_run = 0u;
+      done.push_back(c);
+      pinned.put(it->data);
+      ids.erase(it->id);
+      it = queue.erase(it);
+    }
+    for (rainbow::Slot& s : slots) {
+      if (s.state != rainbow::SlotState::kResident) continue;
+      const bool cancelled = cancel_wanted.erase(s.id) > 0u;
+      const bool late = s.deadline > 0.0 && t >= s.deadline;
+      if (cancelled || late) {
+        const int rc = stop(s, cancelled ? RAINBOW_E_CANCELLED : RAINBOW_E_TIMEOUT);
+        if (rc) return rc;
+      }
+    }
+    cancel_wanted.clear();
+    return RAINBOW_OK;
+  }
+
+  // Admission: memory from the arena, zeroed and filled on the copy stream;
+  // the compute stream waits for that copy and splits the bodies into the
+  // system's streams before the step that deposits it.
   int admit() {
-    while (!queue.empty() && slots.size() < max_slots) {
+    uint32_t occupied = 0;
+    for (const auto& s : slots)
+      if (s.state == rainbow::SlotState::kResident) ++occupied;
+    while (!queue.empty() && occupied < max_slots) {
       rainbow::Pending& p = queue.front();
       const uint64_t need = rainbow::request_bytes(p.n);
       if (used + need > budget) break;
+      bool grew = false;
+      void* block = arena->alloc(need, &grew);
+      if (block == nullptr && arena->trim()) block = arena->alloc(need, &grew);
+      if (block == nullptr) break;  // fragmented: wait for frees
       rainbow::Slot s;
       s.id = p.id;
       s.n = p.n;
       s.steps = p.steps;
-      s.left = p.steps;
       s.submit_s = p.submit_s;
+      s.deadline = p.deadline;
       s.bytes = need;
-      int rc = rainbow::alloc_system(p.n, physics, 0.0f, engine.stream, &s.storage);
-      if (rc) return rc;
-      rc = rainbow::upload_system(s.storage.desc, p.data, 32ull * p.n, engine.stream, &engine.io_buf);
-      if (rc) {
-        rainbow::free_system(&s.storage);
-        return rc;
+      s.block = block;
+      s.upload = p.data;
+      s.desc = rainbow::describe_system(block, p.n, physics, 0.0f);
+      hipError_t e = events.get(&s.loaded);
+      if (e == hipSuccess) e = hipMemsetAsync(block, 0, need, copy);
+      if (e == hipSuccess) e = hipMemcpyAsync(s.desc.tpos, p.data, 32ull * p.n, hipMemcpyHostToDevice, copy);
+      if (e == hipSuccess) e = hipEventRecord(s.loaded, copy);
+      if (e == hipSuccess) e = hipStreamWaitEvent(engine.stream, s.loaded, 0);
+      if (e == hipSuccess) e = rainbow::launch_split_particles(s.desc, s.desc.tpos, engine.stream);
+      if (e != hipSuccess) {
+        arena->release(block);
+        events.put(s.loaded);
+        return fail(RAINBOW_E_HIP, "admission: %s", hipGetErrorString(e));
       }
-      pinned.put(p.data);
       s.admit_s = rainbow::now_s();
       used += need;
       queue.pop_front();
-      slots.push_back(std::move(s));
+      slots.push_back(s);
+      fresh_ids.push_back(s.id);
+      ++occupied;
       dirty = true;
+      // Requests of zero steps are complete once their bodies are in.
+      if (s.steps == 0u) {
+        const int rc = finish(slots.back());
+        if (rc) return rc;
+      }
+    }
+    return RAINBOW_OK;
+  }
+
+  std::vector<uint64_t> fresh_ids;  // admitted since the last install
+
+  // The set of resident systems changed: install it before the next step.
+  int reconfigure() {
+    std::vector<rainbow::SysDesc> descs;
+    std::vector<uint32_t> fr;
+    descs.reserve(slots.size());
+    for (const auto& s : slots) {
+      if (s.state != rainbow::SlotState::kResident) continue;
+      if (std::find(fresh_ids.begin(), fresh_ids.end(), s.id) != fresh_ids.end())
+        fr.push_back(static_cast<uint32_t>(descs.size()));
+      descs.push_back(s.desc);
+    }
+    fresh_ids.clear();
+    dirty = false;
+    return engine.configure_async(descs, fr);
+  }
+
+  // After a slot's last step: gather its bodies into caller order behind
+  // that step, copy them out on the copy stream.
+  int finish(rainbow::Slot& s) {
+    s.result = pinned.get(32ull * s.n);
+    if (s.result == nullptr) return fail(RAINBOW_E_NOMEM, "no pinned host memory for a %u-body result", s.n);
+    RB_HIP(events.get(&s.stepped));
+    hipError_t e = rainbow::launch_join_particles(s.desc, s.desc.tpos, engine.stream);
+    if (e == hipSuccess) e = hipEventRecord(s.stepped, engine.stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(copy, s.stepped, 0);
+    if (e == hipSuccess) e = hipMemcpyAsync(s.result, s.desc.tpos, 32ull * s.n, hipMemcpyDeviceToHost, copy);
+    if (e == hipSuccess) e = events.get(&s.done);
+    if (e == hipSuccess) e = hipEventRecord(s.done, copy);
+    if (e != hipSuccess) return fail(RAINBOW_E_HIP, "result copy: %s", hipGetErrorString(e));
+    s.state = rainbow::SlotState::kDraining;
+    dirty = true;
+    return RAINBOW_OK;
+  }
+
+  // One step of every resident system, timed by an event pair.
+  int step_once() {
+    rainbow::InFlight f{nullptr, nullptr};
+    RB_HIP(events.get(&f.start, true));
+    RB_HIP(events.get(&f.stop, true));
+    RB_HIP(hipEventRecord(f.start, engine.stream));
+    int rc = engine.step_eager(1u);
+    if (rc) return rc;
+    RB_HIP(hipEventRecord(f.stop, engine.stream));
+    inflight.push_back(f);
+    stats[4] += 1.0;
+    for (rainbow::Slot& s : slots) {
+      if (s.state != rainbow::SlotState::kResident) continue;
+      s.run += 1u;
+      if (s.run >= s.steps && (rc = finish(s)) != 0) return rc;
     }
     return RAINBOW_OK;
   }
@@ -229,7 +565,22 @@ int rainbow_server_create(uint64_t budget_bytes, uint32_t max_slots, const rainb
   s->budget = budget_bytes;
   s->max_slots = max_slots;
   int rc = s->engine.open(options);
-  if (rc == RAINBOW_OK) rc = s->reconfigure();
+  if (rc == RAINBOW_OK && hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking) != hipSuccess)
+    rc = fail(RAINBOW_E_HIP, "copy stream creation failed");
+  if (rc == RAINBOW_OK) rc = s->engine.configure(std::vector<rainbow::SysDesc>());
+  if (rc == RAINBOW_OK) {
+    // Table capacity for the most the budget can hold: at least 76 bytes
+    // of storage per body and 128 per finest cell, at most 8^6 finest cells
+    // per system.
+    const uint64_t bodies = std::min<uint64_t>(budget_bytes / 76u, 0xFFFFFFFFull);
+    const uint64_t finest = std::min<uint64_t>(budget_bytes / 128u, 262144ull * max_slots);
+    rc = s->engine.reserve(bodies, max_slots, finest);
+  }
+  if (rc == RAINBOW_OK) {
+    s->engine.start_incremental();
+    s->arena = new (std::nothrow) rainbow::Arena(budget_bytes);
+    if (s->arena == nullptr) rc = fail(RAINBOW_E_NOMEM, "out of host memory");
+  }
   if (rc != RAINBOW_OK) {
     delete s;
     return rc;
@@ -243,19 +594,34 @@ void rainbow_server_destroy(rainbow_server_t* s) { delete s; }
 int rainbow_server_submit(rainbow_server_t* s, const rainbow_request_t* r) {
   if (s == nullptr || r == nullptr) return fail(RAINBOW_E_INVALID, "null argument");
   if (r->particles == 0u || r->data == nullptr) return fail(RAINBOW_E_INVALID, "empty request");
+  if (!(r->timeout_s >= 0.0)) return fail(RAINBOW_E_INVALID, "timeout must be >= 0");
   const uint64_t need = rainbow::request_bytes(r->particles);
   if (need > s->budget)
     return fail(RAINBOW_E_BUDGET, "request of %u bodies needs %llu bytes, budget is %llu", r->particles,
                 static_cast<unsigned long long>(need), static_cast<unsigned long long>(s->budget));
+  if (s->ids.count(r->id) != 0u)
+    return fail(RAINBOW_E_INVALID, "request id %llu is already in use", static_cast<unsigned long long>(r->id));
   rainbow::Pending p;
   p.id = r->id;
   p.n = r->particles;
   p.steps = r->steps;
   p.submit_s = rainbow::now_s();
+  p.deadline = r->timeout_s > 0.0 ? p.submit_s + r->timeout_s : 0.0;
   p.data = s->pinned.get(32ull * r->particles);
   if (p.data == nullptr) return fail(RAINBOW_E_NOMEM, "no pinned host memory for the request");
   std::memcpy(p.data, r->data, 32ull * r->particles);
   s->queue.push_back(p);
+  s->ids.insert(r->id);
+  return RAINBOW_OK;
+}
+
+int rainbow_server_cancel(rainbow_server_t* s, uint64_t id) {
+  if (s == nullptr) return fail(RAINBOW_E_INVALID, "null handle");
+  bool live = false;
+  for (const auto& p : s->queue) live = live || p.id == id;
+  for (const auto& sl : s->slots) live = live || (sl.id == id && sl.state == rainbow::SlotState::kResident);
+  if (!live) return fail(RAINBOW_E_STATE, "request %llu is not queued or running", static_cast<unsigned long long>(id));
+  s->cancel_wanted.insert(id);
   return RAINBOW_OK;
 }
 
@@ -263,38 +629,61 @@ int rainbow_server_poll(rainbow_server_t* s, uint32_t* stepped) {
   if (s == nullptr) return fail(RAINBOW_E_INVALID, "null handle");
   if (stepped) *stepped = 0u;
   s->stats[5] += 1.0;
-  double t0 = rainbow::now_s();
-  int rc = s->admit();
-  if (rc) return rc;
-  double t1 = rainbow::now_s();
-  s->stats[0] += t1 - t0;
-  // Requests of zero steps complete as soon as they are resident.
-  for (size_t i = s->slots.size(); i-- > 0;)
-    if (s->slots[i].left == 0u && (rc = s->retire(i, t1)) != 0) return rc;
-  t0 = rainbow::now_s();
-  s->stats[3] += t0 - t1;
-  if (s->dirty && (rc = s->reconfigure()) != 0) return rc;
-  t1 = rainbow::now_s();
-  s->stats[1] += t1 - t0;
-  if (s->slots.empty()) return RAINBOW_OK;
-  uint32_t k = rainbow::kServeSteps;
-  for (const auto& slot : s->slots) k = std::min(k, slot.left);
-  if ((rc = s->engine.step_eager(k)) != 0) return rc;
-  if ((rc = s->engine.sync()) != 0) return rc;
-  const double t = rainbow::now_s();
-  s->stats[2] += t - t1;
-  s->stats[4] += k;
-  if (stepped) *stepped = static_cast<uint32_t>(s->slots.size());
-  for (auto& slot : s->slots) slot.left -= k;
-  for (size_t i = s->slots.size(); i-- > 0;)
-    if (s->slots[i].left == 0u && (rc = s->retire(i, t)) != 0) return rc;
-  s->stats[3] += rainbow::now_s() - t;
+  uint32_t steps = 0;
+  int rc;
+  for (;;) {
+    double t0 = rainbow::now_s();
+    if ((rc = s->reap_steps(false)) != 0 || (rc = s->retire(t0)) != 0 || (rc = s->expire(t0)) != 0) return rc;
+    double t1 = rainbow::now_s();
+    s->stats[3] += t1 - t0;
+    if ((rc = s->admit()) != 0) return rc;
+    t0 = rainbow::now_s();
+    s->stats[0] += t0 - t1;
+    if (s->dirty && (rc = s->reconfigure()) != 0) return rc;
+    t1 = rainbow::now_s();
+    s->stats[1] += t1 - t0;
+    bool any = false;
+    for (const auto& sl : s->slots) any = any || sl.state == rainbow::SlotState::kResident;
+    if (!any || steps >= rainbow::kStepsPerPoll) break;
+    if (s->inflight.size() >= rainbow::kWindow) {
+      // The device is busy with the window: wait for its oldest step (new
+      // arrivals are admitted by the next poll).
+      if ((rc = s->reap_steps(true)) != 0) return rc;
+      s->stats[2] += rainbow::now_s() - t1;
+      if (steps > 0u) break;
+      continue;
+    }
+    if ((rc = s->step_once()) != 0) return rc;
+    ++steps;
+    s->stats[2] += rainbow::now_s() - t1;
+  }
+  if (stepped) *stepped = steps;
   return RAINBOW_OK;
 }
 
-int rainbow_server_stats(rainbow_server_t* s, double out[6]) {
+int rainbow_server_drain(rainbow_server_t* s) {
+  if (s == nullptr) return fail(RAINBOW_E_INVALID, "null handle");
+  for (;;) {
+    int rc = rainbow_server_poll(s, nullptr);
+    if (rc) return rc;
+    if (s->queue.empty() && s->slots.empty()) return RAINBOW_OK;
+    bool resident = false;
+    for (const auto& sl : s->slots) resident = resident || sl.state == rainbow::SlotState::kResident;
+    if (!resident) {
+      // Only copies and stops left: wait for them.
+      if (hipStreamSynchronize(s->engine.stream) != hipSuccess || hipStreamSynchronize(s->copy) != hipSuccess)
+        return fail(RAINBOW_E_HIP, "drain: stream synchronize failed");
+      if (s->queue.empty()) {
+        if ((rc = s->retire(rainbow::now_s())) != 0) return rc;
+        if (s->slots.empty()) return RAINBOW_OK;
+      }
+    }
+  }
+}
+
+int rainbow_server_stats(rainbow_server_t* s, double out[8]) {
   if (s == nullptr || out == nullptr) return fail(RAINBOW_E_INVALID, "null argument");
-  for (int k = 0; k < 6; ++k) out[k] = s->stats[k];
+  for (int k = 0; k < 8; ++k) out[k] = s->stats[k];
   return RAINBOW_OK;
 }
 
@@ -319,6 +708,7 @@ int rainbow_server_result(rainbow_server_t* s, uint64_t id, void* dst, size_t by
   std::memcpy(dst, it->second.data, bytes);
   s->pinned.put(it->second.data);
   s->results.erase(it);
+  s->ids.erase(id);
   return RAINBOW_OK;
 }
 

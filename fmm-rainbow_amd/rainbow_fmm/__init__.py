@@ -30,7 +30,7 @@ HEADER_PATH = PACKAGE_DIR.parent / "include" / "rainbow_fmm.h"
 
 PARTICLES, MOMENTS, FIELDS = 0, 1, 2
 OPT_COMPENSATED = 1
-OK, E_INVALID, E_HIP, E_NOMEM, E_BUDGET, E_STATE = 0, -1, -2, -3, -4, -5
+OK, E_INVALID, E_HIP, E_NOMEM, E_BUDGET, E_STATE, E_CANCELLED, E_TIMEOUT = 0, -1, -2, -3, -4, -5, -6, -7
 
 
 class RainbowError(RuntimeError):
@@ -77,6 +77,7 @@ class Request(ctypes.Structure):
         ("particles", ctypes.c_uint32),
         ("steps", ctypes.c_uint32),
         ("data", ctypes.c_void_p),
+        ("timeout_s", ctypes.c_double),
     ]
 
 
@@ -88,6 +89,8 @@ class Completion(ctypes.Structure):
         ("submit_s", ctypes.c_double),
         ("admit_s", ctypes.c_double),
         ("finish_s", ctypes.c_double),
+        ("status", ctypes.c_int32),
+        ("steps_run", ctypes.c_uint32),
     ]
 
 
@@ -164,6 +167,8 @@ _SIGNATURES = {
     "rainbow_server_request_bytes": (_u64, [_u32]),
     "rainbow_server_submit": (_i32, [_vp, _P(Request)]),
     "rainbow_server_poll": (_i32, [_vp, _P(_u32)]),
+    "rainbow_server_cancel": (_i32, [_vp, _u64]),
+    "rainbow_server_drain": (_i32, [_vp]),
     "rainbow_server_stats": (_i32, [_vp, _P(_f64)]),
     "rainbow_server_completions": (_i32, [_vp, _P(Completion), _u32, _P(_u32)]),
     "rainbow_server_result": (_i32, [_vp, _u64, _vp, _sz]),
@@ -599,10 +604,17 @@ class Server:
     def request_bytes(n: int) -> int:
         return lib().rainbow_server_request_bytes(n)
 
-    def submit(self, rid: int, particles: np.ndarray, steps: int):
+    def submit(self, rid: int, particles: np.ndarray, steps: int, timeout_s: float = 0.0):
         a = np.ascontiguousarray(particles, dtype=np.float32)
-        r = Request(rid, a.shape[0], steps, _ptr(a))
+        r = Request(rid, a.shape[0], steps, _ptr(a), timeout_s)
         _check(lib().rainbow_server_submit(self._h, ctypes.byref(r)))
+
+    def cancel(self, rid: int):
+        _check(lib().rainbow_server_cancel(self._h, rid))
+
+    def drain(self):
+        """Polls until every submitted request has completed."""
+        _check(lib().rainbow_server_drain(self._h))
 
     def poll(self) -> int:
         n = _u32()
@@ -622,10 +634,10 @@ class Server:
 
     def stats(self) -> dict:
         """Cumulative host seconds per scheduling activity, steps and polls."""
-        buf = (_f64 * 6)()
+        buf = (_f64 * 8)()
         _check(lib().rainbow_server_stats(self._h, buf))
         return {"admit_s": buf[0], "configure_s": buf[1], "step_s": buf[2], "retire_s": buf[3],
-                "steps": int(buf[4]), "polls": int(buf[5])}
+                "steps": int(buf[4]), "polls": int(buf[5]), "gpu_step_s": buf[6], "wait_s": buf[7]}
 
     def state(self):
         q, r, b = _u32(), _u32(), _u64()

@@ -170,7 +170,9 @@ enum {
   RAINBOW_E_HIP = -2,     /* a HIP runtime call failed             */
   RAINBOW_E_NOMEM = -3,   /* device or host allocation failed      */
   RAINBOW_E_BUDGET = -4,  /* request exceeds the admission budget  */
-  RAINBOW_E_STATE = -5    /* call not valid in the current state   */
+  RAINBOW_E_STATE = -5,   /* call not valid in the current state   */
+  RAINBOW_E_CANCELLED = -6, /* a served request was cancelled       */
+  RAINBOW_E_TIMEOUT = -7    /* a served request ran out of time     */
 };
 
 enum { RAINBOW_PARTICLES = 0, RAINBOW_MOMENTS = 1, RAINBOW_FIELDS = 2 };
@@ -388,26 +390,37 @@ int rainbow_batch_autotune(rainbow_batch_t* b, uint32_t steps, uint32_t* interva
 typedef struct rainbow_server rainbow_server_t;
 
 /* A request: body count, steps to run, and its particles (32 N bytes,
- * reference layout, caller-owned, copied at submit). */
+ * reference layout, caller-owned, copied at submit).  timeout_s > 0: the
+ * request is dropped (status RAINBOW_E_TIMEOUT) if it has not completed
+ * that many seconds after submit; 0 = no limit. */
 typedef struct rainbow_request {
   uint64_t id;
   uint32_t particles;
   uint32_t steps;
   const void* data;
+  double timeout_s;
 } rainbow_request_t;
 
-/* Completion record. */
+/* Completion record.  status RAINBOW_OK: the result is ready for
+ * rainbow_server_result; RAINBOW_E_CANCELLED / RAINBOW_E_TIMEOUT: the
+ * request stopped after steps_run of its steps and has no result. */
 typedef struct rainbow_completion {
   uint64_t id;
   uint32_t particles;
   uint32_t steps;
-  double submit_s;   /* host clock at submit                */
-  double admit_s;    /* host clock when it got a slot       */
-  double finish_s;   /* host clock when its last step ended */
+  double submit_s;   /* host clock at submit                          */
+  double admit_s;    /* host clock when it got a slot (0: never)       */
+  double finish_s;   /* host clock when its completion was observed    */
+  int32_t status;
+  uint32_t steps_run;
 } rainbow_completion_t;
 
 /* A server with `budget_bytes` of device memory for resident systems and at
- * most `max_slots` systems in flight. */
+ * most `max_slots` systems in flight.  Resident systems live in an arena of
+ * large device chunks grown as the working set grows (kept until destroy),
+ * so steady-state admission allocates nothing; the batched step's work
+ * tables are reserved for the whole budget at create.  Uploads and result
+ * copies run on a second stream of the same priority, ordered by events. */
 int rainbow_server_create(uint64_t budget_bytes, uint32_t max_slots,
                           const rainbow_params_t* params, uint32_t options,
                           rainbow_server_t** out);
@@ -415,25 +428,36 @@ void rainbow_server_destroy(rainbow_server_t* s);
 /* Device bytes a request of `particles` bodies holds while resident. */
 uint64_t rainbow_server_request_bytes(uint32_t particles);
 /* Queues a request (copies its particles).  RAINBOW_E_BUDGET if it could
- * never fit the budget. */
+ * never fit the budget; RAINBOW_E_INVALID for an empty request, a negative
+ * timeout or an id still in use (queued, running or with an untaken
+ * result). */
 int rainbow_server_submit(rainbow_server_t* s, const rainbow_request_t* r);
-/* One scheduling iteration: retire finished systems, admit queued requests
- * while budget and slots allow, run batched steps over every resident
- * system (up to 8, and no further than the earliest last step of a
- * resident request).  Writes the number of systems stepped. */
+/* Cancels a queued or running request: it stops at the next step boundary
+ * and completes with status RAINBOW_E_CANCELLED.  RAINBOW_E_STATE if the
+ * request is not queued or running. */
+int rainbow_server_cancel(rainbow_server_t* s, uint64_t id);
+/* One scheduling pass, without waiting for the device unless the step
+ * window is full: retire requests whose results have arrived, drop
+ * cancelled and expired ones, admit queued requests while budget and slots
+ * allow, and enqueue batched steps of every resident system (up to 3 steps
+ * ahead of the device, up to 16 per pass); a request leaves the batch right
+ * after its own last step.  Writes the number of steps enqueued. */
 int rainbow_server_poll(rainbow_server_t* s, uint32_t* stepped);
-/* Cumulative host seconds of the scheduling iterations: out[0] admitting
- * (allocation and upload), out[1] reconfiguring the batch, out[2] stepping
- * (launch and wait), out[3] retiring (download and free); out[4] batched
- * steps run, out[5] iterations. */
-int rainbow_server_stats(rainbow_server_t* s, double out[6]);
+/* Polls until every submitted request has completed. */
+int rainbow_server_drain(rainbow_server_t* s);
+/* Cumulative: out[0] host seconds admitting, out[1] reconfiguring the
+ * batch, out[2] enqueuing steps (and waiting on a full window), out[3]
+ * retiring; out[4] batched steps run, out[5] polls; out[6] device seconds
+ * of the steps (events around each); out[7] host seconds blocked on the
+ * device. */
+int rainbow_server_stats(rainbow_server_t* s, double out[8]);
 /* Pops up to `cap` completions; writes how many. */
 int rainbow_server_completions(rainbow_server_t* s, rainbow_completion_t* out,
                                uint32_t cap, uint32_t* n);
 /* Copies a completed request's final particles out (32 N bytes), once. */
 int rainbow_server_result(rainbow_server_t* s, uint64_t id, void* dst,
                           size_t bytes);
-/* Requests queued, resident, device bytes resident. */
+/* Requests queued, resident (running or finishing), device bytes resident. */
 int rainbow_server_state(rainbow_server_t* s, uint32_t* queued,
                          uint32_t* resident, uint64_t* bytes);
 

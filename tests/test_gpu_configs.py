@@ -18,6 +18,10 @@ the failure paths of the C ABI.
   with NaN and infinite coordinates (bit for bit, NaNs compared as NaN), the
   harness CLI's finiteness line, and a fixed-point scale changed between
   steps.
+* The server's request life cycle: cancelled (queued and running) and
+  expired requests, a zero-step request, duplicate ids, and a stream of
+  arrivals through room for a few systems at a time (arena reuse, systems
+  joining and leaving between steps), every result bit for bit.
 """
 
 import re
@@ -307,3 +311,71 @@ def test_scale_change_between_steps():
     assert_bits(s.moments(), m2, "moments at the new scale")
     assert_bits(s.fields(), f2, "fields at the new scale")
     s.close()
+
+
+def test_server_cancel_timeout_and_zero_steps():
+    say("server: cancel queued and running requests, timeouts, a zero-step request")
+    prm = rf.physics(**CFG)
+    sizes = {1: (60000, 40), 2: (30000, 500), 3: (20000, 5), 4: (40000, 7), 5: (16384, 0), 6: (25000, 3)}
+    data = {j: rf.ic_plummer(n, 700 + j, 0.1, 1.5, 1.0) for j, (n, _) in sizes.items()}
+    # Room for two systems at a time: request 4 waits in the queue.
+    budget = rf.Server.request_bytes(60000) + rf.Server.request_bytes(30000)
+    srv = rf.Server(budget, 2, prm)
+    for j in (1, 2, 3, 4, 5):
+        srv.submit(j, data[j], sizes[j][1])
+    with pytest.raises(rf.RainbowError) as e:
+        srv.submit(3, data[3], 1)
+    assert e.value.code == rf.E_INVALID and "in use" in str(e.value)
+    srv.poll()
+    srv.cancel(4)  # still queued
+    srv.cancel(2)  # running (500 steps)
+    srv.submit(6, data[6], 3, timeout_s=1e-9)  # expires before it can run
+    srv.drain()
+    done = {c.id: c for c in srv.completions()}
+    assert sorted(done) == [1, 2, 3, 4, 5, 6]
+    assert done[4].status == rf.E_CANCELLED and done[4].steps_run == 0
+    assert done[2].status == rf.E_CANCELLED and done[2].steps_run < 500
+    assert done[6].status == rf.E_TIMEOUT and done[6].steps_run == 0
+    for j in (1, 3, 5):
+        n, steps = sizes[j]
+        assert done[j].status == rf.OK and done[j].steps_run == steps
+        op, _, _ = orc.step(slots_for(n, **CFG), data[j], steps)
+        assert_bits(srv.result(j, n), op, f"request {j} beside cancelled ones")
+    for j in (2, 4, 6):
+        with pytest.raises(rf.RainbowError):
+            srv.result(j, sizes[j][0])
+    with pytest.raises(rf.RainbowError) as e:
+        srv.cancel(1)
+    assert e.value.code == rf.E_STATE
+    assert srv.state() == (0, 0, 0)
+    st = srv.stats()
+    assert st["steps"] > 0 and st["gpu_step_s"] > 0.0
+    srv.close()
+
+
+def test_server_steady_arrivals_reuse_the_arena():
+    # Many more requests than fit at once: every admission after the first
+    # few reuses arena memory freed by a finished request, and systems join
+    # and leave between steps while others run.
+    say("server: 40 requests through room for 5, results equal the oracle's")
+    prm = rf.physics(**CFG)
+    rng = np.random.default_rng(31)
+    reqs = [(j, int(rng.integers(1000, 40000)), int(rng.integers(1, 12))) for j in range(40)]
+    data = {j: rf.ic_uniform_ball(n, 300 + j, 1.0) for j, n, _ in reqs}
+    budget = 5 * rf.Server.request_bytes(40000)
+    srv = rf.Server(budget, 64, prm)
+    for j, n, steps in reqs[:10]:
+        srv.submit(j, data[j], steps)
+    nxt = 10
+    while nxt < len(reqs):
+        srv.poll()
+        srv.submit(reqs[nxt][0], data[reqs[nxt][0]], reqs[nxt][2])
+        nxt += 1
+    srv.drain()
+    done = {c.id: c for c in srv.completions()}
+    assert len(done) == len(reqs)
+    for j, n, steps in reqs:
+        assert done[j].status == rf.OK and done[j].steps_run == steps
+        op, _, _ = orc.step(slots_for(n, **CFG), data[j], steps)
+        assert_bits(srv.result(j, n), op, f"request {j}")
+    srv.close()
