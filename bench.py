@@ -18,10 +18,11 @@ step, OpenMP) on the same input for a bounded number of steps.
 
 import argparse
 import json
-import os
 import sys
 import time
 from pathlib import Path
+
+import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path[:0] = [str(ROOT / "fmm-rainbow_amd"), str(ROOT / "tests")]
@@ -30,7 +31,12 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 
 
-FLOPS_PER_PAIR = 50  # pair_field: 40 f32 operations (sqrt, div counted once); 10 additions into the field
+# Algorithmic FP32 operations per non-empty source pair: the reference's
+# pair_field plus add_field (src/rainbow_fmm.wgsl:100-118, 163-203) as
+# written, 60 with the square root and each division counted once (the
+# source's centre of mass and mass share, which the reference forms per
+# pair, included; the kernels form them once per cell).
+FLOPS_PER_PAIR = 60
 
 
 def algorithmic_bytes(n, depth, stats, finest_nonempty):
@@ -117,7 +123,7 @@ def roofline_of(phase, ms, bytes_per, flops_per):
     """HBM roofline, or for a tree launch the FP32 compute roofline."""
     if phase in flops_per:
         achieved = flops_per[phase] / (ms * 1e-3) / 1e12
-        return {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+        return {"bound": "fp32-valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "flops_per_launch": flops_per[phase]}
     achieved = bytes_per[phase] / (ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -177,135 +183,207 @@ def cpu_baseline(parts, cfg, budget_s):
     }
 
 
-def batch_form(args):
-    """256 uneven Plummer systems advanced together (one launch sequence per
-    step, graph-captured) against the same systems stepped one after another
-    with their own solvers."""
-    import numpy as np
+def cfg4_counts(systems):
+    rng = np.random.default_rng(4)
+    return [int(round(np.exp(rng.uniform(np.log(4096), np.log(262144))))) for _ in range(systems)]
 
+
+def batch_leg(args):
+    """256 uneven Plummer systems (N log-uniform in [4096, 262144], seed
+    4000 + i) advanced together, one launch sequence per step replayed from
+    captured graphs, against the same systems stepped one after another on
+    their own solvers.  Both legs are timed the same way: every solver is
+    first warmed through a full graph chunk and the sort offset (so no graph
+    is captured inside a timed region), then its K steps are timed with
+    HIP events on its own stream; the sequential GPU time is the sum of the
+    solvers' event times.  The host clock around the whole sequential loop
+    (each solver synchronised before the next starts) is reported too."""
     import rainbow_fmm as rf
 
-    rng = np.random.default_rng(4)
-    counts = [int(round(np.exp(rng.uniform(np.log(4096), np.log(262144))))) for _ in range(args.systems)]
+    counts = cfg4_counts(args.systems)
     prm = rf.physics(dt=5e-4, gravity=1.0, softening=1e-3)
     inputs = [rf.ic_plummer(n, 4000 + i, 0.1, 1.5, 1.0) for i, n in enumerate(counts)]
     total = sum(counts)
+    steps = args.batch_steps
     b = rf.Batch(counts, prm)
     for i, p in enumerate(inputs):
         b.upload(i, p)
     b.step(args.warmup)
     b.sync()
-    ms_batch = b.time_steps(args.steps)
+    ms_batch = b.time_steps(steps)
     b.close()
-    # Sequential plain launches: each system on its own solver.
     solvers = []
     for i, n in enumerate(counts):
         s = rf.Solver(n, prm)
         s.upload(inputs[i])
-        s.step(args.warmup)
-        s.sync()
+        s.step(24)  # a 16-step graph chunk after the first sort, both parities captured
         solvers.append(s)
-    # One after another: each solver's steps complete before the next
-    # solver starts (separate streams would otherwise overlap them).
+    for s in solvers:
+        s.sync()
+    seq_gpu_ms = sum(s.time_steps(steps) for s in solvers)
     t0 = time.perf_counter()
     for s in solvers:
-        s.step(args.steps)
+        s.step(steps)
         s.sync()
-    seq_s = time.perf_counter() - t0
+    seq_wall = time.perf_counter() - t0
     for s in solvers:
         s.close()
-    batch_rate = total * args.steps / (ms_batch / 1e3)
-    seq_rate = total * args.steps / seq_s
+    batch_rate = total * steps / (ms_batch / 1e3)
     return {
-        "metric": "batched particle-steps/s, 256 uneven Plummer systems",
+        "workload": f"{len(counts)} Plummer systems a=0.1, N log-uniform in [4096, 262144] (seed 4, system i "
+                    f"seeded 4000+i), {total} bodies, dt 5e-4, softening 1e-3; {steps} timed steps",
         "value": batch_rate,
         "unit": "particle-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_batch / args.steps,
-        "higher_is_better": True,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": f"{len(counts)} Plummer systems, N log-uniform in [4096, 262144] (seed 4), "
-                               f"{total} bodies, dt 5e-4, softening 1e-3"},
-        "sequential_value": seq_rate,
-        "speedup_vs_sequential": batch_rate / seq_rate,
+        "ms_per_step": ms_batch / steps,
+        "sequential_value": total * steps / (seq_gpu_ms / 1e3),
+        "sequential_wall_value": total * steps / seq_wall,
+        "speedup": batch_rate / (total * steps / (seq_gpu_ms / 1e3)),
+        "speedup_wall": batch_rate / (total * steps / seq_wall),
     }
 
 
-def serve_form(args):
-    """Poisson arrivals of N-body requests into the continuous-batching
-    server; latency = completion - arrival."""
-    import numpy as np
-
+def serve_pool(npool, nmax):
+    """Distinct request inputs: N log-uniform in [16384, nmax] (one at nmax),
+    a seeded Plummer sphere (a = 0.1) or uniform ball, seed 5000 + j."""
     import rainbow_fmm as rf
 
     rng = np.random.default_rng(5)
-    nreq = args.requests
-    reqs = []
-    for j in range(nreq):
-        n = int(round(np.exp(rng.uniform(np.log(16384), np.log(args.serve_nmax)))))
-        steps = int(rng.integers(1, 201))
-        kind = "plummer" if rng.uniform() < 0.5 else "ball"
-        reqs.append((j, n, steps, kind))
-    data = {}
-    for j, n, steps, kind in reqs:
-        data[j] = (rf.ic_plummer(n, 5000 + j, 0.1, 1.5, 1.0) if kind == "plummer"
-                   else rf.ic_uniform_ball(n, 5000 + j, 1.0))
-    prm = rf.physics(dt=5e-4, gravity=1.0, softening=1e-3)
-    budget = int(args.budget_gb * 1e9)
-    # Capacity probe: a mix of the request sizes stepped as one batch.
-    probe = [n for _, n, _, _ in reqs[:32]]
-    b = rf.Batch(probe, prm)
-    for i, n in enumerate(probe):
-        b.upload(i, data[i])
-    b.step(2)
-    b.sync()
-    ms = b.time_steps(10)
-    b.close()
-    capacity = sum(probe) * 10 / (ms / 1e3)
-    mean_work = float(np.mean([n * s for _, n, s, _ in reqs]))
-    lam = args.load * capacity / mean_work
-    gaps = rng.exponential(1.0 / lam, size=nreq)
-    arrivals = np.cumsum(gaps) - gaps[0]
-    srv = rf.Server(budget, args.max_slots, prm)
+    pool = []
+    for j in range(npool):
+        n = int(round(np.exp(rng.uniform(np.log(16384), np.log(nmax)))))
+        if j == npool // 3:
+            n = nmax
+        plummer = rng.uniform() < 0.5
+        data = rf.host_array(n)  # pinned: requests are uploaded by DMA straight from it
+        data[:] = rf.ic_plummer(n, 5000 + j, 0.1, 1.5, 1.0) if plummer else rf.ic_uniform_ball(n, 5000 + j, 1.0)
+        pool.append((n, "plummer" if plummer else "ball", data))
+    return pool
+
+
+def run_server(srv, pool, reqs, arrivals):
+    """Submits reqs (pool index, steps) at their arrival times (seconds from
+    the start) while polling; returns (wall s, {id: completion}, t0)."""
     done = {}
     t0 = time.monotonic()
     nxt = 0
-    work = 0
-    while len(done) < nreq:
+    while len(done) < len(reqs):
         now = time.monotonic() - t0
-        while nxt < nreq and arrivals[nxt] <= now:
-            j, n, steps, _ = reqs[nxt]
-            srv.submit(j, data[j], steps)
+        while nxt < len(reqs) and arrivals[nxt] <= now:
+            k, steps = reqs[nxt]
+            srv.submit(nxt, pool[k][2], steps, borrow=True)
             nxt += 1
         stepped = srv.poll()
         for c in srv.completions():
             done[c.id] = c
-            work += c.particles * c.steps
-        if stepped == 0 and nxt < nreq:
-            time.sleep(max(0.0, min(0.001, arrivals[nxt] - (time.monotonic() - t0))))
-    wall = time.monotonic() - t0
-    lat = np.array([done[j].finish_s - (t0 + arrivals[j]) for j in range(nreq)])
-    host = srv.stats()
+            if c.status == 0:
+                srv.result_view(c.id)  # the client takes the result in place
+                srv.release(c.id)
+        if stepped == 0 and nxt < len(reqs):
+            queued, resident, _ = srv.state()
+            wait = arrivals[nxt] - (time.monotonic() - t0)
+            if queued == 0 and resident == 0 and wait > 0:
+                time.sleep(min(wait, 0.0005))
+    return time.monotonic() - t0, done, t0
+
+
+def serve_leg(args):
+    """The continuous-batching server on mixed-N requests (N log-uniform in
+    [16384, 2097152], 1..200 steps, Plummer or uniform ball, 200 GB budget):
+
+    * saturated: every request submitted at once; aggregate particle-steps/s
+      over the wall time (uploads, steps and downloads included) and the
+      device's busy fraction (event time of the steps over the wall);
+    * sequential: the same requests one after another on plain solvers
+      (upload, steps, download; solvers created and warmed beforehand), on
+      the same host clock; the ratio is the served speedup;
+    * Poisson: arrivals at `load` x the saturated rate for `arrival_s`
+      seconds; p50/p99 of (completion - arrival), next to the same requests'
+      solo latency (upload + steps + download on a plain solver)."""
+    import rainbow_fmm as rf
+
+    prm = rf.physics(dt=5e-4, gravity=1.0, softening=1e-3)
+    pool = serve_pool(args.serve_pool, args.serve_nmax)
+    rng = np.random.default_rng(55)
+    budget = int(args.budget_gb * 1e9)
+
+    # Solo latency model per pool entry: upload + download (host clock) and
+    # per-step GPU time (graph replays), from a warmed plain solver.
+    solvers = []
+    for n, _, data in pool:
+        s = rf.Solver(n, prm)
+        s.upload(data)
+        s.step(24)
+        s.sync()
+        solvers.append(s)
+    updown, per_step = [], []
+    for (n, _, data), s in zip(pool, solvers):
+        t0 = time.perf_counter()
+        s.upload(data)
+        s.particles()
+        updown.append(time.perf_counter() - t0)
+        per_step.append(s.time_steps(32) / 32 / 1e3)
+
+    # Saturated: all at once.
+    sat = [(int(rng.integers(0, len(pool))), int(rng.integers(1, 201))) for _ in range(args.requests)]
+    work = sum(pool[k][0] * st for k, st in sat)
+    srv = rf.Server(budget, args.max_slots, prm)
+    wall, done, _ = run_server(srv, pool, sat, np.zeros(len(sat)))
+    st_sat = srv.stats()
     srv.close()
+    sat_rate = work / wall
+
+    # Sequential: the same requests one after another on plain solvers.
+    t0 = time.perf_counter()
+    solo_lat = []
+    for k, steps in sat:
+        t1 = time.perf_counter()
+        s = solvers[k]
+        s.upload(pool[k][2])
+        s.step(steps)
+        s.particles()
+        solo_lat.append(time.perf_counter() - t1)
+    seq_wall = time.perf_counter() - t0
+    seq_rate = work / seq_wall
+
+    # Poisson arrivals at load x the saturated rate.
+    mean_work = float(np.mean([pool[k][0] * st for k, st in sat]))
+    lam = args.load * sat_rate / mean_work
+    nreq = max(8, int(lam * args.arrival_s))
+    reqs = [(int(rng.integers(0, len(pool))), int(rng.integers(1, 201))) for _ in range(nreq)]
+    gaps = rng.exponential(1.0 / lam, size=nreq)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    srv = rf.Server(budget, args.max_slots, prm)
+    wall_p, done_p, t0p = run_server(srv, pool, reqs, arrivals)
+    st_p = srv.stats()
+    srv.close()
+    for s in solvers:
+        s.close()
+    lat = np.array([done_p[j].finish_s - (t0p + arrivals[j]) for j in range(nreq)])
+    solo = np.array([updown[k] + st * per_step[k] for k, st in reqs])
+    work_p = sum(pool[k][0] * st for k, st in reqs)
+    ok = all(c.status == 0 for c in done_p.values()) and all(c.status == 0 for c in done.values())
     return {
-        "metric": "serving: p50/p99 request latency and aggregate particle-steps/s",
-        "value": work / wall,
-        "unit": "particle-steps/s",
-        "n_gpus": 1,
+        "workload": f"requests from {len(pool)} seeded inputs (N log-uniform in [16384, {args.serve_nmax}], one "
+                    f"at {args.serve_nmax}; Plummer a=0.1 or uniform ball), steps U{{1..200}}, dt 5e-4, "
+                    f"softening 1e-3, budget {args.budget_gb:g} GB, at most {args.max_slots} resident",
         "p50_ms": float(np.percentile(lat, 50) * 1e3),
         "p99_ms": float(np.percentile(lat, 99) * 1e3),
-        "wall_s": wall,
-        "server_host_s": {k: round(v, 4) if isinstance(v, float) else v for k, v in host.items()},
-        "higher_is_better": True,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": f"{nreq} requests, Poisson arrivals at {lam:.1f}/s ({args.load} x probed capacity "
-                               f"{capacity:.3g} particle-steps/s), N log-uniform in [16384, {args.serve_nmax}], "
-                               f"steps U{{1..200}}, Plummer or uniform ball, budget {args.budget_gb} GB, "
-                               f"max {args.max_slots} resident"},
+        "value": work_p / wall_p,
+        "unit": "particle-steps/s",
+        "poisson": {"requests": nreq, "rate_per_s": lam, "load": args.load, "arrival_s": float(arrivals[-1]),
+                    "wall_s": wall_p, "gpu_busy": st_p["gpu_step_s"] / wall_p,
+                    "solo_p50_ms": float(np.percentile(solo, 50) * 1e3),
+                    "solo_p99_ms": float(np.percentile(solo, 99) * 1e3),
+                    "slowdown_p50": float(np.percentile(lat / solo, 50)),
+                    "slowdown_p99": float(np.percentile(lat / solo, 99))},
+        "saturated": {"requests": len(sat), "value": sat_rate, "wall_s": wall,
+                      "gpu_busy": st_sat["gpu_step_s"] / wall, "steps": st_sat["steps"],
+                      "host_s": {k: round(v, 4) for k, v in st_sat.items() if k.endswith("_s")}},
+        "sequential_value": seq_rate,
+        "speedup": sat_rate / seq_rate,
+        "solo_p50_ms": float(np.percentile(solo_lat, 50) * 1e3),
+        "solo_p99_ms": float(np.percentile(solo_lat, 99) * 1e3),
+        "all_completed": ok,
     }
 
 
@@ -387,9 +465,14 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form", choices=["plain", "batch", "serve", "precision"], default="plain")
+    ap.add_argument("--no-batch", action="store_true", help="leave the batched form out of the plain line")
+    ap.add_argument("--no-serve", action="store_true", help="leave the serving form out of the plain line")
     ap.add_argument("--precision-n", type=int, default=5242880)
     ap.add_argument("--systems", type=int, default=256)
-    ap.add_argument("--requests", type=int, default=96)
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--requests", type=int, default=256, help="requests of the saturated serving run")
+    ap.add_argument("--serve-pool", type=int, default=32, help="distinct seeded request inputs")
+    ap.add_argument("--arrival-s", type=float, default=10.0, help="seconds of Poisson arrivals")
     ap.add_argument("--serve-nmax", type=int, default=2097152)
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--max-slots", type=int, default=256)
@@ -400,16 +483,16 @@ def main():
               "decomposition is a design note, not built.", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
-
     import rainbow_fmm as rf
     import workloads as wl
 
     if args.form == "batch":
-        print(json.dumps(batch_form(args)), flush=True)
+        print(json.dumps({"metric": "batched particle-steps/s, 256 uneven Plummer systems", "n_gpus": 1,
+                          "dtype": "f32", "data": "synthetic", **batch_leg(args)}), flush=True)
         return
     if args.form == "serve":
-        print(json.dumps(serve_form(args)), flush=True)
+        print(json.dumps({"metric": "serving: p50/p99 request latency and aggregate particle-steps/s",
+                          "n_gpus": 1, "dtype": "f32", "data": "synthetic", **serve_leg(args)}), flush=True)
         return
     if args.form == "precision":
         print(json.dumps(precision_form(args)), flush=True)
@@ -475,7 +558,15 @@ def main():
         "wall_s_timed": wall,
     }
     solver.close()
+    # The metric's second half and the batched form, on the same device.
+    if not args.no_serve:
+        print("bench: serving form", file=sys.stderr, flush=True)
+        line["serving"] = serve_leg(args)
+    if not args.no_batch:
+        print("bench: batched form", file=sys.stderr, flush=True)
+        line["batch"] = batch_leg(args)
     if not args.no_cpu:
+        print("bench: CPU baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(parts, cfg, args.cpu_seconds)
     print(json.dumps(line), flush=True)
 

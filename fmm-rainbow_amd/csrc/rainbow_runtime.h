@@ -97,12 +97,21 @@ class Engine {
   int configure(const std::vector<SysDesc>& systems);
   // The same between steps without waiting: systems keep their state, and
   // `fresh` (indices into `systems`, storage zeroed, bodies in place) are
-  // deposited by the next step.
+  // deposited by the next step, as are the systems an earlier install
+  // listed that no step has deposited yet.
   int configure_async(const std::vector<SysDesc>& systems, const std::vector<uint32_t>& fresh);
   // Table capacity for later installs of up to these totals.
   int reserve(uint64_t bodies, uint32_t systems, uint64_t finest);
-  // Systems join only through configure_async from now on (the server).
+  // Systems join only through configure_async / append_systems from now on
+  // (the server).
   void start_incremental();
+  // Appends fresh systems to the set, keeping everyone's place (only the
+  // tables' new tails are copied).
+  int append_systems(const std::vector<SysDesc>& add);
+  // Takes a system out of the set by marking it dead (kernels skip it);
+  // its table entries stay until the next full install.
+  int retire_system(const float4* pos);
+  uint64_t dead_blocks() const;
   int push_descriptors();
   int set_physics(const Physics& ph);
   int enqueue(bool integrate, bool sorting, uint32_t step_parity, bool deposit_first, bool rank_next,
@@ -161,11 +170,19 @@ class Engine {
   void drop_graphs();
 
   int install(const std::vector<SysDesc>& systems, const std::vector<uint32_t>& fresh);
+  void add_tables(uint32_t s);
+  void collect_fresh(const std::vector<const float4*>& listed);
+  int upload_tables(uint32_t s0, uint32_t d0, uint32_t i0, uint32_t b0);
+  uint64_t dead_blocks_ = 0;
+  bool descs_dirty_ = false;  // host descriptors changed since the last copy
+  int flush_descriptors();
 
   DevBuf sys_buf, deposit_buf, fresh_buf, integrate_buf, subtree_buf, list_buf, count_buf, stripe_buf, diag_buf,
       snap_buf;
   // Host copies of the work tables (kept to reuse their storage).
   std::vector<BodyBlock> dep_, integ_, fresh_;
+  // Bodies (pos pointers) of the systems the next step deposits first.
+  std::vector<const float4*> fresh_pending_;
   std::vector<SubtreeBlock> sub_;
   // Pinned staging of the table copies, a ring: a buffer is refilled once
   // the copies that read it have run (its event).
@@ -174,9 +191,10 @@ class Engine {
     size_t cap = 0;
     hipEvent_t done = nullptr;
   };
-  static constexpr uint32_t kTableStages = 4;
+  static constexpr uint32_t kTableStages = 8;
   TableStage stages_[kTableStages];
   uint32_t stage_next_ = 0;
+  int next_stage(size_t bytes, TableStage** out);
 
  public:
   DevBuf io_buf;  // staging of the direct sum

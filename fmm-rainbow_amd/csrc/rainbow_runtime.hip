@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <cstddef>
 #include <cstring>
 #include <new>
 #include <string>
@@ -46,6 +47,12 @@ int fail(int code, const char* fmt, ...) {
 void clear_error() { g_error.clear(); }
 
 const char* last_error() { return g_error.c_str(); }
+
+#define RB_HIP_RC(expr) \
+  do {                    \
+    const int rc_ = (expr); \
+    if (rc_ != RAINBOW_OK) return rc_; \
+  } while (0)
 
 #define RB_HIP(expr)                                                                          \
   do {                                                                                        \
@@ -305,61 +312,182 @@ int Engine::reserve(uint64_t bodies, uint32_t systems, uint64_t finest) {
   return RAINBOW_OK;
 }
 
+// Appends system s's work: deposit and integrate ranges, subtree blocks,
+// and its share of the list capacity of every (level, stripe): a subtree
+// block appends at most 64 / 8 / 1 internal cells of levels depth-1 / -2 /
+// -3 to stripe (block index mod kStripes), the top block of system s
+// appends the levels above to stripe (s mod kStripes); a level never holds
+// more than n / 17 internal cells.
+void Engine::add_tables(uint32_t s) {
+  const SysDesc& d = sys[s];
+  const uint32_t ib = d.n / (kAdaptiveLeafCapacity + 1u);
+  for (uint32_t f = 0; f < d.n; f += kDepositBodies) dep_.push_back(BodyBlock{s, f});
+  for (uint32_t f = 0; f < d.n; f += kIntegrateThreads) integ_.push_back(BodyBlock{s, f});
+  for (uint32_t r = 0; r < level_cells(d.depth - 3u); ++r) {
+    const uint32_t stripe = static_cast<uint32_t>(sub_.size()) % kStripes;
+    caps_host[(d.depth - 1u) * kStripes + stripe] += std::min(64u, ib);
+    caps_host[(d.depth - 2u) * kStripes + stripe] += std::min(8u, ib);
+    caps_host[(d.depth - 3u) * kStripes + stripe] += std::min(1u, ib);
+    sub_.push_back(SubtreeBlock{s, r});
+  }
+  for (uint32_t k = 0; k + 3u < d.depth; ++k) caps_host[k * kStripes + s % kStripes] += std::min(level_cells(k), ib);
+  dmax = std::max(dmax, d.depth);
+  total_bodies += d.n;
+}
+
+// The deposit ranges of the systems awaiting their first deposit: those
+// listed in `listed` (bodies pointers) and those an earlier install listed
+// that no step has deposited yet (installs can follow each other between
+// steps).
+void Engine::collect_fresh(const std::vector<const float4*>& listed) {
+  std::vector<const float4*> waiting = listed;
+  for (const float4* p : fresh_pending_)
+    if (std::find(waiting.begin(), waiting.end(), p) == waiting.end()) waiting.push_back(p);
+  fresh_.clear();
+  std::vector<const float4*> kept;
+  for (uint32_t s = 0; s < sys.size(); ++s) {
+    if (!sys[s].live || std::find(waiting.begin(), waiting.end(), sys[s].pos) == waiting.end()) continue;
+    kept.push_back(sys[s].pos);
+    for (uint32_t f = 0; f < sys[s].n; f += kDepositBodies) fresh_.push_back(BodyBlock{s, f});
+  }
+  fresh_pending_ = kept;
+  n_fresh = static_cast<uint32_t>(fresh_.size());
+}
+
 // Builds the work tables of `systems` on the host and copies them (and the
 // descriptors) to the device in stream order, through a ring of pinned
 // staging buffers; nothing here waits for the device unless a buffer must
 // grow or every staging buffer is still being read.
 int Engine::install(const std::vector<SysDesc>& systems, const std::vector<uint32_t>& fresh) {
   if (systems.size() > 65535u) return fail(RAINBOW_E_INVALID, "at most 65535 systems");
+  std::vector<const float4*> listed;
+  for (uint32_t s : fresh) {
+    if (s >= systems.size()) return fail(RAINBOW_E_INVALID, "fresh system %u out of range", s);
+    listed.push_back(systems[s].pos);
+  }
   sys = systems;
-  const uint32_t nsys = static_cast<uint32_t>(sys.size());
   dep_.clear();
   integ_.clear();
   sub_.clear();
-  fresh_.clear();
-  // Capacity of every (level, stripe) list: a subtree block appends at most
-  // 64 / 8 / 1 internal cells of levels depth-1 / -2 / -3 to stripe (block
-  // index mod kStripes), the top block of system s appends the levels above
-  // to stripe (s mod kStripes); a level never holds more than n / 17
-  // internal cells.
-  std::vector<uint32_t> caps(kLevels * kStripes, 0u);
+  caps_host.assign(kLevels * kStripes, 0u);
   dmax = 0;
   total_bodies = 0;
-  uint64_t total_finest = 0;
-  for (uint32_t s = 0; s < nsys; ++s) {
-    const SysDesc& d = sys[s];
-    total_finest += level_cells(d.depth);
-    const uint32_t ib = d.n / (kAdaptiveLeafCapacity + 1u);
-    for (uint32_t f = 0; f < d.n; f += kDepositBodies) dep_.push_back(BodyBlock{s, f});
-    for (uint32_t f = 0; f < d.n; f += kIntegrateThreads) integ_.push_back(BodyBlock{s, f});
-    for (uint32_t r = 0; r < level_cells(d.depth - 3u); ++r) {
-      const uint32_t stripe = static_cast<uint32_t>(sub_.size()) % kStripes;
-      caps[(d.depth - 1u) * kStripes + stripe] += std::min(64u, ib);
-      caps[(d.depth - 2u) * kStripes + stripe] += std::min(8u, ib);
-      caps[(d.depth - 3u) * kStripes + stripe] += std::min(1u, ib);
-      sub_.push_back(SubtreeBlock{s, r});
+  dead_blocks_ = 0;
+  for (uint32_t s = 0; s < sys.size(); ++s) add_tables(s);
+  collect_fresh(listed);
+  return upload_tables(0u, 0u, 0u, 0u);
+}
+
+// Appends `add` to the set (each fresh: storage zeroed, bodies in place);
+// the systems already in the set keep their places, so only the tables'
+// new tails, the stripe layout and the fresh list are copied.  Falls back
+// to a full install (without the dead systems) when the tables would
+// outgrow their buffers.
+int Engine::append_systems(const std::vector<SysDesc>& add) {
+  if (add.empty()) return RAINBOW_OK;
+  std::vector<const float4*> listed;
+  for (const SysDesc& d : add) listed.push_back(d.pos);
+  uint64_t bodies = 0;
+  for (const SysDesc& d : add) bodies += d.n;
+  const uint64_t dep = dep_.size() + bodies / kDepositBodies + add.size();
+  const uint64_t integ = integ_.size() + bodies / kIntegrateThreads + add.size();
+  uint64_t sub = sub_.size();
+  for (const SysDesc& d : add) sub += level_cells(d.depth - 3u);
+  const bool fits = sys.size() + add.size() <= 65535u && sizeof(SysDesc) * (sys.size() + add.size()) <= sys_buf.cap &&
+                    sizeof(BodyBlock) * dep <= deposit_buf.cap && sizeof(BodyBlock) * integ <= integrate_buf.cap &&
+                    sizeof(SubtreeBlock) * sub <= subtree_buf.cap && sizeof(BodyBlock) * dep <= fresh_buf.cap &&
+                    total_bodies + bodies <= 0xFFFFFFFFull;
+  if (!fits) {
+    std::vector<SysDesc> live;
+    std::vector<uint32_t> fr;
+    for (const SysDesc& d : sys)
+      if (d.live) live.push_back(d);
+    for (const SysDesc& d : add) {
+      fr.push_back(static_cast<uint32_t>(live.size()));
+      live.push_back(d);
     }
-    for (uint32_t k = 0; k + 3u < d.depth; ++k) caps[k * kStripes + s % kStripes] += std::min(level_cells(k), ib);
-    dmax = std::max(dmax, d.depth);
-    total_bodies += d.n;
+    return configure_async(live, fr);
   }
-  for (uint32_t s : fresh) {
-    if (s >= nsys) return fail(RAINBOW_E_INVALID, "fresh system %u out of range", s);
-    for (uint32_t f = 0; f < sys[s].n; f += kDepositBodies) fresh_.push_back(BodyBlock{s, f});
+  const uint32_t s0 = static_cast<uint32_t>(sys.size());
+  const uint32_t d0 = static_cast<uint32_t>(dep_.size()), i0 = static_cast<uint32_t>(integ_.size()),
+                 b0 = static_cast<uint32_t>(sub_.size());
+  for (const SysDesc& d : add) {
+    sys.push_back(d);
+    add_tables(static_cast<uint32_t>(sys.size() - 1u));
   }
-  if (total_bodies > 0xFFFFFFFFull) return fail(RAINBOW_E_INVALID, "at most 2^32 - 1 bodies per batch");
+  collect_fresh(listed);
+  return upload_tables(s0, d0, i0, b0);
+}
+
+// Takes the system with bodies `pos` out of the set: its descriptor's live
+// flag is cleared on the device (in stream order), every kernel of a later
+// step skips it, and its tables stay as tombstones until the next full
+// install.  Once the steps enqueued before this call have run, its storage
+// is no longer read.
+int Engine::retire_system(const float4* pos) {
+  uint32_t s = 0;
+  while (s < sys.size() && !(sys[s].pos == pos && sys[s].live)) ++s;
+  if (s == sys.size()) return fail(RAINBOW_E_INVALID, "system not in the set");
+  sys[s].live = 0u;
+  dead_blocks_ += (sys[s].n + kIntegrateThreads - 1u) / kIntegrateThreads;
+  descs_dirty_ = true;  // copied before the next step's kernels
+  return RAINBOW_OK;
+}
+
+// The descriptor table, after systems were marked dead: one copy before the
+// next step (stream order keeps it after the steps already enqueued).
+int Engine::flush_descriptors() {
+  if (!descs_dirty_ || sys.empty()) return RAINBOW_OK;
+  TableStage* st = nullptr;
+  const size_t bytes = sizeof(SysDesc) * sys.size();
+  int rc = next_stage(bytes, &st);
+  if (rc) return rc;
+  std::memcpy(st->host, sys.data(), bytes);
+  RB_HIP(hipMemcpyAsync(sys_buf.p, st->host, bytes, hipMemcpyHostToDevice, stream));
+  RB_HIP(hipEventRecord(st->done, stream));
+  descs_dirty_ = false;
+  return RAINBOW_OK;
+}
+
+// Integrate ranges of systems taken out of the set, still in the tables.
+uint64_t Engine::dead_blocks() const { return dead_blocks_; }
+
+// The next pinned staging buffer of the ring, with room for `bytes`, once
+// the copies that last read it have run.
+int Engine::next_stage(size_t bytes, TableStage** out) {
+  TableStage& st = stages_[stage_next_];
+  stage_next_ = (stage_next_ + 1u) % kTableStages;
+  if (st.done != nullptr) RB_HIP(hipEventSynchronize(st.done));
+  if (st.cap < bytes) {
+    if (st.host != nullptr) (void)hipHostFree(st.host);
+    st.host = nullptr;
+    st.cap = 0;
+    const size_t want = std::max<size_t>(bytes + bytes / 4u, 1u << 16);
+    RB_HIP(hipHostMalloc(&st.host, want, hipHostMallocDefault));
+    st.cap = want;
+  }
+  if (st.done == nullptr) RB_HIP(hipEventCreateWithFlags(&st.done, hipEventDisableTiming));
+  *out = &st;
+  return RAINBOW_OK;
+}
+
+// Copies the tables from the given first entries on (0: all), the stripe
+// layout and the fresh list to the device; sets the step arguments and the
+// tree launches.
+int Engine::upload_tables(uint32_t s0, uint32_t d0, uint32_t i0, uint32_t b0) {
+  if (descs_dirty_) s0 = 0u;  // dead flags not yet copied: copy every descriptor
+  descs_dirty_ = false;
+  const uint32_t nsys = static_cast<uint32_t>(sys.size());
   n_deposit = static_cast<uint32_t>(dep_.size());
   n_integrate = static_cast<uint32_t>(integ_.size());
   n_subtree = static_cast<uint32_t>(sub_.size());
-  n_fresh = static_cast<uint32_t>(fresh_.size());
   std::vector<uint32_t> stripes(2u * kLevels * kStripes);
   uint32_t list_total = 0;
   for (uint32_t i = 0; i < kLevels * kStripes; ++i) {
     stripes[i] = list_total;
-    stripes[kLevels * kStripes + i] = caps[i];
-    list_total += caps[i];
+    stripes[kLevels * kStripes + i] = caps_host[i];
+    list_total += caps_host[i];
   }
-  caps_host = caps;
   counts_pending = false;
   split_wanted = split_auto;
   int rc;
@@ -384,28 +512,17 @@ int Engine::install(const std::vector<SysDesc>& systems, const std::vector<uint3
   args.compensated = (options & RAINBOW_OPT_COMPENSATED) ? 1u : 0u;
   for (uint32_t l = 1; l <= kLevels - 1u; ++l) {
     uint64_t items = 0;
-    for (uint32_t j = 0; j < kStripes; ++j) items += caps[(l - 1u) * kStripes + j];
+    for (uint32_t j = 0; j < kStripes; ++j) items += caps_host[(l - 1u) * kStripes + j];
     tree_blocks[l] = static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(items, 1u), kTreeMaxBlocks));
   }
   plan_tree();
-  // Stage every table in one pinned buffer of the ring and copy it out.
-  const size_t b_sys = sizeof(SysDesc) * nsys, b_dep = sizeof(BodyBlock) * n_deposit,
-               b_int = sizeof(BodyBlock) * n_integrate, b_sub = sizeof(SubtreeBlock) * n_subtree,
+  // Stage the new parts of every table in one pinned buffer and copy them.
+  const size_t b_sys = sizeof(SysDesc) * (nsys - s0), b_dep = sizeof(BodyBlock) * (n_deposit - d0),
+               b_int = sizeof(BodyBlock) * (n_integrate - i0), b_sub = sizeof(SubtreeBlock) * (n_subtree - b0),
                b_str = sizeof(uint32_t) * stripes.size(), b_fr = sizeof(BodyBlock) * n_fresh;
-  const size_t total = b_sys + b_dep + b_int + b_sub + b_str + b_fr;
-  TableStage& st = stages_[stage_next_];
-  stage_next_ = (stage_next_ + 1u) % kTableStages;
-  if (st.done != nullptr) RB_HIP(hipEventSynchronize(st.done));
-  if (st.cap < total) {
-    if (st.host != nullptr) (void)hipHostFree(st.host);
-    st.host = nullptr;
-    st.cap = 0;
-    const size_t want = std::max<size_t>(total + total / 4u, 1u << 16);
-    RB_HIP(hipHostMalloc(&st.host, want, hipHostMallocDefault));
-    st.cap = want;
-  }
-  if (st.done == nullptr) RB_HIP(hipEventCreateWithFlags(&st.done, hipEventDisableTiming));
-  char* h = static_cast<char*>(st.host);
+  TableStage* st = nullptr;
+  if ((rc = next_stage(b_sys + b_dep + b_int + b_sub + b_str + b_fr, &st)) != RAINBOW_OK) return rc;
+  char* h = static_cast<char*>(st->host);
   auto put = [&](void* dev, const void* src, size_t bytes) -> hipError_t {
     if (bytes == 0u) return hipSuccess;
     std::memcpy(h, src, bytes);
@@ -413,13 +530,13 @@ int Engine::install(const std::vector<SysDesc>& systems, const std::vector<uint3
     h += bytes;
     return e;
   };
-  RB_HIP(put(sys_buf.p, sys.data(), b_sys));
-  RB_HIP(put(deposit_buf.p, dep_.data(), b_dep));
-  RB_HIP(put(integrate_buf.p, integ_.data(), b_int));
-  RB_HIP(put(subtree_buf.p, sub_.data(), b_sub));
+  RB_HIP(put(static_cast<SysDesc*>(sys_buf.p) + s0, sys.data() + s0, b_sys));
+  RB_HIP(put(static_cast<BodyBlock*>(deposit_buf.p) + d0, dep_.data() + d0, b_dep));
+  RB_HIP(put(static_cast<BodyBlock*>(integrate_buf.p) + i0, integ_.data() + i0, b_int));
+  RB_HIP(put(static_cast<SubtreeBlock*>(subtree_buf.p) + b0, sub_.data() + b0, b_sub));
   RB_HIP(put(stripe_buf.p, stripes.data(), b_str));
   RB_HIP(put(fresh_buf.p, fresh_.data(), b_fr));
-  RB_HIP(hipEventRecord(st.done, stream));
+  RB_HIP(hipEventRecord(st->done, stream));
   return RAINBOW_OK;
 }
 
@@ -571,6 +688,7 @@ int Engine::set_tree_fusion(uint32_t hi) {
 int Engine::enqueue(bool integrate, bool sorting, uint32_t step_parity, bool deposit_first, bool rank_next,
                     bool record_counts) {
   if (sys.empty()) return RAINBOW_OK;
+  RB_HIP_RC(flush_descriptors());
   // The sorted bodies reach the body arrays only through integrate.
   sorting = sorting && integrate;
   StepArgs a = args;
@@ -588,6 +706,7 @@ int Engine::enqueue(bool integrate, bool sorting, uint32_t step_parity, bool dep
     RB_HIP(launch_deposit(f, n_fresh, stream));
   }
   n_fresh = 0u;
+  fresh_pending_.clear();
   RB_HIP(launch_upward(a, n_subtree, stream));
   // The top levels run in their own launch, or inside the first tree
   // launch; the sort's scatter follows whichever runs them.

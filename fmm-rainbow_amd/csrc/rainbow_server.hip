@@ -227,10 +227,11 @@ struct Pending {
   uint32_t steps;
   double submit_s;
   double deadline;  // 0: none
-  void* data;       // pinned, 32 n bytes
+  void* data;       // 32 n bytes: pinned from the pool, or the caller's (borrowed)
+  bool borrowed;
 };
 
-enum class SlotState { kResident, kDraining };
+enum class SlotState { kLoading, kResident, kDraining };
 
 struct Slot {
   uint64_t id;
@@ -242,12 +243,12 @@ struct Slot {
   double deadline;
   uint64_t bytes;    // counted against the budget
   void* block;       // arena storage
-  void* upload;      // pinned request data, returned once `loaded` has passed
+  void* upload;      // pinned request data from the pool, returned once `loaded` has passed
   void* result = nullptr;  // pinned, 32 n bytes (a completed request)
   hipEvent_t loaded = nullptr;  // copy stream: storage zeroed and bodies copied in
   hipEvent_t stepped = nullptr; // compute stream: the last step and the gather into caller order
   hipEvent_t done = nullptr;    // the last use of the storage (download, or last step)
-  SlotState state = SlotState::kResident;
+  SlotState state = SlotState::kLoading;
   int32_t status = RAINBOW_OK;
   SysDesc desc;
 };
@@ -279,11 +280,9 @@ struct rainbow_server {
   uint64_t budget = 0;
   uint64_t used = 0;
   uint32_t max_slots = 0;
-  bool dirty = false;
   rainbow::Arena* arena = nullptr;
   std::deque<rainbow::Pending> queue;
   std::vector<rainbow::Slot> slots;  // resident and draining
-  std::vector<uint32_t> fresh;       // indices into the next install's set
   std::deque<rainbow_completion_t> done;
   rainbow::PinnedPool pinned;
   rainbow::EventPool events;
@@ -311,7 +310,8 @@ struct rainbow_server {
       events.put(f.stop, true);
     }
     events.destroy_timed();
-    for (auto& p : queue) pinned.put(p.data);
+    for (auto& p : queue)
+      if (!p.borrowed) pinned.put(p.data);
     for (auto& r : results) pinned.put(r.second.data);
     delete arena;
     if (copy) (void)hipStreamDestroy(copy);
@@ -392,11 +392,13 @@ struct rainbow_server {
   // A resident slot stops (cancelled or timed out) after the steps already
   // enqueued: its storage is free once they have run.
   int stop(rainbow::Slot& s, int32_t status) {
+    const bool loading = s.state == rainbow::SlotState::kLoading;
     s.status = status;
     s.state = rainbow::SlotState::kDraining;
     RB_HIP(events.get(&s.done));
-    RB_HIP(hipEventRecord(s.done, engine.stream));
-    dirty = true;
+    // A loading system never ran: its storage is free after its upload.
+    RB_HIP(hipEventRecord(s.done, loading ? copy : engine.stream));
+    if (!loading) return engine.retire_system(s.desc.pos);
     return RAINBOW_OK;
   }
 
@@ -419,12 +421,12 @@ struct rainbow_server {
       c.status = cancelled ? RAINBOW_E_CANCELLED : RAINBOW_E_TIMEOUT;
       c.steps_run = 0u;
       done.push_back(c);
-      pinned.put(it->data);
+      if (!it->borrowed) pinned.put(it->data);
       ids.erase(it->id);
       it = queue.erase(it);
     }
     for (rainbow::Slot& s : slots) {
-      if (s.state != rainbow::SlotState::kResident) continue;
+      if (s.state == rainbow::SlotState::kDraining) continue;
       const bool cancelled = cancel_wanted.erase(s.id) > 0u;
       const bool late = s.deadline > 0.0 && t >= s.deadline;
       if (cancelled || late) {
@@ -442,7 +444,7 @@ struct rainbow_server {
   int admit() {
     uint32_t occupied = 0;
     for (const auto& s : slots)
-      if (s.state == rainbow::SlotState::kResident) ++occupied;
+      if (s.state != rainbow::SlotState::kDraining) ++occupied;
     while (!queue.empty() && occupied < max_slots) {
       rainbow::Pending& p = queue.front();
       const uint64_t need = rainbow::request_bytes(p.n);
@@ -459,14 +461,12 @@ struct rainbow_server {
       s.deadline = p.deadline;
       s.bytes = need;
       s.block = block;
-      s.upload = p.data;
+      s.upload = p.borrowed ? nullptr : p.data;
       s.desc = rainbow::describe_system(block, p.n, physics, 0.0f);
       hipError_t e = events.get(&s.loaded);
       if (e == hipSuccess) e = hipMemsetAsync(block, 0, need, copy);
       if (e == hipSuccess) e = hipMemcpyAsync(s.desc.tpos, p.data, 32ull * p.n, hipMemcpyHostToDevice, copy);
       if (e == hipSuccess) e = hipEventRecord(s.loaded, copy);
-      if (e == hipSuccess) e = hipStreamWaitEvent(engine.stream, s.loaded, 0);
-      if (e == hipSuccess) e = rainbow::launch_split_particles(s.desc, s.desc.tpos, engine.stream);
       if (e != hipSuccess) {
         arena->release(block);
         events.put(s.loaded);
@@ -476,39 +476,50 @@ struct rainbow_server {
       used += need;
       queue.pop_front();
       slots.push_back(s);
-      fresh_ids.push_back(s.id);
       ++occupied;
-      dirty = true;
-      // Requests of zero steps are complete once their bodies are in.
-      if (s.steps == 0u) {
-        const int rc = finish(slots.back());
-        if (rc) return rc;
-      }
     }
     return RAINBOW_OK;
   }
 
-  std::vector<uint64_t> fresh_ids;  // admitted since the last install
-
-  // The set of resident systems changed: install it before the next step.
-  int reconfigure() {
-    std::vector<rainbow::SysDesc> descs;
-    std::vector<uint32_t> fr;
-    descs.reserve(slots.size());
-    for (const auto& s : slots) {
-      if (s.state != rainbow::SlotState::kResident) continue;
-      if (std::find(fresh_ids.begin(), fresh_ids.end(), s.id) != fresh_ids.end())
-        fr.push_back(static_cast<uint32_t>(descs.size()));
-      descs.push_back(s.desc);
+  // Loaded systems join the set at this step boundary: the compute stream
+  // splits their bodies into the system's streams (the copy has already
+  // finished, so no step waits for an upload still in flight), and the next
+  // step deposits them.
+  int join() {
+    std::vector<rainbow::SysDesc> add;
+    for (rainbow::Slot& s : slots) {
+      if (s.state != rainbow::SlotState::kLoading) continue;
+      const hipError_t q = hipEventQuery(s.loaded);
+      if (q == hipErrorNotReady) continue;
+      if (q != hipSuccess) return fail(RAINBOW_E_HIP, "upload event: %s", hipGetErrorString(q));
+      RB_HIP(rainbow::launch_split_particles(s.desc, s.desc.tpos, engine.stream));
+      // Requests of zero steps are complete once their bodies are in.
+      if (s.steps == 0u) {
+        const int rc = finish(s, false);
+        if (rc) return rc;
+        continue;
+      }
+      s.state = rainbow::SlotState::kResident;
+      add.push_back(s.desc);
     }
-    fresh_ids.clear();
-    dirty = false;
-    return engine.configure_async(descs, fr);
+    return add.empty() ? RAINBOW_OK : engine.append_systems(add);
+  }
+
+  // Systems that left stay in the step's tables as dead entries (skipped by
+  // every kernel); when they cover more than half of the integrate ranges,
+  // the tables are rebuilt from the resident systems alone (the engine
+  // carries over the ones awaiting their first deposit).
+  int compact() {
+    if (engine.dead_blocks() * 2u <= engine.n_integrate) return RAINBOW_OK;
+    std::vector<rainbow::SysDesc> descs;
+    for (const auto& s : slots)
+      if (s.state == rainbow::SlotState::kResident) descs.push_back(s.desc);
+    return engine.configure_async(descs, std::vector<uint32_t>());
   }
 
   // After a slot's last step: gather its bodies into caller order behind
   // that step, copy them out on the copy stream.
-  int finish(rainbow::Slot& s) {
+  int finish(rainbow::Slot& s, bool in_set = true) {
     s.result = pinned.get(32ull * s.n);
     if (s.result == nullptr) return fail(RAINBOW_E_NOMEM, "no pinned host memory for a %u-body result", s.n);
     RB_HIP(events.get(&s.stepped));
@@ -520,8 +531,7 @@ struct rainbow_server {
     if (e == hipSuccess) e = hipEventRecord(s.done, copy);
     if (e != hipSuccess) return fail(RAINBOW_E_HIP, "result copy: %s", hipGetErrorString(e));
     s.state = rainbow::SlotState::kDraining;
-    dirty = true;
-    return RAINBOW_OK;
+    return in_set ? engine.retire_system(s.desc.pos) : RAINBOW_OK;
   }
 
   // One step of every resident system, timed by an event pair.
@@ -595,6 +605,8 @@ int rainbow_server_submit(rainbow_server_t* s, const rainbow_request_t* r) {
   if (s == nullptr || r == nullptr) return fail(RAINBOW_E_INVALID, "null argument");
   if (r->particles == 0u || r->data == nullptr) return fail(RAINBOW_E_INVALID, "empty request");
   if (!(r->timeout_s >= 0.0)) return fail(RAINBOW_E_INVALID, "timeout must be >= 0");
+  if (r->flags & ~static_cast<uint32_t>(RAINBOW_REQUEST_BORROW))
+    return fail(RAINBOW_E_INVALID, "unknown request flags 0x%x", r->flags);
   const uint64_t need = rainbow::request_bytes(r->particles);
   if (need > s->budget)
     return fail(RAINBOW_E_BUDGET, "request of %u bodies needs %llu bytes, budget is %llu", r->particles,
@@ -607,9 +619,14 @@ int rainbow_server_submit(rainbow_server_t* s, const rainbow_request_t* r) {
   p.steps = r->steps;
   p.submit_s = rainbow::now_s();
   p.deadline = r->timeout_s > 0.0 ? p.submit_s + r->timeout_s : 0.0;
-  p.data = s->pinned.get(32ull * r->particles);
-  if (p.data == nullptr) return fail(RAINBOW_E_NOMEM, "no pinned host memory for the request");
-  std::memcpy(p.data, r->data, 32ull * r->particles);
+  p.borrowed = (r->flags & RAINBOW_REQUEST_BORROW) != 0u;
+  if (p.borrowed) {
+    p.data = const_cast<void*>(r->data);
+  } else {
+    p.data = s->pinned.get(32ull * r->particles);
+    if (p.data == nullptr) return fail(RAINBOW_E_NOMEM, "no pinned host memory for the request");
+    std::memcpy(p.data, r->data, 32ull * r->particles);
+  }
   s->queue.push_back(p);
   s->ids.insert(r->id);
   return RAINBOW_OK;
@@ -619,7 +636,7 @@ int rainbow_server_cancel(rainbow_server_t* s, uint64_t id) {
   if (s == nullptr) return fail(RAINBOW_E_INVALID, "null handle");
   bool live = false;
   for (const auto& p : s->queue) live = live || p.id == id;
-  for (const auto& sl : s->slots) live = live || (sl.id == id && sl.state == rainbow::SlotState::kResident);
+  for (const auto& sl : s->slots) live = live || (sl.id == id && sl.state != rainbow::SlotState::kDraining);
   if (!live) return fail(RAINBOW_E_STATE, "request %llu is not queued or running", static_cast<unsigned long long>(id));
   s->cancel_wanted.insert(id);
   return RAINBOW_OK;
@@ -639,7 +656,7 @@ int rainbow_server_poll(rainbow_server_t* s, uint32_t* stepped) {
     if ((rc = s->admit()) != 0) return rc;
     t0 = rainbow::now_s();
     s->stats[0] += t0 - t1;
-    if (s->dirty && (rc = s->reconfigure()) != 0) return rc;
+    if ((rc = s->join()) != 0 || (rc = s->compact()) != 0) return rc;
     t1 = rainbow::now_s();
     s->stats[1] += t1 - t0;
     bool any = false;
@@ -710,6 +727,38 @@ int rainbow_server_result(rainbow_server_t* s, uint64_t id, void* dst, size_t by
   s->results.erase(it);
   s->ids.erase(id);
   return RAINBOW_OK;
+}
+
+int rainbow_server_result_view(rainbow_server_t* s, uint64_t id, const void** data, size_t* bytes) {
+  if (s == nullptr || data == nullptr || bytes == nullptr) return fail(RAINBOW_E_INVALID, "null argument");
+  auto it = s->results.find(id);
+  if (it == s->results.end()) return fail(RAINBOW_E_STATE, "no result for request %llu", static_cast<unsigned long long>(id));
+  *data = it->second.data;
+  *bytes = it->second.bytes;
+  return RAINBOW_OK;
+}
+
+int rainbow_server_release(rainbow_server_t* s, uint64_t id) {
+  if (s == nullptr) return fail(RAINBOW_E_INVALID, "null handle");
+  auto it = s->results.find(id);
+  if (it == s->results.end()) return fail(RAINBOW_E_STATE, "no result for request %llu", static_cast<unsigned long long>(id));
+  s->pinned.put(it->second.data);
+  s->results.erase(it);
+  s->ids.erase(id);
+  return RAINBOW_OK;
+}
+
+void* rainbow_host_alloc(size_t bytes) {
+  void* p = nullptr;
+  if (bytes == 0u || hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  return p;
+}
+
+void rainbow_host_free(void* p) {
+  if (p != nullptr) (void)hipHostFree(p);
 }
 
 int rainbow_server_state(rainbow_server_t* s, uint32_t* queued, uint32_t* resident, uint64_t* bytes) {

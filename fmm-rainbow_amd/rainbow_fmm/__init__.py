@@ -30,6 +30,7 @@ HEADER_PATH = PACKAGE_DIR.parent / "include" / "rainbow_fmm.h"
 
 PARTICLES, MOMENTS, FIELDS = 0, 1, 2
 OPT_COMPENSATED = 1
+REQUEST_BORROW = 1
 OK, E_INVALID, E_HIP, E_NOMEM, E_BUDGET, E_STATE, E_CANCELLED, E_TIMEOUT = 0, -1, -2, -3, -4, -5, -6, -7
 
 
@@ -78,6 +79,8 @@ class Request(ctypes.Structure):
         ("steps", ctypes.c_uint32),
         ("data", ctypes.c_void_p),
         ("timeout_s", ctypes.c_double),
+        ("flags", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 
@@ -168,6 +171,10 @@ _SIGNATURES = {
     "rainbow_server_submit": (_i32, [_vp, _P(Request)]),
     "rainbow_server_poll": (_i32, [_vp, _P(_u32)]),
     "rainbow_server_cancel": (_i32, [_vp, _u64]),
+    "rainbow_server_result_view": (_i32, [_vp, _u64, _P(_vp), _P(_sz)]),
+    "rainbow_server_release": (_i32, [_vp, _u64]),
+    "rainbow_host_alloc": (_vp, [_sz]),
+    "rainbow_host_free": (None, [_vp]),
     "rainbow_server_drain": (_i32, [_vp]),
     "rainbow_server_stats": (_i32, [_vp, _P(_f64)]),
     "rainbow_server_completions": (_i32, [_vp, _P(Completion), _u32, _P(_u32)]),
@@ -604,10 +611,21 @@ class Server:
     def request_bytes(n: int) -> int:
         return lib().rainbow_server_request_bytes(n)
 
-    def submit(self, rid: int, particles: np.ndarray, steps: int, timeout_s: float = 0.0):
-        a = np.ascontiguousarray(particles, dtype=np.float32)
-        r = Request(rid, a.shape[0], steps, _ptr(a), timeout_s)
+    def submit(self, rid: int, particles: np.ndarray, steps: int, timeout_s: float = 0.0, borrow: bool = False):
+        """Queues a request.  borrow=True: the server reads `particles` in
+        place (no copy); keep the array alive and unchanged until the
+        request's completion has been popped (pinned arrays from
+        host_array() upload by DMA)."""
+        if borrow:
+            if particles.dtype != np.float32 or not particles.flags["C_CONTIGUOUS"]:
+                raise ValueError("a borrowed request needs a C-contiguous float32 array")
+            a = particles
+        else:
+            a = np.ascontiguousarray(particles, dtype=np.float32)
+        r = Request(rid, a.shape[0], steps, _ptr(a), timeout_s, REQUEST_BORROW if borrow else 0, 0)
         _check(lib().rainbow_server_submit(self._h, ctypes.byref(r)))
+        if borrow:
+            self._keep[rid] = a
 
     def cancel(self, rid: int):
         _check(lib().rainbow_server_cancel(self._h, rid))
@@ -625,12 +643,26 @@ class Server:
         buf = (Completion * cap)()
         n = _u32()
         _check(lib().rainbow_server_completions(self._h, buf, cap, ctypes.byref(n)))
-        return [buf[i] for i in range(n.value)]
+        out = [buf[i] for i in range(n.value)]
+        for c in out:
+            self._keep.pop(c.id, None)  # a borrowed request's array is free again
+        return out
 
     def result(self, rid: int, n: int) -> np.ndarray:
         out = np.empty((n, 8), dtype=np.float32)
         _check(lib().rainbow_server_result(self._h, rid, _ptr(out), out.nbytes))
         return out
+
+    def result_view(self, rid: int) -> np.ndarray:
+        """The result in the server's pinned buffer, without a copy; valid
+        until release(rid)."""
+        p, b = _vp(), _sz()
+        _check(lib().rainbow_server_result_view(self._h, rid, ctypes.byref(p), ctypes.byref(b)))
+        buf = (ctypes.c_float * (b.value // 4)).from_address(p.value)
+        return np.frombuffer(buf, dtype=np.float32).reshape(-1, 8)
+
+    def release(self, rid: int):
+        _check(lib().rainbow_server_release(self._h, rid))
 
     def stats(self) -> dict:
         """Cumulative host seconds per scheduling activity, steps and polls."""
@@ -643,6 +675,37 @@ class Server:
         q, r, b = _u32(), _u32(), _u64()
         _check(lib().rainbow_server_state(self._h, ctypes.byref(q), ctypes.byref(r), ctypes.byref(b)))
         return q.value, r.value, b.value
+
+
+class HostArray(np.ndarray):
+    """A float32[N, 8] particle array in page-locked host memory from
+    rainbow_host_alloc, freed with the array."""
+
+    def __array_finalize__(self, obj):
+        self._owner = getattr(obj, "_owner", None)
+
+
+class _Pinned:
+    def __init__(self, nbytes):
+        self.ptr = lib().rainbow_host_alloc(nbytes)
+        if not self.ptr:
+            raise RainbowError(E_NOMEM, f"no pinned host memory for {nbytes} bytes")
+        self.nbytes = nbytes
+
+    def __del__(self):
+        if self.ptr:
+            lib().rainbow_host_free(self.ptr)
+            self.ptr = None
+
+
+def host_array(n: int) -> np.ndarray:
+    """An uninitialised float32[n, 8] particle array in pinned host memory
+    (for zero-copy server requests)."""
+    owner = _Pinned(32 * n)
+    buf = (ctypes.c_float * (8 * n)).from_address(owner.ptr)
+    a = np.frombuffer(buf, dtype=np.float32).reshape(n, 8).view(HostArray)
+    a._owner = owner
+    return a
 
 
 def gpu_available() -> bool:

@@ -399,7 +399,20 @@ typedef struct rainbow_request {
   uint32_t steps;
   const void* data;
   double timeout_s;
+  uint32_t flags;  /* RAINBOW_REQUEST_BORROW */
+  uint32_t reserved;
 } rainbow_request_t;
+
+/* Request flag: the server reads `data` in place instead of copying it at
+ * submit; the caller keeps it valid and unchanged until the request's
+ * completion record has been popped.  With pinned memory from
+ * rainbow_host_alloc the upload is a DMA straight from the caller's buffer. */
+enum { RAINBOW_REQUEST_BORROW = 1u };
+
+/* Page-locked host memory for zero-copy requests (NULL on failure), and its
+ * release. */
+void* rainbow_host_alloc(size_t bytes);
+void rainbow_host_free(void* p);
 
 /* Completion record.  status RAINBOW_OK: the result is ready for
  * rainbow_server_result; RAINBOW_E_CANCELLED / RAINBOW_E_TIMEOUT: the
@@ -457,6 +470,11 @@ int rainbow_server_completions(rainbow_server_t* s, rainbow_completion_t* out,
 /* Copies a completed request's final particles out (32 N bytes), once. */
 int rainbow_server_result(rainbow_server_t* s, uint64_t id, void* dst,
                           size_t bytes);
+/* Zero-copy form: the completed result's pinned buffer (32 N bytes, the
+ * reference layout), valid until rainbow_server_release(s, id), which ends
+ * the request (as rainbow_server_result does). */
+int rainbow_server_result_view(rainbow_server_t* s, uint64_t id, const void** data, size_t* bytes);
+int rainbow_server_release(rainbow_server_t* s, uint64_t id);
 /* Requests queued, resident (running or finishing), device bytes resident. */
 int rainbow_server_state(rainbow_server_t* s, uint32_t* queued,
                          uint32_t* resident, uint64_t* bytes);

@@ -361,15 +361,20 @@ def test_server_steady_arrivals_reuse_the_arena():
     prm = rf.physics(**CFG)
     rng = np.random.default_rng(31)
     reqs = [(j, int(rng.integers(1000, 40000)), int(rng.integers(1, 12))) for j in range(40)]
-    data = {j: rf.ic_uniform_ball(n, 300 + j, 1.0) for j, n, _ in reqs}
+    data = {}
+    for j, n, _ in reqs:
+        # Odd requests are borrowed from pinned host arrays (no copy at submit).
+        data[j] = rf.host_array(n) if j % 2 else np.empty((n, 8), np.float32)
+        data[j][:] = rf.ic_uniform_ball(n, 300 + j, 1.0)
     budget = 5 * rf.Server.request_bytes(40000)
     srv = rf.Server(budget, 64, prm)
     for j, n, steps in reqs[:10]:
-        srv.submit(j, data[j], steps)
+        srv.submit(j, data[j], steps, borrow=bool(j % 2))
     nxt = 10
     while nxt < len(reqs):
         srv.poll()
-        srv.submit(reqs[nxt][0], data[reqs[nxt][0]], reqs[nxt][2])
+        j, n, steps = reqs[nxt]
+        srv.submit(j, data[j], steps, borrow=bool(j % 2))
         nxt += 1
     srv.drain()
     done = {c.id: c for c in srv.completions()}
@@ -377,5 +382,10 @@ def test_server_steady_arrivals_reuse_the_arena():
     for j, n, steps in reqs:
         assert done[j].status == rf.OK and done[j].steps_run == steps
         op, _, _ = orc.step(slots_for(n, **CFG), data[j], steps)
-        assert_bits(srv.result(j, n), op, f"request {j}")
+        if j % 3 == 0:
+            assert_bits(srv.result_view(j), op, f"request {j} (result view)")
+            srv.release(j)
+        else:
+            assert_bits(srv.result(j, n), op, f"request {j}")
+    assert srv.state() == (0, 0, 0)
     srv.close()
