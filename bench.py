@@ -212,6 +212,10 @@ def batch_leg(args):
     b.sync()
     ms_batch = b.time_steps(steps)
     b.close()
+    batch_rate = total * steps / (ms_batch / 1e3)
+    if args.batch_no_sequential:
+        return {"workload": f"{len(counts)} Plummer systems, {total} bodies; {steps} timed steps (batched leg only)",
+                "value": batch_rate, "unit": "particle-steps/s", "ms_per_step": ms_batch / steps}
     solvers = []
     for i, n in enumerate(counts):
         s = rf.Solver(n, prm)
@@ -228,7 +232,6 @@ def batch_leg(args):
     seq_wall = time.perf_counter() - t0
     for s in solvers:
         s.close()
-    batch_rate = total * steps / (ms_batch / 1e3)
     return {
         "workload": f"{len(counts)} Plummer systems a=0.1, N log-uniform in [4096, 262144] (seed 4, system i "
                     f"seeded 4000+i), {total} bodies, dt 5e-4, softening 1e-3; {steps} timed steps",
@@ -470,6 +473,7 @@ def main():
     ap.add_argument("--precision-n", type=int, default=5242880)
     ap.add_argument("--systems", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--batch-no-sequential", action="store_true", help="time the batched leg only (profiling)")
     ap.add_argument("--requests", type=int, default=256, help="requests of the saturated serving run")
     ap.add_argument("--serve-pool", type=int, default=32, help="distinct seeded request inputs")
     ap.add_argument("--arrival-s", type=float, default=10.0, help="seconds of Poisson arrivals")
