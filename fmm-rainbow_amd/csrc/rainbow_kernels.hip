@@ -132,13 +132,8 @@ __device__ __forceinline__ uint32_t cell1(float u, uint32_t dim) {
   return static_cast<uint32_t>(floorf(u * static_cast<float>(dim)));
 }
 
-// WGSL i32(f32): truncation, saturated at the i32 range.
-__device__ __forceinline__ int32_t to_i32_sat(float x) {
-  if (!(x == x)) return 0;
-  if (x >= 2147483648.0f) return 2147483647;
-  if (x < -2147483648.0f) return static_cast<int32_t>(0x80000000u);
-  return static_cast<int32_t>(x);
-}
+// WGSL i32(f32) (truncation, saturated at the i32 range) is to_i32_sat of
+// rainbow_math.h.
 
 __device__ __forceinline__ float clampf(float x, float lo, float hi) {
   return fminf(fmaxf(x, lo), hi);
@@ -1867,13 +1862,26 @@ __global__ __launch_bounds__(kIntegrateThreads) void k_diag_partial(StepArgs a, 
   if (threadIdx.x < 8u) partial[8u * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
 }
 
-__global__ __launch_bounds__(64) void k_diag_final(const double* partial, uint32_t nblocks,
-                                                   double* out) {
-  if (threadIdx.x < 8u) {
-    double s = 0.0;
-    for (uint32_t b = 0; b < nblocks; ++b) s = s + partial[8u * b + threadIdx.x];
-    out[threadIdx.x] = s;
+// The block partials summed by one block: each thread sums a strided share
+// of the blocks for all eight quantities, then the threads' sums are added
+// pairwise in LDS.
+__global__ __launch_bounds__(256) void k_diag_final(const double* partial, uint32_t nblocks,
+                                                    double* out) {
+  __shared__ double red[8][256];
+  double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint32_t b = threadIdx.x; b < nblocks; b += 256u)
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s[k] = s[k] + partial[8u * b + k];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) red[k][threadIdx.x] = s[k];
+  __syncthreads();
+  for (uint32_t h = 128u; h > 0u; h >>= 1) {
+    if (threadIdx.x < h)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) red[k][threadIdx.x] = red[k][threadIdx.x] + red[k][threadIdx.x + h];
+    __syncthreads();
   }
+  if (threadIdx.x < 8u) out[threadIdx.x] = red[threadIdx.x][0];
 }
 
 // ------------------------------------------------------------------------
@@ -2307,7 +2315,7 @@ hipError_t launch_diag(const StepArgs& a, uint32_t nblocks, uint32_t sys, double
                        sys, partial);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, stream, partial, nblocks, out);
+  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, stream, partial, nblocks, out);
   return hipGetLastError();
 }
 

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <cstdint>
 
 namespace rainbow {
 
@@ -68,6 +69,25 @@ __device__ __forceinline__ float rsqrt_rn(float x) {
     if (!ok) q = rsqrt_slow(x);
   }
   return q;
+}
+
+// WGSL i32(f32) as the reference evaluates it for the fixed-point sums:
+// truncation toward zero, saturated at the i32 range, NaN to 0.
+__device__ __forceinline__ int32_t to_i32_sat_ref(float x) {
+  if (!(x == x)) return 0;
+  if (x >= 2147483648.0f) return 2147483647;
+  if (x < -2147483648.0f) return static_cast<int32_t>(0x80000000u);
+  return static_cast<int32_t>(x);
+}
+
+// The same in one instruction: the hardware conversion truncates, clamps
+// out-of-range values to the i32 range and turns NaN into 0, so it equals
+// to_i32_sat_ref for every float (bin/math_check compares them over all
+// 2^32 inputs), where the C++ cast's range checks cost a branchy sequence.
+__device__ __forceinline__ int32_t to_i32_sat(float x) {
+  int32_t r;
+  asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
 }
 
 // Unguarded forms, for the checker (exact only inside the domain).

@@ -1,6 +1,8 @@
 """The kernels' correctly rounded sqrt and 1/sqrt forms (Newton steps from the
 hardware estimates, csrc/rainbow_math.h) checked on the device against the
-compiler's sqrtf and 1.0f / sqrtf over all 2^32 float inputs
+compiler's sqrtf and 1.0f / sqrtf over all 2^32 float inputs, and the
+one-instruction saturating f32 -> i32 conversion against its reference form
+(truncation, i32 range clamp, NaN to 0) over the same inputs
 (bin/math_check)."""
 
 import subprocess
@@ -18,3 +20,4 @@ def test_root_forms_exhaustive():
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "inputs 4294967296" in r.stdout
+    assert "i32 conversion, all inputs: 0 mismatches" in r.stdout
