@@ -11,7 +11,9 @@
 //   active  uint8[C]    bit 0: cell was an M2L target in the last solve;
 //                       finest cells, bit 1: non-empty in the last solve
 //                       (only then are that solve's finest buffer cell and
-//                       the cell's source record non-zero)
+//                       the cell's source record non-zero); the finest
+//                       level's flags are stored subtree-major (the 8^3
+//                       cells under each level-(D-3) cell together)
 //   extra   int4[8^D]   finest-level moments of bodies deposited by a block
 //                       that does not own their cell (merged and zeroed by
 //                       upward)
@@ -20,6 +22,7 @@
 //                       and this one (parity 1); a step's upward zeroes the
 //                       buffer the next step deposits into
 //   xflag   uint8[8^D]  the deposit added stragglers of the cell into `extra`
+//                       (subtree-major, as the finest flags of `active`)
 //   m2l     float4[3 level_offset(D)]  M2L sums of the cells above the
 //                       finest level, written by the tree pass's first
 //                       launch and read by its second

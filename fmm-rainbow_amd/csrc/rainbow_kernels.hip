@@ -81,6 +81,17 @@ __device__ __forceinline__ uint32_t lin(uint32_t x, uint32_t y, uint32_t z, uint
 }
 
 // Inverse of lin for dim = 2^lg (shifts, no integer division).
+// Byte flags of the finest level (the target/non-empty flags in `active`
+// and `xflag`) are stored subtree-major: the 8^3 cells under each level-(D-3)
+// cell contiguously, x fastest inside.  Upward's waves (one thread per cell
+// of a subtree) then read 64 contiguous bytes, not eight 8-byte pieces of
+// rows.
+__device__ __forceinline__ uint32_t flag_index(uint32_t x, uint32_t y, uint32_t z, uint32_t depth) {
+  const uint32_t rd = depth - 3u;
+  const uint32_t root = (x >> 3) + ((y >> 3) << rd) + ((z >> 3) << (2u * rd));
+  return (root << 9) | (x & 7u) | ((y & 7u) << 3) | ((z & 7u) << 6);
+}
+
 __device__ __forceinline__ void unlin(uint32_t i, uint32_t lg, uint32_t& x, uint32_t& y, uint32_t& z) {
   const uint32_t m = (1u << lg) - 1u;
   x = i & m;
@@ -298,7 +309,7 @@ __device__ void deposit_block(const StepArgs& a, const SysDesc& d, uint32_t firs
     const unsigned long long packed = static_cast<unsigned long long>(static_cast<uint32_t>(sums.x)) |
                                       (static_cast<unsigned long long>(static_cast<uint32_t>(sums.y)) << 32);
     unsigned long long* const ce = reinterpret_cast<unsigned long long*>(G(d.extra) + key);
-    G(d.xflag)[key] = 1u;
+    G(d.xflag)[flag_index(x, y, z, d.depth)] = 1u;
     if (ranks) {
       const unsigned long long old = atomicAdd(ce, packed);
       keys[s] = kRankExtra | static_cast<uint32_t>(old);
@@ -494,8 +505,10 @@ __device__ __forceinline__ void update_active(const SysDesc& d, uint32_t cell, i
 
 // Finest cells: bit 0 as update_active, bit 1 = non-empty now (the next
 // step's upward reads it to skip the cells empty in both solves).
-__device__ __forceinline__ void update_finest_flags(const SysDesc& d, uint32_t cell, int count, int parent_count,
-                                                    uint8_t prev) {
+// `cell`: the finest cell's global index (its field); `flag`: its flag's
+// place in `active` (subtree-major, see flag_index).
+__device__ __forceinline__ void update_finest_flags(const SysDesc& d, uint32_t cell, uint32_t flag, int count,
+                                                    int parent_count, uint8_t prev) {
   const uint8_t target = (count != 0 && parent_count > kLeafCapacity) ? 1u : 0u;
   const uint8_t now = target | (count != 0 ? 2u : 0u);
   if (prev != now) {
@@ -505,7 +518,7 @@ __device__ __forceinline__ void update_finest_flags(const SysDesc& d, uint32_t c
       G(d.fields)[3u * cell + 1u] = z;
       G(d.fields)[3u * cell + 2u] = z;
     }
-    G(d.active)[cell] = now;
+    G(d.active)[flag] = now;
   }
 }
 
@@ -582,7 +595,8 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
   // nothing).
   const uint32_t ix = t & 7u, iy = (t >> 3) & 7u, iz = t >> 6;
   const uint32_t fcell = lin(8u * rx + ix, 8u * ry + iy, 8u * rz + iz, 1u << D);
-  const uint8_t prev_f = G(d.active)[level_off(D) + fcell];
+  const uint32_t fflag = (blk.root << 9) | t;  // flag_index of the cell
+  const uint8_t prev_f = G(d.active)[level_off(D) + fflag];
   uint8_t prev_1 = 0u, prev_2 = 0u;
   if (t < 64u)
     prev_1 = G(d.active)[level_off(D - 1u) + lin(4u * rx + (t & 3u), 4u * ry + ((t >> 2) & 3u), 4u * rz + (t >> 4), 1u << (D - 1u))];
@@ -597,7 +611,7 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
   const gptr<int4> next = finest_of(d, a.parity ^ 1u);
   int4 m0 = F[fcell];
   const uint32_t owned = static_cast<uint32_t>(m0.x);
-  if (G(d.xflag)[fcell] != 0u) {
+  if (G(d.xflag)[fflag] != 0u) {
     const int4 e = G(d.extra)[fcell];
     m0.x = static_cast<int>(static_cast<uint32_t>(m0.x) + static_cast<uint32_t>(e.x));
     m0.y = static_cast<int>(static_cast<uint32_t>(m0.y) + static_cast<uint32_t>(e.y));
@@ -605,7 +619,7 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
     m0.w = static_cast<int>(static_cast<uint32_t>(m0.w) + static_cast<uint32_t>(e.w));
     F[fcell] = m0;
     G(d.extra)[fcell] = make_int4(0, 0, 0, 0);
-    G(d.xflag)[fcell] = 0u;
+    G(d.xflag)[fflag] = 0u;
   }
   // The other buffer holds the last solve's counts: non-zero only where the
   // cell's flag says it was non-empty (bit 1), so only those are zeroed.
@@ -700,7 +714,8 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
     // A cell empty now and in the last solve keeps its zero record.
     const int4 m = lvl0[t];
     if (m.x != 0 || (prev_f & 2u)) G(d.source)[level_off(D) + fcell] = source_record(m, d);
-    update_finest_flags(d, level_off(D) + fcell, m.x, lvl1[lin(ix >> 1, iy >> 1, iz >> 1, 4u)].x, prev_f);
+    update_finest_flags(d, level_off(D) + fcell, level_off(D) + fflag, m.x, lvl1[lin(ix >> 1, iy >> 1, iz >> 1, 4u)].x,
+                        prev_f);
   }
   if (mine) {
     G(d.source)[lcell] = source_record(lm, d);
