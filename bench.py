@@ -293,9 +293,11 @@ def serve_leg(args):
     """The continuous-batching server on mixed-N requests (N log-uniform in
     [16384, 2097152], 1..200 steps, Plummer or uniform ball, 200 GB budget):
 
-    * saturated: every request submitted at once; aggregate particle-steps/s
-      over the wall time (uploads, steps and downloads included) and the
-      device's busy fraction (event time of the steps over the wall);
+    * saturated: every request submitted at once (four times the resident
+      limit by default, so the set stays full while the backlog lasts);
+      aggregate particle-steps/s over the wall time (uploads, steps and
+      downloads included, ramp and drain too) and the device's busy fraction
+      (event time of the steps over the wall);
     * sequential: the same requests one after another on plain solvers
       (upload, steps, download; solvers created and warmed beforehand), on
       the same host clock; the ratio is the served speedup;
@@ -312,7 +314,7 @@ def serve_leg(args):
     # Solo latency model per pool entry: upload + download (host clock) and
     # per-step GPU time (graph replays), from a warmed plain solver.
     solvers = []
-    for n, _, data in pool:
+    for n, _, data in ([] if args.serve_saturated_only else pool):
         s = rf.Solver(n, prm)
         s.upload(data)
         s.step(24)
@@ -334,6 +336,11 @@ def serve_leg(args):
     st_sat = srv.stats()
     srv.close()
     sat_rate = work / wall
+    if args.serve_saturated_only:
+        return {"workload": f"{len(sat)} requests at once from {len(pool)} seeded inputs (saturated leg only)",
+                "value": sat_rate, "unit": "particle-steps/s", "wall_s": wall,
+                "gpu_busy": st_sat["gpu_step_s"] / wall, "steps": st_sat["steps"],
+                "all_completed": all(c.status == 0 for c in done.values())}
 
     # Sequential: the same requests one after another on plain solvers.
     t0 = time.perf_counter()
@@ -472,10 +479,11 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="leave the serving form out of the plain line")
     ap.add_argument("--precision-n", type=int, default=5242880)
     ap.add_argument("--systems", type=int, default=256)
-    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--batch-steps", type=int, default=100, help="timed steps of the batched leg (the configured 100)")
     ap.add_argument("--batch-no-sequential", action="store_true", help="time the batched leg only (profiling)")
-    ap.add_argument("--requests", type=int, default=256, help="requests of the saturated serving run")
+    ap.add_argument("--requests", type=int, default=1024, help="requests of the saturated serving run")
     ap.add_argument("--serve-pool", type=int, default=32, help="distinct seeded request inputs")
+    ap.add_argument("--serve-saturated-only", action="store_true", help="time the saturated serving leg only (profiling)")
     ap.add_argument("--arrival-s", type=float, default=10.0, help="seconds of Poisson arrivals")
     ap.add_argument("--serve-nmax", type=int, default=2097152)
     ap.add_argument("--budget-gb", type=float, default=200.0)
