@@ -210,12 +210,18 @@ def batch_leg(args):
         b.upload(i, p)
     b.step(args.warmup)
     b.sync()
+    tuned = None
+    if args.batch_tune_steps > 0:
+        # Block shapes, tree split and sort interval picked by the autotuner
+        # (the bodies are restored afterwards).
+        interval = b.autotune(args.batch_tune_steps)
+        tuned = {"integrate_threads": b.shape[0], "tree_chunks": b.shape[1], "sort_interval": interval}
     ms_batch = b.time_steps(steps)
     b.close()
     batch_rate = total * steps / (ms_batch / 1e3)
     if args.batch_no_sequential:
         return {"workload": f"{len(counts)} Plummer systems, {total} bodies; {steps} timed steps (batched leg only)",
-                "value": batch_rate, "unit": "particle-steps/s", "ms_per_step": ms_batch / steps}
+                "value": batch_rate, "unit": "particle-steps/s", "ms_per_step": ms_batch / steps, "tuned": tuned}
     solvers = []
     for i, n in enumerate(counts):
         s = rf.Solver(n, prm)
@@ -242,6 +248,7 @@ def batch_leg(args):
         "sequential_wall_value": total * steps / seq_wall,
         "speedup": batch_rate / (total * steps / (seq_gpu_ms / 1e3)),
         "speedup_wall": batch_rate / (total * steps / seq_wall),
+        "tuned": tuned,
     }
 
 
@@ -481,6 +488,9 @@ def main():
     ap.add_argument("--systems", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=100, help="timed steps of the batched leg (the configured 100)")
     ap.add_argument("--batch-no-sequential", action="store_true", help="time the batched leg only (profiling)")
+    ap.add_argument("--batch-tune-steps", type=int, default=100,
+                    help="steps per candidate of the batch's autotuner, the horizon it tunes for (0: default "
+                         "shapes and measured split)")
     ap.add_argument("--requests", type=int, default=1024, help="requests of the saturated serving run")
     ap.add_argument("--serve-pool", type=int, default=32, help="distinct seeded request inputs")
     ap.add_argument("--serve-saturated-only", action="store_true", help="time the saturated serving leg only (profiling)")

@@ -202,15 +202,18 @@ struct DepositLds {
   uint32_t nused;
 };
 
-// Deposit of one block's bodies first + k * kDepositThreads + thread
-// (positions p[k]) into the finest buffer `parity` of system d; writes the
-// bodies' ranks when `ranks`.  All threads of the block call it.
+// Deposit of one block's bodies first + k * kThreads + thread (positions
+// p[k], k < kDepositBodies / kThreads) into the finest buffer `parity` of
+// system d; writes the bodies' ranks when `ranks`.  All threads of the
+// block call it.
+template <uint32_t kThreads>
 __device__ void deposit_block(const StepArgs& a, const SysDesc& d, uint32_t first,
-                              const float4 (&p)[kDepositPerThread], uint32_t parity, bool ranks,
+                              const float4 (&p)[kDepositBodies / kThreads], uint32_t parity, bool ranks,
                               DepositLds& L, const uint2* own_lds = nullptr) {
+  constexpr uint32_t kPer = kDepositBodies / kThreads;
   uint32_t* const keys = L.keys;
   int4* const vals = L.vals;
-  for (uint32_t s = threadIdx.x; s < kDepositSlots; s += kDepositThreads) {
+  for (uint32_t s = threadIdx.x; s < kDepositSlots; s += kThreads) {
     keys[s] = kEmptyKey;
     vals[s] = make_int4(0, 0, 0, 0);
   }
@@ -220,10 +223,10 @@ __device__ void deposit_block(const StepArgs& a, const SysDesc& d, uint32_t firs
   const uint32_t lane = threadIdx.x & 63u;
   // Per body of this thread: its hash slot and its index among the block's
   // bodies of that cell (used when the step sorts).
-  uint32_t my_slot[kDepositPerThread], my_rank[kDepositPerThread];
+  uint32_t my_slot[kPer], my_rank[kPer];
 #pragma unroll
-  for (uint32_t k = 0; k < kDepositPerThread; ++k) {
-    const uint32_t i = first + k * kDepositThreads + threadIdx.x;
+  for (uint32_t k = 0; k < kPer; ++k) {
+    const uint32_t i = first + k * kThreads + threadIdx.x;
     const bool valid = i < d.n;
     uint32_t key = kEmptyKey;
     int v[4] = {0, 0, 0, 0};
@@ -291,7 +294,7 @@ __device__ void deposit_block(const StepArgs& a, const SysDesc& d, uint32_t firs
   const uint2 own = own_lds != nullptr ? *own_lds : G(d.own)[first / kDepositBodies];
   const gptr<int4> finest = finest_of(d, parity);
   const uint32_t nused = L.nused;
-  for (uint32_t u = threadIdx.x; u < nused; u += kDepositThreads) {
+  for (uint32_t u = threadIdx.x; u < nused; u += kThreads) {
     const uint32_t s = L.used[u];
     const uint32_t key = keys[s];
     const int4 sums = vals[s];
@@ -322,8 +325,8 @@ __device__ void deposit_block(const StepArgs& a, const SysDesc& d, uint32_t firs
   if (!ranks) return;
   __syncthreads();
 #pragma unroll
-  for (uint32_t k = 0; k < kDepositPerThread; ++k) {
-    const uint32_t i = first + k * kDepositThreads + threadIdx.x;
+  for (uint32_t k = 0; k < kPer; ++k) {
+    const uint32_t i = first + k * kThreads + threadIdx.x;
     if (i < d.n) G(d.rank)[i] = keys[my_slot[k]] + my_rank[k];
   }
 }
@@ -343,13 +346,13 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(StepArgs a) {
   const BodyBlock blk = a.deposit_blocks[blockIdx.x];
   const SysDesc d = a.sys[blk.sys];
   if (!d.live) return;
-  float4 p[kDepositPerThread];
+  float4 p[kDepositBodies / kDepositThreads];
 #pragma unroll
-  for (uint32_t k = 0; k < kDepositPerThread; ++k) {
+  for (uint32_t k = 0; k < kDepositBodies / kDepositThreads; ++k) {
     const uint32_t i = blk.first + k * kDepositThreads + threadIdx.x;
     p[k] = i < d.n ? G(d.pos)[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
-  deposit_block(a, d, blk.first, p, a.parity, a.sorting != 0u, L);
+  deposit_block<kDepositThreads>(a, d, blk.first, p, a.parity, a.sorting != 0u, L);
 }
 
 // Morton code of the finest cell of a position.
@@ -999,10 +1002,10 @@ __device__ __forceinline__ bool stage_window_from_children(float4* win, const Sy
 // different LDS banks).
 constexpr uint32_t kAncRow = 220u;
 constexpr uint32_t kAncestorBytes = 216u * 16u + 10u * kAncRow * 4u;
-// Chunks of 8 x 8 pairs per round of the tree item loop (two per wave).
-constexpr uint32_t kTreeChunksPerRound = 8u;
-// LDS of one item: the window and a round's 10 x 64 chunk terms.
-constexpr uint32_t kItemBytes = 216u * 16u + kTreeChunksPerRound * 10u * 64u * 4u;
+// LDS of one item: the window and a round's 10 x 64 terms of each of its
+// kCpr chunks of 8 x 8 pairs (the tree kernel's pair tile: 8 chunks per
+// round, two per wave, or 4, one per wave).
+constexpr uint32_t item_bytes(uint32_t cpr) { return 216u * 16u + cpr * 10u * 64u * 4u; }
 
 // Fields of the internal ancestors A_lo .. A_{level-1} of an item of
 // `level` whose parent P = A_{level-1} has coordinates (px, py, pz), from
@@ -1120,24 +1123,25 @@ __device__ void ancestor_fields(const SysDesc& d, uint32_t lo, uint32_t level, u
 struct TreeBlockOffsets {
   uint64_t pack[kTreeThreads / 64u];
 };
+template <uint32_t kCpr>
 constexpr TreeBlockOffsets make_tree_block_offsets() {
   TreeBlockOffsets t{};
-  constexpr uint32_t waves = kTreeThreads / 64u, per_wave = kTreeChunksPerRound / waves;
+  constexpr uint32_t waves = kTreeThreads / 64u, per_wave = kCpr / waves;
   for (uint32_t w = 0; w < waves; ++w)
-    for (uint32_t r0 = 0; r0 < 32u; r0 += kTreeChunksPerRound)
+    for (uint32_t r0 = 0; r0 < 32u; r0 += kCpr)
       for (uint32_t j = 0; j < per_wave; ++j) {
         const uint32_t q = r0 + w * per_wave + j;
         if (q < 27u) {
           const uint64_t off = 2u * (q % 3u) + 12u * ((q / 3u) % 3u) + 72u * (q / 9u);
-          t.pack[w] |= off << (8u * (r0 / kTreeChunksPerRound * per_wave + j));
+          t.pack[w] |= off << (8u * (r0 / kCpr * per_wave + j));
         }
       }
   return t;
 }
-static_assert(32u / kTreeChunksPerRound * (kTreeChunksPerRound / (kTreeThreads / 64u)) <= 8u,
-              "a wave's block offsets fit in 64 bits");
+template <uint32_t kCpr>
 __device__ __forceinline__ uint64_t tree_block_offsets(uint32_t wave) {
-  constexpr TreeBlockOffsets kT = make_tree_block_offsets();
+  static_assert(32u / kCpr * (kCpr / (kTreeThreads / 64u)) <= 8u, "a wave's block offsets fit in 64 bits");
+  constexpr TreeBlockOffsets kT = make_tree_block_offsets<kCpr>();
   static_assert(kTreeThreads / 64u == 4u, "four waves per tree block");
   return wave == 0u ? kT.pack[0] : wave == 1u ? kT.pack[1] : wave == 2u ? kT.pack[2] : kT.pack[3];
 }
@@ -1210,19 +1214,29 @@ __device__ __forceinline__ float path_field(const PathSums& p, uint32_t level, u
   return f;
 }
 
-// Occupancy: 6 waves per SIMD for the unfused forms (their LDS allows 6
-// blocks per CU), 5 for fused launches, which carry the ancestor path.
-template <bool kComp, uint32_t kKind>
-__global__ __launch_bounds__(kTreeThreads, kKind == kTreeFused ? 5 : 6) void k_tree(StepArgs a, uint32_t lo,
+// kCpr: chunks of 8 x 8 pairs per round (the pair tile).  Occupancy: with
+// 8 chunks per round 6 waves per SIMD for the unfused forms (their LDS
+// allows 6 blocks per CU), 5 for fused launches, which carry the ancestor
+// path; with 4 (half the terms' LDS) 8 for the unfused forms.
+constexpr uint32_t tree_occupancy(uint32_t kind, uint32_t cpr) {
+  return kind == kTreeFused ? 5u : (cpr == 4u ? 8u : 6u);
+}
+template <bool kComp, uint32_t kKind, uint32_t kCpr>
+__global__ __launch_bounds__(kTreeThreads, tree_occupancy(kKind, kCpr)) void k_tree(StepArgs a, uint32_t lo,
                                                                                     uint32_t hi, uint32_t lead) {
+  static_assert(kCpr == 4u || kCpr == 8u, "4 or 8 chunks per round");
   constexpr bool kFused = kKind == kTreeFused;
   constexpr uint32_t kWaves = kTreeThreads / 64u;
-  constexpr uint32_t kChunksPerRound = kTreeChunksPerRound;
+  constexpr uint32_t kChunksPerRound = kCpr;
   // Single level: static LDS.  Fused: one dynamic region of
   // max(kItemBytes, (hi - lo) * kAncestorBytes) bytes, used by the
   // ancestor path and then by the item.
   __shared__ float4 win_s[kFused ? 1 : 216];
-  __shared__ __attribute__((aligned(16))) float terms_s[kFused ? 1 : kChunksPerRound * 10 * 64];
+  // The M2L form's block 0 may run the top levels in this array instead.
+  constexpr uint32_t kTermFloats = kChunksPerRound * 10u * 64u;
+  constexpr uint32_t kTopFloats = (static_cast<uint32_t>(sizeof(TopLds)) + 3u) / 4u;
+  __shared__ __attribute__((aligned(16))) float
+      terms_s[kFused ? 1u : (kKind == kTreeM2l && kTopFloats > kTermFloats ? kTopFloats : kTermFloats)];
   extern __shared__ float4 dyn[];
   __shared__ float pf[12];
   __shared__ uint32_t votes[kTreeThreads / 64u];
@@ -1314,7 +1328,7 @@ __global__ __launch_bounds__(kTreeThreads, kKind == kTreeFused ? 5 : 6) void k_t
   // j; the window offsets of their source-parent blocks come packed from a
   // table built at compile time (one byte per round and j).
   constexpr uint32_t kPerWave = kChunksPerRound / kWaves;
-  const uint64_t block_off = tree_block_offsets(wave_s);
+  const uint64_t block_off = tree_block_offsets<kCpr>(wave_s);
   // Chain threads 0..79: target tid / 10, component tid % 10.
   const bool chain = tid < 80u;
   const uint32_t ct = tid / 10u, ck = tid % 10u;
@@ -1780,8 +1794,11 @@ __device__ __forceinline__ void integrate_body(const SysDesc& d, uint32_t parity
 // system's first block starts at 0 and its last ends at 2^18.
 // kOne: the step holds one system, whose descriptor is then loaded with the
 // block's entry instead of after it (one dependent load less per block).
-template <bool kComp, bool kOne>
-__global__ __launch_bounds__(kDepositThreads) void k_integrate_deposit(StepArgs a) {
+// kThreads: the block's width over its 1024-body range (1024: one body per
+// thread; 512: two, the second 512 after the first).
+template <bool kComp, bool kOne, uint32_t kThreads>
+__global__ __launch_bounds__(kThreads) void k_integrate_deposit(StepArgs a) {
+  constexpr uint32_t kPer = kDepositBodies / kThreads;
   __shared__ DepositLds L;
   __shared__ uint2 own_new;
   reset_lists(a);
@@ -1790,10 +1807,10 @@ __global__ __launch_bounds__(kDepositThreads) void k_integrate_deposit(StepArgs 
   if (!d.live) return;
   const bool staged = a.sorting != 0u;
   const uint32_t dim = 1u << d.depth;
-  float4 p[kDepositPerThread];
+  float4 p[kPer];
 #pragma unroll
-  for (uint32_t k = 0; k < kDepositPerThread; ++k) {
-    const uint32_t i = blk.first + k * kDepositThreads + threadIdx.x;
+  for (uint32_t k = 0; k < kPer; ++k) {
+    const uint32_t i = blk.first + k * kThreads + threadIdx.x;
     p[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (i < d.n) {
       float4 p4 = staged ? G(d.tpos)[i] : G(d.pos)[i];
@@ -1821,7 +1838,7 @@ __global__ __launch_bounds__(kDepositThreads) void k_integrate_deposit(StepArgs 
       p[k] = p4;
     }
   }
-  deposit_block(a, d, blk.first, p, a.parity ^ 1u, a.rank_next != 0u, L, staged ? &own_new : nullptr);
+  deposit_block<kThreads>(a, d, blk.first, p, a.parity ^ 1u, a.rank_next != 0u, L, staged ? &own_new : nullptr);
 }
 
 // ------------------------------------------------------------------------
@@ -2231,10 +2248,10 @@ hipError_t launch_top(const StepArgs& a, hipStream_t stream, const Timing* t) {
 }
 
 // Blocks of a tree launch that fit on the device at once (occupancy x CUs),
-// cached per kernel form and level span (which sets the dynamic LDS size).
-// The tree kernel loops over its items, so a grid of this size keeps every
-// block busy and none is launched only to find no work.
-template <bool kComp, uint32_t kKind>
+// cached per kernel form, pair tile and level span (which sets the dynamic
+// LDS size).  The tree kernel loops over its items, so a grid of this size
+// keeps every block busy and none is launched only to find no work.
+template <bool kComp, uint32_t kKind, uint32_t kCpr>
 static uint32_t tree_resident_blocks(uint32_t span, uint32_t lds) {
   static uint32_t cache[kLevels] = {};
   const uint32_t key = span % kLevels;
@@ -2242,49 +2259,57 @@ static uint32_t tree_resident_blocks(uint32_t span, uint32_t lds) {
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tree<kComp, kKind>, kTreeThreads, lds) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tree<kComp, kKind, kCpr>, kTreeThreads, lds) !=
+          hipSuccess ||
       cus <= 0 || per_cu <= 0)
     return kTreeMaxBlocks;
   cache[key] = static_cast<uint32_t>(cus * per_cu);
   return cache[key];
 }
 
-uint32_t tree_single_level_capacity(bool compensated) {
-  return compensated ? tree_resident_blocks<true, kTreeSingle>(0u, 0u)
-                     : tree_resident_blocks<false, kTreeSingle>(0u, 0u);
+uint32_t tree_single_level_capacity(bool compensated, uint32_t chunks) {
+  if (chunks == 4u)
+    return compensated ? tree_resident_blocks<true, kTreeSingle, 4u>(0u, 0u)
+                       : tree_resident_blocks<false, kTreeSingle, 4u>(0u, 0u);
+  return compensated ? tree_resident_blocks<true, kTreeSingle, 8u>(0u, 0u)
+                     : tree_resident_blocks<false, kTreeSingle, 8u>(0u, 0u);
 }
 
-template <bool kComp, uint32_t kKind>
+template <bool kComp, uint32_t kKind, uint32_t kCpr>
 static hipError_t launch_tree_form(const StepArgs& a, uint32_t lo, uint32_t hi, uint32_t nblocks, uint32_t lead,
                                    uint32_t lds, hipStream_t stream, const Timing* t) {
-  nblocks = std::min(nblocks, tree_resident_blocks<kComp, kKind>(hi - lo, lds)) + lead;
+  nblocks = std::min(nblocks, tree_resident_blocks<kComp, kKind, kCpr>(hi - lo, lds)) + lead;
   if (lds != 0u)
-    RB_LAUNCH_LDS(t, (k_tree<kComp, kKind>), dim3(nblocks), dim3(kTreeThreads), lds, stream, a, lo, hi, lead);
+    RB_LAUNCH_LDS(t, (k_tree<kComp, kKind, kCpr>), dim3(nblocks), dim3(kTreeThreads), lds, stream, a, lo, hi, lead);
   else
-    RB_LAUNCH(t, (k_tree<kComp, kKind>), dim3(nblocks), dim3(kTreeThreads), stream, a, lo, hi, lead);
+    RB_LAUNCH(t, (k_tree<kComp, kKind, kCpr>), dim3(nblocks), dim3(kTreeThreads), stream, a, lo, hi, lead);
   return hipGetLastError();
 }
 
-template <bool kComp>
+template <bool kComp, uint32_t kCpr>
 static hipError_t launch_tree_comp(const StepArgs& a, uint32_t kind, uint32_t lo, uint32_t hi, uint32_t nblocks,
                                    uint32_t lead, hipStream_t stream, const Timing* t) {
   switch (kind) {
-    case kTreePassM2l: return launch_tree_form<kComp, kTreeM2l>(a, lo, hi, nblocks, lead, 0u, stream, t);
-    case kTreePassFinish: return launch_tree_form<kComp, kTreeFinish>(a, lo, hi, nblocks, 0u, 0u, stream, t);
+    case kTreePassM2l: return launch_tree_form<kComp, kTreeM2l, kCpr>(a, lo, hi, nblocks, lead, 0u, stream, t);
+    case kTreePassFinish: return launch_tree_form<kComp, kTreeFinish, kCpr>(a, lo, hi, nblocks, 0u, 0u, stream, t);
     default:
       if (lo < hi)
-        return launch_tree_form<kComp, kTreeFused>(a, lo, hi, nblocks, 0u,
-                                                   std::max(kItemBytes, (hi - lo) * kAncestorBytes), stream, t);
-      return launch_tree_form<kComp, kTreeSingle>(a, lo, hi, nblocks, 0u, 0u, stream, t);
+        return launch_tree_form<kComp, kTreeFused, kCpr>(
+            a, lo, hi, nblocks, 0u, std::max(item_bytes(kCpr), (hi - lo) * kAncestorBytes), stream, t);
+      return launch_tree_form<kComp, kTreeSingle, kCpr>(a, lo, hi, nblocks, 0u, 0u, stream, t);
   }
 }
 
 hipError_t launch_tree(const StepArgs& a, uint32_t kind, uint32_t lo, uint32_t hi, uint32_t nblocks, uint32_t lead,
-                       hipStream_t stream, const Timing* t) {
+                       uint32_t chunks, hipStream_t stream, const Timing* t) {
   if ((nblocks == 0u && lead == 0u) || lo < 2u || hi < lo || hi >= kLevels) return no_launch(stream, t);
   if (lead != 0u && (kind != kTreePassM2l || a.nsys != 1u)) return hipErrorInvalidValue;
-  return a.compensated ? launch_tree_comp<true>(a, kind, lo, hi, nblocks, lead, stream, t)
-                       : launch_tree_comp<false>(a, kind, lo, hi, nblocks, lead, stream, t);
+  if (chunks != 4u && chunks != 8u) return hipErrorInvalidValue;
+  if (chunks == 4u)
+    return a.compensated ? launch_tree_comp<true, 4u>(a, kind, lo, hi, nblocks, lead, stream, t)
+                         : launch_tree_comp<false, 4u>(a, kind, lo, hi, nblocks, lead, stream, t);
+  return a.compensated ? launch_tree_comp<true, 8u>(a, kind, lo, hi, nblocks, lead, stream, t)
+                       : launch_tree_comp<false, 8u>(a, kind, lo, hi, nblocks, lead, stream, t);
 }
 
 hipError_t launch_sort(const StepArgs& a, uint32_t nintegrate, hipStream_t stream) {
@@ -2293,20 +2318,28 @@ hipError_t launch_sort(const StepArgs& a, uint32_t nintegrate, hipStream_t strea
   return hipGetLastError();
 }
 
-hipError_t launch_integrate_deposit(const StepArgs& a, uint32_t ndeposit, hipStream_t stream, const Timing* t) {
-  if (ndeposit == 0u) return no_launch(stream, t);
-  const dim3 grid(ndeposit), block(kDepositThreads);
+template <uint32_t kThreads>
+static hipError_t launch_integrate_width(const StepArgs& a, uint32_t ndeposit, hipStream_t stream, const Timing* t) {
+  const dim3 grid(ndeposit), block(kThreads);
   if (a.compensated) {
     if (a.nsys == 1u)
-      RB_LAUNCH(t, (k_integrate_deposit<true, true>), grid, block, stream, a);
+      RB_LAUNCH(t, (k_integrate_deposit<true, true, kThreads>), grid, block, stream, a);
     else
-      RB_LAUNCH(t, (k_integrate_deposit<true, false>), grid, block, stream, a);
+      RB_LAUNCH(t, (k_integrate_deposit<true, false, kThreads>), grid, block, stream, a);
   } else if (a.nsys == 1u) {
-    RB_LAUNCH(t, (k_integrate_deposit<false, true>), grid, block, stream, a);
+    RB_LAUNCH(t, (k_integrate_deposit<false, true, kThreads>), grid, block, stream, a);
   } else {
-    RB_LAUNCH(t, (k_integrate_deposit<false, false>), grid, block, stream, a);
+    RB_LAUNCH(t, (k_integrate_deposit<false, false, kThreads>), grid, block, stream, a);
   }
   return hipGetLastError();
+}
+
+hipError_t launch_integrate_deposit(const StepArgs& a, uint32_t ndeposit, uint32_t threads, hipStream_t stream,
+                                    const Timing* t) {
+  if (ndeposit == 0u) return no_launch(stream, t);
+  if (threads == 512u) return launch_integrate_width<512u>(a, ndeposit, stream, t);
+  if (threads == 1024u) return launch_integrate_width<1024u>(a, ndeposit, stream, t);
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_accel(const StepArgs& a, uint32_t nblocks, uint32_t sys, float4* out,

@@ -26,16 +26,33 @@ hipError_t launch_top(const StepArgs& a, hipStream_t stream, const Timing* t = n
 //                    target's field from those sums (the finest level
 //                    evaluates its own), L2L and near.
 constexpr uint32_t kTreePassRun = 0u, kTreePassM2l = 1u, kTreePassFinish = 2u;
+// Block shapes of a step (results never depend on them; the autotuner
+// times them):
+//   integrate_threads  width of an integrate+deposit block over its
+//                      1024-body deposit range: 1024 (one body per thread)
+//                      or 512 (two per thread);
+//   tree_chunks        the tree items' pair tile: chunks of 8 targets x 8
+//                      sources evaluated per round, 8 (two per wave, 6
+//                      blocks per CU) or 4 (one per wave, half the terms'
+//                      LDS, 8 blocks per CU).
+struct Shape {
+  uint32_t integrate_threads = 1024u;
+  uint32_t tree_chunks = 8u;
+};
+inline bool shape_valid(const Shape& s) {
+  return (s.integrate_threads == 1024u || s.integrate_threads == 512u) && (s.tree_chunks == 8u || s.tree_chunks == 4u);
+}
 // Blocks of a single-level tree launch resident on the device at once.
-uint32_t tree_single_level_capacity(bool compensated);
+uint32_t tree_single_level_capacity(bool compensated, uint32_t chunks);
 // lead > 0 (kTreePassM2l, one system): the launch also runs the top
 // levels (block 0) and the items of levels 2..depth-3 (blocks 1..lead-1);
-// lo..hi are then the levels whose parents upward lists.
+// lo..hi are then the levels whose parents upward lists.  chunks: the pair
+// tile (Shape::tree_chunks).
 hipError_t launch_tree(const StepArgs& a, uint32_t kind, uint32_t lo, uint32_t hi, uint32_t nblocks, uint32_t lead,
-                       hipStream_t stream, const Timing* t = nullptr);
+                       uint32_t chunks, hipStream_t stream, const Timing* t = nullptr);
 // Integrate plus the next step's deposit (into finest buffer parity ^ 1),
-// one block per deposit range.
-hipError_t launch_integrate_deposit(const StepArgs& a, uint32_t ndeposit, hipStream_t stream,
+// one block of `threads` (Shape::integrate_threads) per deposit range.
+hipError_t launch_integrate_deposit(const StepArgs& a, uint32_t ndeposit, uint32_t threads, hipStream_t stream,
                                     const Timing* t = nullptr);
 // The counting sort's scatter into the staging arrays (after upward and top
 // of a step run with a.sorting set; that step's integrate reads them).

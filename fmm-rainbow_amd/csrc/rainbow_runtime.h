@@ -135,6 +135,9 @@ class Engine {
   int set_sort_interval(uint32_t interval);
   int set_tree_fusion(uint32_t hi);
   int set_tree_split(uint32_t starts);  // fixes the split (no measured choice)
+  // Block shapes of the integrate and tree launches (rainbow_launch.h).
+  int set_shape(const Shape& s);
+  // Times the block shapes, tree splits and sort intervals; keeps the fastest.
   int autotune(uint32_t steps, uint32_t* chosen);
   int snapshot(bool restore);
 
@@ -152,6 +155,7 @@ class Engine {
   uint32_t tree_blocks[kLevels + 1] = {};
   std::vector<TreeLaunch> tree_launches;
   uint32_t tree_split = kDefaultTreeSplit;
+  Shape shape;
   // Until a split is set explicitly, the first eager step after a
   // configuration records its per-level internal-cell counts and the split
   // is chosen from them (adapt_split).

@@ -650,7 +650,8 @@ int Engine::adapt_split(bool wait) {
   RB_HIP(hipEventSynchronize(ev_counts));
   counts_pending = false;
   split_wanted = false;
-  const uint32_t cap = 2u * tree_single_level_capacity((options & RAINBOW_OPT_COMPENSATED) != 0u) / 3u;
+  const uint32_t cap =
+      2u * tree_single_level_capacity((options & RAINBOW_OPT_COMPENSATED) != 0u, shape.tree_chunks) / 3u;
   uint32_t starts = 1u << 2;
   for (uint32_t level = 3u; level <= dmax && level < kLevels; ++level) {
     uint64_t items = 0;  // internal parents (level - 1) of this level's targets
@@ -665,6 +666,20 @@ int Engine::adapt_split(bool wait) {
   // then all fields; 1M-body Plummer: 25.6 us against 28.2).
   if (dmax >= 4u && starts == ((1u << 2) | (1u << dmax))) starts = 1u;
   return apply_tree_split(starts);
+}
+
+// A new shape drops the captured graphs; a measured split is measured
+// again (it depends on the tree launches' resident blocks).
+int Engine::set_shape(const Shape& s) {
+  if (!shape_valid(s))
+    return fail(RAINBOW_E_INVALID, "block shape: integrate threads must be 1024 or 512 and tree chunks 8 or 4 (got %u, %u)",
+                s.integrate_threads, s.tree_chunks);
+  if (s.integrate_threads == shape.integrate_threads && s.tree_chunks == shape.tree_chunks) return RAINBOW_OK;
+  RB_HIP(hipStreamSynchronize(stream));
+  drop_graphs();
+  shape = s;
+  if (split_auto && !counts_pending) split_wanted = true;
+  return RAINBOW_OK;
 }
 
 // Levels 2..hi in one launch, every deeper level alone.
@@ -717,7 +732,7 @@ int Engine::enqueue(bool integrate, bool sorting, uint32_t step_parity, bool dep
   }
   for (size_t i = 0; i < tree_launches.size(); ++i) {
     const TreeLaunch& t = tree_launches[i];
-    RB_HIP(launch_tree(a, t.kind, t.lo, t.hi, t.blocks, t.lead, stream));
+    RB_HIP(launch_tree(a, t.kind, t.lo, t.hi, t.blocks, t.lead, shape.tree_chunks, stream));
     if (top_in_tree && i == 0u && sorting) RB_HIP(launch_sort(a, n_integrate, stream));
   }
   if (record_counts) {
@@ -727,7 +742,7 @@ int Engine::enqueue(bool integrate, bool sorting, uint32_t step_parity, bool dep
                           stream));
     RB_HIP(hipEventRecord(ev_counts, stream));
   }
-  if (integrate) RB_HIP(launch_integrate_deposit(a, n_deposit, stream));
+  if (integrate) RB_HIP(launch_integrate_deposit(a, n_deposit, shape.integrate_threads, stream));
   return RAINBOW_OK;
 }
 
@@ -892,13 +907,41 @@ int Engine::snapshot(bool restore) {
   return RAINBOW_OK;
 }
 
-// Times `steps` steps at each candidate sort interval from the same
-// starting state and keeps the fastest; the bodies are restored afterwards.
+// Times `steps` steps from the same starting state for each candidate,
+// in turn: the block shapes (each at two tree splits), every tree split,
+// every sort interval; keeps the fastest of each.  The bodies are restored
+// afterwards.
 int Engine::autotune(uint32_t steps, uint32_t* chosen) {
   static const uint32_t kCandidates[] = {1u, 2u, 4u, 8u, 16u, 32u, 0u};
+  static const Shape kShapes[] = {{1024u, 8u}, {512u, 8u}, {1024u, 4u}, {512u, 4u}};
   if (steps == 0u) return fail(RAINBOW_E_INVALID, "steps must be >= 1");
   int rc = snapshot(false);
   if (rc) return rc;
+  // Block shapes, each at the two-launch form and at one launch per level
+  // (the split that suits a shape depends on its resident blocks; the
+  // search below refines it for the chosen shape).
+  {
+    const uint32_t per_level = ((1u << (dmax + 1u)) - 1u) & 0x7Cu;
+    const uint32_t splits[2] = {1u, per_level};
+    float best = 0.0f;
+    Shape best_shape = shape;
+    for (const Shape& c : kShapes) {
+      if ((rc = set_shape(c)) != 0) return rc;
+      for (uint32_t sp : splits) {
+        if ((rc = set_tree_split(sp)) != 0) return rc;
+        float ms = 0.0f;
+        if ((rc = time_steps(std::min<uint32_t>(steps, 4u), &ms)) != 0) return rc;  // warm graphs
+        if ((rc = snapshot(true)) != 0) return rc;
+        if ((rc = time_steps(steps, &ms)) != 0) return rc;
+        if ((rc = snapshot(true)) != 0) return rc;
+        if (best == 0.0f || ms < best) {
+          best = ms;
+          best_shape = c;
+        }
+      }
+    }
+    if ((rc = set_shape(best_shape)) != 0) return rc;
+  }
   // Tree launch split: every partition of levels 2..dmax into runs, and the
   // two-launch form (mask 1, tried last).
   {
@@ -1023,10 +1066,11 @@ int Engine::phase_ms(uint32_t steps, float* out, uint32_t n) {
     }
     for (size_t i = 0; i < tree_launches.size(); ++i) {
       const TreeLaunch& t = tree_launches[i];
-      if (err == hipSuccess) err = launch_tree(a, t.kind, t.lo, t.hi, t.blocks, t.lead, stream, &tm[k++]);
+      if (err == hipSuccess)
+        err = launch_tree(a, t.kind, t.lo, t.hi, t.blocks, t.lead, shape.tree_chunks, stream, &tm[k++]);
       if (top_in_tree && i == 0u && sorting) sort_timed();
     }
-    if (err == hipSuccess) err = launch_integrate_deposit(a, n_deposit, stream, &tm[k++]);
+    if (err == hipSuccess) err = launch_integrate_deposit(a, n_deposit, shape.integrate_threads, stream, &tm[k++]);
     if (sorting && err == hipSuccess) {
       err = hipEventSynchronize(sort_ev[1]);
       float ms = 0.0f;
@@ -1366,6 +1410,19 @@ int rainbow_hip_tree_launches(rainbow_hip_t* h, uint32_t* lo_hi, uint32_t n) {
   return static_cast<int>(tl.size());
 }
 
+int rainbow_hip_set_shape(rainbow_hip_t* h, uint32_t integrate_threads, uint32_t tree_chunks) {
+  RB_CHECK_HANDLE(h);
+  return h->batch.engine.set_shape(rainbow::Shape{integrate_threads, tree_chunks});
+}
+
+int rainbow_hip_get_shape(rainbow_hip_t* h, uint32_t* integrate_threads, uint32_t* tree_chunks) {
+  RB_CHECK_HANDLE(h);
+  if (integrate_threads == nullptr || tree_chunks == nullptr) return fail(RAINBOW_E_INVALID, "null output");
+  *integrate_threads = h->batch.engine.shape.integrate_threads;
+  *tree_chunks = h->batch.engine.shape.tree_chunks;
+  return RAINBOW_OK;
+}
+
 int rainbow_hip_autotune(rainbow_hip_t* h, uint32_t steps, uint32_t* interval) {
   RB_CHECK_HANDLE(h);
   return h->batch.engine.autotune(steps, interval);
@@ -1526,6 +1583,19 @@ int rainbow_batch_set_tree_fusion(rainbow_batch_t* b, uint32_t hi) {
 int rainbow_batch_set_tree_split(rainbow_batch_t* b, uint32_t starts) {
   RB_CHECK_HANDLE(b);
   return b->batch.engine.set_tree_split(starts);
+}
+
+int rainbow_batch_set_shape(rainbow_batch_t* b, uint32_t integrate_threads, uint32_t tree_chunks) {
+  RB_CHECK_HANDLE(b);
+  return b->batch.engine.set_shape(rainbow::Shape{integrate_threads, tree_chunks});
+}
+
+int rainbow_batch_get_shape(rainbow_batch_t* b, uint32_t* integrate_threads, uint32_t* tree_chunks) {
+  RB_CHECK_HANDLE(b);
+  if (integrate_threads == nullptr || tree_chunks == nullptr) return fail(RAINBOW_E_INVALID, "null output");
+  *integrate_threads = b->batch.engine.shape.integrate_threads;
+  *tree_chunks = b->batch.engine.shape.tree_chunks;
+  return RAINBOW_OK;
 }
 
 int rainbow_batch_autotune(rainbow_batch_t* b, uint32_t steps, uint32_t* interval) {

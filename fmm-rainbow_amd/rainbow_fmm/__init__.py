@@ -150,6 +150,10 @@ _SIGNATURES = {
     "rainbow_hip_pair_counts": (_i32, [_vp, _P(_u64)]),
     "rainbow_hip_set_sort_interval": (_i32, [_vp, _u32]),
     "rainbow_hip_autotune": (_i32, [_vp, _u32, _P(_u32)]),
+    "rainbow_hip_set_shape": (_i32, [_vp, _u32, _u32]),
+    "rainbow_hip_get_shape": (_i32, [_vp, _P(_u32), _P(_u32)]),
+    "rainbow_batch_set_shape": (_i32, [_vp, _u32, _u32]),
+    "rainbow_batch_get_shape": (_i32, [_vp, _P(_u32), _P(_u32)]),
     "rainbow_batch_set_sort_interval": (_i32, [_vp, _u32]),
     "rainbow_batch_autotune": (_i32, [_vp, _u32, _P(_u32)]),
     "rainbow_hip_diagnostics": (_i32, [_vp, _P(_f64)]),
@@ -485,7 +489,21 @@ class Solver:
             tree.append(f"tree_{levels}" if form == "run" else f"tree_{form}_{levels}")
         return ["deposit", "upward", "top"] + tree + ["integrate", "sort"]
 
+    def set_shape(self, integrate_threads: int = 1024, tree_chunks: int = 8):
+        """Block shapes: integrate+deposit block width (1024 or 512 threads
+        over a 1024-body range) and the tree's pair tile (8 or 4 chunks of
+        8 x 8 pairs per round)."""
+        _check(lib().rainbow_hip_set_shape(self._h, integrate_threads, tree_chunks))
+
+    @property
+    def shape(self) -> tuple[int, int]:
+        t, c = _u32(), _u32()
+        _check(lib().rainbow_hip_get_shape(self._h, ctypes.byref(t), ctypes.byref(c)))
+        return t.value, c.value
+
     def autotune(self, steps: int = 32) -> int:
+        """Times block shapes, tree splits and sort intervals; keeps the
+        fastest of each and returns the sort interval (see `shape`)."""
         v = _u32()
         _check(lib().rainbow_hip_autotune(self._h, steps, ctypes.byref(v)))
         return v.value
@@ -575,6 +593,15 @@ class Batch:
 
     def set_tree_split(self, starts: int):
         _check(lib().rainbow_batch_set_tree_split(self._h, starts))
+
+    def set_shape(self, integrate_threads: int = 1024, tree_chunks: int = 8):
+        _check(lib().rainbow_batch_set_shape(self._h, integrate_threads, tree_chunks))
+
+    @property
+    def shape(self) -> tuple[int, int]:
+        t, c = _u32(), _u32()
+        _check(lib().rainbow_batch_get_shape(self._h, ctypes.byref(t), ctypes.byref(c)))
+        return t.value, c.value
 
     def autotune(self, steps: int = 32) -> int:
         v = _u32()

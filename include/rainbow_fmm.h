@@ -298,11 +298,26 @@ int rainbow_hip_set_tree_fusion(rainbow_hip_t* h, uint32_t hi);
  * launches or a negative status. */
 int rainbow_hip_tree_launches(rainbow_hip_t* h, uint32_t* lo_hi, uint32_t n);
 
-/* Autotuner: from the same starting state, times `steps` steps at each tree
+/* Block shapes of the step's kernels (no counterpart in the reference,
+ * whose workgroup sizes are fixed in the WGSL: src/rainbow_fmm.wgsl:394,
+ * 408, 424, 457, 521, 585, 613).  integrate_threads: the width of an
+ * integrate+deposit block over its 1024-body deposit range, 1024 (one body
+ * per thread, the default) or 512 (two per thread).  tree_chunks: the tree
+ * items' pair tile, chunks of 8 targets x 8 sources evaluated per round,
+ * 8 (the default; two per wave, 6 blocks per CU) or 4 (one per wave, half
+ * the LDS of terms, 8 blocks per CU).  Results do not depend on the shape;
+ * RAINBOW_E_INVALID for other values. */
+int rainbow_hip_set_shape(rainbow_hip_t* h, uint32_t integrate_threads, uint32_t tree_chunks);
+int rainbow_hip_get_shape(rainbow_hip_t* h, uint32_t* integrate_threads, uint32_t* tree_chunks);
+
+/* Autotuner: from the same starting state, times `steps` steps at each
+ * block shape (the four combinations above, each at the two-launch form
+ * and at one launch per level) and keeps the fastest, then at each tree
  * split (every partition of levels 2..depth into runs, and the two-launch
- * form) and keeps the fastest, then at each candidate
- * sort interval (1, 2, 4, 8, 16, 32, never), keeps the fastest and writes
- * it to *interval.  The bodies are restored afterwards. */
+ * form), then at each candidate sort interval (1, 2, 4, 8, 16, 32, never);
+ * keeps the fastest of each and writes the interval to *interval (the
+ * shape: rainbow_hip_get_shape).  The split is fixed afterwards.  The
+ * bodies are restored afterwards. */
 int rainbow_hip_autotune(rainbow_hip_t* h, uint32_t steps, uint32_t* interval);
 
 /* Number of cells active in the last solve at each level (M2L targets:
@@ -377,7 +392,10 @@ int rainbow_batch_synchronize(rainbow_batch_t* b);
 /* Total bodies over all systems. */
 uint64_t rainbow_batch_particles(rainbow_batch_t* b);
 /* As rainbow_hip_set_sort_interval / rainbow_hip_set_tree_fusion /
- * rainbow_hip_set_tree_split / rainbow_hip_autotune, for the batch. */
+ * rainbow_hip_set_tree_split / rainbow_hip_set_shape / rainbow_hip_get_shape
+ * / rainbow_hip_autotune, for the batch. */
+int rainbow_batch_set_shape(rainbow_batch_t* b, uint32_t integrate_threads, uint32_t tree_chunks);
+int rainbow_batch_get_shape(rainbow_batch_t* b, uint32_t* integrate_threads, uint32_t* tree_chunks);
 int rainbow_batch_set_sort_interval(rainbow_batch_t* b, uint32_t interval);
 int rainbow_batch_set_tree_fusion(rainbow_batch_t* b, uint32_t hi);
 int rainbow_batch_set_tree_split(rainbow_batch_t* b, uint32_t starts);

@@ -366,7 +366,91 @@ def test_autotune_restores_state():
     before = s.particles()
     chosen = s.autotune(8)
     assert chosen in (0, 1, 2, 4, 8, 16, 32)
+    assert s.shape in ((1024, 8), (512, 8), (1024, 4), (512, 4))
     assert_bits(s.particles(), before, "bodies after autotune")
+    # The tuned solver still steps bit-exactly.
+    s.step(3)
+    op, _, _ = orc.step(slots_for(n, 5e-4, 1.0, 1e-3), parts, 8)
+    assert_bits(s.particles(), op, "particles after autotune and 3 steps")
+    s.close()
+
+
+SHAPES = [(1024, 8), (512, 8), (1024, 4), (512, 4)]
+
+
+@pytest.mark.parametrize("threads,chunks", SHAPES)
+@pytest.mark.parametrize("split", [None, 1, 0x7D])
+def test_block_shapes_bit_exact(threads, chunks, split):
+    """Every block shape with the measured split, the two-launch form and
+    one launch per level: solve + 9 steps (a sort inside) equal the oracle."""
+    say(f"shape {threads}x{chunks} split {split}")
+    n = 180000
+    parts = rf.ic_plummer(n, 95, 0.1, 1.5, 1.0)
+    dt, g, soft = 5e-4, 1.0, 1e-3
+    s = rf.Solver(n, rf.physics(dt=dt, gravity=g, softening=soft))
+    s.set_shape(threads, chunks)
+    if split is not None:
+        s.set_tree_split(split)
+    assert s.shape == (threads, chunks)
+    s.upload(parts)
+    s.step(9)
+    op, om, of = orc.step(slots_for(n, dt, g, soft), parts, 9)
+    assert_bits(s.particles(), op, "particles")
+    assert_bits(s.moments(), om, "moments")
+    assert_bits(s.fields(), of, "fields")
+    s.close()
+
+
+@pytest.mark.parametrize("threads,chunks", [(512, 4), (1024, 4)])
+def test_block_shapes_compensated_and_fused(threads, chunks):
+    """The compensated form and a fused run of levels (2..5 with ancestor
+    recomputation, 6 alone) under the smaller tiles."""
+    say(f"shape {threads}x{chunks} compensated, fused 2-5")
+    n = 120000
+    parts = rf.ic_plummer(n, 96, 0.1, 1.5, 1.0)
+    dt, g, soft = 5e-4, 1.0, 1e-3
+    s = rf.Solver(n, rf.physics(dt=dt, gravity=g, softening=soft), compensated=True)
+    s.set_shape(threads, chunks)
+    s.set_tree_fusion(5)
+    s.upload(parts)
+    s.step(4)
+    op, om, of = orc.step(slots_for(n, dt, g, soft), parts, 4, compensated=True)
+    assert_bits(s.particles(), op, "particles")
+    assert_bits(s.fields(), of, "fields")
+    s.close()
+
+
+def test_batch_shapes_equal_solo():
+    """A batch of uneven systems under the 512 x 4 shape equals each system
+    stepped alone under the default shape."""
+    say("batch shape 512x4 = solo")
+    counts = [4096, 23000, 70000, 131072, 9000]
+    prm = rf.physics(dt=5e-4, gravity=1.0, softening=1e-3)
+    inputs = [rf.ic_plummer(n, 700 + i, 0.1, 1.5, 1.0) for i, n in enumerate(counts)]
+    b = rf.Batch(counts, prm)
+    b.set_shape(512, 4)
+    assert b.shape == (512, 4)
+    for i, p in enumerate(inputs):
+        b.upload(i, p)
+    b.step(10)
+    for i, n in enumerate(counts):
+        s = rf.Solver(n, prm)
+        s.upload(inputs[i])
+        s.step(10)
+        assert_bits(b.download(i, rf.PARTICLES), s.particles(), f"system {i}")
+        s.close()
+    b.close()
+
+
+def test_shape_rejects_other_values():
+    say("shape validation")
+    s = rf.Solver(5000, rf.physics(dt=5e-4, gravity=1.0, softening=1e-3))
+    for bad in [(256, 8), (1024, 16), (2048, 4), (512, 0)]:
+        with pytest.raises(rf.RainbowError) as e:
+            s.set_shape(*bad)
+        assert e.value.code == rf.E_INVALID
+        assert "block shape" in str(e.value)
+    assert s.shape == (1024, 8)
     s.close()
 
 
