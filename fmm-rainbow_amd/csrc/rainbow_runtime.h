@@ -59,6 +59,9 @@ struct SystemStorage {
 
 // Bytes of the allocation of a system of n bodies.
 size_t system_storage_bytes(uint32_t n);
+// Offset in that storage from which it must start zeroed (the buffers
+// before it are written before they are read).
+size_t system_zeroed_from(uint32_t n);
 // Allocates and zeroes a system's storage; scale 0 = planner scale.
 int alloc_system(uint32_t n, const Physics& ph, float scale, hipStream_t stream, SystemStorage* out);
 void free_system(SystemStorage* s);

@@ -102,9 +102,11 @@ void apply_physics(SysDesc& s, const Physics& ph) {
 
 namespace {
 // Byte offsets of a system's buffers in its allocation (each 256-aligned).
+// The buffers every step writes before reading come first; those that must
+// start zeroed (the tree, the flags, the owned ranges) follow from `zero`.
 struct StorageLayout {
   size_t pos, vel, perm, moments, fields, source, active, extra, finest_alt, m2l, xflag;
-  size_t rank, staging, tperm, cstart, xstart, sstart, own, bytes;
+  size_t rank, staging, tperm, cstart, xstart, sstart, own, zero, bytes;
 };
 
 StorageLayout storage_layout(uint32_t n) {
@@ -121,9 +123,22 @@ StorageLayout storage_layout(uint32_t n) {
     at += align_up(bytes);
     return here;
   };
+  // Bodies (written by the upload's split), the sort's ranks, staging and
+  // offsets (written by each sorting step's deposit, upward and top before
+  // its scatter reads them).
   L.pos = take(16u * nb);
   L.vel = take(16u * nb);
   L.perm = take(4u * nb);
+  L.rank = take(4u * nb);
+  L.staging = take(32u * nb);  // tpos then tvel, contiguous
+  L.tperm = take(4u * nb);
+  L.cstart = take(4u * finest);
+  L.xstart = take(4u * finest);
+  L.sstart = take(4u * (finest / 512u));
+  // Zeroed at admission: moments (the deposit adds into a zeroed finest
+  // buffer), fields and source records (zero outside the target set),
+  // flags, the straggler sums, the owned ranges ((0, 0): none).
+  L.zero = at;
   L.moments = take(16u * cells);
   L.fields = take(48u * cells);
   L.source = take(16u * cells);
@@ -132,12 +147,6 @@ StorageLayout storage_layout(uint32_t n) {
   L.finest_alt = take(16u * finest);
   L.m2l = take(48u * above);
   L.xflag = take(finest);
-  L.rank = take(4u * nb);
-  L.staging = take(32u * nb);  // tpos then tvel, contiguous
-  L.tperm = take(4u * nb);
-  L.cstart = take(4u * finest);
-  L.xstart = take(4u * finest);
-  L.sstart = take(4u * (finest / 512u));
   L.own = take(8u * blocks);
   L.bytes = at;
   return L;
@@ -145,6 +154,7 @@ StorageLayout storage_layout(uint32_t n) {
 }  // namespace
 
 size_t system_storage_bytes(uint32_t n) { return storage_layout(n).bytes; }
+size_t system_zeroed_from(uint32_t n) { return storage_layout(n).zero; }
 
 SysDesc describe_system(void* base, uint32_t n, const Physics& ph, float scale) {
   const Plan plan = make_plan(n);

@@ -438,7 +438,7 @@ struct rainbow_server {
     return RAINBOW_OK;
   }
 
-  // Admission: memory from the arena, zeroed and filled on the copy stream;
+  // Admission: memory from the arena, cleared and filled on the copy stream;
   // the compute stream waits for that copy and splits the bodies into the
   // system's streams before the step that deposits it.
   int admit() {
@@ -463,8 +463,11 @@ struct rainbow_server {
       s.block = block;
       s.upload = p.borrowed ? nullptr : p.data;
       s.desc = rainbow::describe_system(block, p.n, physics, 0.0f);
+      // Only the part that must start zeroed is cleared: the bodies and the
+      // sort's buffers are written before they are read.
+      const size_t zero = rainbow::system_zeroed_from(p.n);
       hipError_t e = events.get(&s.loaded);
-      if (e == hipSuccess) e = hipMemsetAsync(block, 0, need, copy);
+      if (e == hipSuccess) e = hipMemsetAsync(static_cast<char*>(block) + zero, 0, need - zero, copy);
       if (e == hipSuccess) e = hipMemcpyAsync(s.desc.tpos, p.data, 32ull * p.n, hipMemcpyHostToDevice, copy);
       if (e == hipSuccess) e = hipEventRecord(s.loaded, copy);
       if (e != hipSuccess) {
