@@ -18,6 +18,8 @@ the failure paths of the C ABI.
   with NaN and infinite coordinates (bit for bit, NaNs compared as NaN), the
   harness CLI's finiteness line, and a fixed-point scale changed between
   steps.
+* One system of 150,000,000 bodies (4.8 GB of particles, every body
+  buffer past 4 GiB): two steps bit for bit against the oracle.
 * The server's request life cycle: cancelled (queued and running) and
   expired requests, a zero-step request, duplicate ids, and a stream of
   arrivals through room for a few systems at a time (arena reuse, systems
@@ -98,6 +100,33 @@ def test_cfg3_clumps_compensated_bit_exact():
     assert_bits(s.particles(), op, "particles after 2 compensated steps")
     assert_bits(s.fields(), of2, "fields after 2 compensated steps")
     s.close()
+
+
+def test_system_past_4gib_buffers_bit_exact():
+    """150M bodies: byte offsets of every body buffer pass 2^32, the deposit
+    has 146,485 ranges and the fixed-point scale is the planner's smallest
+    (the sums stay within i32).  Inputs from a vectorized generator (a
+    uniform cube of half-width 1.2 with small random velocities), so the
+    test's host time stays in seconds."""
+    n = 150_000_000
+    say(f"{n} bodies: 2 steps")
+    rng = np.random.default_rng(150)
+    parts = np.empty((n, 8), dtype=np.float32)
+    parts[:, 0:3] = rng.uniform(-1.2, 1.2, size=(n, 3)).astype(np.float32)
+    parts[:, 3] = rng.uniform(0.0, 1.0, size=n).astype(np.float32)
+    parts[:, 4:7] = rng.normal(0.0, 0.05, size=(n, 3)).astype(np.float32)
+    parts[:, 7] = 0.0
+    assert parts.nbytes > 4 * 2**30
+    s = rf.Solver(n, rf.physics(**CFG))
+    s.upload(parts)
+    s.step(2)
+    got = s.particles()
+    moments = s.moments()
+    s.close()
+    op, om, _ = orc.step(slots_for(n, **CFG), parts, 2)
+    del parts
+    assert_bits(moments, om, "moments")
+    assert_bits(got, op, "particles")
 
 
 def cfg4_counts():
