@@ -158,7 +158,9 @@ class Arena {
 };
 
 // Page-locked host buffers, reused: requests are copied into one at submit
-// and results land in one, so transfers run as DMA at full bus speed.
+// and results land in one, so transfers run as DMA at full bus speed.  At
+// most kPinnedKeepBytes of unused buffers are kept (the largest go first).
+constexpr size_t kPinnedKeepBytes = 16ull << 30;
 class PinnedPool {
  public:
   ~PinnedPool() {
@@ -170,6 +172,7 @@ class PinnedPool {
     if (it != free_.end() && it->first <= 2u * bytes) {
       void* p = it->second;
       size_[p] = it->first;
+      free_bytes_ -= it->first;
       free_.erase(it);
       return p;
     }
@@ -186,12 +189,20 @@ class PinnedPool {
     auto it = size_.find(p);
     if (it == size_.end()) return;
     free_.emplace(it->second, p);
+    free_bytes_ += it->second;
     size_.erase(it);
+    while (free_bytes_ > kPinnedKeepBytes && !free_.empty()) {
+      auto last = std::prev(free_.end());
+      free_bytes_ -= last->first;
+      (void)hipHostFree(last->second);
+      free_.erase(last);
+    }
   }
 
  private:
   std::multimap<size_t, void*> free_;
   std::unordered_map<void*, size_t> size_;  // buffers in use
+  size_t free_bytes_ = 0;
 };
 
 // Events, reused.
