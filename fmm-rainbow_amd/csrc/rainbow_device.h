@@ -139,9 +139,12 @@ struct StepArgs {
   uint32_t rank_next;  // the fused deposit writes ranks: the next step sorts
 };
 
-constexpr uint32_t kDepositThreads = 1024;  // one body per thread
-constexpr uint32_t kDepositPerThread = 1;
-constexpr uint32_t kDepositBodies = kDepositThreads * kDepositPerThread;  // per block
+// Deposit tile: the bodies of one deposit/integrate block (its owned Morton
+// range of finest cells is kept per tile).  The standalone deposit runs
+// kDepositThreads threads, one body each; the fused integrate+deposit runs
+// 1024 or 512 threads over the same tile (Shape::integrate_threads).
+constexpr uint32_t kDepositBodies = 1024;
+constexpr uint32_t kDepositThreads = 1024;
 constexpr uint32_t kDepositSlots = 2048;  // LDS hash slots, 2x bodies per block
 constexpr uint32_t kDepositSlotBits = 31u - static_cast<uint32_t>(__builtin_clz(kDepositSlots));
 static_assert((1u << kDepositSlotBits) == kDepositSlots, "hash table size is a power of two");

@@ -1,28 +1,33 @@
 // gfx950 kernels of the p=1 Cartesian uniform-octree FMM step.
 //
-// One step is five passes over a batch of systems (a plain solve is a batch
-// of one), launched in order on one stream and captured in a HIP graph:
+// One step runs these passes over a batch of systems (a plain solve is a
+// batch of one), launched in order on one stream and captured in HIP
+// graphs:
 //
-//   deposit    P2M: finest-cell binning and fixed-point moment sums,
-//              pre-aggregated per block in an LDS hash table; a block stores
-//              the cells of its own Morton range and adds the others into a
-//              straggler accumulator with integer atomics (exact in any
-//              order)
 //   upward     M2M: one block per 8x8x8 finest-cell subtree merges the
-//              stragglers, sums three levels in LDS, writes each cell's
-//              source record (centre of mass and G m), lists the internal
-//              cells (count > 16), zeroes the fields of cells that stopped
-//              being M2L targets and the finest-level buffer of the next step
+//              straggler sums of the last deposit, sums three levels in LDS,
+//              writes each cell's source record (centre of mass and G m),
+//              lists the internal cells (count > 16), zeroes the fields of
+//              cells that stopped being M2L targets and the finest-level
+//              buffer of the next deposit
 //   top        the levels above the subtrees and the fields of level 1,
-//              one block per system
-//   tree       for levels 2..depth, in a few launches over runs of levels:
-//              M2L, L2L and the near field of every M2L target (a non-empty
-//              child of an internal cell), four waves per internal parent
+//              one block per system (for one system it runs inside the
+//              first tree launch)
+//   tree       for levels 2..depth, M2L, L2L and the near field of every
+//              M2L target (a non-empty child of an internal cell), four
+//              waves per internal parent; in runs of levels, or in two
+//              launches (the M2L sums above the finest level, then every
+//              level's fields from them)
 //   integrate  adaptive descent, local expansion, own-leaf monopole,
-//              symplectic Euler kick-drift and the soft absorbing boundary
-// A sorting step also reorders the bodies by Morton code: a scatter into
-// staging arrays between top and the tree pass, from which that step's
-// integrate reads them.
+//     +deposit symplectic Euler kick-drift and the soft absorbing boundary,
+//              then the next step's P2M of the new positions: finest-cell
+//              binning and fixed-point sums pre-aggregated per block in an
+//              LDS hash table; a block stores the cells of its own Morton
+//              range and adds the others into a straggler accumulator with
+//              integer atomics (exact in any order)
+// A solve's first deposit runs alone (k_deposit).  A sorting step (every few
+// steps) also reorders the bodies by Morton code: a scatter into staging
+// arrays after top, from which that step's integrate reads them.
 //
 // The reference runs the same physics as 2d+5 dispatches
 // (native/main.rs:411-421): clear_cells, deposit, upward per level, m2l_all,
@@ -1229,7 +1234,7 @@ __global__ __launch_bounds__(kTreeThreads, tree_occupancy(kKind, kCpr)) void k_t
   constexpr uint32_t kWaves = kTreeThreads / 64u;
   constexpr uint32_t kChunksPerRound = kCpr;
   // Single level: static LDS.  Fused: one dynamic region of
-  // max(kItemBytes, (hi - lo) * kAncestorBytes) bytes, used by the
+  // max(item_bytes(kCpr), (hi - lo) * kAncestorBytes) bytes, used by the
   // ancestor path and then by the item.
   __shared__ float4 win_s[kFused ? 1 : 216];
   // The M2L form's block 0 may run the top levels in this array instead.
