@@ -87,10 +87,13 @@ struct alignas(16) SysDesc {
   float control_strength;  // control_settings.x
   float control_radius;    // control_settings.y
   uint32_t live;           // 0: every kernel skips the system
+  uint32_t sort_now;       // this step sorts the system even when the step
+                           // as a whole does not (its first step after it
+                           // joined a running set)
   float control_point[4];
   float control_axis_mode[4];
 };
-static_assert(sizeof(SysDesc) == 224, "SysDesc layout");
+static_assert(sizeof(SysDesc) == 240, "SysDesc layout");
 
 // Work list of the particle passes: a block covers bodies [first, first +
 // block size) of one system.

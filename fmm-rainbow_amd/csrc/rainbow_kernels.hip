@@ -163,6 +163,12 @@ __device__ __forceinline__ void load_field(gptr<const float4> fields, uint32_t c
 }
 
 // The finest level of the step's parity (see SysDesc::finest_alt).
+// Whether this step sorts system d: every system on a sorting step, a
+// system that just joined a running set on its first step.
+__device__ __forceinline__ bool sorts(const StepArgs& a, const SysDesc& d) {
+  return a.sorting != 0u || d.sort_now != 0u;
+}
+
 __device__ __forceinline__ gptr<int4> finest_of(const SysDesc& d, uint32_t parity) {
   return parity ? G(d.finest_alt) : G(d.moments) + level_off(d.depth);
 }
@@ -357,7 +363,7 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit(StepArgs a) {
     const uint32_t i = blk.first + k * kDepositThreads + threadIdx.x;
     p[k] = i < d.n ? G(d.pos)[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
-  deposit_block<kDepositThreads>(a, d, blk.first, p, a.parity, a.sorting != 0u, L);
+  deposit_block<kDepositThreads>(a, d, blk.first, p, a.parity, sorts(a, d), L);
 }
 
 // Morton code of the finest cell of a position.
@@ -382,7 +388,7 @@ __device__ __forceinline__ uint32_t compact3(uint32_t v) {
 __global__ __launch_bounds__(kIntegrateThreads) void k_scatter(StepArgs a) {
   const BodyBlock blk = a.integrate_blocks[blockIdx.x];
   const SysDesc d = a.sys[blk.sys];
-  if (!d.live) return;
+  if (!d.live || !sorts(a, d)) return;
   const uint32_t i = blk.first + threadIdx.x;
   if (i >= d.n) return;
   const float4 p = G(d.pos)[i];
@@ -634,7 +640,7 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
   if (prev_f & 2u) next[fcell] = make_int4(0, 0, 0, 0);
   lvl0[t] = m0;
   __syncthreads();
-  if (a.sorting) {
+  if (sorts(a, d)) {
     // Offsets of the subtree's cells in Morton order (the subtree's 512
     // finest cells are the contiguous Morton range root << 9 ..): thread u
     // takes Morton cell u; wave prefix sums, then the waves' totals.
@@ -782,7 +788,7 @@ __device__ void top_levels(const StepArgs& a, uint32_t s, TopLds& L) {
     }
     __syncthreads();
   }
-  if (a.sorting) {
+  if (sorts(a, d)) {
     // Offsets of the subtrees (level r cells, Morton order) in the system:
     // 512 values (the absent ones zero), kThreads per pass; wave prefix
     // sums, then the waves' totals.
@@ -1810,7 +1816,7 @@ __global__ __launch_bounds__(kThreads) void k_integrate_deposit(StepArgs a) {
   const BodyBlock blk = a.deposit_blocks[blockIdx.x];
   const SysDesc d = a.sys[kOne ? 0u : blk.sys];
   if (!d.live) return;
-  const bool staged = a.sorting != 0u;
+  const bool staged = sorts(a, d);
   const uint32_t dim = 1u << d.depth;
   float4 p[kPer];
 #pragma unroll

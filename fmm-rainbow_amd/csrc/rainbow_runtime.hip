@@ -713,9 +713,22 @@ int Engine::set_tree_fusion(uint32_t hi) {
 int Engine::enqueue(bool integrate, bool sorting, uint32_t step_parity, bool deposit_first, bool rank_next,
                     bool record_counts) {
   if (sys.empty()) return RAINBOW_OK;
-  RB_HIP_RC(flush_descriptors());
   // The sorted bodies reach the body arrays only through integrate.
   sorting = sorting && integrate;
+  // Systems that joined a running set since the last step arrive in caller
+  // order: when this step does not sort everyone, it sorts them alone
+  // (their descriptors carry the flag for this step only).
+  std::vector<uint32_t> joined;
+  if (integrate && !sorting && sort_interval != 0u && !deposit_first && n_fresh > 0u) {
+    for (uint32_t s = 0; s < sys.size(); ++s)
+      if (sys[s].live && std::find(fresh_pending_.begin(), fresh_pending_.end(), sys[s].pos) != fresh_pending_.end()) {
+        sys[s].sort_now = 1u;
+        joined.push_back(s);
+      }
+    if (!joined.empty()) descs_dirty_ = true;
+  }
+  const bool any_sort = sorting || !joined.empty();
+  RB_HIP_RC(flush_descriptors());
   StepArgs a = args;
   a.sorting = sorting ? 1u : 0u;
   a.parity = step_parity;
@@ -725,7 +738,8 @@ int Engine::enqueue(bool integrate, bool sorting, uint32_t step_parity, bool dep
     RB_HIP(launch_deposit(a, n_deposit, stream));
   } else if (n_fresh > 0u) {
     // Systems that joined since the last step: their own first deposit
-    // (ranks too when this step sorts), into this step's finest buffer.
+    // (ranks too when this step sorts them), into this step's finest
+    // buffer.
     StepArgs f = a;
     f.deposit_blocks = static_cast<const BodyBlock*>(fresh_buf.p);
     RB_HIP(launch_deposit(f, n_fresh, stream));
@@ -738,12 +752,12 @@ int Engine::enqueue(bool integrate, bool sorting, uint32_t step_parity, bool dep
   const bool top_in_tree = !tree_launches.empty() && tree_launches[0].lead != 0u;
   if (!top_in_tree) {
     RB_HIP(launch_top(a, stream));
-    if (sorting) RB_HIP(launch_sort(a, n_integrate, stream));
+    if (any_sort) RB_HIP(launch_sort(a, n_integrate, stream));
   }
   for (size_t i = 0; i < tree_launches.size(); ++i) {
     const TreeLaunch& t = tree_launches[i];
     RB_HIP(launch_tree(a, t.kind, t.lo, t.hi, t.blocks, t.lead, shape.tree_chunks, stream));
-    if (top_in_tree && i == 0u && sorting) RB_HIP(launch_sort(a, n_integrate, stream));
+    if (top_in_tree && i == 0u && any_sort) RB_HIP(launch_sort(a, n_integrate, stream));
   }
   if (record_counts) {
     // The step's internal-cell counts, for the measured tree split (the
@@ -753,6 +767,9 @@ int Engine::enqueue(bool integrate, bool sorting, uint32_t step_parity, bool dep
     RB_HIP(hipEventRecord(ev_counts, stream));
   }
   if (integrate) RB_HIP(launch_integrate_deposit(a, n_deposit, shape.integrate_threads, stream));
+  // The next step's descriptors clear the joined systems' flag again.
+  for (uint32_t s : joined) sys[s].sort_now = 0u;
+  if (!joined.empty()) descs_dirty_ = true;
   return RAINBOW_OK;
 }
 
