@@ -385,8 +385,12 @@ __device__ __forceinline__ uint32_t compact3(uint32_t v) {
 // Scatter of the counting sort (bodies in Morton order of their finest
 // cell): body i goes to slot sstart[subtree] + cstart[cell] + its index in
 // the cell, in the staging buffers.
-__global__ __launch_bounds__(kIntegrateThreads) void k_scatter(StepArgs a) {
-  const BodyBlock blk = a.integrate_blocks[blockIdx.x];
+// kThreads bodies per block from `blocks`: the integrate ranges (every
+// system, 256 bodies each) on a sorting step, or the deposit ranges of the
+// systems that joined (1024 each) when only they sort.
+template <uint32_t kThreads>
+__global__ __launch_bounds__(kThreads) void k_scatter(StepArgs a, const BodyBlock* blocks) {
+  const BodyBlock blk = blocks[blockIdx.x];
   const SysDesc d = a.sys[blk.sys];
   if (!d.live || !sorts(a, d)) return;
   const uint32_t i = blk.first + threadIdx.x;
@@ -2325,7 +2329,15 @@ hipError_t launch_tree(const StepArgs& a, uint32_t kind, uint32_t lo, uint32_t h
 
 hipError_t launch_sort(const StepArgs& a, uint32_t nintegrate, hipStream_t stream) {
   if (nintegrate == 0u) return hipSuccess;
-  hipLaunchKernelGGL(k_scatter, dim3(nintegrate), dim3(kIntegrateThreads), 0, stream, a);
+  hipLaunchKernelGGL(k_scatter<kIntegrateThreads>, dim3(nintegrate), dim3(kIntegrateThreads), 0, stream, a,
+                     a.integrate_blocks);
+  return hipGetLastError();
+}
+
+hipError_t launch_sort_ranges(const StepArgs& a, const BodyBlock* deposit_ranges, uint32_t nranges,
+                              hipStream_t stream) {
+  if (nranges == 0u) return hipSuccess;
+  hipLaunchKernelGGL(k_scatter<kDepositBodies>, dim3(nranges), dim3(kDepositBodies), 0, stream, a, deposit_ranges);
   return hipGetLastError();
 }
 

@@ -57,6 +57,10 @@ hipError_t launch_integrate_deposit(const StepArgs& a, uint32_t ndeposit, uint32
 // The counting sort's scatter into the staging arrays (after upward and top
 // of a step run with a.sorting set; that step's integrate reads them).
 hipError_t launch_sort(const StepArgs& a, uint32_t nintegrate, hipStream_t stream);
+// The same scatter over given deposit ranges (1024 bodies each): the systems
+// that joined, when only they sort.
+hipError_t launch_sort_ranges(const StepArgs& a, const BodyBlock* deposit_ranges, uint32_t nranges,
+                              hipStream_t stream);
 // FP64 direct sum at `nt` target bodies given by caller index.  It works in
 // one scratch buffer laid out by direct_layout(nt): the target list (the
 // caller copies it in), each target's slot, the partial sums of the body

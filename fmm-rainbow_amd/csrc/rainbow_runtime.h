@@ -198,8 +198,11 @@ class Engine {
     size_t cap = 0;
     hipEvent_t done = nullptr;
   };
-  static constexpr uint32_t kTableStages = 8;
+  static constexpr uint32_t kTableStages = 32;
   TableStage stages_[kTableStages];
+  // Staging buffers outgrown while their copies may still be queued; freed
+  // with the engine (a pinned free waits for the device).
+  std::vector<void*> retired_stages_;
   uint32_t stage_next_ = 0;
   int next_stage(size_t bytes, TableStage** out);
 
@@ -219,6 +222,12 @@ class Engine {
   hipEvent_t ev_b = nullptr;
   bool split_wanted = false;   // a measured split choice is due
   bool counts_pending = false;  // counts_host is being filled (ev_counts)
+  // Changes of the set (or of the block shape) since the engine opened, and
+  // the one the pending counts were recorded at: a measurement in flight
+  // still applies when the set changes under it (under steady churn one
+  // would never finish otherwise), and the split is measured again after.
+  uint64_t set_gen_ = 0;
+  uint64_t counts_gen_ = 0;
   uint32_t* counts_host = nullptr;  // pinned copy of the list counts
   hipEvent_t ev_counts = nullptr;
   std::vector<uint32_t> caps_host;  // list capacity per level and stripe

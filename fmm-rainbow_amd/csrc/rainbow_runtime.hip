@@ -249,6 +249,7 @@ Engine::~Engine() {
     if (st.done) (void)hipEventDestroy(st.done);
     if (st.host) (void)hipHostFree(st.host);
   }
+  for (void* p : retired_stages_) (void)hipHostFree(p);
   for (hipEvent_t e : {ev_a, ev_b, ev_counts})
     if (e) (void)hipEventDestroy(e);
   if (counts_host) (void)hipHostFree(counts_host);
@@ -463,13 +464,16 @@ int Engine::flush_descriptors() {
 uint64_t Engine::dead_blocks() const { return dead_blocks_; }
 
 // The next pinned staging buffer of the ring, with room for `bytes`, once
-// the copies that last read it have run.
+// the copies that last read it have run.  The ring is deep (a step that
+// changes the set uses up to three), so the wait is rare; an outgrown
+// buffer is kept until the engine goes (freeing pinned memory would wait
+// for the device) and a larger one takes its place.
 int Engine::next_stage(size_t bytes, TableStage** out) {
   TableStage& st = stages_[stage_next_];
   stage_next_ = (stage_next_ + 1u) % kTableStages;
   if (st.done != nullptr) RB_HIP(hipEventSynchronize(st.done));
   if (st.cap < bytes) {
-    if (st.host != nullptr) (void)hipHostFree(st.host);
+    if (st.host != nullptr) retired_stages_.push_back(st.host);
     st.host = nullptr;
     st.cap = 0;
     const size_t want = std::max<size_t>(bytes + bytes / 4u, 1u << 16);
@@ -498,7 +502,7 @@ int Engine::upload_tables(uint32_t s0, uint32_t d0, uint32_t i0, uint32_t b0) {
     stripes[kLevels * kStripes + i] = caps_host[i];
     list_total += caps_host[i];
   }
-  counts_pending = false;
+  ++set_gen_;
   split_wanted = split_auto;
   int rc;
   if ((rc = sys_buf.ensure(sizeof(SysDesc) * nsys)) || (rc = deposit_buf.ensure(sizeof(BodyBlock) * n_deposit)) ||
@@ -659,7 +663,10 @@ int Engine::adapt_split(bool wait) {
   }
   RB_HIP(hipEventSynchronize(ev_counts));
   counts_pending = false;
-  split_wanted = false;
+  if (!split_auto) return RAINBOW_OK;  // a split was fixed since the counts were recorded
+  // Counts recorded before the latest set change still choose a split (the
+  // set changes little per step); a fresh measurement follows.
+  split_wanted = counts_gen_ != set_gen_;
   const uint32_t cap =
       2u * tree_single_level_capacity((options & RAINBOW_OPT_COMPENSATED) != 0u, shape.tree_chunks) / 3u;
   uint32_t starts = 1u << 2;
@@ -688,7 +695,8 @@ int Engine::set_shape(const Shape& s) {
   RB_HIP(hipStreamSynchronize(stream));
   drop_graphs();
   shape = s;
-  if (split_auto && !counts_pending) split_wanted = true;
+  ++set_gen_;
+  if (split_auto) split_wanted = true;
   return RAINBOW_OK;
 }
 
@@ -744,6 +752,13 @@ int Engine::enqueue(bool integrate, bool sorting, uint32_t step_parity, bool dep
     f.deposit_blocks = static_cast<const BodyBlock*>(fresh_buf.p);
     RB_HIP(launch_deposit(f, n_fresh, stream));
   }
+  // The scatter: every system's bodies on a sorting step, else those of the
+  // systems that joined (their deposit ranges).
+  const uint32_t fresh_ranges = n_fresh;
+  auto scatter = [&]() -> hipError_t {
+    return sorting ? launch_sort(a, n_integrate, stream)
+                   : launch_sort_ranges(a, static_cast<const BodyBlock*>(fresh_buf.p), fresh_ranges, stream);
+  };
   n_fresh = 0u;
   fresh_pending_.clear();
   RB_HIP(launch_upward(a, n_subtree, stream));
@@ -752,12 +767,12 @@ int Engine::enqueue(bool integrate, bool sorting, uint32_t step_parity, bool dep
   const bool top_in_tree = !tree_launches.empty() && tree_launches[0].lead != 0u;
   if (!top_in_tree) {
     RB_HIP(launch_top(a, stream));
-    if (any_sort) RB_HIP(launch_sort(a, n_integrate, stream));
+    if (any_sort) RB_HIP(scatter());
   }
   for (size_t i = 0; i < tree_launches.size(); ++i) {
     const TreeLaunch& t = tree_launches[i];
     RB_HIP(launch_tree(a, t.kind, t.lo, t.hi, t.blocks, t.lead, shape.tree_chunks, stream));
-    if (top_in_tree && i == 0u && any_sort) RB_HIP(launch_sort(a, n_integrate, stream));
+    if (top_in_tree && i == 0u && any_sort) RB_HIP(scatter());
   }
   if (record_counts) {
     // The step's internal-cell counts, for the measured tree split (the
@@ -890,7 +905,10 @@ int Engine::step_eager(uint32_t steps) {
     const bool record = split_wanted && !counts_pending;
     rc = enqueue(true, sorting, parity, pending_deposit, rank_next, record);
     if (rc) return rc;
-    if (record) counts_pending = true;
+    if (record) {
+      counts_pending = true;
+      counts_gen_ = set_gen_;
+    }
     parity ^= 1u;
     since_sort = after;
     pending_deposit = false;
