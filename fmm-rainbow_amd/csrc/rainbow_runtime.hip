@@ -736,7 +736,13 @@ int Engine::enqueue(bool integrate, bool sorting, uint32_t step_parity, bool dep
     if (!joined.empty()) descs_dirty_ = true;
   }
   const bool any_sort = sorting || !joined.empty();
-  RB_HIP_RC(flush_descriptors());
+  const int flushed = flush_descriptors();
+  // The copy just staged carries the flags for this step; the host copy
+  // drops them at once, so the next step's descriptors come without them
+  // whatever happens below.
+  for (uint32_t s : joined) sys[s].sort_now = 0u;
+  if (!joined.empty()) descs_dirty_ = true;
+  if (flushed != RAINBOW_OK) return flushed;
   StepArgs a = args;
   a.sorting = sorting ? 1u : 0u;
   a.parity = step_parity;
@@ -782,9 +788,6 @@ int Engine::enqueue(bool integrate, bool sorting, uint32_t step_parity, bool dep
     RB_HIP(hipEventRecord(ev_counts, stream));
   }
   if (integrate) RB_HIP(launch_integrate_deposit(a, n_deposit, shape.integrate_threads, stream));
-  // The next step's descriptors clear the joined systems' flag again.
-  for (uint32_t s : joined) sys[s].sort_now = 0u;
-  if (!joined.empty()) descs_dirty_ = true;
   return RAINBOW_OK;
 }
 
