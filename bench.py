@@ -306,8 +306,10 @@ def serve_leg(args):
       downloads included, ramp and drain too) and the device's busy fraction
       (event time of the steps over the wall);
     * sequential: the same requests one after another on plain solvers
-      (upload, steps, download; solvers created and warmed beforehand), on
-      the same host clock; the ratio is the served speedup;
+      (upload from and download into pinned buffers, as the server does;
+      solvers created and warmed beforehand), on the same host clock; the
+      ratio is the served speedup (also reported against the same loop
+      downloading into fresh pageable arrays, the plain API's default);
     * Poisson: arrivals at `load` x the saturated rate for `arrival_s`
       seconds; p50/p99 of (completion - arrival), next to the same requests'
       solo latency (upload + steps + download on a plain solver)."""
@@ -319,19 +321,21 @@ def serve_leg(args):
     budget = int(args.budget_gb * 1e9)
 
     # Solo latency model per pool entry: upload + download (host clock) and
-    # per-step GPU time (graph replays), from a warmed plain solver.
-    solvers = []
+    # per-step GPU time (graph replays), from a warmed plain solver.  Plain
+    # solvers download into pinned buffers, as the server does.
+    solvers, results = [], []
     for n, _, data in ([] if args.serve_saturated_only else pool):
         s = rf.Solver(n, prm)
         s.upload(data)
         s.step(24)
         s.sync()
         solvers.append(s)
+        results.append(rf.host_array(n))
     updown, per_step = [], []
-    for (n, _, data), s in zip(pool, solvers):
+    for (n, _, data), s, out in zip(pool, solvers, results):
         t0 = time.perf_counter()
         s.upload(data)
-        s.particles()
+        s.particles(out)
         updown.append(time.perf_counter() - t0)
         per_step.append(s.time_steps(32) / 32 / 1e3)
 
@@ -349,18 +353,23 @@ def serve_leg(args):
                 "gpu_busy": st_sat["gpu_step_s"] / wall, "steps": st_sat["steps"],
                 "all_completed": all(c.status == 0 for c in done.values())}
 
-    # Sequential: the same requests one after another on plain solvers.
-    t0 = time.perf_counter()
-    solo_lat = []
-    for k, steps in sat:
-        t1 = time.perf_counter()
-        s = solvers[k]
-        s.upload(pool[k][2])
-        s.step(steps)
-        s.particles()
-        solo_lat.append(time.perf_counter() - t1)
-    seq_wall = time.perf_counter() - t0
-    seq_rate = work / seq_wall
+    # Sequential: the same requests one after another on plain solvers,
+    # results into pinned buffers (the comparison's baseline), then once more
+    # into fresh pageable arrays (the plain API's default download).
+    def sequential(pinned):
+        t0 = time.perf_counter()
+        lat = []
+        for k, steps in sat:
+            t1 = time.perf_counter()
+            s = solvers[k]
+            s.upload(pool[k][2])
+            s.step(steps)
+            s.particles(results[k] if pinned else None)
+            lat.append(time.perf_counter() - t1)
+        return work / (time.perf_counter() - t0), lat
+
+    seq_rate, solo_lat = sequential(True)
+    seq_rate_pageable, _ = sequential(False)
 
     # Poisson arrivals at load x the saturated rate.
     mean_work = float(np.mean([pool[k][0] * st for k, st in sat]))
@@ -398,6 +407,8 @@ def serve_leg(args):
                       "host_s": {k: round(v, 4) for k, v in st_sat.items() if k.endswith("_s")}},
         "sequential_value": seq_rate,
         "speedup": sat_rate / seq_rate,
+        "sequential_pageable_value": seq_rate_pageable,
+        "speedup_vs_pageable": sat_rate / seq_rate_pageable,
         "solo_p50_ms": float(np.percentile(solo_lat, 50) * 1e3),
         "solo_p99_ms": float(np.percentile(solo_lat, 99) * 1e3),
         "all_completed": ok,

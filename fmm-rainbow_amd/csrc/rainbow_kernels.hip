@@ -1161,6 +1161,18 @@ __device__ __forceinline__ uint64_t tree_block_offsets(uint32_t wave) {
   return wave == 0u ? kT.pack[0] : wave == 1u ? kT.pack[1] : wave == 2u ? kT.pack[2] : kT.pack[3];
 }
 
+// LDS word of the term of pair (target tt, source cc) for component k in
+// chunk slot u: rows of 8 sources, component-major ([u][k][tt][cc]), so a
+// component's 64 pair lanes store 64 consecutive words; each row's two
+// halves swap places when bit 2 ^ bit 3 of its chain thread (8 k + tt) is
+// set, so that any 8 (and any 16) consecutive chain threads read their
+// 16-byte halves from distinct banks.
+__device__ __forceinline__ uint32_t term_index(uint32_t u, uint32_t k, uint32_t tt, uint32_t cc) {
+  const uint32_t reader = 8u * k + tt;
+  const uint32_t swap = ((reader >> 2) ^ (reader >> 3)) & 1u;
+  return u * 640u + reader * 8u + 4u * ((cc >> 2) ^ swap) + (cc & 3u);
+}
+
 // Forms of the tree kernel:
 //   kTreeSingle  one level; the parent's field from memory.
 //   kTreeFused   levels lo..hi; the parent's field recomputed along the
@@ -1344,9 +1356,9 @@ __global__ __launch_bounds__(kTreeThreads, tree_occupancy(kKind, kCpr)) void k_t
   // table built at compile time (one byte per round and j).
   constexpr uint32_t kPerWave = kChunksPerRound / kWaves;
   const uint64_t block_off = tree_block_offsets<kCpr>(wave_s);
-  // Chain threads 0..79: target tid / 10, component tid % 10.
+  // Chain threads 0..79: target tid % 8, component tid / 8.
   const bool chain = tid < 80u;
-  const uint32_t ct = tid / 10u, ck = tid % 10u;
+  const uint32_t ct = tid & 7u, ck = tid >> 3;
 
   // Each item's list entry is loaded one item ahead; the system descriptor
   // is reloaded only when the system changes.
@@ -1532,7 +1544,7 @@ __global__ __launch_bounds__(kTreeThreads, tree_occupancy(kKind, kCpr)) void k_t
         // Terms of chunk `slot` stored per chain (target tt, component k)
         // as 8 consecutive sources: the chains then read contiguous runs.
 #pragma unroll
-        for (int k = 0; k < 10; ++k) terms[((slot * 8u + tt) * 10u + k) * 8u + cc] = t[k];
+        for (uint32_t k = 0; k < 10u; ++k) terms[term_index(slot, k, tt, cc)] = t[k];
       }
       __syncthreads();
       // The chains add the round's chunks in order; the L2L term goes in
@@ -1542,8 +1554,8 @@ __global__ __launch_bounds__(kTreeThreads, tree_occupancy(kKind, kCpr)) void k_t
         for (uint32_t u = 0; u < kChunksPerRound; ++u) {
           if (kKind != kTreeM2l && r0 + u == 27u) ch.reset(ch.value() + inh);
           if (kKind == kTreeM2l && !chunk_has_terms(r0 + u)) continue;
-          const float4 v0 = *reinterpret_cast<const float4*>(&terms[((u * 8u + ct) * 10u + ck) * 8u]);
-          const float4 v1 = *reinterpret_cast<const float4*>(&terms[((u * 8u + ct) * 10u + ck) * 8u + 4u]);
+          const float4 v0 = *reinterpret_cast<const float4*>(&terms[term_index(u, ck, ct, 0u)]);
+          const float4 v1 = *reinterpret_cast<const float4*>(&terms[term_index(u, ck, ct, 4u)]);
           ch.add(v0.x);
           ch.add(v0.y);
           ch.add(v0.z);

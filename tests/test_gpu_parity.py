@@ -81,6 +81,11 @@ def test_cfg1_ten_steps_bit_exact(golden):
     assert_bits(s.moments(), om, "moments")
     assert_bits(s.fields(), of, "fields")
     assert orc.fnv1a(p) == golden["cfg1"]["step_particles_digest"]
+    pinned = rf.host_array(c["n"])
+    assert s.particles(pinned) is pinned
+    assert_bits(pinned, op, "particles into a pinned buffer")
+    with pytest.raises(ValueError):
+        s.particles(np.empty((c["n"], 4), np.float32))
     s.close()
 
 
