@@ -47,6 +47,8 @@
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "rainbow_device.h"
 #include "rainbow_launch.h"
 #include "rainbow_math.h"
@@ -2280,9 +2282,12 @@ hipError_t launch_top(const StepArgs& a, hipStream_t stream, const Timing* t) {
 // keeps every block busy and none is launched only to find no work.
 template <bool kComp, uint32_t kKind, uint32_t kCpr>
 static uint32_t tree_resident_blocks(uint32_t span, uint32_t lds) {
-  static uint32_t cache[kLevels] = {};
+  // Handles may be driven from different host threads: the cache entries
+  // are atomics (every thread computes the same value).
+  static std::atomic<uint32_t> cache[kLevels] = {};
   const uint32_t key = span % kLevels;
-  if (cache[key] != 0u) return cache[key];
+  const uint32_t known = cache[key].load(std::memory_order_relaxed);
+  if (known != 0u) return known;
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
@@ -2290,8 +2295,9 @@ static uint32_t tree_resident_blocks(uint32_t span, uint32_t lds) {
           hipSuccess ||
       cus <= 0 || per_cu <= 0)
     return kTreeMaxBlocks;
-  cache[key] = static_cast<uint32_t>(cus * per_cu);
-  return cache[key];
+  const uint32_t blocks = static_cast<uint32_t>(cus * per_cu);
+  cache[key].store(blocks, std::memory_order_relaxed);
+  return blocks;
 }
 
 uint32_t tree_single_level_capacity(bool compensated, uint32_t chunks) {

@@ -814,6 +814,10 @@ hipError_t Engine::zero_finest() {
 // last_rank is set.
 int Engine::capture(uint32_t steps, uint32_t sort_every, bool last_rank, uint32_t start_parity,
                     hipGraphExec_t* out) {
+  // Descriptor changes go out before the capture: a staged copy recorded
+  // into the graph would replay whatever its staging buffer holds later.
+  int frc = flush_descriptors();
+  if (frc != RAINBOW_OK) return frc;
   hipGraph_t graph = nullptr;
   RB_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
   int rc = RAINBOW_OK;
