@@ -954,13 +954,15 @@ __device__ __forceinline__ float l2l_term(uint32_t k, float4 pa, float4 pj0, flo
 // (P = the parent's coordinates); cells outside the level read as empty.
 // Returns false when a source this thread staged has a coordinate that is
 // not finite or exceeds 2^60 in magnitude.
+// kStride threads stage the window, thread `tid` every kStride-th record.
+template <uint32_t kStride = kTreeThreads>
 __device__ __forceinline__ bool stage_window(float4* win, const SysDesc& d, uint32_t level, uint32_t px,
                                              uint32_t py, uint32_t pz, uint32_t tid) {
   bool bounded = true;
   const uint32_t dim = 1u << level;
   const int dim_i = static_cast<int>(dim);
   const gptr<float4> src = G(d.source) + level_off(level);
-  for (uint32_t w = tid; w < 216u; w += kTreeThreads) {
+  for (uint32_t w = tid; w < 216u; w += kStride) {
     const uint32_t wz = w / 36u, wy = (w / 6u) % 6u, wx = w % 6u;
     const int qx = 2 * static_cast<int>(px) - 2 + static_cast<int>(wx);
     const int qy = 2 * static_cast<int>(py) - 2 + static_cast<int>(wy);
@@ -994,11 +996,12 @@ __device__ __forceinline__ int4 children_moment(const SysDesc& d, uint32_t l, ui
 
 // stage_window with the sources' records formed from their children's
 // moments (the same integer sums, so the same records).
+template <uint32_t kStride = kTreeThreads>
 __device__ __forceinline__ bool stage_window_from_children(float4* win, const SysDesc& d, uint32_t level, uint32_t px,
                                                            uint32_t py, uint32_t pz, uint32_t tid) {
   bool bounded = true;
   const int dim_i = static_cast<int>(1u << level);
-  for (uint32_t w = tid; w < 216u; w += kTreeThreads) {
+  for (uint32_t w = tid; w < 216u; w += kStride) {
     const uint32_t wz = w / 36u, wy = (w / 6u) % 6u, wx = w % 6u;
     const int qx = 2 * static_cast<int>(px) - 2 + static_cast<int>(wx);
     const int qy = 2 * static_cast<int>(py) - 2 + static_cast<int>(wy);
@@ -1571,14 +1574,315 @@ __global__ __launch_bounds__(kTreeThreads, tree_occupancy(kKind, kCpr)) void k_t
       __syncthreads();
     }
 
-    // Store each chain's component of its (non-empty) target's field.
+    // Store each chain's component of its (non-empty) target's field; the
+    // target's count is the one lane 8 ct of the wave loaded for the pairs.
+    const int ccount = __shfl(tcount, static_cast<int>(ct << 3));
     if (chain) {
       const uint32_t x = 2u * px + (ct & 1u), y = 2u * py + ((ct >> 1) & 1u), z = 2u * pz + (ct >> 2);
       const uint32_t g = off + lin(x, y, z, dim);
-      if ((derived ? children_moment(d, level, x, y, z).x : cell_count(d, a.parity, level, lin(x, y, z, dim))) != 0) {
+      if (ccount != 0) {
         const gptr<float> f = reinterpret_cast<gptr<float>>(G(kKind == kTreeM2l ? d.m2l : d.fields) + 3u * g);
         f[ck] = ch.value();
         if (kKind != kTreeM2l && ck == 9u) {
+          f[10] = 0.0f;
+          f[11] = 0.0f;
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// tree pass with wave items: one item per wave
+// ------------------------------------------------------------------------
+
+// The unfused forms of k_tree (kTreeSingle, kTreeM2l, kTreeFinish) with an
+// item per wave instead of per block: each wave of a block walks items of
+// its own, stages the item's 6^3 source window in its own LDS, evaluates
+// one chunk of 8 x 8 pair terms at a time into its own term tile (lane:
+// target lane / 8, source lane mod 8) and then adds that chunk to the
+// item's 80 chains (lane l: target l mod 8, component l / 8; lanes 0..15
+// also component 8 + l / 8), in the reference order.  After the work table
+// no block barrier is taken, so no wave waits for another's item.  A wave's
+// LDS accesses execute in order, so a tile is rewritten only after the
+// previous chunk's reads.
+struct WaveTreeLds {
+  float4 win[216];
+  float terms[640];
+};
+
+// Orders a wave's LDS accesses across its lanes: the compiler moves no
+// memory access over it (a wave's LDS instructions then execute in program
+// order, so a lane reads what other lanes of its wave wrote before it).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+constexpr uint32_t kTreeWaves = kTreeThreads / 64u;
+constexpr uint32_t kWaveTreeOccupancy = 6u;  // blocks per CU (LDS: 4 tiles + the work table)
+
+template <bool kComp, uint32_t kKind>
+__global__ __launch_bounds__(kTreeThreads, kWaveTreeOccupancy) void k_tree_w(StepArgs a, uint32_t lo, uint32_t hi,
+                                                                              uint32_t lead) {
+  static_assert(kKind == kTreeSingle || kKind == kTreeM2l || kKind == kTreeFinish, "the unfused forms");
+  __shared__ __attribute__((aligned(16))) WaveTreeLds wl[kTreeWaves];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = tid & 63u;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+
+  // The M2L form of a one-system step can carry the top levels (lead > 0):
+  // block 0 runs the top levels' pass; the first lead - 1 waves of the
+  // blocks after it take the items of levels 2..r (r = depth - 3; parents
+  // at levels 1..r-1) directly, one possible parent each, and the other
+  // waves the listed items of levels lo..hi.
+  if constexpr (kKind == kTreeM2l) {
+    static_assert(sizeof(TopLds) <= sizeof(wl), "the top levels' LDS fits the wave tiles");
+    if (lead != 0u && blockIdx.x == 0u) {
+      top_levels<kComp, kTreeThreads>(a, 0u, *reinterpret_cast<TopLds*>(wl));
+      return;
+    }
+  }
+  const uint32_t lead_blocks = kKind == kTreeM2l && lead != 0u ? 1u : 0u;
+  const uint32_t ndirect = kKind == kTreeM2l && lead != 0u ? lead - 1u : 0u;
+  const uint32_t gw = (blockIdx.x - lead_blocks) * kTreeWaves + wave;
+  const uint32_t nwaves = (gridDim.x - lead_blocks) * kTreeWaves;
+  const bool direct = gw < ndirect;
+  const uint32_t nlisted = nwaves > ndirect ? nwaves - ndirect : 0u;
+
+  // The work table, as in k_tree: per level and stripe the inclusive prefix
+  // of the stripe counts and the stripe's list offset.
+  __shared__ uint32_t tab_incl[kLevels][kStripes];
+  __shared__ uint32_t tab_off[kLevels][kStripes];
+  if (kKind == kTreeSingle) lo = hi;
+  if (tid < (hi - lo + 1u) * kStripes) {
+    const uint32_t level = lo + tid / kStripes, j = tid % kStripes, e = (level - 1u) * kStripes + j;
+    uint32_t incl = min(G(a.list_count)[e], G(a.stripe_cap)[e]);
+    const uint32_t off = G(a.stripe_off)[e];
+#pragma unroll
+    for (uint32_t o = 1u; o < kStripes; o <<= 1) {
+      const uint32_t v = __shfl_up(incl, o, kStripes);
+      if (j >= o) incl += v;
+    }
+    tab_incl[level][j] = incl;
+    tab_off[level][j] = off;
+  }
+  __syncthreads();
+  uint32_t total = 0u;
+  for (uint32_t level = lo; level <= hi; ++level) total += tab_incl[level][kStripes - 1u];
+  auto locate = [&](uint32_t it, uint32_t& level) -> uint32_t {
+    uint32_t rem = it;
+    for (level = hi;; --level) {
+      const uint32_t tl = tab_incl[level][kStripes - 1u];
+      if (rem < tl || level == lo) break;
+      rem -= tl;
+    }
+    const bool below = lane < kStripes && tab_incl[level][lane & (kStripes - 1u)] <= rem;
+    const uint32_t j = static_cast<uint32_t>(__popcll(__ballot(below)));
+    const uint32_t base = j == 0u ? 0u : tab_incl[level][j - 1u];
+    return tab_off[level][j] + (rem - base);
+  };
+
+  WaveTreeLds& W = wl[wave];
+  // Pair lane: target tt, source cc of the chunk (as in k_tree).
+  const uint32_t tt = lane >> 3, cc = lane & 7u;
+  const uint32_t wtx = 2u + (tt & 1u), wty = 2u + ((tt >> 1) & 1u), wtz = 2u + (tt >> 2);
+  const uint32_t lane_src = (cc & 1u) + 6u * ((cc >> 1) & 1u) + 36u * (cc >> 2);
+  const uint32_t lane_tgt = wtx + 6u * (wty + 6u * wtz);
+  uint32_t near_q = 0u;
+  {
+    auto adjacent = [](uint32_t c, uint32_t w) -> uint32_t {
+      return 2u | (c == 1u && w == 2u ? 1u : 0u) | (c == 0u && w == 3u ? 4u : 0u);
+    };
+    const uint32_t mx = adjacent(cc & 1u, wtx), my = adjacent((cc >> 1) & 1u, wty), mz = adjacent(cc >> 2, wtz);
+    const uint32_t ys = (my & 1u) | ((my & 2u) << 2) | ((my & 4u) << 4);
+    const uint32_t zs = (mz & 1u) | ((mz & 2u) << 8) | ((mz & 4u) << 16);
+    near_q = (mx * ys) * zs;
+  }
+  // Chain lanes: chain A (target ct, component ka), chain B for lanes 0..15
+  // (component kb = 8 + lane / 8).
+  const uint32_t ct = lane & 7u, ka = lane >> 3;
+  const bool has_b = lane < 16u;
+  const uint32_t kb = has_b ? 8u + (lane >> 3) : 9u;
+
+  uint32_t next_level = lo, next_item = 0u;
+  uint32_t it0 = gw - ndirect, stride = nlisted, items = total;
+  if (direct) {
+    // Wave k: parent k of level 1, then k - 8 of level 2, ...
+    uint32_t k = gw, pl = 1u;
+    while (k >= (1u << (3u * pl))) {
+      k -= 1u << (3u * pl);
+      ++pl;
+    }
+    next_level = pl + 1u;
+    next_item = pack_item(0u, k);
+    it0 = 0u;
+    stride = 1u;
+    items = 1u;
+  } else if (it0 < total) {
+    next_item = G(a.lists)[locate(it0, next_level)];
+  }
+  uint32_t cur_sys = 0xFFFFFFFFu;
+  SysDesc d;
+  for (uint32_t it = it0; it < items; it += stride) {
+    const uint32_t level = __builtin_amdgcn_readfirstlane(next_level);
+    const uint32_t item = __builtin_amdgcn_readfirstlane(next_item);
+    if (!direct && it + stride < items) next_item = G(a.lists)[locate(it + stride, next_level)];
+    const uint32_t s = item >> 16;
+    if (s >= a.nsys) continue;
+    const uint32_t pidx = item & 0xFFFFu;
+    if (s != cur_sys) {
+      d = a.sys[s];
+      cur_sys = s;
+    }
+    if (direct && !d.live) continue;
+    if (kKind == kTreeM2l && level >= d.depth) continue;
+    const bool full = kKind != kTreeFinish || level == d.depth;
+    const uint32_t dim = 1u << level;
+    const uint32_t pdim = dim >> 1u;
+    const uint32_t off = level_off(level);
+    uint32_t px, py, pz;
+    unlin(pidx, level - 1u, px, py, pz);
+
+    PathSums path{};
+    if (kKind == kTreeFinish) path = path_loads(d, level, px, py, pz, lane);
+    // Chain lanes' target, and its M2L sum above the finest level.
+    const uint32_t cx = 2u * px + (ct & 1u), cy = 2u * py + ((ct >> 1) & 1u), cz = 2u * pz + (ct >> 2);
+    const uint32_t cg = off + lin(cx, cy, cz, dim);
+    float m2l_a = 0.0f, m2l_b = 0.0f;
+    if (kKind == kTreeFinish && !full) {
+      const gptr<float> m = reinterpret_cast<gptr<float>>(G(d.m2l) + 3u * cg);
+      m2l_a = m[ka];
+      m2l_b = m[kb];
+    }
+    const uint32_t tx = 2u * px + (tt & 1u), ty = 2u * py + ((tt >> 1) & 1u), tz = 2u * pz + (tt >> 2);
+    const bool derived = direct && level + 3u < d.depth;
+    const int tcount = derived ? children_moment(d, level, tx, ty, tz).x : cell_count(d, a.parity, level, lin(tx, ty, tz, dim));
+    if (direct) {
+      uint32_t parent_count = 0u;
+#pragma unroll
+      for (int l = 0; l < 64; l += 8) parent_count += static_cast<uint32_t>(__builtin_amdgcn_readlane(tcount, l));
+      if (parent_count <= static_cast<uint32_t>(kLeafCapacity)) continue;
+    }
+    wave_sync();  // the last item's window reads come first
+    const bool bounded = derived ? stage_window_from_children<64u>(W.win, d, level, px, py, pz, lane)
+                                 : stage_window<64u>(W.win, d, level, px, py, pz, lane);
+    wave_sync();
+    const float tcx = center1(tx, dim), tcy = center1(ty, dim), tcz = center1(tz, dim);
+    const bool leaf = tcount != 0 && (tcount <= kLeafCapacity || level == d.depth);
+    const bool any_leaf = __any(leaf);
+    const float cell_epsilon = (0.06f * kWorldHalfExtent) * inv_pow2(dim);
+    const float eps2_m2l = cell_epsilon * cell_epsilon;
+    const float near_epsilon = fmaxf(d.softening, (0.72f * kWorldHalfExtent) * inv_pow2(dim));
+    const float eps2_near = near_epsilon * near_epsilon;
+    const bool fast = __all(bounded) && eps2_m2l >= 0x1p-94f && eps2_near >= 0x1p-94f && eps2_m2l <= 0x1p100f &&
+                      eps2_near <= 0x1p100f;
+
+    // L2L: each chain's component of the parent's field shifted to its
+    // target's centre (the parent's field from the ancestors' M2L sums in
+    // the finish form, from memory in the single-level form).
+    float inh_a = 0.0f, inh_b = 0.0f;
+    if (kKind != kTreeM2l) {
+      float4 pa, pj0, pj1;
+      if (kKind == kTreeFinish) {
+        const float f = path_field(path, level, px, py, pz, lane);
+        auto at = [&](int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(f), l)); };
+        pa = make_float4(at(0), at(1), at(2), at(3));
+        pj0 = make_float4(at(4), at(5), at(6), at(7));
+        pj1 = make_float4(at(8), at(9), 0.0f, 0.0f);
+      } else {
+        load_field(G(d.fields), level_off(level - 1u) + pidx, pa, pj0, pj1);
+      }
+      const float dx = center1(cx, dim) - center1(px, pdim);
+      const float dy = center1(cy, dim) - center1(py, pdim);
+      const float dz = center1(cz, dim) - center1(pz, pdim);
+      inh_a = l2l_term(ka, pa, pj0, pj1, dx, dy, dz);
+      inh_b = l2l_term(kb, pa, pj0, pj1, dx, dy, dz);
+    }
+    Chain<kComp> cha, chb;
+    if (kKind == kTreeFinish && !full) {
+      cha.reset(m2l_a);
+      chb.reset(m2l_b);
+    }
+    bool inh_added = kKind == kTreeM2l;
+    // Chunks 0..26: source-parent blocks (pz, py, px order), the parent's
+    // own block (13) without a pair; 27..30: the near offsets of leaf
+    // targets.  Chunks without a pair are skipped (their terms are zero).
+#pragma unroll 1
+    for (uint32_t q = full ? 0u : 27u; q < 31u; ++q) {
+      if (q < 27u ? q == 13u : (kKind == kTreeM2l || !any_leaf)) continue;
+      if (!inh_added && q >= 27u) {
+        cha.reset(cha.value() + inh_a);
+        chb.reset(chb.value() + inh_b);
+        inh_added = true;
+      }
+      uint32_t widx = 0u;
+      bool active = false;
+      float eps2 = eps2_m2l;
+      if (q < 27u) {
+        widx = 2u * (q % 3u) + 12u * ((q / 3u) % 3u) + 72u * (q / 9u) + lane_src;
+        active = tcount != 0 && ((near_q >> q) & 1u) == 0u;
+      } else {
+        const uint32_t idx = (q - 27u) * 8u + cc;
+        if (idx < 27u) {
+          widx = lane_tgt + (idx % 3u + 6u * ((idx / 3u) % 3u) + 36u * (idx / 9u)) - 43u;
+          active = leaf && idx != 13u;
+        }
+        eps2 = eps2_near;
+      }
+      float t[10];
+      if (fast) {
+        float4 sr = W.win[widx];
+        if (!active) sr.w = 0.0f;
+        pair_terms<true>(sr, tcx, tcy, tcz, eps2, t);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 10; ++k) t[k] = 0.0f;
+        const float4 sr = W.win[widx];
+        if (active && sr.w != 0.0f) pair_terms<false>(sr, tcx, tcy, tcz, eps2, t);
+      }
+      wave_sync();  // the last chunk's term reads come first
+#pragma unroll
+      for (uint32_t k = 0; k < 10u; ++k) W.terms[term_index(0u, k, tt, cc)] = t[k];
+      wave_sync();
+      {
+        const float4 v0 = *reinterpret_cast<const float4*>(&W.terms[term_index(0u, ka, ct, 0u)]);
+        const float4 v1 = *reinterpret_cast<const float4*>(&W.terms[term_index(0u, ka, ct, 4u)]);
+        cha.add(v0.x);
+        cha.add(v0.y);
+        cha.add(v0.z);
+        cha.add(v0.w);
+        cha.add(v1.x);
+        cha.add(v1.y);
+        cha.add(v1.z);
+        cha.add(v1.w);
+      }
+      if (has_b) {
+        const float4 v0 = *reinterpret_cast<const float4*>(&W.terms[term_index(0u, kb, ct, 0u)]);
+        const float4 v1 = *reinterpret_cast<const float4*>(&W.terms[term_index(0u, kb, ct, 4u)]);
+        chb.add(v0.x);
+        chb.add(v0.y);
+        chb.add(v0.z);
+        chb.add(v0.w);
+        chb.add(v1.x);
+        chb.add(v1.y);
+        chb.add(v1.z);
+        chb.add(v1.w);
+      }
+    }
+    if (!inh_added) {
+      cha.reset(cha.value() + inh_a);
+      chb.reset(chb.value() + inh_b);
+    }
+    // Store each chain's component of its (non-empty) target's field; the
+    // target's count is the one lane 8 ct loaded for the pairs.
+    const int ccount = __shfl(tcount, static_cast<int>(ct << 3));
+    if (ccount != 0) {
+      const gptr<float> f = reinterpret_cast<gptr<float>>(G(kKind == kTreeM2l ? d.m2l : d.fields) + 3u * cg);
+      f[ka] = cha.value();
+      if (has_b) {
+        f[kb] = chb.value();
+        if (kKind != kTreeM2l && kb == 9u) {
           f[10] = 0.0f;
           f[11] = 0.0f;
         }
@@ -2300,7 +2604,28 @@ static uint32_t tree_resident_blocks(uint32_t span, uint32_t lds) {
   return blocks;
 }
 
+// Resident blocks of the wave-item tree kernel k_tree_w<kComp, kKind>.
+template <bool kComp, uint32_t kKind>
+static uint32_t tree_w_resident_blocks() {
+  static std::atomic<uint32_t> cache{0u};
+  const uint32_t known = cache.load(std::memory_order_relaxed);
+  if (known != 0u) return known;
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_tree_w<kComp, kKind>, kTreeThreads, 0) != hipSuccess ||
+      cus <= 0 || per_cu <= 0)
+    return kTreeMaxBlocks;
+  const uint32_t blocks = static_cast<uint32_t>(cus * per_cu);
+  cache.store(blocks, std::memory_order_relaxed);
+  return blocks;
+}
+
 uint32_t tree_single_level_capacity(bool compensated, uint32_t chunks) {
+  // Wave items: a block's four waves each take an item.
+  if (chunks == 1u)
+    return kTreeWaves * (compensated ? tree_w_resident_blocks<true, kTreeSingle>()
+                                     : tree_w_resident_blocks<false, kTreeSingle>());
   if (chunks == 4u)
     return compensated ? tree_resident_blocks<true, kTreeSingle, 4u>(0u, 0u)
                        : tree_resident_blocks<false, kTreeSingle, 4u>(0u, 0u);
@@ -2317,6 +2642,29 @@ static hipError_t launch_tree_form(const StepArgs& a, uint32_t lo, uint32_t hi, 
   else
     RB_LAUNCH(t, (k_tree<kComp, kKind, kCpr>), dim3(nblocks), dim3(kTreeThreads), stream, a, lo, hi, lead);
   return hipGetLastError();
+}
+
+// Wave items: `items` items (an upper bound on the launch's parents) on at
+// most the resident waves, after the lead block and direct waves.
+template <bool kComp, uint32_t kKind>
+static hipError_t launch_tree_wave_form(const StepArgs& a, uint32_t lo, uint32_t hi, uint32_t items, uint32_t lead,
+                                        hipStream_t stream, const Timing* t) {
+  const uint32_t resident_waves = kTreeWaves * tree_w_resident_blocks<kComp, kKind>();
+  const uint32_t ndirect = lead != 0u ? lead - 1u : 0u;
+  const uint32_t listed = std::max(1u, std::min(items, resident_waves));
+  const uint32_t nblocks = (lead != 0u ? 1u : 0u) + (ndirect + listed + kTreeWaves - 1u) / kTreeWaves;
+  RB_LAUNCH(t, (k_tree_w<kComp, kKind>), dim3(nblocks), dim3(kTreeThreads), stream, a, lo, hi, lead);
+  return hipGetLastError();
+}
+
+template <bool kComp>
+static hipError_t launch_tree_wave(const StepArgs& a, uint32_t kind, uint32_t lo, uint32_t hi, uint32_t items,
+                                   uint32_t lead, hipStream_t stream, const Timing* t) {
+  switch (kind) {
+    case kTreePassM2l: return launch_tree_wave_form<kComp, kTreeM2l>(a, lo, hi, items, lead, stream, t);
+    case kTreePassFinish: return launch_tree_wave_form<kComp, kTreeFinish>(a, lo, hi, items, 0u, stream, t);
+    default: return launch_tree_wave_form<kComp, kTreeSingle>(a, lo, hi, items, 0u, stream, t);
+  }
 }
 
 template <bool kComp, uint32_t kCpr>
@@ -2337,7 +2685,13 @@ hipError_t launch_tree(const StepArgs& a, uint32_t kind, uint32_t lo, uint32_t h
                        uint32_t chunks, hipStream_t stream, const Timing* t) {
   if ((nblocks == 0u && lead == 0u) || lo < 2u || hi < lo || hi >= kLevels) return no_launch(stream, t);
   if (lead != 0u && (kind != kTreePassM2l || a.nsys != 1u)) return hipErrorInvalidValue;
-  if (chunks != 4u && chunks != 8u) return hipErrorInvalidValue;
+  if (chunks != 1u && chunks != 4u && chunks != 8u) return hipErrorInvalidValue;
+  // Wave items serve the unfused forms; a fused run of levels (which
+  // recomputes the ancestor path with the whole block) takes the block form.
+  if (chunks == 1u && (kind != kTreePassRun || lo == hi))
+    return a.compensated ? launch_tree_wave<true>(a, kind, lo, hi, nblocks, lead, stream, t)
+                         : launch_tree_wave<false>(a, kind, lo, hi, nblocks, lead, stream, t);
+  if (chunks == 1u) chunks = 8u;
   if (chunks == 4u)
     return a.compensated ? launch_tree_comp<true, 4u>(a, kind, lo, hi, nblocks, lead, stream, t)
                          : launch_tree_comp<false, 4u>(a, kind, lo, hi, nblocks, lead, stream, t);

@@ -33,15 +33,27 @@ constexpr uint32_t kTreePassRun = 0u, kTreePassM2l = 1u, kTreePassFinish = 2u;
 //                      or 512 (two per thread);
 //   tree_chunks        the tree items' pair tile: chunks of 8 targets x 8
 //                      sources evaluated per round, 8 (two per wave, 6
-//                      blocks per CU) or 4 (one per wave, half the terms'
-//                      LDS, 8 blocks per CU).
+//                      blocks per CU), 4 (one per wave, half the terms'
+//                      LDS, 8 blocks per CU) or 1 (wave items: each wave
+//                      walks items of its own, one chunk at a time, with no
+//                      block barrier; unfused launches only, a fused run
+//                      of levels takes 8), or 0: per launch, wave items for
+//                      a launch of at least kWaveItemsMin items (measured
+//                      in the last recorded step), 8 otherwise.
 struct Shape {
   uint32_t integrate_threads = 1024u;
   uint32_t tree_chunks = 8u;
 };
 inline bool shape_valid(const Shape& s) {
-  return (s.integrate_threads == 1024u || s.integrate_threads == 512u) && (s.tree_chunks == 8u || s.tree_chunks == 4u);
+  return (s.integrate_threads == 1024u || s.integrate_threads == 512u) &&
+         (s.tree_chunks == 8u || s.tree_chunks == 4u || s.tree_chunks == 1u || s.tree_chunks == 0u);
 }
+// Wave items pay off once a launch holds many rounds of resident blocks'
+// items (on MI355X: batches of 262144-body systems run faster with block
+// items up to 32 systems, about 20k items per step, and the 256-system
+// batch of 18.6M bodies faster with wave items on its levels of 12.7k and
+// 22.4k items).
+constexpr uint32_t kWaveItemsMin = 8192u;
 // Blocks of a single-level tree launch resident on the device at once.
 uint32_t tree_single_level_capacity(bool compensated, uint32_t chunks);
 // lead > 0 (kTreePassM2l, one system): the launch also runs the top

@@ -45,8 +45,10 @@ constexpr uint32_t kDefaultTreeSplit = (1u << 2) | (1u << 6);
 
 // One launch of the tree pass: form (kTreePass*) and levels lo..hi; lead
 // > 0: the launch also runs the top levels (see launch_tree).
+// chunks: the launch's pair tile (Shape::tree_chunks, resolved per launch
+// when that is 0).
 struct TreeLaunch {
-  uint32_t kind, lo, hi, blocks, lead;
+  uint32_t kind, lo, hi, blocks, lead, chunks;
 };
 
 // One device allocation holding a system's bodies and cells, and its
@@ -156,6 +158,9 @@ class Engine {
   uint32_t n_subtree = 0;
   uint32_t n_fresh = 0;  // deposit blocks of systems that joined and await their first deposit
   uint32_t tree_blocks[kLevels + 1] = {};
+  // Items (internal parents) per level in the last measured step, 0 before
+  // any measurement; they choose each launch's tile under tree_chunks 0.
+  uint32_t level_items[kLevels + 1] = {};
   std::vector<TreeLaunch> tree_launches;
   uint32_t tree_split = kDefaultTreeSplit;
   Shape shape;

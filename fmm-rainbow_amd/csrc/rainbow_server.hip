@@ -589,6 +589,10 @@ int rainbow_server_create(uint64_t budget_bytes, uint32_t max_slots, const rainb
   s->budget = budget_bytes;
   s->max_slots = max_slots;
   int rc = s->engine.open(options);
+  // Tree tiles chosen per launch from the measured items: a full served
+  // set is throughput-bound (many systems' items per launch), where wave
+  // items run faster than block items; a few resident systems are not.
+  if (rc == RAINBOW_OK) rc = s->engine.set_shape(rainbow::Shape{1024u, 0u});
   if (rc == RAINBOW_OK && hipStreamCreateWithFlags(&s->copy, hipStreamNonBlocking) != hipSuccess)
     rc = fail(RAINBOW_E_HIP, "copy stream creation failed");
   if (rc == RAINBOW_OK) rc = s->engine.configure(std::vector<rainbow::SysDesc>());

@@ -494,7 +494,8 @@ class Solver:
     def set_shape(self, integrate_threads: int = 1024, tree_chunks: int = 8):
         """Block shapes: integrate+deposit block width (1024 or 512 threads
         over a 1024-body range) and the tree's pair tile (8 or 4 chunks of
-        8 x 8 pairs per round)."""
+        8 x 8 pairs per round; 1: an item per wave; 0: chosen per launch
+        from the measured items)."""
         _check(lib().rainbow_hip_set_shape(self._h, integrate_threads, tree_chunks))
 
     @property

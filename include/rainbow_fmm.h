@@ -304,9 +304,14 @@ int rainbow_hip_tree_launches(rainbow_hip_t* h, uint32_t* lo_hi, uint32_t n);
  * integrate+deposit block over its 1024-body deposit range, 1024 (one body
  * per thread, the default) or 512 (two per thread).  tree_chunks: the tree
  * items' pair tile, chunks of 8 targets x 8 sources evaluated per round,
- * 8 (the default; two per wave, 6 blocks per CU) or 4 (one per wave, half
- * the LDS of terms, 8 blocks per CU).  Results do not depend on the shape;
- * RAINBOW_E_INVALID for other values. */
+ * 8 (the default; two per wave, 6 blocks per CU), 4 (one per wave, half
+ * the LDS of terms, 8 blocks per CU), 1 (wave items: each wave walks
+ * items of its own, one chunk at a time, without block barriers; faster
+ * when many items share a launch, as in large batches; fused runs of
+ * levels keep 8) or 0 (per launch: wave items for a launch that held at
+ * least 8192 items in the last measured step, 8 otherwise; the server's
+ * shape).  Results do not depend on the shape; RAINBOW_E_INVALID for other
+ * values. */
 int rainbow_hip_set_shape(rainbow_hip_t* h, uint32_t integrate_threads, uint32_t tree_chunks);
 int rainbow_hip_get_shape(rainbow_hip_t* h, uint32_t* integrate_threads, uint32_t* tree_chunks);
 

@@ -380,7 +380,7 @@ def test_autotune_restores_state():
     s.close()
 
 
-SHAPES = [(1024, 8), (512, 8), (1024, 4), (512, 4)]
+SHAPES = [(1024, 8), (512, 8), (1024, 4), (512, 4), (1024, 1), (512, 1), (1024, 0)]
 
 
 @pytest.mark.parametrize("threads,chunks", SHAPES)
@@ -406,10 +406,11 @@ def test_block_shapes_bit_exact(threads, chunks, split):
     s.close()
 
 
-@pytest.mark.parametrize("threads,chunks", [(512, 4), (1024, 4)])
+@pytest.mark.parametrize("threads,chunks", [(512, 4), (1024, 4), (1024, 1)])
 def test_block_shapes_compensated_and_fused(threads, chunks):
     """The compensated form and a fused run of levels (2..5 with ancestor
-    recomputation, 6 alone) under the smaller tiles."""
+    recomputation, 6 alone) under the smaller tiles, and under wave items
+    (the fused run then takes block items, level 6 wave items)."""
     say(f"shape {threads}x{chunks} compensated, fused 2-5")
     n = 120000
     parts = rf.ic_plummer(n, 96, 0.1, 1.5, 1.0)
@@ -425,16 +426,17 @@ def test_block_shapes_compensated_and_fused(threads, chunks):
     s.close()
 
 
-def test_batch_shapes_equal_solo():
-    """A batch of uneven systems under the 512 x 4 shape equals each system
-    stepped alone under the default shape."""
-    say("batch shape 512x4 = solo")
+@pytest.mark.parametrize("threads,chunks", [(512, 4), (1024, 1)])
+def test_batch_shapes_equal_solo(threads, chunks):
+    """A batch of uneven systems under a smaller tile or wave items equals
+    each system stepped alone under the default shape."""
+    say(f"batch shape {threads}x{chunks} = solo")
     counts = [4096, 23000, 70000, 131072, 9000]
     prm = rf.physics(dt=5e-4, gravity=1.0, softening=1e-3)
     inputs = [rf.ic_plummer(n, 700 + i, 0.1, 1.5, 1.0) for i, n in enumerate(counts)]
     b = rf.Batch(counts, prm)
-    b.set_shape(512, 4)
-    assert b.shape == (512, 4)
+    b.set_shape(threads, chunks)
+    assert b.shape == (threads, chunks)
     for i, p in enumerate(inputs):
         b.upload(i, p)
     b.step(10)
@@ -450,7 +452,7 @@ def test_batch_shapes_equal_solo():
 def test_shape_rejects_other_values():
     say("shape validation")
     s = rf.Solver(5000, rf.physics(dt=5e-4, gravity=1.0, softening=1e-3))
-    for bad in [(256, 8), (1024, 16), (2048, 4), (512, 0)]:
+    for bad in [(256, 8), (1024, 16), (2048, 4), (512, 3), (1024, 2)]:
         with pytest.raises(rf.RainbowError) as e:
             s.set_shape(*bad)
         assert e.value.code == rf.E_INVALID
