@@ -1246,6 +1246,143 @@ __device__ __forceinline__ float path_field(const PathSums& p, uint32_t level, u
   return f;
 }
 
+// Orders a wave's LDS accesses across its lanes: the compiler moves no
+// memory access over it (a wave's LDS instructions then execute in program
+// order, so a lane reads what other lanes of its wave wrote before it).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// LDS of a finish item above its system's finest level run by one wave: the
+// 4^3 cells around its targets (their near sources) and one chunk's terms.
+struct ShortWaveLds {
+  float4 win[64];
+  float terms[640];
+};
+
+// A finish item above its system's finest level (every target's M2L sum is
+// in the m2l buffer), by one wave: the parent's field from the level-1
+// field and the ancestors' M2L sums, then per target its M2L sum + the L2L
+// term + the near field of a leaf target, the same operations and order as
+// k_tree's finish form (near sources dz, dy, dx; the centre skipped).
+// Chain lanes as in k_tree_w: lane l sums component l / 8 of target l mod
+// 8, lanes 0..15 also component 8 + l / 8.
+template <bool kComp>
+__device__ void finish_short_wave(const StepArgs& a, const SysDesc& d, uint32_t level, uint32_t pidx, uint32_t lane,
+                                  ShortWaveLds& W) {
+  const uint32_t dim = 1u << level, pdim = dim >> 1u, off = level_off(level);
+  uint32_t px, py, pz;
+  unlin(pidx, level - 1u, px, py, pz);
+  const PathSums path = path_loads(d, level, px, py, pz, lane);
+  const uint32_t ct = lane & 7u, ka = lane >> 3;
+  const bool has_b = lane < 16u;
+  const uint32_t kb = has_b ? 8u + (lane >> 3) : 9u;
+  const uint32_t cx = 2u * px + (ct & 1u), cy = 2u * py + ((ct >> 1) & 1u), cz = 2u * pz + (ct >> 2);
+  const uint32_t cg = off + lin(cx, cy, cz, dim);
+  const gptr<float> msum = reinterpret_cast<gptr<float>>(G(d.m2l) + 3u * cg);
+  const float m2l_a = msum[ka], m2l_b = msum[kb];
+  const uint32_t tt = lane >> 3, cc = lane & 7u;
+  const uint32_t tx = 2u * px + (tt & 1u), ty = 2u * py + ((tt >> 1) & 1u), tz = 2u * pz + (tt >> 2);
+  const int tcount = cell_count(d, a.parity, level, lin(tx, ty, tz, dim));
+  // Near window: cell (2P - 1 + (lane & 3), 2P - 1 + (lane >> 2 & 3),
+  // 2P - 1 + (lane >> 4)); cells outside the level are empty.
+  bool bounded = true;
+  {
+    const int dim_i = static_cast<int>(dim);
+    const int qx = 2 * static_cast<int>(px) - 1 + static_cast<int>(lane & 3u);
+    const int qy = 2 * static_cast<int>(py) - 1 + static_cast<int>((lane >> 2) & 3u);
+    const int qz = 2 * static_cast<int>(pz) - 1 + static_cast<int>(lane >> 4);
+    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (qx >= 0 && qy >= 0 && qz >= 0 && qx < dim_i && qy < dim_i && qz < dim_i)
+      v = G(d.source)[off + lin(static_cast<uint32_t>(qx), static_cast<uint32_t>(qy), static_cast<uint32_t>(qz), dim)];
+    wave_sync();  // the last item's window reads come first
+    W.win[lane] = v;
+    bounded = fabsf(v.x) <= 0x1p60f && fabsf(v.y) <= 0x1p60f && fabsf(v.z) <= 0x1p60f;
+  }
+  // The chains start from the M2L sum + the L2L term of the parent's field.
+  const float f = path_field(path, level, px, py, pz, lane);
+  auto at = [&](int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(f), l)); };
+  const float4 pa = make_float4(at(0), at(1), at(2), at(3));
+  const float4 pj0 = make_float4(at(4), at(5), at(6), at(7));
+  const float4 pj1 = make_float4(at(8), at(9), 0.0f, 0.0f);
+  const float dx = center1(cx, dim) - center1(px, pdim);
+  const float dy = center1(cy, dim) - center1(py, pdim);
+  const float dz = center1(cz, dim) - center1(pz, pdim);
+  Chain<kComp> cha, chb;
+  cha.reset(m2l_a + l2l_term(ka, pa, pj0, pj1, dx, dy, dz));
+  chb.reset(m2l_b + l2l_term(kb, pa, pj0, pj1, dx, dy, dz));
+  const bool leaf = tcount != 0 && tcount <= kLeafCapacity;
+  if (__any(leaf)) {
+    const float near_epsilon = fmaxf(d.softening, (0.72f * kWorldHalfExtent) * inv_pow2(dim));
+    const float eps2 = near_epsilon * near_epsilon;
+    const bool fast = __all(bounded) && eps2 >= 0x1p-94f && eps2 <= 0x1p100f;
+    const float tcx = center1(tx, dim), tcy = center1(ty, dim), tcz = center1(tz, dim);
+    wave_sync();
+#pragma unroll 1
+    for (uint32_t q = 0; q < 4u; ++q) {
+      const uint32_t idx = q * 8u + cc;
+      uint32_t widx = 0u;
+      bool active = false;
+      if (idx < 27u) {
+        widx = ((tt & 1u) + idx % 3u) + 4u * (((tt >> 1) & 1u) + (idx / 3u) % 3u) + 16u * ((tt >> 2) + idx / 9u);
+        active = leaf && idx != 13u;
+      }
+      float t[10];
+      if (fast) {
+        float4 sr = W.win[widx];
+        if (!active) sr.w = 0.0f;
+        pair_terms<true>(sr, tcx, tcy, tcz, eps2, t);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 10; ++k) t[k] = 0.0f;
+        const float4 sr = W.win[widx];
+        if (active && sr.w != 0.0f) pair_terms<false>(sr, tcx, tcy, tcz, eps2, t);
+      }
+      wave_sync();  // the last chunk's term reads come first
+#pragma unroll
+      for (uint32_t k = 0; k < 10u; ++k) W.terms[term_index(0u, k, tt, cc)] = t[k];
+      wave_sync();
+      {
+        const float4 v0 = *reinterpret_cast<const float4*>(&W.terms[term_index(0u, ka, ct, 0u)]);
+        const float4 v1 = *reinterpret_cast<const float4*>(&W.terms[term_index(0u, ka, ct, 4u)]);
+        cha.add(v0.x);
+        cha.add(v0.y);
+        cha.add(v0.z);
+        cha.add(v0.w);
+        cha.add(v1.x);
+        cha.add(v1.y);
+        cha.add(v1.z);
+        cha.add(v1.w);
+      }
+      if (has_b) {
+        const float4 v0 = *reinterpret_cast<const float4*>(&W.terms[term_index(0u, kb, ct, 0u)]);
+        const float4 v1 = *reinterpret_cast<const float4*>(&W.terms[term_index(0u, kb, ct, 4u)]);
+        chb.add(v0.x);
+        chb.add(v0.y);
+        chb.add(v0.z);
+        chb.add(v0.w);
+        chb.add(v1.x);
+        chb.add(v1.y);
+        chb.add(v1.z);
+        chb.add(v1.w);
+      }
+    }
+  }
+  if (__shfl(tcount, static_cast<int>(ct << 3)) != 0) {
+    const gptr<float> fo = reinterpret_cast<gptr<float>>(G(d.fields) + 3u * cg);
+    fo[ka] = cha.value();
+    if (has_b) {
+      fo[kb] = chb.value();
+      if (kb == 9u) {
+        fo[10] = 0.0f;
+        fo[11] = 0.0f;
+      }
+    }
+  }
+}
+
 // kCpr: chunks of 8 x 8 pairs per round (the pair tile).  Occupancy: with
 // 8 chunks per round 6 waves per SIMD for the unfused forms (their LDS
 // allows 6 blocks per CU), 5 for fused launches, which carry the ancestor
@@ -1369,6 +1506,35 @@ __global__ __launch_bounds__(kTreeThreads, tree_occupancy(kKind, kCpr)) void k_t
   // is reloaded only when the system changes.
   uint32_t next_level = lo, next_item = 0u;
   uint32_t it0 = first_block, stride = nlisted, items = total;
+  if constexpr (kKind == kTreeFinish && kTermFloats * 4u >= kWaves * sizeof(ShortWaveLds)) {
+    // One system: the items of the highest level (its finest) are long (M2L
+    // and near) and taken one per block; the blocks after them take the
+    // items above the finest level (L2L and near only) four at a time, one
+    // per wave, so that the launch needs about one round of resident blocks.
+    if (a.nsys == 1u && hi > lo && gridDim.x >= 2u) {
+      const uint32_t nlong = tab_incl[hi][kStripes - 1u];
+      const uint32_t ngroups = (total - nlong + kWaves - 1u) / kWaves;
+      const uint32_t lblocks = ngroups == 0u ? gridDim.x : min(nlong, gridDim.x - 1u);
+      if (blockIdx.x >= lblocks) {
+        const SysDesc d0 = a.sys[0];
+        if (!d0.live || d0.depth != hi) return;
+        ShortWaveLds& W = reinterpret_cast<ShortWaveLds*>(terms_s)[wave_s];
+        const uint32_t sblocks = gridDim.x - lblocks;
+        for (uint32_t g = blockIdx.x - lblocks; g < ngroups; g += sblocks) {
+          const uint32_t it = nlong + g * kWaves + wave_s;
+          if (it >= total) continue;
+          uint32_t level = lo;
+          const uint32_t item = __builtin_amdgcn_readfirstlane(G(a.lists)[locate(it, level)]);
+          level = __builtin_amdgcn_readfirstlane(level);
+          if ((item >> 16) != 0u) continue;
+          finish_short_wave<kComp>(a, d0, level, item & 0xFFFFu, lane, W);
+        }
+        return;
+      }
+      stride = lblocks;
+      items = nlong;
+    }
+  }
   if (direct) {
     // Block 1 + k: parent k of level 1, then k - 8 of level 2, ...
     uint32_t k = blockIdx.x - 1u, pl = 1u;
@@ -1611,14 +1777,6 @@ struct WaveTreeLds {
   float terms[640];
 };
 
-// Orders a wave's LDS accesses across its lanes: the compiler moves no
-// memory access over it (a wave's LDS instructions then execute in program
-// order, so a lane reads what other lanes of its wave wrote before it).
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 constexpr uint32_t kTreeWaves = kTreeThreads / 64u;
 constexpr uint32_t kWaveTreeOccupancy = 6u;  // blocks per CU (LDS: 4 tiles + the work table)
 
