@@ -211,6 +211,10 @@ def batch_leg(args):
     b.step(args.warmup)
     b.sync()
     tuned = None
+    if args.batch_shape:
+        # A fixed block shape (integrate threads, tree chunks) instead of the
+        # autotuner's pick, e.g. for profiling the tuned step alone.
+        b.set_shape(*(int(x) for x in args.batch_shape.split(",")))
     if args.batch_tune_steps > 0:
         # Block shapes, tree split and sort interval picked by the autotuner
         # (the bodies are restored afterwards).
@@ -499,6 +503,7 @@ def main():
     ap.add_argument("--systems", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=100, help="timed steps of the batched leg (the configured 100)")
     ap.add_argument("--batch-no-sequential", action="store_true", help="time the batched leg only (profiling)")
+    ap.add_argument("--batch-shape", default="", help="fixed batch block shape 'threads,chunks' (with --batch-tune-steps 0)")
     ap.add_argument("--batch-tune-steps", type=int, default=100,
                     help="steps per candidate of the batch's autotuner, the horizon it tunes for (0: default "
                          "shapes and measured split)")
