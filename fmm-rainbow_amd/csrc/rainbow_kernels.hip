@@ -1511,10 +1511,15 @@ __global__ __launch_bounds__(kTreeThreads, tree_occupancy(kKind, kCpr)) void k_t
     // and near) and taken one per block; the blocks after them take the
     // items above the finest level (L2L and near only) four at a time, one
     // per wave, so that the launch needs about one round of resident blocks.
+    // When the items need more than one round, the blocks are split in
+    // proportion to the work (a long item about three groups of four short
+    // ones), the long items still at most one per block.
     if (a.nsys == 1u && hi > lo && gridDim.x >= 2u) {
       const uint32_t nlong = tab_incl[hi][kStripes - 1u];
       const uint32_t ngroups = (total - nlong + kWaves - 1u) / kWaves;
-      const uint32_t lblocks = ngroups == 0u ? gridDim.x : min(nlong, gridDim.x - 1u);
+      const uint64_t wlong = 3ull * nlong;
+      const uint32_t share = static_cast<uint32_t>(uint64_t{gridDim.x} * wlong / (wlong + ngroups + 1ull));
+      const uint32_t lblocks = ngroups == 0u ? gridDim.x : min(nlong, min(max(share, 1u), gridDim.x - 1u));
       if (blockIdx.x >= lblocks) {
         const SysDesc d0 = a.sys[0];
         if (!d0.live || d0.depth != hi) return;

@@ -635,6 +635,25 @@ def test_two_launch_tree_ball_and_uneven_batch(compensated):
     b.close()
 
 
+def test_finish_launch_beyond_one_round():
+    """A 2,097,152-body uniform ball: about 4,200 finest-level items, more
+    than the finish launch's resident blocks, so the launch splits its
+    blocks between the long items (several per block) and the groups of
+    short items (several per block); solve + 2 steps equal the oracle."""
+    say("two-launch tree beyond one round of blocks, uniform ball 2M")
+    n = 2097152
+    parts = rf.ic_uniform_ball(n, 21, 1.0)
+    dt, g, soft = 5e-4, 1.0, 1e-3
+    s = rf.Solver(n, rf.physics(dt=dt, gravity=g, softening=soft))
+    s.set_tree_split(1)
+    s.upload(parts)
+    s.step(2)
+    op, om, of = orc.step(slots_for(n, dt, g, soft), parts, 2)
+    assert_bits(s.particles(), op, "particles")
+    assert_bits(s.fields(), of, "fields")
+    s.close()
+
+
 @pytest.mark.parametrize("compensated", [False, True])
 def test_tree_fusion_ball_all_levels(compensated):
     say(f"tree fusion 2..6, uniform ball 400k, compensated={compensated}")
