@@ -344,6 +344,17 @@ __device__ void deposit_block(const StepArgs& a, const SysDesc& d, uint32_t firs
   }
 }
 
+// Workgroups are dealt to the 8 XCDs in turn (workgroup b to XCD b mod 8),
+// each XCD with its own L2.  Giving XCD x a contiguous eighth of the body
+// tiles (tiles are in Morton order within each system) keeps the tree
+// cells its tiles read, and the finest cells they deposit into, in its L2.
+// A bijection of [0, n): the tiles of XCD x start at x q + min(x, r),
+// q = n / 8, r = n mod 8.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
+  const uint32_t x = b & 7u, k = b >> 3, q = n >> 3, r = n & 7u;
+  return x * q + min(x, r) + k;
+}
+
 // Zeroes the work-list counters of the step whose upward follows (their
 // last reader, the previous tree pass, has finished).
 __device__ __forceinline__ void reset_lists(const StepArgs& a) {
@@ -356,7 +367,7 @@ __device__ __forceinline__ void reset_lists(const StepArgs& a) {
 __global__ __launch_bounds__(kDepositThreads) void k_deposit(StepArgs a) {
   __shared__ DepositLds L;
   reset_lists(a);
-  const BodyBlock blk = a.deposit_blocks[blockIdx.x];
+  const BodyBlock blk = a.deposit_blocks[xcd_tile(blockIdx.x, gridDim.x)];
   const SysDesc d = a.sys[blk.sys];
   if (!d.live) return;
   float4 p[kDepositBodies / kDepositThreads];
@@ -392,7 +403,7 @@ __device__ __forceinline__ uint32_t compact3(uint32_t v) {
 // systems that joined (1024 each) when only they sort.
 template <uint32_t kThreads>
 __global__ __launch_bounds__(kThreads) void k_scatter(StepArgs a, const BodyBlock* blocks) {
-  const BodyBlock blk = blocks[blockIdx.x];
+  const BodyBlock blk = blocks[xcd_tile(blockIdx.x, gridDim.x)];
   const SysDesc d = a.sys[blk.sys];
   if (!d.live || !sorts(a, d)) return;
   const uint32_t i = blk.first + threadIdx.x;
@@ -594,7 +605,8 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
   int4* const lvl1 = lv + 512;
   int4* const lvl2 = lv + 576;
   int4* const lvl3 = lv + 584;
-  const SubtreeBlock blk = a.subtree_blocks[blockIdx.x];
+  const uint32_t sb = xcd_tile(blockIdx.x, gridDim.x);
+  const SubtreeBlock blk = a.subtree_blocks[sb];
   const SysDesc d = a.sys[blk.sys];
   if (!d.live) return;
   const uint32_t t = threadIdx.x;
@@ -604,7 +616,7 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
   unlin(blk.root, r, rx, ry, rz);
   const gptr<int4> M = G(d.moments);
   if (t < 4u) ps.n[t] = 0u;
-  const uint32_t stripe = blockIdx.x % kStripes;
+  const uint32_t stripe = sb % kStripes;  // the stripe the table gave this subtree
   const uint32_t lev_of[4] = {D - 1u, D - 2u, r, 0u};
   prefetch_stripes(a, ps, lev_of, 3u, stripe, t);
 
@@ -2298,7 +2310,7 @@ __global__ __launch_bounds__(kThreads) void k_integrate_deposit(StepArgs a) {
   __shared__ DepositLds L;
   __shared__ uint2 own_new;
   reset_lists(a);
-  const BodyBlock blk = a.deposit_blocks[blockIdx.x];
+  const BodyBlock blk = a.deposit_blocks[xcd_tile(blockIdx.x, gridDim.x)];
   const SysDesc d = a.sys[kOne ? 0u : blk.sys];
   if (!d.live) return;
   const bool staged = sorts(a, d);

@@ -334,7 +334,15 @@ void Engine::add_tables(uint32_t s) {
   const uint32_t ib = d.n / (kAdaptiveLeafCapacity + 1u);
   for (uint32_t f = 0; f < d.n; f += kDepositBodies) dep_.push_back(BodyBlock{s, f});
   for (uint32_t f = 0; f < d.n; f += kIntegrateThreads) integ_.push_back(BodyBlock{s, f});
-  for (uint32_t r = 0; r < level_cells(d.depth - 3u); ++r) {
+  // Subtrees in Morton order of their roots, as the bodies are: the upward
+  // blocks an XCD runs (a contiguous eighth of the table, see xcd_tile)
+  // then read the finest cells that XCD's integrate blocks deposited.
+  const uint32_t rd = d.depth - 3u;
+  for (uint32_t m = 0; m < level_cells(rd); ++m) {
+    uint32_t r = 0u;
+    for (uint32_t b = 0; b < rd; ++b)
+      r |= (((m >> (3u * b)) & 1u) << b) | (((m >> (3u * b + 1u)) & 1u) << (rd + b)) |
+           (((m >> (3u * b + 2u)) & 1u) << (2u * rd + b));
     const uint32_t stripe = static_cast<uint32_t>(sub_.size()) % kStripes;
     caps_host[(d.depth - 1u) * kStripes + stripe] += std::min(64u, ib);
     caps_host[(d.depth - 2u) * kStripes + stripe] += std::min(8u, ib);
