@@ -371,7 +371,9 @@ def test_autotune_restores_state():
     before = s.particles()
     chosen = s.autotune(8)
     assert chosen in (0, 1, 2, 4, 8, 16, 32)
-    assert s.shape in ((1024, 8), (512, 8), (1024, 4), (512, 4))
+    # The tuner times both block widths at every pair tile (8, 4, 1 and the
+    # per-launch choice 0) and keeps the fastest.
+    assert s.shape in {(t, c) for t in (1024, 512) for c in (8, 4, 1, 0)}
     assert_bits(s.particles(), before, "bodies after autotune")
     # The tuned solver still steps bit-exactly.
     s.step(3)
