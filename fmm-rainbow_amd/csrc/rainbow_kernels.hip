@@ -79,6 +79,14 @@ __device__ __forceinline__ gptr<T> G(T* p) {
   return (gptr<T>)p;
 }
 
+// An empty asm taking uniform values in SGPRs: each value is loaded (and
+// waited for) before this point.  Holding a descriptor's words before the
+// branch on its live flag makes them load in one round trip; otherwise the
+// compiler loads the flag alone and the other words after the branch.
+__device__ __forceinline__ void in_sgpr(uint32_t x) { asm volatile("" ::"s"(x)); }
+__device__ __forceinline__ void in_sgpr(float x) { asm volatile("" ::"s"(x)); }
+__device__ __forceinline__ void in_sgpr(const void* p) { asm volatile("" ::"s"(p)); }
+
 __device__ __forceinline__ uint32_t level_off(uint32_t level) {
   return ((1u << (3u * level)) - 1u) / 7u;
 }
@@ -594,6 +602,10 @@ __device__ __forceinline__ void write_item(const StepArgs& a, const PushStage& p
 // cells of a subtree rooted at level r = depth-3 sums levels depth-1..r in
 // LDS, writes them with their source records, updates the target flags of
 // the levels whose parents it holds, and lists the internal cells.
+// kOne: the step holds one system, whose subtree table lists its 8^r roots
+// in Morton order: the block's root follows from its index, and the system
+// descriptor is loaded at entry (no dependent load of the table entry).
+template <bool kOne>
 __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
   static_assert(kUpwardThreads == 512u, "one thread per finest cell of the subtree");
   // Levels of the subtree in one LDS array: 8^3 finest cells at 0, then
@@ -606,8 +618,27 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
   int4* const lvl2 = lv + 576;
   int4* const lvl3 = lv + 584;
   const uint32_t sb = xcd_tile(blockIdx.x, gridDim.x);
-  const SubtreeBlock blk = a.subtree_blocks[sb];
-  const SysDesc d = a.sys[blk.sys];
+  SubtreeBlock blk;
+  SysDesc d;
+  if constexpr (kOne) {
+    d = a.sys[0];
+    blk.sys = 0u;
+    blk.root = lin(compact3(sb), compact3(sb >> 1), compact3(sb >> 2), 1u << (d.depth - 3u));
+  } else {
+    blk = a.subtree_blocks[sb];
+    d = a.sys[blk.sys];
+  }
+  in_sgpr(d.live);
+  in_sgpr(d.depth);
+  in_sgpr(d.n);
+  in_sgpr(d.scale);
+  in_sgpr(d.gravity);
+  in_sgpr(d.moments);
+  in_sgpr(d.active);
+  in_sgpr(d.xflag);
+  in_sgpr(d.extra);
+  in_sgpr(d.finest_alt);
+  in_sgpr(d.source);
   if (!d.live) return;
   const uint32_t t = threadIdx.x;
   const uint32_t D = d.depth;
@@ -618,7 +649,16 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
   if (t < 4u) ps.n[t] = 0u;
   const uint32_t stripe = sb % kStripes;  // the stripe the table gave this subtree
   const uint32_t lev_of[4] = {D - 1u, D - 2u, r, 0u};
-  prefetch_stripes(a, ps, lev_of, 3u, stripe, t);
+  // The stripe's capacity and list offset per staged level (constant between
+  // configurations): uniform loads kept in registers until the items are
+  // written, in flight with the cells' loads.
+  uint32_t scap[3], soff[3];
+#pragma unroll
+  for (uint32_t k = 0; k < 3u; ++k) {
+    const uint32_t si = lev_of[k] * kStripes + stripe;
+    scap[k] = G(a.stripe_cap)[si];
+    soff[k] = G(a.stripe_off)[si];
+  }
 
   // This thread's cells: finest cell t (x fastest, rows of 8 contiguous
   // moments), level D-1 cell t (t < 64), level D-2 cell t - 64 (64 <= t <
@@ -629,12 +669,16 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
   const uint32_t fcell = lin(8u * rx + ix, 8u * ry + iy, 8u * rz + iz, 1u << D);
   const uint32_t fflag = (blk.root << 9) | t;  // flag_index of the cell
   const uint8_t prev_f = G(d.active)[level_off(D) + fflag];
-  uint8_t prev_1 = 0u, prev_2 = 0u;
-  if (t < 64u)
-    prev_1 = G(d.active)[level_off(D - 1u) + lin(4u * rx + (t & 3u), 4u * ry + ((t >> 2) & 3u), 4u * rz + (t >> 4), 1u << (D - 1u))];
-  else if (t < 72u)
-    prev_2 = G(d.active)[level_off(D - 2u) + lin(2u * rx + ((t - 64u) & 1u), 2u * ry + (((t - 64u) >> 1) & 1u),
-                                                   2u * rz + ((t - 64u) >> 2), 1u << (D - 2u))];
+  // The flag of this thread's cell of level D-1 (t < 64) or D-2 (t < 72).
+  uint8_t prev_l = 0u;
+  if (t < 72u) {
+    const uint32_t u = t - 64u;
+    const uint32_t idx = t < 64u ? level_off(D - 1u) + lin(4u * rx + (t & 3u), 4u * ry + ((t >> 2) & 3u),
+                                                            4u * rz + (t >> 4), 1u << (D - 1u))
+                                 : level_off(D - 2u) + lin(2u * rx + (u & 1u), 2u * ry + ((u >> 1) & 1u),
+                                                            2u * rz + (u >> 2), 1u << (D - 2u));
+    prev_l = G(d.active)[idx];
+  }
   // Finest level (this step's buffer) plus the sums deposited into `extra`
   // (wrapping i32 adds, exact in any order), written back when non-zero;
   // `extra` and the other buffer's cells are zeroed for the next step's
@@ -714,7 +758,7 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
     lcell = level_off(D - 1u) + local;
     lm = lvl1[t];
     lparent = lvl2[lin(cx >> 1, cy >> 1, cz >> 1, 2u)].x;
-    lprev = prev_1;
+    lprev = prev_l;
     my_level = 0u;
     mine = true;
   } else if (t < 72u) {
@@ -724,7 +768,7 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
     lcell = level_off(D - 2u) + local;
     lm = lvl2[u];
     lparent = lvl3[0].x;
-    lprev = prev_2;
+    lprev = prev_l;
     my_level = 1u;
     mine = true;
   } else if (t == 72u) {
@@ -755,7 +799,12 @@ __global__ __launch_bounds__(kUpwardThreads) void k_upward(StepArgs a) {
     if (my_level < 2u) update_active(d, lcell, lm.x, lparent, lprev);
   }
   __syncthreads();
-  if (pushes) write_item(a, ps, my_level, ps.base[my_level] + my_slot, my_item);
+  if (pushes) {
+    const uint32_t slot = ps.base[my_level] + my_slot;
+    const uint32_t cap = my_level == 0u ? scap[0] : (my_level == 1u ? scap[1] : scap[2]);
+    const uint32_t off = my_level == 0u ? soff[0] : (my_level == 1u ? soff[1] : soff[2]);
+    if (slot < cap) G(a.lists)[off + slot] = my_item;
+  }
 }
 
 // LDS of the top levels' pass.
@@ -2742,7 +2791,10 @@ hipError_t launch_deposit(const StepArgs& a, uint32_t nblocks, hipStream_t strea
 
 hipError_t launch_upward(const StepArgs& a, uint32_t nblocks, hipStream_t stream, const Timing* t) {
   if (nblocks == 0u) return no_launch(stream, t);
-  RB_LAUNCH(t, k_upward, dim3(nblocks), dim3(kUpwardThreads), stream, a);
+  if (a.nsys == 1u)
+    RB_LAUNCH(t, k_upward<true>, dim3(nblocks), dim3(kUpwardThreads), stream, a);
+  else
+    RB_LAUNCH(t, k_upward<false>, dim3(nblocks), dim3(kUpwardThreads), stream, a);
   return hipGetLastError();
 }
 
