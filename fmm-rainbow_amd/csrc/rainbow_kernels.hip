@@ -1491,6 +1491,11 @@ __global__ __launch_bounds__(kTreeThreads, tree_occupancy(kKind, kCpr)) void k_t
   const bool direct = kKind == kTreeM2l && blockIdx.x < lead;
   const uint32_t first_block = blockIdx.x - (kKind == kTreeM2l ? lead : 0u);
   const uint32_t nlisted = gridDim.x - (kKind == kTreeM2l ? lead : 0u);
+  // System 0's descriptor is loaded at entry, with the work table: for a
+  // one-system step it is the items' descriptor, so the first item need not
+  // load it after its list entry (a launch always has a system 0).
+  SysDesc d = a.sys[0];
+  uint32_t cur_sys = a.nsys == 1u ? 0u : 0xFFFFFFFFu;
 
   // Items of levels hi, hi-1, ..., lo in that order.  The work table holds,
   // per level and stripe, the inclusive prefix of the stripe counts and the
@@ -1582,7 +1587,7 @@ __global__ __launch_bounds__(kTreeThreads, tree_occupancy(kKind, kCpr)) void k_t
       const uint32_t share = static_cast<uint32_t>(uint64_t{gridDim.x} * wlong / (wlong + ngroups + 1ull));
       const uint32_t lblocks = ngroups == 0u ? gridDim.x : min(nlong, min(max(share, 1u), gridDim.x - 1u));
       if (blockIdx.x >= lblocks) {
-        const SysDesc d0 = a.sys[0];
+        const SysDesc& d0 = d;
         if (!d0.live || d0.depth != hi) return;
         ShortWaveLds& W = reinterpret_cast<ShortWaveLds*>(terms_s)[wave_s];
         const uint32_t sblocks = gridDim.x - lblocks;
@@ -1616,8 +1621,6 @@ __global__ __launch_bounds__(kTreeThreads, tree_occupancy(kKind, kCpr)) void k_t
   } else if (first_block < total) {
     next_item = G(a.lists)[locate(first_block, next_level)];
   }
-  uint32_t cur_sys = 0xFFFFFFFFu;
-  SysDesc d;
   for (uint32_t it = it0; it < items; it += stride) {
     const uint32_t level = __builtin_amdgcn_readfirstlane(next_level);
     const uint32_t item = __builtin_amdgcn_readfirstlane(next_item);
@@ -1873,6 +1876,9 @@ __global__ __launch_bounds__(kTreeThreads, kWaveTreeOccupancy) void k_tree_w(Ste
   const uint32_t nwaves = (gridDim.x - lead_blocks) * kTreeWaves;
   const bool direct = gw < ndirect;
   const uint32_t nlisted = nwaves > ndirect ? nwaves - ndirect : 0u;
+  // System 0's descriptor is loaded at entry (as in k_tree).
+  SysDesc d = a.sys[0];
+  uint32_t cur_sys = a.nsys == 1u ? 0u : 0xFFFFFFFFu;
 
   // The work table, as in k_tree: per level and stripe the inclusive prefix
   // of the stripe counts and the stripe's list offset.
@@ -1946,8 +1952,6 @@ __global__ __launch_bounds__(kTreeThreads, kWaveTreeOccupancy) void k_tree_w(Ste
   } else if (it0 < total) {
     next_item = G(a.lists)[locate(it0, next_level)];
   }
-  uint32_t cur_sys = 0xFFFFFFFFu;
-  SysDesc d;
   for (uint32_t it = it0; it < items; it += stride) {
     const uint32_t level = __builtin_amdgcn_readfirstlane(next_level);
     const uint32_t item = __builtin_amdgcn_readfirstlane(next_item);
@@ -2358,9 +2362,24 @@ __global__ __launch_bounds__(kThreads) void k_integrate_deposit(StepArgs a) {
   constexpr uint32_t kPer = kDepositBodies / kThreads;
   __shared__ DepositLds L;
   __shared__ uint2 own_new;
-  reset_lists(a);
+  // The block's entry and descriptor are loaded before block 0 clears the
+  // list counts (loads issued after a store could read it, so they would
+  // become vector loads), and held before the live test: one scalar round
+  // trip.
   const BodyBlock blk = a.deposit_blocks[xcd_tile(blockIdx.x, gridDim.x)];
   const SysDesc d = a.sys[kOne ? 0u : blk.sys];
+  in_sgpr(blk.first);
+  in_sgpr(d.live);
+  in_sgpr(d.n);
+  in_sgpr(d.depth);
+  in_sgpr(d.sort_now);
+  in_sgpr(d.pos);
+  in_sgpr(d.vel);
+  in_sgpr(d.moments);
+  in_sgpr(d.fields);
+  in_sgpr(d.source);
+  in_sgpr(d.finest_alt);
+  reset_lists(a);
   if (!d.live) return;
   const bool staged = sorts(a, d);
   const uint32_t dim = 1u << d.depth;
