@@ -408,6 +408,53 @@ def test_block_shapes_bit_exact(threads, chunks, split):
     s.close()
 
 
+def _random_case(seed):
+    """One seeded configuration: bodies (log-uniform in [1000, 400000]),
+    initial condition, physics, steps, summation form, block shape, sort
+    interval and tree split."""
+    rng = np.random.default_rng(seed)
+    n = int(round(np.exp(rng.uniform(np.log(1000), np.log(400000)))))
+    kind = ["plummer", "ball", "clumps"][int(rng.integers(0, 3))]
+    if kind == "plummer":
+        parts = rf.ic_plummer(n, 600 + seed, float(rng.uniform(0.05, 0.3)), 1.5, 1.0)
+    elif kind == "ball":
+        parts = rf.ic_uniform_ball(n, 600 + seed, float(rng.uniform(0.3, 1.5)))
+    else:
+        parts = rf.ic_clumps(n, 600 + seed, int(rng.integers(2, 65)), 0.02, 10.0, 1.0)
+    phys = dict(dt=float(rng.uniform(1e-4, 3e-3)), gravity=float(rng.uniform(0.2, 2.0)),
+                softening=float(np.exp(rng.uniform(np.log(5e-4), np.log(2e-2)))),
+                speed=float(rng.choice([1.0, 1.5])))
+    steps = int(rng.integers(1, 13))
+    compensated = bool(rng.uniform() < 0.3)
+    shape = SHAPES[int(rng.integers(0, len(SHAPES)))]
+    interval = int(rng.choice([0, 1, 3, 8, 20]))
+    split = [None, 1, 0x7C, 4 | (int(rng.integers(0, 32)) << 2)][int(rng.integers(0, 4))]
+    return n, kind, parts, phys, steps, compensated, shape, interval, split
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_random_configurations_bit_exact(seed):
+    """Seeded random configurations against the oracle: the performance
+    settings (shape, sort interval, tree split) must not change any bit, and
+    every combination of size, distribution, physics and form must match."""
+    n, kind, parts, phys, steps, comp, shape, interval, split = _random_case(seed)
+    say(f"random seed {seed}: {kind} n={n} {phys} steps={steps} compensated={comp} shape={shape} "
+        f"sort interval={interval} split={split}")
+    s = rf.Solver(n, rf.physics(**phys), compensated=comp)
+    s.set_shape(*shape)
+    s.set_sort_interval(interval)
+    if split is not None:
+        s.set_tree_split(split)
+    s.upload(parts)
+    s.step(steps)
+    slots = orc.build_plans(n, **phys)
+    op, om, of = orc.step(slots, parts, steps, compensated=comp)
+    assert_bits(s.particles(), op, "particles")
+    assert_bits(s.moments(), om, "moments")
+    assert_bits(s.fields(), of, "fields")
+    s.close()
+
+
 @pytest.mark.parametrize("threads,chunks", [(512, 4), (1024, 4), (1024, 1)])
 def test_block_shapes_compensated_and_fused(threads, chunks):
     """The compensated form and a fused run of levels (2..5 with ancestor
