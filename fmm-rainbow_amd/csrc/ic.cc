@@ -13,6 +13,10 @@
 
 #include "rainbow_fmm.h"
 
+namespace rainbow {
+int fail(int code, const char* fmt, ...);  // sets the thread's error message (rainbow_runtime.hip)
+}
+
 namespace {
 
 constexpr double kPi = 3.14159265358979323846;
@@ -100,7 +104,8 @@ void write_out(const std::vector<Body>& bodies, void* out) {
 extern "C" {
 
 int rainbow_ic_uniform_ball(uint32_t n, uint64_t seed, double radius, void* out) {
-  if (out == nullptr || n == 0u || !(radius > 0.0)) return RAINBOW_E_INVALID;
+  if (out == nullptr || n == 0u || !(radius > 0.0))
+    return rainbow::fail(RAINBOW_E_INVALID, "uniform ball: need an output buffer, n >= 1 and radius > 0");
   SplitMix64 rng(seed);
   std::vector<Body> bodies(n);
   for (uint32_t i = 0; i < n; ++i) {
@@ -114,7 +119,8 @@ int rainbow_ic_uniform_ball(uint32_t n, uint64_t seed, double radius, void* out)
 }
 
 int rainbow_ic_plummer(uint32_t n, uint64_t seed, double a, double rmax, double gravity, void* out) {
-  if (out == nullptr || n == 0u || !(a > 0.0) || !(rmax > 0.0)) return RAINBOW_E_INVALID;
+  if (out == nullptr || n == 0u || !(a > 0.0) || !(rmax > 0.0))
+    return rainbow::fail(RAINBOW_E_INVALID, "Plummer sphere: need an output buffer, n >= 1, a > 0 and rmax > 0");
   SplitMix64 rng(seed);
   std::vector<Body> bodies(n);
   for (uint32_t i = 0; i < n; ++i) {
@@ -129,7 +135,8 @@ int rainbow_ic_plummer(uint32_t n, uint64_t seed, double a, double rmax, double 
 
 int rainbow_ic_clumps(uint32_t n, uint64_t seed, uint32_t clumps, double a, double rcut_over_a,
                       double gravity, void* out) {
-  if (out == nullptr || n == 0u || clumps == 0u || clumps > n || !(a > 0.0)) return RAINBOW_E_INVALID;
+  if (out == nullptr || n == 0u || clumps == 0u || clumps > n || !(a > 0.0))
+    return rainbow::fail(RAINBOW_E_INVALID, "clumps: need an output buffer, 1 <= clumps <= n and a > 0");
   SplitMix64 rng(seed);
   std::vector<double> w(clumps), centre(3u * clumps);
   double wsum = 0.0;

@@ -145,7 +145,8 @@ uint32_t rainbow_fnv1a(const void* data, size_t bytes);
  * synthetic inputs of the benchmark workloads.  SplitMix64 stream, f64
  * generation rounded to f32, equal masses 1/N, hue uniform in [0,1),
  * excitation 0.  `out` receives 32 N bytes in the reference layout.
- * Return 0 or RAINBOW_E_INVALID. */
+ * Return 0 or RAINBOW_E_INVALID (with rainbow_last_error() naming the
+ * violated bound: n >= 1, a positive radius or scale, 1 <= clumps <= n). */
 /* Uniform in a ball of `radius`, at rest. */
 int rainbow_ic_uniform_ball(uint32_t n, uint64_t seed, double radius, void* out);
 /* Plummer sphere of scale radius a and total mass 1 truncated at rmax, with

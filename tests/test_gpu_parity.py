@@ -233,6 +233,48 @@ def test_batch_equals_solo():
     b.close()
 
 
+@pytest.mark.parametrize("seed", range(4))
+def test_random_batches_bit_exact(seed):
+    """Seeded random batches: 3..10 systems of 1..300000 bodies (Plummer,
+    ball or clumps) under shared random physics, steps, summation form,
+    shape, sort interval and tree split; every system against the oracle."""
+    rng = np.random.default_rng(700 + seed)
+    k = int(rng.integers(3, 11))
+    counts = [int(round(np.exp(rng.uniform(0.0, np.log(300000))))) for _ in range(k)]
+    inputs = []
+    for i, n in enumerate(counts):
+        kind = int(rng.integers(0, 3))
+        if kind == 0:
+            inputs.append(rf.ic_plummer(n, 7000 + 16 * seed + i, 0.1, 1.5, 1.0))
+        elif kind == 1:
+            inputs.append(rf.ic_uniform_ball(n, 7000 + 16 * seed + i, float(rng.uniform(0.3, 1.5))))
+        else:
+            inputs.append(rf.ic_clumps(n, 7000 + 16 * seed + i, min(n, int(rng.integers(2, 33))), 0.02, 10.0, 1.0))
+    phys = dict(dt=float(rng.uniform(1e-4, 3e-3)), gravity=float(rng.uniform(0.2, 2.0)),
+                softening=float(np.exp(rng.uniform(np.log(5e-4), np.log(2e-2)))))
+    steps = int(rng.integers(1, 13))
+    comp = bool(rng.uniform() < 0.3)
+    shape = SHAPES[int(rng.integers(0, len(SHAPES)))]
+    interval = int(rng.choice([0, 1, 3, 8, 20]))
+    split = [None, 1, 0x7C, 4 | (int(rng.integers(0, 32)) << 2)][int(rng.integers(0, 4))]
+    say(f"random batch {seed}: counts {counts} {phys} steps={steps} compensated={comp} shape={shape} "
+        f"sort interval={interval} split={split}")
+    b = rf.Batch(counts, rf.physics(**phys), compensated=comp)
+    b.set_shape(*shape)
+    b.set_sort_interval(interval)
+    if split is not None:
+        b.set_tree_split(split)
+    for i, p in enumerate(inputs):
+        b.upload(i, p)
+    b.step(steps)
+    for i, n in enumerate(counts):
+        op, om, of = orc.step(orc.build_plans(n, **phys), inputs[i], steps, compensated=comp)
+        assert_bits(b.download(i, rf.PARTICLES), op, f"system {i} (N={n}) particles")
+        assert_bits(b.download(i, rf.MOMENTS), om, f"system {i} (N={n}) moments")
+        assert_bits(b.download(i, rf.FIELDS), of, f"system {i} (N={n}) fields")
+    b.close()
+
+
 def test_graph_replay_consistency():
     say("graph replays: 16-step graph vs single steps vs eager oracle")
     n = 100000

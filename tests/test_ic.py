@@ -55,3 +55,21 @@ def test_clumps():
     x = p[:, :3]
     assert np.abs(x).max() < 0.5 + 0.2 + 1e-3
     assert np.isfinite(p).all()
+
+
+def test_invalid_arguments_are_rejected_with_a_message():
+    import pytest
+
+    cases = [
+        (lambda: rf.ic_uniform_ball(100, 1, 0.0), "uniform ball"),
+        (lambda: rf.ic_plummer(100, 1, -0.1, 1.5, 1.0), "Plummer sphere"),
+        (lambda: rf.ic_plummer(100, 1, 0.1, 0.0, 1.0), "Plummer sphere"),
+        (lambda: rf.ic_clumps(9, 1, 32, 0.02, 10.0, 1.0), "clumps"),
+        (lambda: rf.ic_clumps(100, 1, 0, 0.02, 10.0, 1.0), "clumps"),
+    ]
+    for call, what in cases:
+        with pytest.raises(rf.RainbowError) as e:
+            call()
+        assert e.value.code == rf.E_INVALID and what in str(e.value)
+    # A clump per body is allowed.
+    assert rf.ic_clumps(9, 1, 9, 0.02, 10.0, 1.0).shape == (9, 8)
